@@ -1,0 +1,316 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+* static closed forms against hand-derived golden vectors and scipy's
+  textbook routines (BFS, Dijkstra, connected components), SSWP against the
+  threshold characterisation and brute-force path enumeration;
+* the tree checker against planted faults;
+* the cpu_ref store against the brute-force multiset model and golden ops;
+* the cpu_ref incremental engine against the static closed form after every
+  update of seeded sliding-window streams (all four algorithms, single and
+  batched), with tree validity each time.
+"""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse import csgraph
+
+import oracle
+from oracle import BFS, SSSP, SSWP, WCC
+from oracle.model import MultigraphModel
+import streams
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")
+MAXL_BFS, MAXL_W = 65536, 134217728
+
+
+def _golden():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("g", _golden()["graphs"], ids=lambda g: g["name"])
+def test_static_known_answers(g):
+    e = np.array(g["edges"], dtype=np.uint64)
+    for name, algo in oracle.ALGOS.items():
+        if name not in g:
+            continue
+        got = oracle.static_values(algo, g["V"], e[:, 0], e[:, 1], e[:, 2], g["root"], g.get("init"))
+        assert got.tolist() == g[name], (g["name"], name)
+
+
+def _random_graph(V, m, seed, wmax=20):
+    rng = np.random.default_rng(seed)
+    s = rng.integers(0, V, m).astype(np.uint64)
+    d = rng.integers(0, V, m).astype(np.uint64)
+    w = rng.integers(1, wmax + 1, m).astype(np.uint64)
+    return s, d, w
+
+
+def _min_weight_csr(V, s, d, w):
+    """Directed CSR keeping the minimum weight of parallel edges (scipy sums duplicates)."""
+    best = {}
+    for a, b, c in zip(s.tolist(), d.tolist(), w.tolist()):
+        if a == b:
+            continue
+        best[(a, b)] = min(best.get((a, b), c), c)
+    if not best:
+        return sp.csr_matrix((V, V))
+    k = np.array(list(best.keys()))
+    return sp.csr_matrix((np.array(list(best.values()), dtype=np.float64), (k[:, 0], k[:, 1])), shape=(V, V))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_static_vs_scipy(seed):
+    V, m = 300, 900 + 150 * seed
+    s, d, w = _random_graph(V, m, seed)
+    root = int(s[0])
+    A = _min_weight_csr(V, s, d, w)
+    # BFS levels: unweighted shortest paths
+    U = A.copy()
+    U.data[:] = 1.0
+    lev = csgraph.shortest_path(U, method="D", directed=True, indices=root, unweighted=True)
+    exp = np.where(np.isinf(lev), MAXL_BFS, lev).astype(np.uint64)
+    assert np.array_equal(oracle.static_values(BFS, V, s, d, w, root), exp)
+    # SSSP: Dijkstra
+    dist = csgraph.dijkstra(A, directed=True, indices=root)
+    exp = np.where(np.isinf(dist), MAXL_W, dist).astype(np.uint64)
+    assert np.array_equal(oracle.static_values(SSSP, V, s, d, w, root), exp)
+    # WCC: min vertex id per undirected component
+    n, lab = csgraph.connected_components(sp.csr_matrix((np.ones(m), (s.astype(np.int64), d.astype(np.int64))),
+                                                        shape=(V, V)), directed=False)
+    first = np.full(n, V)
+    for v in range(V):
+        first[lab[v]] = min(first[lab[v]], v)
+    assert np.array_equal(oracle.static_values(WCC, V, s, d), first[lab].astype(np.uint64))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_sswp_threshold_characterisation(seed):
+    """width(v) >= t  <=>  v reachable from root over edges with weight >= t."""
+    V, m = 200, 700
+    s, d, w = _random_graph(V, m, 100 + seed, wmax=50)
+    root = int(s[0])
+    width = oracle.static_values(SSWP, V, s, d, w, root)
+    assert width[root] == MAXL_W
+    for t in sorted(set(w.tolist())):
+        keep = w >= t
+        A = sp.csr_matrix((np.ones(int(keep.sum())), (s[keep].astype(np.int64), d[keep].astype(np.int64))),
+                          shape=(V, V))
+        reach = np.zeros(V, bool)
+        reach[csgraph.breadth_first_order(A, root, directed=True, return_predecessors=False)] = True
+        assert np.array_equal(width >= t, reach), t
+    # unreached vertices have width 0, never a positive value
+    assert np.all((width > 0) == np.isin(np.arange(V), csgraph.breadth_first_order(
+        sp.csr_matrix((np.ones(m), (s.astype(np.int64), d.astype(np.int64))), shape=(V, V)), root,
+        return_predecessors=False)))
+
+
+def _brute_paths(V, edges, root):
+    """All simple paths from root (tiny graphs): BFS/SSSP minima and SSWP maxima."""
+    adj = {}
+    for a, b, c in edges:
+        adj.setdefault(a, []).append((b, c))
+    best = {"bfs": [MAXL_BFS] * V, "sssp": [MAXL_W] * V, "sswp": [0] * V}
+    best["bfs"][root] = 0
+    best["sssp"][root] = 0
+    best["sswp"][root] = MAXL_W
+
+    def rec(u, seen, hops, dist, width):
+        for v, c in adj.get(u, []):
+            if v in seen:
+                continue
+            h, dd, ww = hops + 1, dist + c, min(width, c)
+            best["bfs"][v] = min(best["bfs"][v], h)
+            best["sssp"][v] = min(best["sssp"][v], dd)
+            best["sswp"][v] = max(best["sswp"][v], ww)
+            rec(v, seen | {v}, h, dd, ww)
+
+    rec(root, {root}, 0, 0, MAXL_W)
+    return best
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_static_vs_brute_force(seed):
+    V, m = 7, 14
+    s, d, w = _random_graph(V, m, 500 + seed, wmax=9)
+    root = int(s[0])
+    exp = _brute_paths(V, list(zip(s.tolist(), d.tolist(), w.tolist())), root)
+    for name in ("bfs", "sssp", "sswp"):
+        got = oracle.static_values(oracle.ALGOS[name], V, s, d, w, root)
+        assert got.tolist() == exp[name], name
+
+
+def test_bfs_sentinel_caps_long_paths():
+    """s+1 < 65536 never holds at depth 65535, so deeper vertices stay 65536."""
+    V = 66000
+    s = np.arange(V - 1, dtype=np.uint64)
+    d = s + np.uint64(1)
+    got = oracle.static_values(BFS, V, s, d, None, 0)
+    exp = np.minimum(np.arange(V), MAXL_BFS).astype(np.uint64)
+    assert np.array_equal(got, exp)
+    g = oracle.RefGraph(V, V, symmetric=False, dual=True)
+    g.add_edges(s, d)
+    L = oracle.RefLabels(g, BFS, 0)
+    L.build()
+    assert np.array_equal(L.read()[0], exp)
+
+
+def test_tree_checker_catches_planted_faults():
+    g = [x for x in _golden()["graphs"] if x["name"] == "diamond"][0]
+    e = np.array(g["edges"], dtype=np.uint64)
+    s, d, w = e[:, 0], e[:, 1], e[:, 2]
+    V = g["V"]
+    vals = np.array(g["sssp"], dtype=np.uint64)
+    par = np.array([V, 0, 0, 1, 3], dtype=np.uint64)     # 3 via 1 (2+4) is valid
+    assert oracle.check_tree(SSSP, V, s, d, w, 0, vals, par) == 0
+    par2 = par.copy(); par2[3] = 2                         # 5+1 == 6 also valid (tie)
+    assert oracle.check_tree(SSSP, V, s, d, w, 0, vals, par2) == 0
+    bad = par.copy(); bad[4] = 1                           # no edge 1->4
+    assert oracle.check_tree(SSSP, V, s, d, w, 0, vals, bad) == 5
+    bad = par.copy(); bad[0] = 1                           # root must have no parent
+    assert oracle.check_tree(SSSP, V, s, d, w, 0, vals, bad) == 1
+    badv = vals.copy(); badv[4] = 12                       # value inconsistent with edge
+    assert oracle.check_tree(SSSP, V, s, d, w, 0, badv, par) == 5
+    # cycle with consistent values: WCC labels, 0<->1 pointing at each other
+    cs = np.array([0, 1], np.uint64); cd = np.array([1, 0], np.uint64)
+    cv = np.array([5, 5, 2], np.uint64)
+    init = np.array([7, 8, 2], np.uint64)
+    cp = np.array([1, 0, 3], np.uint64)
+    assert oracle.check_tree(WCC, 3, cs, cd, None, 0, cv, cp, init=init) != 0
+
+
+@pytest.mark.parametrize("case", _golden()["store_ops"], ids=lambda c: c["name"])
+def test_store_golden_ops(case):
+    g = oracle.RefGraph(case["V"], 16, symmetric=case["symmetric"], dual=True, weighted=case["weighted"])
+    m = MultigraphModel(case["V"], symmetric=case["symmetric"], weighted=case["weighted"])
+    for op, a, b, w, directed, expect in case["ops"]:
+        f = g.add_edge if op == "add" else g.del_edge
+        fm = m.add_edge if op == "add" else m.del_edge
+        assert f(a, b, w, directed) == expect
+        assert fm(a, b, w, directed) == expect
+    for a, b, w, c in case["final_counts"]:
+        assert g.edge_num(a, b, w) == c
+        assert m.count(a, b, w) == c
+
+
+@pytest.mark.parametrize("symmetric,weighted", [(False, False), (False, True), (True, False)])
+def test_store_matches_model_with_hubs(symmetric, weighted):
+    """Random add/del mix incl. a hub list above the 512-entry index threshold."""
+    rng = np.random.default_rng(7)
+    V = 1500
+    g = oracle.RefGraph(V, 0, symmetric=symmetric, dual=True, weighted=weighted)
+    m = MultigraphModel(V, symmetric=symmetric, weighted=weighted)
+    for k in range(6000):
+        a = 0 if rng.random() < 0.5 else int(rng.integers(0, V))
+        b = int(rng.integers(0, V))
+        w = int(rng.integers(1, 4))
+        directed = True if not symmetric else bool(rng.random() < 0.2)
+        if rng.random() < 0.65:
+            assert g.add_edge(a, b, w, directed) == m.add_edge(a, b, w, directed), k
+        else:
+            assert g.del_edge(a, b, w, directed) == m.del_edge(a, b, w, directed), k
+    assert g.degree(0) == m.degree(0)
+    assert g.degree(0) > 512
+    s1, d1, w1 = g.live_edges()
+    s2, d2, w2 = m.live_edges()
+    assert sorted(zip(s1.tolist(), d1.tolist(), w1.tolist())) == sorted(zip(s2.tolist(), d2.tolist(), w2.tolist()))
+
+
+def _window_stream(kind, V, E, load_frac, seed=1, wseed=2):
+    if kind == "rmat":
+        s, d = streams.rmat(int(np.log2(V)), E, seed)
+    else:
+        s, d = streams.erdos_renyi(V, E, seed)
+    w = streams.weights(E, wseed)
+    return s, d, w, int(E * load_frac)
+
+
+CASES = [(BFS, "rmat", False), (SSSP, "rmat", True), (SSWP, "rmat", True), (WCC, "rmat", False),
+         (WCC, "er", False), (BFS, "er", False)]
+
+
+@pytest.mark.parametrize("algo,kind,weighted", CASES)
+def test_cpu_ref_plain_stream_matches_static(algo, kind, weighted):
+    V, E = 512, 4096
+    s, d, w, n0 = _window_stream(kind, V, E, 0.9)
+    sym = algo == WCC
+    directed = not sym
+    g = oracle.RefGraph(V, E, symmetric=sym, dual=not sym, weighted=weighted)
+    m = MultigraphModel(V, symmetric=sym, weighted=weighted)
+    g.add_edges(s[:n0], d[:n0], w[:n0], directed)
+    for i in range(n0):
+        m.add_edge(int(s[i]), int(d[i]), int(w[i]), directed)
+    root = streams.max_out_degree_root(s[:n0], V)
+    L = oracle.RefLabels(g, algo, root)
+    L.build()
+    for k in range(E - n0):
+        i, j = n0 + k, k
+        ww = int(w[i]) if weighted else 0
+        old = g.add_edge(int(s[i]), int(d[i]), ww, directed)
+        assert old == m.add_edge(int(s[i]), int(d[i]), ww, directed)
+        if old == 0:
+            L.inc_add(int(s[i]), int(d[i]), ww, directed)
+        wj = int(w[j]) if weighted else 0
+        old = g.del_edge(int(s[j]), int(d[j]), wj, directed)
+        assert old == m.del_edge(int(s[j]), int(d[j]), wj, directed)
+        if old == 1:
+            L.inc_del(int(s[j]), int(d[j]), wj, directed)
+        ls, ld, lw = m.live_edges()
+        exp = oracle.static_values(algo, V, ls, ld, lw, root)
+        vals, par, _ = L.read()
+        assert np.array_equal(vals, exp), k
+        assert oracle.check_tree(algo, V, ls, ld, lw, root, vals, par) == 0, k
+
+
+@pytest.mark.parametrize("algo,kind,weighted", CASES)
+@pytest.mark.parametrize("batch", [7, 64])
+def test_cpu_ref_batched_stream_matches_static(algo, kind, weighted, batch):
+    V, E = 512, 4096
+    s, d, w, n0 = _window_stream(kind, V, E, 0.8, seed=3)
+    sym = algo == WCC
+    directed = not sym
+    g = oracle.RefGraph(V, E, symmetric=sym, dual=not sym, weighted=weighted)
+    g.add_edges(s[:n0], d[:n0], w[:n0], directed)
+    root = streams.max_out_degree_root(s[:n0], V)
+    L = oracle.RefLabels(g, algo, root)
+    L.build()
+    wz = w if weighted else np.zeros_like(w)
+    for b0 in range(n0, E, batch):
+        b1 = min(b0 + batch, E)
+        idx = np.arange(b0, b1)
+        olds = [g.add_edge(int(s[i]), int(d[i]), int(wz[i]), directed) for i in idx]
+        new = idx[np.array(olds) == 0]
+        if len(new):
+            L.inc_add_batch(s[new], d[new], wz[new], directed)
+        didx = idx - n0
+        olds = [g.del_edge(int(s[i]), int(d[i]), int(wz[i]), directed) for i in didx]
+        gone = didx[np.array(olds) == 1]
+        if len(gone):
+            L.inc_del_batch(s[gone], d[gone], wz[gone], directed)
+        ls, ld, lw = g.live_edges()
+        exp = oracle.static_values(algo, V, ls, ld, lw, root)
+        vals, par, _ = L.read()
+        assert np.array_equal(vals, exp), b0
+        assert oracle.check_tree(algo, V, ls, ld, lw, root, vals, par) == 0, b0
+
+
+def test_cpu_ref_recompute_fallback_keeps_values():
+    """The deletion fallback (graph.hpp:1252) is a full build: values unchanged."""
+    V = 5000
+    # a long chain 0->1->...->V-1: deleting 0->1 invalidates > 0.05V but the
+    # 2,000,000 floor keeps the reference from recomputing; check values anyway
+    s = np.arange(V - 1, dtype=np.uint64)
+    d = s + np.uint64(1)
+    g = oracle.RefGraph(V, V, False, True, False)
+    g.add_edges(s, d)
+    g.add_edge(0, V // 2)
+    L = oracle.RefLabels(g, BFS, 0)
+    L.build()
+    assert g.del_edge(0, 1) == 1
+    L.inc_del(0, 1)
+    ls, ld, lw = g.live_edges()
+    assert np.array_equal(L.read()[0], oracle.static_values(BFS, V, ls, ld, lw, 0))
