@@ -1,0 +1,195 @@
+/*
+ * risgraph_amd.h -- C ABI of the MI355X (gfx950) incremental graph hot path.
+ *
+ * This is the drop-in boundary under the Graph<> interface of the reference
+ * (core/graph.hpp).  Plain pointers and sizes only; every call returns an
+ * int status (RG_OK on success) and rg_last_error() holds a per-thread
+ * message.  Handles own their device memory.  Calls on one graph handle are
+ * serialised by the library (the reference engine is not reentrant either:
+ * graph.hpp:1728-1730 shares active sets between calls).
+ *
+ * Vertex ids are u64 at the boundary and must be < 2^31 (the device keeps
+ * u32 ids; the reference allows 48-bit ids, type.hpp:23).  Labels are u64 at
+ * the boundary and must be < 2^32 (every reference algorithm's labels are:
+ * 65536, 2^27 and vertex ids).  Empty parent = V (graph.hpp:90).
+ */
+#ifndef RISGRAPH_AMD_H
+#define RISGRAPH_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define RG_OK 0
+#define RG_EINVAL_VERTEX 1   /* graph.hpp:205, 231 "VertexId error." */
+#define RG_ENOMEM 2
+#define RG_EALGO 3           /* unrecognised algorithm */
+#define RG_EDEVICE 4         /* HIP error or device-side failure */
+#define RG_EINVAL 5          /* bad argument */
+
+/* algorithms (driver lambdas: bfs_inc.cpp:77-92, sssp_inc.cpp:82-97,
+ * sswp_inc.cpp:82-97, wcc_inc.cpp:82-97) */
+#define RG_BFS 0
+#define RG_SSSP 1
+#define RG_SSWP 2
+#define RG_WCC 3
+
+/* Edge<u64> (type.hpp:55-64): src, dst, data (weight; 0 when unweighted). */
+typedef struct rg_edge {
+    uint64_t src, dst, data;
+} rg_edge;
+
+typedef struct rg_graph rg_graph;
+typedef struct rg_labels rg_labels;
+typedef struct rg_stream rg_stream;
+
+/* Per-thread description of the last error. */
+const char *rg_last_error(void);
+
+/* Graph(vertices, hint_edges, symmetric, dual) -- graph.hpp:61-100.
+ * `weighted` selects Graph<u64> (weights part of edge identity, type.hpp:32-35)
+ * versus Graph<void>.  The slot pool is sized from hint_edges. */
+int rg_graph_create(uint64_t vertices, uint64_t hint_edges, int symmetric, int dual,
+                    int weighted, rg_graph **out);
+int rg_graph_destroy(rg_graph *g);
+/* vertices, slot-pool use/capacity (entries), hash-pool use/capacity */
+int rg_graph_info(rg_graph *g, uint64_t *vertices, uint64_t *slots_used, uint64_t *slots_cap,
+                  uint64_t *hash_used, uint64_t *hash_cap);
+
+/* add_edge / del_edge -- graph.hpp:203-253 over IndexedEdgeStorage::update_edge
+ * (storage.hpp:345-398).  delta = +1 add, -1 del.  *old = the reference return
+ * value (old multiplicity; undirected: old(d->s) - (s == d) on add). */
+int rg_edge_update(rg_graph *g, uint64_t src, uint64_t dst, uint64_t data, int delta, int directed,
+                   uint64_t *old);
+/* n edge updates with the same delta, applied as one device batch (the
+ * reference batch drivers' parallel add_edge / del_edge loops,
+ * bfs_inc_batch.cpp:142-173).  old_out[i] receives each return value. */
+int rg_edge_update_batch(rg_graph *g, const rg_edge *edges, uint64_t n, int delta, int directed,
+                         uint64_t *old_out);
+/* get_outgoing_degree / get_incoming_degree (graph.hpp:155-164) and
+ * get_edge_num (graph.hpp:198-201). */
+int rg_degree(rg_graph *g, uint64_t v, int incoming, uint64_t *out);
+int rg_edge_num(rg_graph *g, uint64_t src, uint64_t dst, uint64_t data, uint64_t *out);
+
+/* alloc_vertex_tree_array (graph.hpp:255-262) bound to one algorithm.
+ * root is used by BFS/SSSP/SSWP; init_values (length V, may be NULL) gives
+ * WCC labels (all_inc.cpp:172-178 uses a permutation; NULL = identity).
+ * trace_modified keeps the set of vertices whose label changed
+ * (graph.hpp:117-130).  Labels are all "empty" until rg_build. */
+int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_values,
+                     int trace_modified, rg_labels **out);
+int rg_labels_destroy(rg_labels *L);
+/* Strategy knobs of do_update_tree_del (graph.hpp:1252, 1324, 1900) plus the
+ * work size below which a whole update runs in one persistent workgroup.
+ * Defaults: rescue while level <= 5 or frontier < 16384; full recompute when
+ * level <= 10 and subtree+frontier > max(0.05*V, 2000000); ancestor walks
+ * capped at walk_limit steps (0 = min(V/10, 65536)); block_max = 4096.
+ * Values never depend on these; only the work done does. */
+int rg_labels_set_tuning(rg_labels *L, uint32_t rescue_level, uint32_t rescue_frontier,
+                         uint32_t recompute_level, uint32_t recompute_abs, double recompute_frac,
+                         uint32_t walk_limit, uint32_t block_max);
+
+/* build_tree -- graph.hpp:873-985: full fixpoint from the init labels. */
+int rg_build(rg_labels *L, uint64_t *result);
+/* update_tree_add (single and batch) -- graph.hpp:1124-1218.  The edges are
+ * already in the store (the caller ran rg_edge_update and saw old == 0). */
+int rg_inc_add(rg_labels *L, const rg_edge *edges, uint64_t n, int directed, uint64_t *result);
+/* update_tree_del (single and batch) -- graph.hpp:1622-1718.  The edges are
+ * already removed from the store (old == 1). */
+int rg_inc_del(rg_labels *L, const rg_edge *edges, uint64_t n, int directed, uint64_t *result);
+/* need_update_tree_add / need_update_tree_del (graph.hpp:987-1000,
+ * 1220-1233) for n edges at once: unsafe[i] = 1 when edge i would change a
+ * label (kind[i] = +1 add, -1 del).  Reads the current labels. */
+int rg_classify(rg_labels *L, const rg_edge *edges, const int8_t *kind, uint64_t n, int directed,
+                uint8_t *unsafe);
+
+/* labels[first .. first+count): value and parent vertex (V = empty). */
+int rg_labels_read(rg_labels *L, uint64_t first, uint64_t count, uint64_t *values, uint64_t *parents);
+/* get_modified() (graph.hpp:117-125): vertices whose label changed since the
+ * last clear.  *n = count written (<= cap); *all = 1 after a full rebuild
+ * (every vertex may have changed). */
+int rg_modified_read(rg_labels *L, uint64_t *vids, uint64_t cap, uint64_t *n, int *all);
+/* clear_modified() (graph.hpp:127-130). */
+int rg_modified_clear(rg_labels *L);
+
+/* Device-resident synthetic streams (edge arrays that never leave HBM).
+ * kind 0 = R-MAT (scale_or_v = scale, a,b,c probabilities), 1 = Erdos-Renyi
+ * (scale_or_v = V).  Weights are generated when wseed != 0. */
+int rg_stream_create(int kind, uint64_t scale_or_v, uint64_t n_edges, uint64_t seed, double a,
+                     double b, double c, uint64_t wseed, rg_stream **out);
+int rg_stream_destroy(rg_stream *s);
+/* copy stream edges [first, first+n) to host arrays (any may be NULL) */
+int rg_stream_read(rg_stream *s, uint64_t first, uint64_t n, uint64_t *src, uint64_t *dst, uint32_t *w);
+/* vertex with the largest out-degree among stream edges [0, n) (ties: lowest id) */
+int rg_stream_max_degree_vertex(rg_stream *s, uint64_t n, uint64_t V, uint64_t *out);
+
+/* Bulk add_edge of stream edges [first, first+n) (the drivers' load loop,
+ * bfs_inc.cpp:59-69).  No labels are touched. */
+int rg_stream_load(rg_graph *g, rg_stream *s, uint64_t first, uint64_t n, int directed);
+
+/* One sliding-window batch on device-resident data, the fused form of a
+ * batch driver round (bfs_inc_batch.cpp:142-178): add stream edges
+ * [ins_first, ins_first+n_ins), delete [del_first, del_first+n_del), then
+ * repair every labels instance in L[0..nL) (inserts propagate first).
+ * n_ins = n_del = 1 is one plain update (bfs_inc.cpp:160-189). */
+typedef struct rg_batch_stats {
+    uint64_t seeds;          /* insert edges that reached the engine */
+    uint64_t roots;          /* delete edges whose multiplicity hit 0 */
+    uint64_t levels;         /* engine levels run (all labels) */
+    uint64_t grid_steps;     /* passes that ran grid-wide */
+    uint64_t rebuilds;       /* full recomputes taken by the deletion fallback */
+} rg_batch_stats;
+int rg_stream_apply(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first,
+                    uint64_t n_ins, uint64_t del_first, uint64_t n_del, int directed,
+                    rg_batch_stats *stats);
+
+/* Run `n_steps` plain updates back to back (step k: add ins_first+k, then
+ * delete del_first+k, each followed by its repair), the per-update GPU path
+ * of bfs_inc.cpp:160-189.  lat_ns (may be NULL) receives each step's wall
+ * time in ns. */
+int rg_stream_plain(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first,
+                    uint64_t del_first, uint64_t n_steps, int directed, double *lat_ns,
+                    rg_batch_stats *stats);
+
+/* Device timing of the most recent rg_stream_apply batch: GPU milliseconds
+ * between events recorded before its store kernels and after its last
+ * engine kernel. */
+int rg_last_gpu_ms(rg_graph *g, double *ms);
+
+/* Serving loop: the device-side replacement of the latency-aware scheduler
+ * of the *_inc_rt drivers (bfs_inc_rt.cpp:154-431).  Updates u = 0..n-1
+ * alternate add (stream edge ins_first + u/2) and delete (del_first + u/2),
+ * the all_inc stream (all_inc.cpp:254-300), and arrive by a Poisson process
+ * of `rate` updates/s from `clients` simulated clients (client u % clients).
+ * Every loop iteration admits all arrived updates up to the current batch
+ * cap, applies them as one device batch to the store and every labels
+ * instance in L, and completes them together.  With target_ms > 0 the cap
+ * adapts like the reference's num_changes_threshold (x0.9 when a batch's
+ * oldest request waited past half the target, else +16 / x1.01, clamped to
+ * [1, cap_max]); batch_cap = 1 is the plain one-update-at-a-time path.
+ * Latency = completion time - arrival time (no coordinated omission). */
+typedef struct rg_serve_params {
+    uint64_t ins_first, del_first, n_updates;
+    double rate;
+    uint64_t seed;
+    uint32_t clients;
+    uint32_t batch_cap;       /* initial cap (1 = plain path) */
+    uint32_t cap_max;
+    double target_ms;         /* 0 = fixed cap */
+    int directed;
+} rg_serve_params;
+typedef struct rg_serve_result {
+    double wall_s, updates_per_s;
+    double p50_ms, p99_ms, p999_ms, max_ms, mean_ms;
+    double mean_batch;
+    uint64_t batches, max_batch, grid_steps, rebuilds;
+} rg_serve_result;
+int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_params *p,
+             rg_serve_result *r, double *lat_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

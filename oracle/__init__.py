@@ -48,6 +48,8 @@ def lib():
             "or_add_edge": (u64, [vp, u64, u64, u64, ctypes.c_int]),
             "or_del_edge": (u64, [vp, u64, u64, u64, ctypes.c_int]),
             "or_add_edges": (None, [vp, u64, vp, vp, vp, ctypes.c_int]),
+            "or_bulk_load": (ctypes.c_int, [vp, u64, vp, vp, vp, ctypes.c_int]),
+            "or_labels_assign": (ctypes.c_int64, [vp, vp, vp, vp]),
             "or_degree": (u64, [vp, u64, ctypes.c_int]),
             "or_edge_num": (u64, [vp, u64, u64, u64]),
             "or_live_edges": (u64, [vp, vp, vp, vp]),
@@ -134,6 +136,13 @@ class RefGraph:
         src, dst, w = _u64(src), _u64(dst), _u64(w)
         lib().or_add_edges(self.h, len(src), _p(src), _p(dst), _p(w), int(directed))
 
+    def bulk_load(self, src, dst, w=None, directed=True):
+        """Same store as add_edges, built by sorting (graph must be empty)."""
+        src, dst, w = _u64(src), _u64(dst), _u64(w)
+        rc = lib().or_bulk_load(self.h, len(src), _p(src), _p(dst), _p(w), int(directed))
+        if rc != 0:
+            raise ValueError("or_bulk_load: graph not empty or id out of range")
+
     def degree(self, v, incoming=False):
         return int(lib().or_degree(self.h, v, int(incoming)))
 
@@ -200,6 +209,13 @@ class RefLabels:
         pw = np.empty(V, np.uint64)
         lib().or_labels_read(self.h, _p(v), _p(p), _p(pw))
         return v, p, pw
+
+    def assign(self, values, parents):
+        """Start from a given fixpoint (values, parent vertices) instead of build()."""
+        values, parents = _u64(values), _u64(parents)
+        bad = int(lib().or_labels_assign(self.g.h, self.h, _p(values), _p(parents)))
+        if bad:
+            raise ValueError(f"or_labels_assign: no consistent parent edge at vertex {bad - 1}")
 
     def recomputes(self):
         return int(lib().or_recompute_count(self.h))

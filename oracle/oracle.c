@@ -477,6 +477,101 @@ void or_add_edges(or_graph *g, uint64_t n, const uint64_t *src, const uint64_t *
     for (uint64_t i = 0; i < n; i++) or_add_edge(g, src[i], dst[i], w ? w[i] : 0, directed);
 }
 
+/* Bulk load into an empty graph: the same multiset store (entries, counts,
+ * degrees, hub indexes) that n sequential or_add_edge calls build, made by
+ * counting sort + per-list sort and run-length merge.  Entries sit in
+ * (nbr, data) order instead of insertion order, which changes no value. */
+typedef struct { uint64_t nbr, data; } key_t_;
+
+static int key_cmp(const void *a, const void *b)
+{
+    const key_t_ *x = a, *y = b;
+    if (x->nbr != y->nbr) return x->nbr < y->nbr ? -1 : 1;
+    if (x->data != y->data) return x->data < y->data ? -1 : 1;
+    return 0;
+}
+
+static void bulk_store(or_store *st, uint64_t V, uint64_t m, const uint64_t *lv,
+                       const uint64_t *lnbr, const uint64_t *ldata)
+{
+    uint64_t *off = calloc(V + 1, sizeof(uint64_t));
+    for (uint64_t i = 0; i < m; i++) off[lv[i] + 1]++;
+    for (uint64_t v = 0; v < V; v++) off[v + 1] += off[v];
+    uint64_t *pos = malloc((V + 1) * sizeof(uint64_t));
+    memcpy(pos, off, (V + 1) * sizeof(uint64_t));
+    key_t_ *k = malloc((m ? m : 1) * sizeof(key_t_));
+    for (uint64_t i = 0; i < m; i++) {
+        uint64_t p = pos[lv[i]]++;
+        k[p].nbr = lnbr[i];
+        k[p].data = ldata[i];
+    }
+    free(pos);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (uint64_t v = 0; v < V; v++) {
+        uint64_t b = off[v], e = off[v + 1];
+        if (b == e) continue;
+        qsort(k + b, e - b, sizeof(key_t_), key_cmp);
+        or_adj *a = &st->v[v];
+        uint64_t uniq = 1;
+        for (uint64_t i = b + 1; i < e; i++) uniq += key_cmp(&k[i - 1], &k[i]) != 0;
+        a->a = malloc(uniq * sizeof(or_ent));
+        a->cap = uniq;
+        a->len = 0;
+        for (uint64_t i = b; i < e; i++) {
+            if (i == b || key_cmp(&k[i - 1], &k[i]) != 0) {
+                a->a[a->len].nbr = k[i].nbr;
+                a->a[a->len].data = k[i].data;
+                a->a[a->len].num = 0;
+                a->len++;
+            }
+            a->a[a->len - 1].num = (uint16_t)(a->a[a->len - 1].num + 1);
+        }
+        a->degree = e - b;
+        if (a->len >= OR_INDEX_THRESHOLD) {
+            uint64_t cap = 1024;
+            while (cap < 2 * a->len) cap <<= 1;
+            hidx_rebuild(a, cap);
+        }
+    }
+    free(k);
+    free(off);
+}
+
+int or_bulk_load(or_graph *g, uint64_t n, const uint64_t *src, const uint64_t *dst,
+                 const uint64_t *w, int directed)
+{
+    for (uint64_t v = 0; v < g->V; v++)
+        if (g->out.v[v].len || (!g->symmetric && g->in.v[v].len)) return -1;
+    for (uint64_t i = 0; i < n; i++)
+        if (src[i] >= g->V || dst[i] >= g->V) return -1;
+    /* list ops of add_edge (graph.hpp:203-227): out[src] gets (dst, w);
+     * an undirected add also gives out[dst] (src, w) */
+    uint64_t per = directed ? 1 : 2;
+    uint64_t m = per * n;
+    uint64_t *lv = malloc((m ? m : 1) * sizeof(uint64_t));
+    uint64_t *ln = malloc((m ? m : 1) * sizeof(uint64_t));
+    uint64_t *ld = malloc((m ? m : 1) * sizeof(uint64_t));
+    for (int pass = 0; pass < (g->symmetric ? 1 : 2); pass++) {
+        /* pass 0: out-lists; pass 1: in-lists (directed graphs) */
+        uint64_t k = 0;
+        for (uint64_t i = 0; i < n; i++) {
+            uint64_t data = g->weighted && w ? w[i] : 0;
+            uint64_t a = src[i], b = dst[i];
+            if (pass == 0) { lv[k] = a; ln[k] = b; }
+            else { lv[k] = b; ln[k] = a; }
+            ld[k++] = data;
+            if (!directed) {
+                if (pass == 0) { lv[k] = b; ln[k] = a; }
+                else { lv[k] = a; ln[k] = b; }
+                ld[k++] = data;
+            }
+        }
+        bulk_store(pass == 0 ? &g->out : &g->in, g->V, m, lv, ln, ld);
+    }
+    free(lv); free(ln); free(ld);
+    return 0;
+}
+
 uint64_t or_degree(or_graph *g, uint64_t v, int inc)
 {
     return (inc ? incoming(g) : outgoing(g))->v[v].degree;
@@ -537,6 +632,40 @@ void or_labels_read(or_labels *L, uint64_t *values, uint64_t *parents, uint64_t 
 }
 
 uint64_t or_recompute_count(or_labels *L) { return L->recomputes; }
+
+/* Set the labels to a given fixpoint (values + parent vertices, V = empty)
+ * instead of running build_tree.  Each parent edge's weight is resolved
+ * from the store as a live in-edge (p -> v, w) with f(value[p], w) ==
+ * value[v].  Returns 0, or 1 + the first vertex without such an edge. */
+int64_t or_labels_assign(or_graph *g, or_labels *L, const uint64_t *values, const uint64_t *parents)
+{
+    or_store *is = g->symmetric ? &g->out : &g->in;
+    int64_t bad = 0;
+    for (uint64_t v = 0; v < g->V; v++) {
+        L->data[v] = values[v];
+        L->pnbr[v] = parents[v] >= g->V ? g->V : parents[v];
+        L->pdata[v] = 0;
+    }
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (uint64_t v = 0; v < g->V; v++) {
+        uint64_t p = L->pnbr[v];
+        if (p == g->V) continue;
+        or_adj *a = &is->v[v];
+        int ok = 0;
+        for (uint64_t i = 0; i < a->len && !ok; i++) {
+            or_ent e = a->a[i];
+            if (e.num && e.nbr == p && cand(L->algo, values[p], e.data) == values[v]) {
+                L->pdata[v] = e.data;
+                ok = 1;
+            }
+        }
+        if (!ok) {
+#pragma omp critical
+            if (!bad || (int64_t)v + 1 < bad) bad = (int64_t)v + 1;
+        }
+    }
+    return bad;
+}
 
 static inline uint64_t init_of(or_graph *g, or_labels *L, uint64_t v)
 {

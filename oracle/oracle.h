@@ -67,6 +67,11 @@ uint64_t or_del_edge(or_graph *g, uint64_t s, uint64_t d, uint64_t w, int direct
 /* Bulk import of n edges, same result as n sequential or_add_edge calls. */
 void or_add_edges(or_graph *g, uint64_t n, const uint64_t *src,
                   const uint64_t *dst, const uint64_t *w, int directed);
+/* Bulk import into an EMPTY graph: same store as n sequential adds (list
+ * entries in (nbr, data) order).  Returns -1 if the graph is not empty or an
+ * id is out of range. */
+int or_bulk_load(or_graph *g, uint64_t n, const uint64_t *src, const uint64_t *dst,
+                 const uint64_t *w, int directed);
 uint64_t or_degree(or_graph *g, uint64_t v, int incoming);
 uint64_t or_edge_num(or_graph *g, uint64_t s, uint64_t d, uint64_t w);
 /* Number of (src,dst,w) entries with num>0 (out store); fills arrays when non-NULL. */
@@ -87,6 +92,10 @@ uint64_t or_update_del_batch(or_graph *g, or_labels *L, uint64_t n, const uint64
 void or_labels_read(or_labels *L, uint64_t *values, uint64_t *parents, uint64_t *pweights);
 /* Number of full recomputes the deletion fallback took (graph.hpp:1252). */
 uint64_t or_recompute_count(or_labels *L);
+/* Start from a known fixpoint instead of build_tree: values and parent
+ * vertices (V = empty); parent weights are resolved from the store.  Returns
+ * 0, or 1 + the first vertex whose parent has no consistent live edge. */
+int64_t or_labels_assign(or_graph *g, or_labels *L, const uint64_t *values, const uint64_t *parents);
 
 /* One plain sliding-window step of the reference drivers (bfs_inc.cpp:160-189):
  * add (s1,d1,w1); if old==0 update_tree_add; del (s2,d2,w2); if old==1

@@ -1,0 +1,346 @@
+"""risgraph_amd -- MI355X-native incremental graph hot path (Python side).
+
+Thin ctypes mirror of the C ABI in ``include/risgraph_amd.h``.  The compute
+path is the HIP library ``risgraph_amd/librisgraph_amd.so`` (built by
+``risgraph_amd/build.py``); there is no CPU fallback: importing the bindings
+without the library, or calling them without a GPU, raises.
+
+Class and method names follow the reference ``Graph<>`` interface
+(core/graph.hpp): ``add_edge`` / ``del_edge`` return the old multiplicity,
+``build_tree``, ``update_tree_add`` / ``update_tree_del`` (single edge or
+batch), ``need_update_tree_add`` / ``need_update_tree_del``,
+``get_modified`` / ``clear_modified``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+BFS, SSSP, SSWP, WCC = 0, 1, 2, 3
+ALGOS = {"bfs": BFS, "sssp": SSSP, "sswp": SSWP, "wcc": WCC}
+RG_OK, RG_EINVAL_VERTEX, RG_ENOMEM, RG_EALGO, RG_EDEVICE, RG_EINVAL = range(6)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librisgraph_amd.so")
+
+u64 = ctypes.c_uint64
+u64p = ctypes.POINTER(ctypes.c_uint64)
+vp = ctypes.c_void_p
+ci = ctypes.c_int
+
+
+class RgEdge(ctypes.Structure):
+    _fields_ = [("src", u64), ("dst", u64), ("data", u64)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("seeds", u64), ("roots", u64), ("levels", u64), ("grid_steps", u64),
+                ("rebuilds", u64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class ServeParams(ctypes.Structure):
+    _fields_ = [("ins_first", u64), ("del_first", u64), ("n_updates", u64), ("rate", ctypes.c_double),
+                ("seed", u64), ("clients", ctypes.c_uint32), ("batch_cap", ctypes.c_uint32),
+                ("cap_max", ctypes.c_uint32), ("target_ms", ctypes.c_double), ("directed", ci)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("updates_per_s", ctypes.c_double),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("p999_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("mean_batch", ctypes.c_double),
+                ("batches", u64), ("max_batch", u64), ("grid_steps", u64), ("rebuilds", u64)]
+
+    def as_dict(self):
+        return {k: (float(getattr(self, k)) if t is ctypes.c_double else int(getattr(self, k)))
+                for k, t in self._fields_}
+
+
+class RgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rg error {code}: {msg}")
+        self.code = code
+
+
+# exported symbols and signatures (must match include/risgraph_amd.h)
+SIGNATURES = {
+    "rg_last_error": (ctypes.c_char_p, []),
+    "rg_graph_create": (ci, [u64, u64, ci, ci, ci, ctypes.POINTER(vp)]),
+    "rg_graph_destroy": (ci, [vp]),
+    "rg_graph_info": (ci, [vp, u64p, u64p, u64p, u64p, u64p]),
+    "rg_edge_update": (ci, [vp, u64, u64, u64, ci, ci, u64p]),
+    "rg_edge_update_batch": (ci, [vp, vp, u64, ci, ci, vp]),
+    "rg_degree": (ci, [vp, u64, ci, u64p]),
+    "rg_edge_num": (ci, [vp, u64, u64, u64, u64p]),
+    "rg_labels_create": (ci, [vp, ci, u64, vp, ci, ctypes.POINTER(vp)]),
+    "rg_labels_destroy": (ci, [vp]),
+    "rg_labels_set_tuning": (ci, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                  ctypes.c_uint32, ctypes.c_double, ctypes.c_uint32,
+                                  ctypes.c_uint32]),
+    "rg_build": (ci, [vp, u64p]),
+    "rg_inc_add": (ci, [vp, vp, u64, ci, u64p]),
+    "rg_inc_del": (ci, [vp, vp, u64, ci, u64p]),
+    "rg_classify": (ci, [vp, vp, vp, u64, ci, vp]),
+    "rg_labels_read": (ci, [vp, u64, u64, vp, vp]),
+    "rg_modified_read": (ci, [vp, vp, u64, u64p, ctypes.POINTER(ci)]),
+    "rg_modified_clear": (ci, [vp]),
+    "rg_stream_create": (ci, [ci, u64, u64, u64, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_double, u64, ctypes.POINTER(vp)]),
+    "rg_stream_destroy": (ci, [vp]),
+    "rg_stream_read": (ci, [vp, u64, u64, vp, vp, vp]),
+    "rg_stream_max_degree_vertex": (ci, [vp, u64, u64, u64p]),
+    "rg_stream_load": (ci, [vp, vp, u64, u64, ci]),
+    "rg_stream_apply": (ci, [vp, vp, ci, vp, u64, u64, u64, u64, ci, vp]),
+    "rg_stream_plain": (ci, [vp, vp, ci, vp, u64, u64, u64, ci, vp, vp]),
+    "rg_last_gpu_ms": (ci, [vp, ctypes.POINTER(ctypes.c_double)]),
+    "rg_serve": (ci, [vp, vp, ci, vp, vp, vp, vp]),
+}
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library; raises if it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: run risgraph_amd/build.py (hipcc, gfx950) first")
+        L = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != RG_OK:
+        raise RgError(rc, load_library().rg_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(vp)
+
+
+def _edges(src, dst, w=None):
+    src = np.atleast_1d(np.asarray(src, dtype=np.uint64))
+    dst = np.atleast_1d(np.asarray(dst, dtype=np.uint64))
+    e = np.zeros((len(src), 3), dtype=np.uint64)
+    e[:, 0] = src
+    e[:, 1] = dst
+    if w is not None:
+        e[:, 2] = np.atleast_1d(np.asarray(w, dtype=np.uint64))
+    return np.ascontiguousarray(e)
+
+
+class Stream:
+    """Device-resident synthetic edge stream (R-MAT or Erdos-Renyi)."""
+
+    def __init__(self, kind, scale_or_v, n_edges, seed=1, a=0.57, b=0.19, c=0.19, wseed=0):
+        h = vp()
+        _check(load_library().rg_stream_create(0 if kind == "rmat" else 1, scale_or_v, n_edges, seed,
+                                               a, b, c, wseed, ctypes.byref(h)))
+        self.h = h
+        self.n = n_edges
+        self.weighted = bool(wseed)
+
+    def read(self, first=0, n=None):
+        n = self.n - first if n is None else n
+        s = np.empty(n, np.uint64)
+        d = np.empty(n, np.uint64)
+        w = np.empty(n, np.uint32) if self.weighted else None
+        _check(load_library().rg_stream_read(self.h, first, n, _ptr(s), _ptr(d), _ptr(w)))
+        return s, d, w
+
+    def max_degree_vertex(self, n, V):
+        out = u64()
+        _check(load_library().rg_stream_max_degree_vertex(self.h, n, V, ctypes.byref(out)))
+        return int(out.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().rg_stream_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """Graph<EdgeData> (core/graph.hpp:41-100) backed by the HBM slab store."""
+
+    def __init__(self, vertices, hint_edges=0, symmetric=False, dual=True, weighted=False):
+        h = vp()
+        _check(load_library().rg_graph_create(vertices, hint_edges, int(symmetric), int(dual),
+                                              int(weighted), ctypes.byref(h)))
+        self.h = h
+        self.V = vertices
+        self.symmetric = symmetric
+        self.weighted = weighted
+        self._labels = []
+
+    def close(self):
+        for L in self._labels:
+            L.close()
+        self._labels = []
+        if getattr(self, "h", None):
+            load_library().rg_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- edge store (graph.hpp:203-253) ---
+    def add_edge(self, src, dst, data=0, directed=True):
+        old = u64()
+        _check(load_library().rg_edge_update(self.h, src, dst, data, 1, int(directed), ctypes.byref(old)))
+        return int(old.value)
+
+    def del_edge(self, src, dst, data=0, directed=True):
+        old = u64()
+        _check(load_library().rg_edge_update(self.h, src, dst, data, -1, int(directed), ctypes.byref(old)))
+        return int(old.value)
+
+    def update_edges(self, src, dst, w=None, delta=1, directed=True):
+        e = _edges(src, dst, w)
+        old = np.empty(len(e), np.uint64)
+        _check(load_library().rg_edge_update_batch(self.h, _ptr(e), len(e), delta, int(directed), _ptr(old)))
+        return old
+
+    def get_degree(self, v, incoming=False):
+        out = u64()
+        _check(load_library().rg_degree(self.h, v, int(incoming), ctypes.byref(out)))
+        return int(out.value)
+
+    def get_edge_num(self, src, dst, data=0):
+        out = u64()
+        _check(load_library().rg_edge_num(self.h, src, dst, data, ctypes.byref(out)))
+        return int(out.value)
+
+    def info(self):
+        vals = [u64() for _ in range(5)]
+        _check(load_library().rg_graph_info(self.h, *[ctypes.byref(v) for v in vals]))
+        keys = ("vertices", "slots_used", "slots_cap", "hash_used", "hash_cap")
+        return {k: int(v.value) for k, v in zip(keys, vals)}
+
+    def load_stream(self, stream, first, n, directed=True):
+        _check(load_library().rg_stream_load(self.h, stream.h, first, n, int(directed)))
+
+    # --- labels (graph.hpp:255-262) ---
+    def alloc_vertex_tree_array(self, algo, root=0, init=None, trace_modified=True):
+        L = Labels(self, algo, root, init, trace_modified)
+        self._labels.append(L)
+        return L
+
+    def apply_stream(self, labels, stream, ins_first, n_ins, del_first, n_del, directed=True):
+        arr = (vp * max(1, len(labels)))(*[L.h for L in labels])
+        st = BatchStats()
+        _check(load_library().rg_stream_apply(self.h, arr, len(labels), stream.h, ins_first, n_ins,
+                                              del_first, n_del, int(directed), ctypes.byref(st)))
+        return st.as_dict()
+
+    def stream_plain(self, labels, stream, ins_first, del_first, n_steps, directed=True, latency=True):
+        arr = (vp * max(1, len(labels)))(*[L.h for L in labels])
+        st = BatchStats()
+        lat = np.empty(n_steps, np.float64) if latency else None
+        _check(load_library().rg_stream_plain(self.h, arr, len(labels), stream.h, ins_first, del_first,
+                                              n_steps, int(directed), _ptr(lat), ctypes.byref(st)))
+        return lat, st.as_dict()
+
+    def serve(self, labels, stream, ins_first, del_first, n_updates, rate, batch_cap=1, cap_max=0,
+              target_ms=0.0, seed=4, clients=256, directed=True, latencies=False):
+        """Poisson-arrival serving loop (rg_serve); returns (result dict, latencies_ms or None)."""
+        arr = (vp * max(1, len(labels)))(*[L.h for L in labels])
+        p = ServeParams(ins_first, del_first, n_updates, rate, seed, clients, batch_cap,
+                        cap_max or batch_cap, target_ms, int(directed))
+        r = ServeResult()
+        lat = np.empty(n_updates, np.float64) if latencies else None
+        _check(load_library().rg_serve(self.h, arr, len(labels), stream.h, ctypes.byref(p),
+                                       ctypes.byref(r), _ptr(lat)))
+        return r.as_dict(), lat
+
+    def last_gpu_ms(self):
+        ms = ctypes.c_double()
+        _check(load_library().rg_last_gpu_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+
+class Labels:
+    """One algorithm's dependency tree: VertexTree array (graph.hpp:54-59)."""
+
+    def __init__(self, g, algo, root=0, init=None, trace_modified=True):
+        self.g = g
+        self.algo = algo
+        self.root = root
+        init_arr = None if init is None else np.ascontiguousarray(init, dtype=np.uint64)
+        h = vp()
+        _check(load_library().rg_labels_create(g.h, algo, root, _ptr(init_arr), int(trace_modified),
+                                               ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().rg_labels_destroy(self.h)
+            self.h = None
+
+    def set_tuning(self, rescue_level=5, rescue_frontier=16384, recompute_level=10,
+                   recompute_abs=2000000, recompute_frac=0.05, walk_limit=0, block_max=4096):
+        _check(load_library().rg_labels_set_tuning(self.h, rescue_level, rescue_frontier, recompute_level,
+                                                   recompute_abs, recompute_frac, walk_limit, block_max))
+
+    def build_tree(self):
+        r = u64()
+        _check(load_library().rg_build(self.h, ctypes.byref(r)))
+        return int(r.value)
+
+    def update_tree_add(self, src, dst, w=None, directed=True):
+        e = _edges(src, dst, w)
+        r = u64()
+        _check(load_library().rg_inc_add(self.h, _ptr(e), len(e), int(directed), ctypes.byref(r)))
+        return int(r.value)
+
+    def update_tree_del(self, src, dst, w=None, directed=True):
+        e = _edges(src, dst, w)
+        r = u64()
+        _check(load_library().rg_inc_del(self.h, _ptr(e), len(e), int(directed), ctypes.byref(r)))
+        return int(r.value)
+
+    def classify(self, src, dst, w, kind, directed=True):
+        e = _edges(src, dst, w)
+        k = np.ascontiguousarray(kind, dtype=np.int8)
+        out = np.empty(len(e), np.uint8)
+        _check(load_library().rg_classify(self.h, _ptr(e), _ptr(k), len(e), int(directed), _ptr(out)))
+        return out.astype(bool)
+
+    def need_update_tree_add(self, src, dst, w=0, directed=True):
+        return bool(self.classify([src], [dst], [w], [1], directed)[0])
+
+    def need_update_tree_del(self, src, dst, w=0, directed=True):
+        return bool(self.classify([src], [dst], [w], [-1], directed)[0])
+
+    def read(self, first=0, count=None):
+        count = self.g.V - first if count is None else count
+        v = np.empty(count, np.uint64)
+        p = np.empty(count, np.uint64)
+        _check(load_library().rg_labels_read(self.h, first, count, _ptr(v), _ptr(p)))
+        return v, p
+
+    def get_modified(self):
+        n = u64()
+        allf = ci()
+        _check(load_library().rg_modified_read(self.h, None, 0, ctypes.byref(n), ctypes.byref(allf)))
+        out = np.empty(int(n.value), np.uint64)
+        _check(load_library().rg_modified_read(self.h, _ptr(out), len(out), ctypes.byref(n), ctypes.byref(allf)))
+        return out[: int(n.value)], bool(allf.value)
+
+    def clear_modified(self):
+        _check(load_library().rg_modified_clear(self.h))

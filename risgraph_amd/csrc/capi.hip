@@ -1,0 +1,1007 @@
+// capi.hip -- host orchestration behind the C ABI (include/risgraph_amd.h).
+//
+// One HIP stream per graph.  Store batches run as two kernels (expand+link,
+// apply); each labels instance then runs its engine: a persistent
+// single-workgroup launch while the work of a pass is small, grid-wide
+// passes otherwise, with the control block copied back after each launch to
+// pick the next one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/risgraph_amd.h"
+#include "rg_internal.h"
+
+using namespace rg;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct RgError {
+    int code;
+    std::string msg;
+};
+
+void hipchk(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw RgError{RG_EDEVICE, std::string(what) + ": " + hipGetErrorString(e)};
+}
+#define HC(x) hipchk((x), #x)
+
+template <class T>
+T *dalloc(uint64_t n)
+{
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess)
+        throw RgError{RG_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed"};
+    return (T *)p;
+}
+
+template <class T>
+T *halloc(uint64_t n)
+{
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess) throw RgError{RG_ENOMEM, "hipHostMalloc failed"};
+    return (T *)p;
+}
+
+template <class T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <class T>
+void hfree(T *&p)
+{
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+}
+
+int guard(const std::function<void()> &f)
+{
+    try {
+        f();
+        return RG_OK;
+    } catch (const RgError &e) {
+        g_err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return RG_EDEVICE;
+    }
+}
+
+constexpr uint64_t MAX_V = 0x7FFFFFFFull;
+
+}  // namespace
+
+struct rg_graph {
+    uint64_t V = 0, hint = 0;
+    int symmetric = 0, dual = 0, weighted = 0;
+    hipStream_t st = nullptr;
+    Slots slots{};
+    List lists[2]{};
+    Pool *pool = nullptr;
+    uint64_t slot_cap = 0, hash_cap = 0;
+    EdgeOps ops{};
+    ListOps lops{};
+    uint32_t op_cap = 0;
+    uint32_t *err = nullptr;          // device: [0] bad ids
+    uint32_t *h_status = nullptr;     // pinned: err, oom
+    // pinned staging for host-side batches
+    uint32_t *h_src = nullptr, *h_dst = nullptr, *h_w = nullptr, *h_old = nullptr;
+    uint8_t *h_kind = nullptr, *h_flag = nullptr;
+    uint8_t *d_flag = nullptr;
+    uint32_t h_cap = 0;
+    hipEvent_t ev[4]{};
+    double last_ms = 0;
+    std::recursive_mutex mu;
+};
+
+struct rg_labels {
+    rg_graph *g = nullptr;
+    int algo = 0;
+    uint32_t root = 0;
+    int trace = 0;
+    uint64_t *val = nullptr;
+    uint32_t *init = nullptr;
+    int32_t *inv = nullptr;
+    uint32_t *stamp = nullptr, *mstamp = nullptr;
+    uint32_t *q[3]{};
+    uint64_t *cq = nullptr;
+    uint32_t cq_cap = 0;
+    uint32_t *mlist = nullptr;
+    Ctl *ctl = nullptr;
+    Ctl *hc = nullptr;     // pinned mirror
+    Tuning tune{};
+    uint32_t block_max = 4096;
+    uint32_t *init_active = nullptr;
+};
+
+struct rg_stream {
+    uint64_t n = 0;
+    uint64_t *src = nullptr, *dst = nullptr;
+    uint32_t *w = nullptr;
+    hipStream_t st = nullptr;
+};
+
+namespace {
+
+void alloc_list(List &l, uint64_t V, hipStream_t st)
+{
+    l.off = dalloc<uint64_t>(V);
+    l.len = dalloc<uint32_t>(V);
+    l.cap = dalloc<uint32_t>(V);
+    l.deg = dalloc<uint32_t>(V);
+    l.hoff = dalloc<uint64_t>(V);
+    l.hcap = dalloc<uint32_t>(V);
+    l.hcnt = dalloc<uint32_t>(V);
+    l.head = dalloc<int32_t>(V);
+    HC(hipMemsetAsync(l.off, 0, V * 8, st));
+    HC(hipMemsetAsync(l.len, 0, V * 4, st));
+    HC(hipMemsetAsync(l.cap, 0, V * 4, st));
+    HC(hipMemsetAsync(l.deg, 0, V * 4, st));
+    HC(hipMemsetAsync(l.hoff, 0xFF, V * 8, st));
+    HC(hipMemsetAsync(l.hcap, 0, V * 4, st));
+    HC(hipMemsetAsync(l.hcnt, 0, V * 4, st));
+    HC(hipMemsetAsync(l.head, 0xFF, V * 4, st));
+}
+
+void free_list(List &l)
+{
+    dfree(l.off); dfree(l.len); dfree(l.cap); dfree(l.deg);
+    dfree(l.hoff); dfree(l.hcap); dfree(l.hcnt); dfree(l.head);
+}
+
+void free_ops(rg_graph *g)
+{
+    dfree(g->ops.src); dfree(g->ops.dst); dfree(g->ops.w); dfree(g->ops.kind);
+    dfree(g->ops.old); dfree(g->ops.slot);
+    dfree(g->lops.v); dfree(g->lops.nbr); dfree(g->lops.w); dfree(g->lops.edge);
+    dfree(g->lops.delta); dfree(g->lops.flags); dfree(g->lops.next);
+    dfree(g->d_flag);
+    g->op_cap = 0;
+}
+
+void ensure_ops(rg_graph *g, uint64_t n)
+{
+    if (n > 0xFFFFFFFFull / 4) throw RgError{RG_EINVAL, "batch too large"};
+    if (n <= g->op_cap) return;
+    uint32_t cap = std::max<uint32_t>((uint32_t)n, 1024);
+    HC(hipStreamSynchronize(g->st));
+    free_ops(g);
+    g->ops.src = dalloc<uint32_t>(cap);
+    g->ops.dst = dalloc<uint32_t>(cap);
+    g->ops.w = dalloc<uint32_t>(cap);
+    g->ops.kind = dalloc<uint8_t>(cap);
+    g->ops.old = dalloc<uint32_t>(cap);
+    g->ops.slot = dalloc<uint32_t>(cap);
+    uint64_t lc = 4ull * cap;
+    g->lops.v = dalloc<uint32_t>(lc);
+    g->lops.nbr = dalloc<uint32_t>(lc);
+    g->lops.w = dalloc<uint32_t>(lc);
+    g->lops.edge = dalloc<uint32_t>(lc);
+    g->lops.delta = dalloc<int8_t>(lc);
+    g->lops.flags = dalloc<uint8_t>(lc);
+    g->lops.next = dalloc<int32_t>(lc);
+    g->d_flag = dalloc<uint8_t>(cap);
+    g->op_cap = cap;
+}
+
+void ensure_host(rg_graph *g, uint64_t n)
+{
+    if (n <= g->h_cap) return;
+    uint32_t cap = std::max<uint32_t>((uint32_t)n, 1024);
+    hfree(g->h_src); hfree(g->h_dst); hfree(g->h_w); hfree(g->h_old); hfree(g->h_kind); hfree(g->h_flag);
+    g->h_src = halloc<uint32_t>(cap);
+    g->h_dst = halloc<uint32_t>(cap);
+    g->h_w = halloc<uint32_t>(cap);
+    g->h_old = halloc<uint32_t>(cap);
+    g->h_kind = halloc<uint8_t>(cap);
+    g->h_flag = halloc<uint8_t>(cap);
+    g->h_cap = cap;
+}
+
+StoreArgs store_args(rg_graph *g)
+{
+    StoreArgs a{};
+    a.V = (uint32_t)g->V;
+    a.symmetric = g->symmetric;
+    a.weighted = g->weighted;
+    a.slots = g->slots;
+    a.lists[0] = g->lists[0];
+    a.lists[1] = g->lists[1];
+    a.pool = g->pool;
+    a.ops = g->ops;
+    a.lops = g->lops;
+    a.err = g->err;
+    return a;
+}
+
+// upload host edges as edge ops [0, n) with the given kind; validates ids
+void upload_edges(rg_graph *g, const rg_edge *e, uint64_t n, uint8_t kind)
+{
+    ensure_ops(g, n);
+    ensure_host(g, n);
+    for (uint64_t i = 0; i < n; i++) {
+        if (e[i].src >= g->V || e[i].dst >= g->V) throw RgError{RG_EINVAL_VERTEX, "VertexId error."};
+        if (g->weighted && e[i].data > 0xFFFFFFFFull) throw RgError{RG_EINVAL, "edge weight exceeds 32 bits"};
+        g->h_src[i] = (uint32_t)e[i].src;
+        g->h_dst[i] = (uint32_t)e[i].dst;
+        g->h_w[i] = g->weighted ? (uint32_t)e[i].data : 0;
+        g->h_kind[i] = kind;
+    }
+    HC(hipMemcpyAsync(g->ops.src, g->h_src, n * 4, hipMemcpyHostToDevice, g->st));
+    HC(hipMemcpyAsync(g->ops.dst, g->h_dst, n * 4, hipMemcpyHostToDevice, g->st));
+    HC(hipMemcpyAsync(g->ops.w, g->h_w, n * 4, hipMemcpyHostToDevice, g->st));
+    HC(hipMemcpyAsync(g->ops.kind, g->h_kind, n, hipMemcpyHostToDevice, g->st));
+    g->ops.n = (uint32_t)n;
+    g->lops.n = (uint32_t)(4 * n);
+}
+
+void store_status_fetch(rg_graph *g)
+{
+    HC(hipMemcpyAsync(&g->h_status[0], g->err, 4, hipMemcpyDeviceToHost, g->st));
+    HC(hipMemcpyAsync(&g->h_status[1], &g->pool->oom, 4, hipMemcpyDeviceToHost, g->st));
+}
+
+void store_status_check(rg_graph *g)
+{
+    if (g->h_status[0]) {
+        uint32_t e = g->h_status[0];
+        g->h_status[0] = 0;
+        HC(hipMemsetAsync(g->err, 0, 4, g->st));
+        if (e & 1) throw RgError{RG_EINVAL_VERTEX, "VertexId error."};
+        throw RgError{RG_EDEVICE, "store op chain corrupt"};
+    }
+    if (g->h_status[1]) throw RgError{RG_ENOMEM, "adjacency slot or hash pool exhausted"};
+}
+
+void run_store(rg_graph *g)
+{
+    launch_store_batch(store_args(g), g->st);
+    HC(hipGetLastError());
+    store_status_fetch(g);
+}
+
+EngineArgs engine_args(rg_labels *L)
+{
+    rg_graph *g = L->g;
+    EngineArgs a{};
+    a.V = (uint32_t)g->V;
+    a.algo = L->algo;
+    a.trace = L->trace;
+    a.slots = g->slots;
+    a.lists[0] = g->lists[0];
+    a.lists[1] = g->symmetric ? g->lists[0] : g->lists[1];
+    if (g->symmetric) {
+        a.undirected = 0;
+        a.npush = 1; a.push_l[0] = 0;
+        a.npull = 1; a.pull_l[0] = 0;
+    } else if (L->algo == WCC) {
+        // WCC over the undirected view of a directed store
+        a.undirected = 1;
+        a.npush = 2; a.push_l[0] = 0; a.push_l[1] = 1;
+        a.npull = 2; a.pull_l[0] = 1; a.pull_l[1] = 0;
+    } else {
+        a.undirected = 0;
+        a.npush = 1; a.push_l[0] = 0;
+        a.npull = 1; a.pull_l[0] = 1;
+    }
+    a.val = L->val;
+    a.init = L->init;
+    a.init_active = L->init_active;
+    a.root = L->root;
+    a.inv = L->inv;
+    a.stamp = L->stamp;
+    a.mstamp = L->mstamp;
+    a.q[0] = L->q[0]; a.q[1] = L->q[1]; a.q[2] = L->q[2];
+    a.cq = L->cq;
+    a.cq_cap = L->cq_cap;
+    a.mlist = L->mlist;
+    a.ctl = L->ctl;
+    a.ops = g->ops;
+    a.tune = L->tune;
+    return a;
+}
+
+uint64_t host_work(const Ctl &h, uint64_t V)
+{
+    switch (h.phase) {
+    case PH_SEED:
+    case PH_DEL_ROOTS: return h.n_ops;
+    case PH_DEL_RESET:
+    case PH_DEL_PULL: return h.n_tree;
+    case PH_BUILD_INIT: return V;
+    default: return h.n_cur;
+    }
+}
+
+// Run one labels instance's engine from `phase0` to completion.
+void engine_run(rg_labels *L, uint32_t phase0, bool del_pending, uint32_t n_ops, bool new_idx,
+                rg_batch_stats *stats)
+{
+    rg_graph *g = L->g;
+    Ctl &h = *L->hc;
+    h.phase = phase0;
+    h.stage = 0;
+    h.level = 0;
+    h.del_pending = del_pending ? 1u : 0u;
+    h.n_cur = h.n_next = h.n_tree = h.n_chunk = 0;
+    h.n_ops = n_ops;
+    h.sel_cur = 0; h.sel_next = 1; h.sel_tree = 2;
+    h.epoch += 1;
+    if (new_idx || del_pending) {
+        h.idx += 1;
+        if (h.idx >= 0x7FFFFFF0) h.idx = 1;  // marks are cleared after every run
+    }
+    h.flags &= F_ALL_MODIFIED;
+    HC(hipMemcpyAsync(L->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
+    EngineArgs a = engine_args(L);
+    uint64_t guard_iter = 0;
+    for (;;) {
+        uint64_t work = host_work(h, g->V);
+        if (h.phase != PH_BUILD_INIT && h.stage == 0 && work <= L->block_max) {
+            launch_engine_block(a, L->block_max, g->st);
+        } else {
+            launch_engine_grid_step(a, h, g->st);
+            if (stats) stats->grid_steps++;
+        }
+        HC(hipGetLastError());
+        HC(hipMemcpyAsync(&h, L->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->st));
+        HC(hipStreamSynchronize(g->st));
+        if (h.flags & F_QUEUE_OVERFLOW) throw RgError{RG_EDEVICE, "engine chunk queue overflow"};
+        if (h.phase == PH_DONE) break;
+        if (h.phase == PH_REBUILD) {
+            // deletion fallback (graph.hpp:1252-1321): full recompute
+            h.phase = PH_BUILD_INIT;
+            h.stage = 0;
+            h.del_pending = 0;
+            h.n_cur = h.n_next = h.n_tree = h.n_chunk = 0;
+            h.sel_cur = 0; h.sel_next = 1; h.sel_tree = 2;
+            h.epoch += 1;
+            h.flags = (h.flags & ~F_NEED_REBUILD) | F_ALL_MODIFIED;
+            HC(hipMemcpyAsync(L->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
+            if (stats) stats->rebuilds++;
+        }
+        if (++guard_iter > (1ull << 26)) throw RgError{RG_EDEVICE, "engine did not converge"};
+    }
+    if (stats) stats->levels += h.levels;
+    h.levels = 0;
+}
+
+void labels_free(rg_labels *L)
+{
+    dfree(L->val); dfree(L->init); dfree(L->inv); dfree(L->stamp); dfree(L->mstamp);
+    dfree(L->q[0]); dfree(L->q[1]); dfree(L->q[2]); dfree(L->cq); dfree(L->mlist); dfree(L->ctl);
+    dfree(L->init_active);
+    hfree(L->hc);
+}
+
+}  // namespace
+
+// =====================================================================
+// C ABI
+// =====================================================================
+extern "C" {
+
+const char *rg_last_error(void) { return g_err.c_str(); }
+
+int rg_graph_create(uint64_t vertices, uint64_t hint_edges, int symmetric, int dual, int weighted,
+                    rg_graph **out)
+{
+    if (!out) return RG_EINVAL;
+    *out = nullptr;
+    rg_graph *g = new rg_graph();
+    int rc = guard([&] {
+        if (vertices == 0 || vertices > MAX_V) throw RgError{RG_EINVAL_VERTEX, "vertex count must be in [1, 2^31)"};
+        g->V = vertices;
+        g->hint = hint_edges ? hint_edges : vertices * 16;   // graph.hpp:64
+        g->symmetric = symmetric;
+        g->dual = dual;
+        g->weighted = weighted;
+        HC(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
+        for (auto &e : g->ev) HC(hipEventCreate(&e));
+        alloc_list(g->lists[0], vertices, g->st);
+        if (!symmetric) alloc_list(g->lists[1], vertices, g->st);
+        else g->lists[1] = g->lists[0];
+        const char *env = getenv("RG_SLOT_FACTOR");
+        double f = env ? atof(env) : 6.0;
+        g->slot_cap = std::max<uint64_t>(1ull << 20, (uint64_t)(f * (double)g->hint)) + 4 * vertices;
+        g->hash_cap = std::max<uint64_t>(1ull << 20, 4 * g->hint);
+        g->slots.nbr = dalloc<uint32_t>(g->slot_cap);
+        g->slots.cnt = dalloc<uint32_t>(g->slot_cap);
+        g->slots.wt = weighted ? dalloc<uint32_t>(g->slot_cap) : nullptr;
+        g->slots.hkey = dalloc<uint64_t>(g->hash_cap);
+        g->slots.hval = dalloc<uint32_t>(g->hash_cap);
+        g->pool = dalloc<Pool>(1);
+        Pool p{};
+        p.slot_cap = g->slot_cap;
+        p.hash_cap = g->hash_cap;
+        HC(hipMemcpyAsync(g->pool, &p, sizeof(Pool), hipMemcpyHostToDevice, g->st));
+        g->err = dalloc<uint32_t>(4);
+        HC(hipMemsetAsync(g->err, 0, 16, g->st));
+        g->h_status = halloc<uint32_t>(4);
+        memset(g->h_status, 0, 16);
+        HC(hipStreamSynchronize(g->st));
+    });
+    if (rc != RG_OK) {
+        rg_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return RG_OK;
+}
+
+int rg_graph_destroy(rg_graph *g)
+{
+    if (!g) return RG_OK;
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    free_list(g->lists[0]);
+    if (!g->symmetric) free_list(g->lists[1]);
+    dfree(g->slots.nbr); dfree(g->slots.cnt); dfree(g->slots.wt); dfree(g->slots.hkey); dfree(g->slots.hval);
+    dfree(g->pool); dfree(g->err);
+    free_ops(g);
+    hfree(g->h_src); hfree(g->h_dst); hfree(g->h_w); hfree(g->h_old); hfree(g->h_kind); hfree(g->h_flag);
+    hfree(g->h_status);
+    for (auto &e : g->ev) if (e) (void)hipEventDestroy(e);
+    if (g->st) (void)hipStreamDestroy(g->st);
+    delete g;
+    return RG_OK;
+}
+
+int rg_graph_info(rg_graph *g, uint64_t *vertices, uint64_t *slots_used, uint64_t *slots_cap,
+                  uint64_t *hash_used, uint64_t *hash_cap)
+{
+    if (!g) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        Pool p{};
+        HC(hipMemcpyAsync(&p, g->pool, sizeof(Pool), hipMemcpyDeviceToHost, g->st));
+        HC(hipStreamSynchronize(g->st));
+        if (vertices) *vertices = g->V;
+        if (slots_used) *slots_used = std::min<uint64_t>(p.slot_top, p.slot_cap);
+        if (slots_cap) *slots_cap = p.slot_cap;
+        if (hash_used) *hash_used = std::min<uint64_t>(p.hash_top, p.hash_cap);
+        if (hash_cap) *hash_cap = p.hash_cap;
+    });
+}
+
+int rg_edge_update_batch(rg_graph *g, const rg_edge *edges, uint64_t n, int delta, int directed,
+                         uint64_t *old_out)
+{
+    if (!g || (n && !edges) || (delta != 1 && delta != -1)) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (n == 0) return;
+        uint8_t kind = (delta > 0 ? K_INS : K_DEL) | (directed ? 0 : K_UNDIR);
+        upload_edges(g, edges, n, kind);
+        run_store(g);
+        HC(hipMemcpyAsync(g->h_old, g->ops.old, n * 4, hipMemcpyDeviceToHost, g->st));
+        HC(hipStreamSynchronize(g->st));
+        store_status_check(g);
+        if (old_out)
+            for (uint64_t i = 0; i < n; i++) old_out[i] = g->h_old[i];
+    });
+}
+
+int rg_edge_update(rg_graph *g, uint64_t src, uint64_t dst, uint64_t data, int delta, int directed,
+                   uint64_t *old)
+{
+    rg_edge e{src, dst, data};
+    return rg_edge_update_batch(g, &e, 1, delta, directed, old);
+}
+
+int rg_degree(rg_graph *g, uint64_t v, int incoming, uint64_t *out)
+{
+    if (!g || !out) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (v >= g->V) throw RgError{RG_EINVAL_VERTEX, "VertexId error."};
+        const List &l = (incoming && !g->symmetric) ? g->lists[1] : g->lists[0];
+        uint32_t d = 0;
+        HC(hipMemcpyAsync(&d, l.deg + v, 4, hipMemcpyDeviceToHost, g->st));
+        HC(hipStreamSynchronize(g->st));
+        *out = d;
+    });
+}
+
+int rg_edge_num(rg_graph *g, uint64_t src, uint64_t dst, uint64_t data, uint64_t *out)
+{
+    if (!g || !out) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (src >= g->V || dst >= g->V) throw RgError{RG_EINVAL_VERTEX, "VertexId error."};
+        uint64_t off = 0;
+        uint32_t len = 0;
+        HC(hipMemcpyAsync(&off, g->lists[0].off + src, 8, hipMemcpyDeviceToHost, g->st));
+        HC(hipMemcpyAsync(&len, g->lists[0].len + src, 4, hipMemcpyDeviceToHost, g->st));
+        HC(hipStreamSynchronize(g->st));
+        std::vector<uint32_t> nb(len), cnt(len), w(len);
+        if (len) {
+            HC(hipMemcpyAsync(nb.data(), g->slots.nbr + off, len * 4ull, hipMemcpyDeviceToHost, g->st));
+            HC(hipMemcpyAsync(cnt.data(), g->slots.cnt + off, len * 4ull, hipMemcpyDeviceToHost, g->st));
+            if (g->weighted)
+                HC(hipMemcpyAsync(w.data(), g->slots.wt + off, len * 4ull, hipMemcpyDeviceToHost, g->st));
+            HC(hipStreamSynchronize(g->st));
+        }
+        *out = 0;
+        for (uint32_t i = 0; i < len; i++)
+            if ((nb[i] & NBR_MASK) == dst && (!g->weighted || w[i] == data)) *out = cnt[i];
+    });
+}
+
+int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_values, int trace_modified,
+                     rg_labels **out)
+{
+    if (!g || !out) return RG_EINVAL;
+    if (algo < RG_BFS || algo > RG_WCC) {
+        g_err = "unknown algorithm";
+        return RG_EALGO;
+    }
+    *out = nullptr;
+    rg_labels *L = new rg_labels();
+    int rc = guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (algo != RG_WCC && root >= g->V) throw RgError{RG_EINVAL_VERTEX, "root out of range"};
+        uint64_t V = g->V;
+        L->g = g;
+        L->algo = algo;
+        L->root = (uint32_t)root;
+        L->trace = trace_modified;
+        L->val = dalloc<uint64_t>(V);
+        L->init = dalloc<uint32_t>(V);
+        L->inv = dalloc<int32_t>(V);
+        L->stamp = dalloc<uint32_t>(V);
+        L->mstamp = dalloc<uint32_t>(V);
+        for (auto &q : L->q) q = dalloc<uint32_t>(V);
+        L->cq_cap = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull,
+                                                  g->slot_cap / CHUNK + 2 * std::min<uint64_t>(V, g->slot_cap / LONG_LIST) + 1024);
+        L->cq = dalloc<uint64_t>(L->cq_cap);
+        L->mlist = dalloc<uint32_t>(V);
+        L->ctl = dalloc<Ctl>(1);
+        L->hc = halloc<Ctl>(1);
+        memset(L->hc, 0, sizeof(Ctl));
+        L->hc->mepoch = 1;
+        HC(hipMemsetAsync(L->inv, 0, V * 4, g->st));
+        HC(hipMemsetAsync(L->stamp, 0, V * 4, g->st));
+        HC(hipMemsetAsync(L->mstamp, 0, V * 4, g->st));
+        // labels start as (init, empty parent) like alloc_vertex_tree_array
+        if (algo == RG_WCC) {
+            L->init_active = nullptr;
+            if (init_values) {
+                std::vector<uint32_t> tmp(V);
+                for (uint64_t v = 0; v < V; v++) {
+                    if (init_values[v] > 0xFFFFFFFEull) throw RgError{RG_EINVAL, "WCC init label exceeds 32 bits"};
+                    tmp[v] = (uint32_t)init_values[v];
+                }
+                HC(hipMemcpyAsync(L->init, tmp.data(), V * 4, hipMemcpyHostToDevice, g->st));
+                HC(hipStreamSynchronize(g->st));
+            } else {
+                EngineArgs a = engine_args(L);
+                launch_engine_fill_init(a, g->st);
+            }
+            // every vertex starts active (wcc_inc.cpp:94-97)
+            L->init_active = dalloc<uint32_t>(V);
+            HC(hipMemsetAsync(L->init_active, 0xFF, V * 4, g->st));
+        } else {
+            EngineArgs a = engine_args(L);
+            launch_engine_fill_init(a, g->st);
+        }
+        L->tune.rescue_level = 5;
+        L->tune.rescue_frontier = 16384;
+        L->tune.recompute_level = 10;
+        L->tune.recompute_abs = 2000000;
+        L->tune.recompute_frac = 0.05f;
+        L->tune.walk_limit = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(V / 10, 1u << 16));
+        if (const char *bm = getenv("RG_BLOCK_MAX")) L->block_max = (uint32_t)atoi(bm);
+        HC(hipMemcpyAsync(L->ctl, L->hc, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
+        HC(hipStreamSynchronize(g->st));
+    });
+    if (rc != RG_OK) {
+        labels_free(L);
+        delete L;
+        return rc;
+    }
+    *out = L;
+    return RG_OK;
+}
+
+int rg_labels_destroy(rg_labels *L)
+{
+    if (!L) return RG_OK;
+    if (L->g && L->g->st) (void)hipStreamSynchronize(L->g->st);
+    labels_free(L);
+    delete L;
+    return RG_OK;
+}
+
+int rg_labels_set_tuning(rg_labels *L, uint32_t rescue_level, uint32_t rescue_frontier,
+                         uint32_t recompute_level, uint32_t recompute_abs, double recompute_frac,
+                         uint32_t walk_limit, uint32_t block_max)
+{
+    if (!L || recompute_frac < 0) return RG_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(L->g->mu);
+    L->tune.rescue_level = rescue_level;
+    L->tune.rescue_frontier = rescue_frontier;
+    L->tune.recompute_level = recompute_level;
+    L->tune.recompute_abs = recompute_abs;
+    L->tune.recompute_frac = (float)recompute_frac;
+    L->tune.walk_limit = walk_limit ? walk_limit
+                                    : (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(L->g->V / 10, 1u << 16));
+    L->block_max = block_max;
+    return RG_OK;
+}
+
+int rg_build(rg_labels *L, uint64_t *result)
+{
+    if (!L) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(L->g->mu);
+        L->hc->flags |= F_ALL_MODIFIED;
+        rg_batch_stats st{};
+        engine_run(L, PH_BUILD_INIT, false, 0, false, &st);
+        if (result) *result = st.levels;
+    });
+}
+
+int rg_inc_add(rg_labels *L, const rg_edge *edges, uint64_t n, int directed, uint64_t *result)
+{
+    if (!L || (n && !edges)) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (result) *result = 0;
+        if (!n) return;
+        upload_edges(g, edges, n, K_INS | K_FORCE | K_NOSTORE | (directed ? 0 : K_UNDIR));
+        rg_batch_stats st{};
+        engine_run(L, PH_SEED, false, (uint32_t)n, false, &st);
+        if (result) *result = st.levels;
+    });
+}
+
+int rg_inc_del(rg_labels *L, const rg_edge *edges, uint64_t n, int directed, uint64_t *result)
+{
+    if (!L || (n && !edges)) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (result) *result = 0;
+        if (!n) return;
+        upload_edges(g, edges, n, K_DEL | K_FORCE | K_NOSTORE | (directed ? 0 : K_UNDIR));
+        rg_batch_stats st{};
+        engine_run(L, PH_DEL_ROOTS, false, (uint32_t)n, true, &st);
+        if (result) *result = st.levels;
+    });
+}
+
+int rg_classify(rg_labels *L, const rg_edge *edges, const int8_t *kind, uint64_t n, int directed,
+                uint8_t *unsafe)
+{
+    if (!L || (n && (!edges || !kind || !unsafe))) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (!n) return;
+        upload_edges(g, edges, n, K_NOSTORE | (directed ? 0 : K_UNDIR));
+        for (uint64_t i = 0; i < n; i++)
+            g->h_kind[i] = (uint8_t)(K_NOSTORE | (directed ? 0 : K_UNDIR) | (kind[i] > 0 ? K_INS : K_DEL));
+        HC(hipMemcpyAsync(g->ops.kind, g->h_kind, n, hipMemcpyHostToDevice, g->st));
+        EngineArgs a = engine_args(L);
+        launch_classify(a, (uint32_t)n, g->d_flag, g->st);
+        HC(hipGetLastError());
+        HC(hipMemcpyAsync(g->h_flag, g->d_flag, n, hipMemcpyDeviceToHost, g->st));
+        HC(hipStreamSynchronize(g->st));
+        memcpy(unsafe, g->h_flag, n);
+    });
+}
+
+int rg_labels_read(rg_labels *L, uint64_t first, uint64_t count, uint64_t *values, uint64_t *parents)
+{
+    if (!L) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (first + count > g->V || first + count < first) throw RgError{RG_EINVAL_VERTEX, "range out of bounds"};
+        if (!count) return;
+        std::vector<uint64_t> tmp(count);
+        HC(hipMemcpyAsync(tmp.data(), L->val + first, count * 8, hipMemcpyDeviceToHost, g->st));
+        HC(hipStreamSynchronize(g->st));
+        for (uint64_t i = 0; i < count; i++) {
+            uint32_t p = (uint32_t)tmp[i];
+            if (values) values[i] = tmp[i] >> 32;
+            if (parents) parents[i] = p == NILP ? g->V : p;
+        }
+    });
+}
+
+int rg_modified_read(rg_labels *L, uint64_t *vids, uint64_t cap, uint64_t *n, int *all)
+{
+    if (!L || !n) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        uint64_t cnt = std::min<uint64_t>(L->hc->n_mod, g->V);
+        if (all) *all = (L->hc->flags & F_ALL_MODIFIED) ? 1 : 0;
+        uint64_t k = std::min(cnt, cap);
+        if (k && vids) {
+            std::vector<uint32_t> tmp(k);
+            HC(hipMemcpyAsync(tmp.data(), L->mlist, k * 4, hipMemcpyDeviceToHost, g->st));
+            HC(hipStreamSynchronize(g->st));
+            for (uint64_t i = 0; i < k; i++) vids[i] = tmp[i];
+        }
+        *n = vids ? k : cnt;
+    });
+}
+
+int rg_modified_clear(rg_labels *L)
+{
+    if (!L) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        L->hc->n_mod = 0;
+        L->hc->mepoch += 1;
+        L->hc->flags &= ~F_ALL_MODIFIED;
+        HC(hipMemcpyAsync(L->ctl, L->hc, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
+        HC(hipStreamSynchronize(g->st));
+    });
+}
+
+int rg_stream_create(int kind, uint64_t scale_or_v, uint64_t n_edges, uint64_t seed, double a, double b,
+                     double c, uint64_t wseed, rg_stream **out)
+{
+    if (!out || (kind != 0 && kind != 1)) return RG_EINVAL;
+    *out = nullptr;
+    rg_stream *s = new rg_stream();
+    int rc = guard([&] {
+        if (kind == 0 && (scale_or_v < 1 || scale_or_v > 31)) throw RgError{RG_EINVAL, "R-MAT scale must be 1..31"};
+        if (kind == 1 && (scale_or_v < 1 || scale_or_v > MAX_V)) throw RgError{RG_EINVAL, "vertex count out of range"};
+        HC(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        s->n = n_edges;
+        s->src = dalloc<uint64_t>(n_edges);
+        s->dst = dalloc<uint64_t>(n_edges);
+        if (kind == 0) launch_gen_rmat(s->src, s->dst, n_edges, 0, (int)scale_or_v, seed, a, b, c, s->st);
+        else launch_gen_er(s->src, s->dst, n_edges, 0, scale_or_v, seed, s->st);
+        if (wseed) {
+            s->w = dalloc<uint32_t>(n_edges);
+            launch_gen_weights(s->w, n_edges, 0, wseed, s->st);
+        }
+        HC(hipGetLastError());
+        HC(hipStreamSynchronize(s->st));
+    });
+    if (rc != RG_OK) {
+        rg_stream_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return RG_OK;
+}
+
+int rg_stream_destroy(rg_stream *s)
+{
+    if (!s) return RG_OK;
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    dfree(s->src); dfree(s->dst); dfree(s->w);
+    if (s->st) (void)hipStreamDestroy(s->st);
+    delete s;
+    return RG_OK;
+}
+
+int rg_stream_read(rg_stream *s, uint64_t first, uint64_t n, uint64_t *src, uint64_t *dst, uint32_t *w)
+{
+    if (!s) return RG_EINVAL;
+    return guard([&] {
+        if (first + n > s->n) throw RgError{RG_EINVAL, "range out of bounds"};
+        if (src) HC(hipMemcpy(src, s->src + first, n * 8, hipMemcpyDeviceToHost));
+        if (dst) HC(hipMemcpy(dst, s->dst + first, n * 8, hipMemcpyDeviceToHost));
+        if (w) {
+            if (!s->w) throw RgError{RG_EINVAL, "stream has no weights"};
+            HC(hipMemcpy(w, s->w + first, n * 4, hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int rg_stream_max_degree_vertex(rg_stream *s, uint64_t n, uint64_t V, uint64_t *out)
+{
+    if (!s || !out) return RG_EINVAL;
+    return guard([&] {
+        if (n > s->n) throw RgError{RG_EINVAL, "range out of bounds"};
+        uint32_t *deg = dalloc<uint32_t>(V);
+        unsigned long long *best = dalloc<unsigned long long>(1);
+        launch_max_degree(s->src, n, V, deg, best, s->st);
+        unsigned long long b = 0;
+        hipError_t e = hipMemcpyAsync(&b, best, 8, hipMemcpyDeviceToHost, s->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+        dfree(deg);
+        dfree(best);
+        hipchk(e, "max degree");
+        *out = 0xFFFFFFFFull - (b & 0xFFFFFFFFull);
+    });
+}
+
+int rg_stream_load(rg_graph *g, rg_stream *s, uint64_t first, uint64_t n, int directed)
+{
+    if (!g || !s) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (first + n > s->n) throw RgError{RG_EINVAL, "range out of bounds"};
+        HC(hipStreamSynchronize(s->st));
+        const uint64_t B = 1ull << 24;
+        uint8_t kind = K_INS | (directed ? 0 : K_UNDIR);
+        for (uint64_t b = 0; b < n; b += B) {
+            uint32_t m = (uint32_t)std::min(B, n - b);
+            ensure_ops(g, m);
+            launch_ops_from_stream(s->src, s->dst, g->weighted ? s->w : nullptr, first + b, m, kind, g->ops, 0, g->st);
+            g->ops.n = m;
+            g->lops.n = 4 * m;
+            run_store(g);
+        }
+        HC(hipStreamSynchronize(g->st));
+        store_status_check(g);
+    });
+}
+
+static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first, uint64_t n_ins,
+                        uint64_t del_first, uint64_t n_del, int directed, rg_batch_stats *stats)
+{
+    if (ins_first + n_ins > s->n || del_first + n_del > s->n) throw RgError{RG_EINVAL, "range out of bounds"};
+    for (int i = 0; i < nL; i++)
+        if (!L[i] || L[i]->g != g) throw RgError{RG_EINVAL, "labels belong to another graph"};
+    uint64_t n = n_ins + n_del;
+    if (!n) return;
+    ensure_ops(g, n);
+    uint8_t ud = directed ? 0 : K_UNDIR;
+    const uint32_t *w = g->weighted ? s->w : nullptr;
+    if (g->weighted && !w) throw RgError{RG_EINVAL, "weighted graph needs a weighted stream"};
+    launch_ops_from_stream(s->src, s->dst, w, ins_first, (uint32_t)n_ins, K_INS | ud, g->ops, 0, g->st);
+    launch_ops_from_stream(s->src, s->dst, w, del_first, (uint32_t)n_del, K_DEL | ud, g->ops, (uint32_t)n_ins, g->st);
+    g->ops.n = (uint32_t)n;
+    g->lops.n = (uint32_t)(4 * n);
+    HC(hipEventRecord(g->ev[0], g->st));
+    run_store(g);
+    HC(hipEventRecord(g->ev[1], g->st));
+    for (int i = 0; i < nL; i++) {
+        uint32_t ph = n_ins ? PH_SEED : PH_DEL_ROOTS;
+        engine_run(L[i], ph, n_ins && n_del, (uint32_t)n, n_del > 0, stats);
+    }
+    HC(hipEventRecord(g->ev[2], g->st));
+    HC(hipEventSynchronize(g->ev[2]));
+    store_status_check(g);
+    float ms = 0;
+    HC(hipEventElapsedTime(&ms, g->ev[0], g->ev[2]));
+    g->last_ms = ms;
+    if (stats) {
+        stats->seeds += n_ins;
+        stats->roots += n_del;
+    }
+}
+
+int rg_stream_apply(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first, uint64_t n_ins,
+                    uint64_t del_first, uint64_t n_del, int directed, rg_batch_stats *stats)
+{
+    if (!g || !s || (nL && !L) || nL < 0) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        HC(hipStreamSynchronize(s->st));
+        apply_batch(g, L, nL, s, ins_first, n_ins, del_first, n_del, directed, stats);
+    });
+}
+
+int rg_stream_plain(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first, uint64_t del_first,
+                    uint64_t n_steps, int directed, double *lat_ns, rg_batch_stats *stats)
+{
+    if (!g || !s || (nL && !L) || nL < 0) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        HC(hipStreamSynchronize(s->st));
+        for (uint64_t k = 0; k < n_steps; k++) {
+            auto t0 = std::chrono::steady_clock::now();
+            apply_batch(g, L, nL, s, ins_first + k, 1, del_first + k, 1, directed, stats);
+            auto t1 = std::chrono::steady_clock::now();
+            if (lat_ns) lat_ns[k] = (double)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+        }
+    });
+}
+
+int rg_last_gpu_ms(rg_graph *g, double *ms)
+{
+    if (!g || !ms) return RG_EINVAL;
+    *ms = g->last_ms;
+    return RG_OK;
+}
+
+static double percentile(std::vector<double> &v, double q)
+{
+    if (v.empty()) return 0;
+    size_t k = (size_t)std::min<double>((double)v.size() - 1, std::ceil(q * (double)v.size()) - 1);
+    std::nth_element(v.begin(), v.begin() + k, v.end());
+    return v[k];
+}
+
+int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_params *p, rg_serve_result *r,
+             double *lat_ms)
+{
+    if (!g || !s || !p || !r || (nL && !L) || nL < 0 || p->rate <= 0 || p->batch_cap == 0) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        HC(hipStreamSynchronize(s->st));
+        const uint64_t n = p->n_updates;
+        // Poisson arrivals: exponential gaps from a counter-based uniform draw
+        std::vector<double> arr(n), lat(n);
+        double t = 0;
+        uint64_t key = p->seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+        for (uint64_t u = 0; u < n; u++) {
+            uint64_t z = key + (u + 1) * 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            double uni = ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+            t += -std::log(uni) / p->rate;
+            arr[u] = t;
+        }
+        rg_batch_stats st{};
+        double cap = p->batch_cap;
+        const double cap_max = std::max<double>(p->cap_max ? p->cap_max : p->batch_cap, 1.0);
+        uint64_t next = 0, batches = 0, max_batch = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        auto now = [&] {
+            return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        };
+        while (next < n) {
+            double tn = now();
+            while (arr[next] > tn) tn = now();   // idle until the next request arrives
+            uint64_t arrived = (uint64_t)(std::upper_bound(arr.begin() + next, arr.end(), tn) - arr.begin());
+            uint64_t end = std::min<uint64_t>(arrived, next + (uint64_t)std::max(1.0, std::floor(cap)));
+            // updates [next, end): adds are the even indices, deletes the odd ones
+            uint64_t a0 = (next + 1) / 2, a1 = (end + 1) / 2;
+            uint64_t d0 = next / 2, d1 = end / 2;
+            apply_batch(g, L, nL, s, p->ins_first + a0, a1 - a0, p->del_first + d0, d1 - d0, p->directed, &st);
+            double tc = now();
+            for (uint64_t u = next; u < end; u++) lat[u] = (tc - arr[u]) * 1e3;
+            if (p->target_ms > 0) {
+                double waited = (tc - arr[next]) * 1e3;
+                if (waited > 0.5 * p->target_ms) cap = std::max(1.0, cap * 0.9);
+                else cap = std::min(cap_max, cap < 1000 ? cap + 16 : cap * 1.01);
+            }
+            batches++;
+            max_batch = std::max<uint64_t>(max_batch, end - next);
+            next = end;
+        }
+        double wall = now();
+        r->wall_s = wall;
+        r->updates_per_s = wall > 0 ? (double)n / wall : 0;
+        r->batches = batches;
+        r->max_batch = max_batch;
+        r->mean_batch = batches ? (double)n / (double)batches : 0;
+        r->grid_steps = st.grid_steps;
+        r->rebuilds = st.rebuilds;
+        double sum = 0;
+        for (double x : lat) sum += x;
+        r->mean_ms = n ? sum / (double)n : 0;
+        if (lat_ms) std::copy(lat.begin(), lat.end(), lat_ms);
+        std::vector<double> tmp = lat;
+        r->p50_ms = percentile(tmp, 0.50);
+        r->p99_ms = percentile(tmp, 0.99);
+        r->p999_ms = percentile(tmp, 0.999);
+        r->max_ms = tmp.empty() ? 0 : *std::max_element(tmp.begin(), tmp.end());
+    });
+}
+
+}  // extern "C"

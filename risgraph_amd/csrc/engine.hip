@@ -1,0 +1,696 @@
+// engine.hip -- incremental dependency-tree kernels (gfx950).
+//
+// Device restatement of the reference engine in core/graph.hpp:
+//   seeding          update_tree_add single/batch      graph.hpp:1124-1218
+//   push relax       do_update_tree_add                graph.hpp:1002-1116
+//   delete roots     update_tree_del single/batch      graph.hpp:1622-1718
+//   rescue           do_update_tree_del + has_parents  graph.hpp:1323-1361, 1889-1907
+//   subtree expand   do_update_tree_del                graph.hpp:1364-1417
+//   reset/pull/push  do_update_tree_del                graph.hpp:1420-1608
+//   full build       build_tree                        graph.hpp:873-985
+//   label update     update_label (16-B CAS form)      graph.hpp:1861-1887
+//
+// The label word is 64 bits: value << 32 | parent vertex.  A relax is a CAS
+// loop that only writes when update_func strictly improves, exactly as the
+// reference's CAS form does, so equal-value races never re-parent a vertex
+// and parent pointers stay acyclic.  Parent identity is the vertex only; a
+// delete of any (p -> v) edge whose multiplicity reached 0 while v's parent is
+// p marks v, and the rescue step re-parents v onto a surviving parallel edge.
+//
+// Work is level-synchronous over vertex queues.  A vertex pass gives each
+// queued vertex a 16-lane group; lists longer than 128 entries are split
+// into 512-entry chunk items handled a wave each in the chunk pass, so hub
+// vertices spread over the whole chip.  The same pass functions run either in
+// one persistent 1024-thread workgroup (small frontiers: one launch for the
+// whole update) or as one grid-wide launch per pass (large frontiers); the
+// phase state machine lives in the device control block so the two modes
+// hand over at any pass boundary.
+#include "rg_internal.h"
+
+namespace rg {
+
+// ---------------------------------------------------------------------
+// algorithm lambdas (bfs_inc.cpp:77-92, sssp_inc.cpp:82-97,
+// sswp_inc.cpp:82-97, wcc_inc.cpp:82-97)
+// ---------------------------------------------------------------------
+template <int A>
+__device__ __forceinline__ uint64_t cand(uint32_t s, uint32_t w)
+{
+    if constexpr (A == BFS) return (uint64_t)s + 1;
+    else if constexpr (A == SSSP) return (uint64_t)s + w;
+    else if constexpr (A == SSWP) return s < w ? s : w;
+    else return s;
+}
+
+template <int A>
+__device__ __forceinline__ bool better(uint64_t c, uint32_t d)
+{
+    if constexpr (A == SSWP) return c > (uint64_t)d;
+    else return c < (uint64_t)d;
+}
+
+__device__ __forceinline__ uint64_t pack(uint32_t v, uint32_t p) { return ((uint64_t)v << 32) | p; }
+__device__ __forceinline__ uint32_t vval(uint64_t x) { return (uint32_t)(x >> 32); }
+__device__ __forceinline__ uint32_t vpar(uint64_t x) { return (uint32_t)x; }
+
+__device__ __forceinline__ uint32_t ald(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t ald(const int32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ast(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ast(int32_t *p, int32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// snapshot of the control block taken at a pass boundary
+struct Snap {
+    uint32_t phase, stage, level, n_cur, n_tree, n_chunk, n_ops;
+    uint32_t sel_cur, sel_next, sel_tree, epoch, mepoch;
+    int32_t idx;
+};
+
+__device__ __forceinline__ void take_snap(const EngineArgs &a, Snap &s)
+{
+    Ctl *c = a.ctl;
+    s.phase = ald(&c->phase);
+    s.stage = ald(&c->stage);
+    s.level = ald(&c->level);
+    s.n_cur = ald(&c->n_cur);
+    s.n_tree = ald(&c->n_tree);
+    s.n_chunk = ald(&c->n_chunk);
+    s.n_ops = ald(&c->n_ops);
+    s.sel_cur = ald(&c->sel_cur);
+    s.sel_next = ald(&c->sel_next);
+    s.sel_tree = ald(&c->sel_tree);
+    s.epoch = ald(&c->epoch);
+    s.mepoch = ald(&c->mepoch);
+    s.idx = ald(&c->idx);
+}
+
+// strict-improvement CAS on the packed label word (update_label, CAS form)
+template <int A>
+__device__ __forceinline__ bool relax(uint64_t *val, uint32_t dst, uint64_t c, uint32_t parent)
+{
+    unsigned long long *p = (unsigned long long *)&val[dst];
+    unsigned long long cur = *p;
+    while (better<A>(c, vval(cur))) {
+        unsigned long long nw = pack((uint32_t)c, parent);
+        unsigned long long prev = atomicCAS(p, cur, nw);
+        if (prev == cur) return true;
+        cur = prev;
+    }
+    return false;
+}
+
+// bounded append to a vertex queue (capacity V; dedupe keeps it in range)
+__device__ __forceinline__ void q_append(const EngineArgs &a, uint32_t *q, uint32_t *count, uint32_t v)
+{
+    uint32_t pos = atomicAdd(count, 1u);
+    if (pos < a.V) q[pos] = v;
+    else atomicOr(&a.ctl->flags, F_QUEUE_OVERFLOW);
+}
+
+__device__ __forceinline__ void push_next(const EngineArgs &a, const Snap &s, uint32_t v)
+{
+    if (atomicExch(&a.stamp[v], s.epoch) != s.epoch) q_append(a, a.q[s.sel_next], &a.ctl->n_next, v);
+}
+
+__device__ __forceinline__ void mark_mod(const EngineArgs &a, const Snap &s, uint32_t v)
+{
+    if (a.trace && atomicExch(&a.mstamp[v], s.mepoch) != s.mepoch) {
+        uint32_t pos = atomicAdd(&a.ctl->n_mod, 1u);
+        if (pos < a.V) a.mlist[pos] = v;
+    }
+}
+
+__device__ __forceinline__ void defer_chunks(const EngineArgs &a, uint32_t v, int L, uint32_t len)
+{
+    uint32_t nch = (len + CHUNK - 1) / CHUNK;
+    uint32_t pos = atomicAdd(&a.ctl->n_chunk, nch);
+    if (pos + nch > a.cq_cap) {
+        atomicOr(&a.ctl->flags, F_QUEUE_OVERFLOW);
+        return;
+    }
+    for (uint32_t k = 0; k < nch; k++)
+        a.cq[pos + k] = (uint64_t)v | ((uint64_t)L << 32) | ((uint64_t)k << 33);
+}
+
+// has_parents (graph.hpp:1889-1907): true when an ancestor of x (x
+// included) carries the current invalidation mark; a walk longer than the cap
+// asks for a full rebuild, like the reference's 0.1*V guard.
+__device__ bool has_parents(const EngineArgs &a, uint32_t x, int32_t idx)
+{
+    uint32_t steps = 0;
+    for (;;) {
+        int32_t iv = a.inv[x];
+        if (iv == idx || iv == -idx) return true;
+        uint32_t p = vpar(a.val[x]);
+        if (p == NILP) return false;
+        x = p;
+        if (++steps > a.tune.walk_limit) {
+            atomicOr(&a.ctl->flags, F_NEED_REBUILD);
+            return true;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// per-entry work shared by vertex groups (16 lanes) and chunk waves (64)
+// ---------------------------------------------------------------------
+template <int A>
+__device__ __forceinline__ void entry_w(const EngineArgs &a, uint64_t slot, uint32_t &w)
+{
+    if constexpr (A == SSSP || A == SSWP) w = a.slots.wt[slot];
+    else w = 0;
+}
+
+template <int A>
+__device__ void push_range(const EngineArgs &a, const Snap &s, uint32_t u, uint32_t uv, uint64_t off,
+                           uint32_t b, uint32_t e, int lane, int stride)
+{
+    for (uint32_t i = b + lane; i < e; i += stride) {
+        uint32_t n = a.slots.nbr[off + i];
+        if (n & DEAD) continue;
+        uint32_t w;
+        entry_w<A>(a, off + i, w);
+        uint64_t c = cand<A>(uv, w);
+        if (!better<A>(c, vval(a.val[n]))) continue;
+        if (relax<A>(a.val, n, c, u)) {
+            push_next(a, s, n);
+            mark_mod(a, s, n);
+        }
+    }
+}
+
+template <int A>
+__device__ void rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uint32_t dv, uint64_t off,
+                             uint32_t b, uint32_t e, int lane, int stride)
+{
+    for (uint32_t i = b + lane; i < e; i += stride) {
+        if (a.inv[d] != s.idx) break;
+        uint32_t n = a.slots.nbr[off + i];
+        if (n & DEAD) continue;
+        uint32_t w;
+        entry_w<A>(a, off + i, w);
+        if (cand<A>(vval(a.val[n]), w) != (uint64_t)dv) continue;
+        if (has_parents(a, n, s.idx)) continue;
+        if (atomicCAS(&a.inv[d], s.idx, -s.idx) == s.idx) {
+            a.val[d] = pack(dv, n);
+            break;
+        }
+    }
+}
+
+__device__ void expand_range(const EngineArgs &a, const Snap &s, uint32_t x, uint64_t off, uint32_t b,
+                             uint32_t e, int lane, int stride)
+{
+    uint32_t *qn = a.q[s.sel_next];
+    for (uint32_t i = b + lane; i < e; i += stride) {
+        uint32_t c = a.slots.nbr[off + i];
+        if (c & DEAD) continue;
+        if (vpar(a.val[c]) != x) continue;
+        if (atomicCAS(&a.inv[c], 0, s.idx) == 0) q_append(a, qn, &a.ctl->n_next, c);
+    }
+}
+
+// best candidate word over a range (min or max of value<<32|parent)
+template <int A>
+__device__ uint64_t pull_range(const EngineArgs &a, uint64_t off, uint32_t b, uint32_t e, int lane,
+                               int stride)
+{
+    uint64_t best = (A == SSWP) ? 0ull : ~0ull;
+    for (uint32_t i = b + lane; i < e; i += stride) {
+        uint32_t n = a.slots.nbr[off + i];
+        if (n & DEAD) continue;
+        uint32_t w;
+        entry_w<A>(a, off + i, w);
+        uint64_t c = cand<A>(vval(a.val[n]), w);
+        if (A != SSWP && c > 0xFFFFFFFFull) continue;
+        uint64_t word = pack((uint32_t)c, n);
+        if (A == SSWP ? word > best : word < best) best = word;
+    }
+    return best;
+}
+
+template <int A>
+__device__ __forceinline__ uint64_t reduce_best(uint64_t x, int width)
+{
+    for (int m = width / 2; m > 0; m >>= 1) {
+        uint64_t y = __shfl_xor((unsigned long long)x, m, width);
+        if (A == SSWP ? y > x : y < x) x = y;
+    }
+    return x;
+}
+
+template <int A>
+__device__ __forceinline__ void pull_apply(const EngineArgs &a, const Snap &s, uint32_t x, uint64_t best)
+{
+    if (A == SSWP ? best == 0ull : best == ~0ull) return;
+    if (relax<A>(a.val, x, vval(best), vpar(best))) mark_mod(a, s, x);
+}
+
+// ---------------------------------------------------------------------
+// passes
+// ---------------------------------------------------------------------
+template <int A>
+__device__ void seed_edge(const EngineArgs &a, const Snap &s, uint32_t src, uint32_t dst, uint32_t w)
+{
+    uint64_t c = cand<A>(vval(a.val[src]), w);
+    if (!better<A>(c, vval(a.val[dst]))) return;
+    if (relax<A>(a.val, dst, c, src)) {
+        push_next(a, s, dst);
+        mark_mod(a, s, dst);
+    }
+}
+
+__device__ __forceinline__ void mark_root(const EngineArgs &a, const Snap &s, uint32_t src, uint32_t dst)
+{
+    if (vpar(a.val[dst]) != src) return;
+    if (atomicCAS(&a.inv[dst], 0, s.idx) == 0) q_append(a, a.q[s.sel_next], &a.ctl->n_next, dst);
+}
+
+template <int A>
+__device__ void pass_ops(const EngineArgs &a, const Snap &s, uint32_t tid, uint32_t nthr)
+{
+    for (uint32_t i = tid; i < s.n_ops; i += nthr) {
+        uint8_t k = a.ops.kind[i];
+        uint32_t src = a.ops.src[i], dst = a.ops.dst[i];
+        uint32_t w = (A == SSSP || A == SSWP) ? a.ops.w[i] : 0;
+        bool both = a.undirected || (k & K_UNDIR);
+        if (s.phase == PH_SEED) {
+            if (!(k & K_INS)) continue;
+            bool seed = k & K_FORCE;
+            if (!seed && a.ops.old[i] == 0 && a.ops.slot[i] != NILP) {
+                uint64_t off = a.lists[0].off[src];
+                seed = a.slots.cnt[off + a.ops.slot[i]] > 0;
+            }
+            if (!seed) continue;
+            seed_edge<A>(a, s, src, dst, w);
+            if (both) seed_edge<A>(a, s, dst, src, w);
+        } else {
+            if (!(k & K_DEL)) continue;
+            if (!((k & K_FORCE) || a.ops.old[i] == 1)) continue;
+            mark_root(a, s, src, dst);
+            if (both) mark_root(a, s, dst, src);
+        }
+    }
+}
+
+template <int A>
+__device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, uint32_t nthr)
+{
+    const uint32_t ph = s.phase;
+    if (ph == PH_BUILD_INIT) {
+        for (uint32_t v = tid; v < a.V; v += nthr) {
+            a.val[v] = pack(a.init[v], NILP);
+            a.inv[v] = 0;
+            bool act = a.init_active ? a.init_active[v] != 0 : v == a.root;
+            if (act) push_next(a, s, v);
+        }
+        return;
+    }
+    if (ph == PH_DEL_RESET) {
+        const uint32_t *qt = a.q[s.sel_tree];
+        for (uint32_t i = tid; i < s.n_tree; i += nthr) {
+            uint32_t x = qt[i];
+            a.val[x] = pack(a.init[x], NILP);
+            a.inv[x] = 0;
+            mark_mod(a, s, x);
+        }
+        return;
+    }
+    const int glane = tid % GROUP;
+    const uint32_t gid = tid / GROUP, ngr = nthr / GROUP;
+    const uint32_t n = (ph == PH_DEL_PULL) ? s.n_tree : s.n_cur;
+    const uint32_t *qv = (ph == PH_DEL_PULL) ? a.q[s.sel_tree] : a.q[s.sel_cur];
+    for (uint32_t gi = gid; gi < n; gi += ngr) {
+        uint32_t v = qv[gi];
+        if (ph == PH_PUSH) {
+            uint32_t uv = vval(a.val[v]);
+            for (int li = 0; li < a.npush; li++) {
+                int L = a.push_l[li];
+                uint32_t len = a.lists[L].len[v];
+                if (len > LONG_LIST) {
+                    if (glane == 0) defer_chunks(a, v, L, len);
+                    continue;
+                }
+                push_range<A>(a, s, v, uv, a.lists[L].off[v], 0, len, glane, GROUP);
+            }
+        } else if (ph == PH_DEL_RESCUE) {
+            if (a.inv[v] != s.idx) continue;
+            uint32_t dv = vval(a.val[v]);
+            for (int li = 0; li < a.npull; li++) {
+                int L = a.pull_l[li];
+                uint32_t len = a.lists[L].len[v];
+                if (len > LONG_LIST) {
+                    if (glane == 0) defer_chunks(a, v, L, len);
+                    continue;
+                }
+                rescue_range<A>(a, s, v, dv, a.lists[L].off[v], 0, len, glane, GROUP);
+            }
+        } else if (ph == PH_DEL_EXPAND) {
+            int32_t iv = a.inv[v];
+            if (iv == -s.idx) {
+                if (glane == 0) a.inv[v] = 0;
+                continue;
+            }
+            if (iv != s.idx) continue;
+            if (glane == 0) q_append(a, a.q[s.sel_tree], &a.ctl->n_tree, v);
+            for (int li = 0; li < a.npush; li++) {
+                int L = a.push_l[li];
+                uint32_t len = a.lists[L].len[v];
+                if (len > LONG_LIST) {
+                    if (glane == 0) defer_chunks(a, v, L, len);
+                    continue;
+                }
+                expand_range(a, s, v, a.lists[L].off[v], 0, len, glane, GROUP);
+            }
+        } else if (ph == PH_DEL_PULL) {
+            uint64_t best = (A == SSWP) ? 0ull : ~0ull;
+            for (int li = 0; li < a.npull; li++) {
+                int L = a.pull_l[li];
+                uint32_t len = a.lists[L].len[v];
+                if (len > LONG_LIST) {
+                    if (glane == 0) defer_chunks(a, v, L, len);
+                    continue;
+                }
+                uint64_t b = pull_range<A>(a, a.lists[L].off[v], 0, len, glane, GROUP);
+                if (A == SSWP ? b > best : b < best) best = b;
+            }
+            best = reduce_best<A>(best, GROUP);
+            if (glane == 0) pull_apply<A>(a, s, v, best);
+        }
+    }
+}
+
+template <int A>
+__device__ void pass_chunks(const EngineArgs &a, const Snap &s, uint32_t tid, uint32_t nthr, uint32_t n_chunk)
+{
+    const uint32_t ph = s.phase;
+    if (ph != PH_PUSH && ph != PH_DEL_RESCUE && ph != PH_DEL_EXPAND && ph != PH_DEL_PULL) return;
+    const int lane = tid % WAVE;
+    const uint32_t wid = tid / WAVE, nw = nthr / WAVE;
+    for (uint32_t ci = wid; ci < n_chunk; ci += nw) {
+        uint64_t item = a.cq[ci];
+        uint32_t v = (uint32_t)item;
+        int L = (int)((item >> 32) & 1);
+        uint32_t k = (uint32_t)(item >> 33);
+        uint32_t len = a.lists[L].len[v];
+        uint64_t off = a.lists[L].off[v];
+        uint32_t b = k * CHUNK, e = b + CHUNK < len ? b + CHUNK : len;
+        if (ph == PH_PUSH) {
+            push_range<A>(a, s, v, vval(a.val[v]), off, b, e, lane, WAVE);
+        } else if (ph == PH_DEL_RESCUE) {
+            if (a.inv[v] != s.idx) continue;
+            rescue_range<A>(a, s, v, vval(a.val[v]), off, b, e, lane, WAVE);
+        } else if (ph == PH_DEL_EXPAND) {
+            expand_range(a, s, v, off, b, e, lane, WAVE);
+        } else {
+            uint64_t best = pull_range<A>(a, off, b, e, lane, WAVE);
+            best = reduce_best<A>(best, WAVE);
+            if (lane == 0) pull_apply<A>(a, s, v, best);
+        }
+    }
+}
+
+// phase transitions, run by one thread after a pass completed
+__device__ void advance(const EngineArgs &a)
+{
+    Ctl *c = a.ctl;
+    uint32_t ph = ald(&c->phase);
+    auto swap_queues = [&]() {
+        uint32_t cur = ald(&c->sel_cur), nxt = ald(&c->sel_next);
+        ast(&c->sel_cur, nxt);
+        ast(&c->sel_next, cur);
+        ast(&c->n_cur, ald(&c->n_next));
+        ast(&c->n_next, 0u);
+        ast(&c->epoch, ald(&c->epoch) + 1);
+        c->levels += 1;
+    };
+    auto after_push = [&]() {
+        if (ald(&c->del_pending)) {
+            ast(&c->del_pending, 0u);
+            ast(&c->phase, (uint32_t)PH_DEL_ROOTS);
+        } else {
+            ast(&c->phase, (uint32_t)PH_DONE);
+        }
+    };
+    auto rescue_on = [&](uint32_t level, uint32_t n) {
+        return level <= a.tune.rescue_level || n < a.tune.rescue_frontier;
+    };
+    auto recompute = [&](uint32_t level, uint32_t n) {
+        uint64_t tot = (uint64_t)ald(&c->n_tree) + n;
+        return level <= a.tune.recompute_level && (double)tot > (double)a.tune.recompute_frac * a.V &&
+               tot > a.tune.recompute_abs;
+    };
+    ast(&c->n_chunk, 0u);
+    ast(&c->stage, 0u);
+    switch (ph) {
+    case PH_SEED:
+    case PH_BUILD_INIT:
+    case PH_PUSH:
+        swap_queues();
+        if (ald(&c->n_cur) == 0) after_push();
+        else ast(&c->phase, (uint32_t)PH_PUSH);
+        break;
+    case PH_DEL_ROOTS: {
+        swap_queues();
+        ast(&c->level, 0u);
+        ast(&c->n_tree, 0u);
+        uint32_t n = ald(&c->n_cur);
+        if (n == 0) ast(&c->phase, (uint32_t)PH_DONE);
+        else if (recompute(0, n)) ast(&c->phase, (uint32_t)PH_REBUILD);
+        else ast(&c->phase, rescue_on(0, n) ? (uint32_t)PH_DEL_RESCUE : (uint32_t)PH_DEL_EXPAND);
+        break;
+    }
+    case PH_DEL_RESCUE:
+        if (ald(&c->flags) & F_NEED_REBUILD) ast(&c->phase, (uint32_t)PH_REBUILD);
+        else ast(&c->phase, (uint32_t)PH_DEL_EXPAND);
+        break;
+    case PH_DEL_EXPAND: {
+        swap_queues();
+        uint32_t level = ald(&c->level) + 1;
+        ast(&c->level, level);
+        uint32_t n = ald(&c->n_cur);
+        if (n == 0) ast(&c->phase, (uint32_t)PH_DEL_RESET);
+        else if (recompute(level, n)) ast(&c->phase, (uint32_t)PH_REBUILD);
+        else ast(&c->phase, rescue_on(level, n) ? (uint32_t)PH_DEL_RESCUE : (uint32_t)PH_DEL_EXPAND);
+        break;
+    }
+    case PH_DEL_RESET:
+        ast(&c->phase, (uint32_t)PH_DEL_PULL);
+        break;
+    case PH_DEL_PULL: {
+        // push to the fixpoint starting from the whole reset subtree
+        uint32_t cur = ald(&c->sel_cur), nxt = ald(&c->sel_next), tree = ald(&c->sel_tree);
+        ast(&c->sel_cur, tree);
+        ast(&c->sel_next, cur);
+        ast(&c->sel_tree, nxt);
+        ast(&c->n_cur, ald(&c->n_tree));
+        ast(&c->n_next, 0u);
+        ast(&c->n_tree, 0u);
+        ast(&c->epoch, ald(&c->epoch) + 1);
+        ast(&c->phase, ald(&c->n_cur) ? (uint32_t)PH_PUSH : (uint32_t)PH_DONE);
+        break;
+    }
+    default:
+        // unknown phase: stop instead of spinning
+        atomicOr(&c->flags, F_QUEUE_OVERFLOW);
+        ast(&c->phase, (uint32_t)PH_DONE);
+        break;
+    }
+}
+
+__device__ __forceinline__ uint32_t phase_work(const EngineArgs &a, const Snap &s)
+{
+    switch (s.phase) {
+    case PH_SEED:
+    case PH_DEL_ROOTS: return s.n_ops;
+    case PH_DEL_RESET:
+    case PH_DEL_PULL: return s.n_tree;
+    case PH_BUILD_INIT: return a.V;
+    default: return s.n_cur;
+    }
+}
+
+// ---------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------
+template <int A>
+__global__ __launch_bounds__(1024) void k_block(EngineArgs a, uint32_t block_max, uint32_t chunk_max)
+{
+    __shared__ Snap snap;
+    __shared__ uint32_t nch;
+    const uint32_t tid = threadIdx.x, nthr = blockDim.x;
+    // every pass either finishes the phase or hands it to grid mode; the cap
+    // only bounds the launch if the state machine were ever corrupted
+    for (uint32_t iter = 0; iter < (1u << 24); iter++) {
+        __threadfence_block();
+        __syncthreads();
+        if (tid == 0) take_snap(a, snap);
+        __syncthreads();
+        const Snap s = snap;
+        if (s.phase == PH_DONE || s.phase == PH_REBUILD) break;
+        if (phase_work(a, s) > block_max) break;
+        if (s.stage == 0) {
+            if (s.phase == PH_SEED || s.phase == PH_DEL_ROOTS) pass_ops<A>(a, s, tid, nthr);
+            else pass_vertices<A>(a, s, tid, nthr);
+            __threadfence_block();
+            __syncthreads();
+            if (tid == 0) nch = ald(&a.ctl->n_chunk);
+            __syncthreads();
+            if (nch > chunk_max) {
+                if (tid == 0) ast(&a.ctl->stage, 1u);
+                break;
+            }
+        } else {
+            if (tid == 0) nch = ald(&a.ctl->n_chunk);
+            __syncthreads();
+        }
+        pass_chunks<A>(a, s, tid, nthr, nch);
+        __threadfence_block();
+        __syncthreads();
+        if (tid == 0) advance(a);
+    }
+}
+
+template <int A>
+__global__ void k_grid_main(EngineArgs a)
+{
+    __shared__ Snap snap;
+    if (threadIdx.x == 0) take_snap(a, snap);
+    __syncthreads();
+    const Snap s = snap;
+    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthr = gridDim.x * blockDim.x;
+    if (s.stage != 0) return;
+    if (s.phase == PH_SEED || s.phase == PH_DEL_ROOTS) pass_ops<A>(a, s, tid, nthr);
+    else pass_vertices<A>(a, s, tid, nthr);
+}
+
+template <int A>
+__global__ void k_grid_chunks(EngineArgs a)
+{
+    __shared__ Snap snap;
+    if (threadIdx.x == 0) take_snap(a, snap);
+    __syncthreads();
+    const Snap s = snap;
+    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthr = gridDim.x * blockDim.x;
+    pass_chunks<A>(a, s, tid, nthr, s.n_chunk);
+}
+
+__global__ void k_advance(EngineArgs a) { advance(a); }
+
+template <int A>
+static void block_launch(const EngineArgs &a, uint32_t block_max, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_block<A>, dim3(1), dim3(1024), 0, st, a, block_max, 64u);
+}
+
+void launch_engine_block(const EngineArgs &a, uint32_t block_max, hipStream_t st)
+{
+    switch (a.algo) {
+    case BFS: block_launch<BFS>(a, block_max, st); break;
+    case SSSP: block_launch<SSSP>(a, block_max, st); break;
+    case SSWP: block_launch<SSWP>(a, block_max, st); break;
+    default: block_launch<WCC>(a, block_max, st); break;
+    }
+}
+
+template <int A>
+static void grid_launch(const EngineArgs &a, const Ctl &h, hipStream_t st)
+{
+    uint64_t threads;
+    switch (h.phase) {
+    case PH_SEED:
+    case PH_DEL_ROOTS: threads = h.n_ops; break;
+    case PH_BUILD_INIT: threads = a.V; break;
+    case PH_DEL_RESET: threads = h.n_tree; break;
+    case PH_DEL_PULL: threads = (uint64_t)h.n_tree * GROUP; break;
+    default: threads = (uint64_t)h.n_cur * GROUP; break;
+    }
+    uint64_t blocks = (threads + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 32768) blocks = 32768;
+    if (h.stage == 0) hipLaunchKernelGGL(k_grid_main<A>, dim3((uint32_t)blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_grid_chunks<A>, dim3(2048), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, a);
+}
+
+void launch_engine_grid_step(const EngineArgs &a, const Ctl &h, hipStream_t st)
+{
+    switch (a.algo) {
+    case BFS: grid_launch<BFS>(a, h, st); break;
+    case SSSP: grid_launch<SSSP>(a, h, st); break;
+    case SSWP: grid_launch<SSWP>(a, h, st); break;
+    default: grid_launch<WCC>(a, h, st); break;
+    }
+}
+
+// need_update_tree_add / need_update_tree_del (graph.hpp:987-1000,
+// 1220-1233) for a batch of edges: an add is unsafe when it strictly improves
+// the target label, a del when the target's parent is the source (either
+// direction for undirected ops or an undirected algorithm).
+template <int A>
+__global__ void k_classify(EngineArgs a, uint32_t n, uint8_t *unsafe)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint8_t k = a.ops.kind[i];
+    uint32_t s = a.ops.src[i], d = a.ops.dst[i];
+    uint32_t w = (A == SSSP || A == SSWP) ? a.ops.w[i] : 0;
+    bool both = a.undirected || (k & K_UNDIR);
+    bool u;
+    if (k & K_INS) {
+        u = better<A>(cand<A>(vval(a.val[s]), w), vval(a.val[d]));
+        if (both) u = u || better<A>(cand<A>(vval(a.val[d]), w), vval(a.val[s]));
+    } else {
+        u = vpar(a.val[d]) == s;
+        if (both) u = u || vpar(a.val[s]) == d;
+    }
+    unsafe[i] = u;
+}
+
+void launch_classify(const EngineArgs &a, uint32_t n, uint8_t *unsafe, hipStream_t st)
+{
+    if (!n) return;
+    dim3 g((n + 255) / 256), b(256);
+    switch (a.algo) {
+    case BFS: hipLaunchKernelGGL(k_classify<BFS>, g, b, 0, st, a, n, unsafe); break;
+    case SSSP: hipLaunchKernelGGL(k_classify<SSSP>, g, b, 0, st, a, n, unsafe); break;
+    case SSWP: hipLaunchKernelGGL(k_classify<SSWP>, g, b, 0, st, a, n, unsafe); break;
+    default: hipLaunchKernelGGL(k_classify<WCC>, g, b, 0, st, a, n, unsafe); break;
+    }
+}
+
+// init labels of the reference drivers: BFS/SSSP root 0 else MAXL, SSWP
+// root MAXL else 0 (MAXL = 65536 for BFS, 2^27 otherwise); WCC takes the
+// host-provided labels (identity or a permutation) and is not filled here.
+__global__ void k_fill_init(uint32_t *init, uint32_t V, int algo, uint32_t root)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        uint32_t x;
+        if (algo == BFS) x = v == root ? 0u : 65536u;
+        else if (algo == SSSP) x = v == root ? 0u : 134217728u;
+        else if (algo == SSWP) x = v == root ? 134217728u : 0u;
+        else x = v;
+        init[v] = x;
+    }
+}
+
+void launch_engine_fill_init(const EngineArgs &a, hipStream_t st)
+{
+    uint64_t blocks = (a.V + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    if (!blocks) blocks = 1;
+    hipLaunchKernelGGL(k_fill_init, dim3((uint32_t)blocks), dim3(256), 0, st, a.init, a.V, a.algo, a.root);
+}
+
+}  // namespace rg

@@ -1,0 +1,187 @@
+// rg_internal.h -- device data layout shared by the store and engine kernels.
+//
+// HBM layout (one graph):
+//   slot pool   nbr[S] (u32: neighbour | DEAD bit), wt[S] (u32, weighted
+//               graphs only), cnt[S] (u32 multiplicity).  Each vertex list is
+//               one contiguous block [off, off+cap) of the pool; `len` entries
+//               are used, tombstones (cnt == 0) keep their place, as
+//               IndexedEdgeStorage does (storage.hpp:369-394).
+//   hash pool   hkey[H] (u64 key = nbr | w << 32, ~0 = empty), hval[H] (u32
+//               local slot) -- per-list open-addressing index built once a
+//               list reaches 512 entries (storage.hpp:304, 358-367).
+//   headers     per direction (out, in): off, len, cap, deg, hoff, hcap,
+//               hcnt, head (batch op chains).
+// One labels instance (one algorithm's dependency tree):
+//   val[V]      u64 packed word value << 32 | parent (parent 0xFFFFFFFF =
+//               empty), the device form of VertexTree (graph.hpp:54-59),
+//               updated by 64-bit CAS with a strict-improvement check.
+//   init[V]     init labels; inv[V] invalidation marks (graph.hpp:1729);
+//               stamp[V] / mstamp[V] frontier and modified dedupe epochs;
+//               q[3][V] vertex queues (cur / next / tree); cq chunk queue of
+//               long lists; mlist[V] modified set; ctl control block.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rg {
+
+enum Algo : int { BFS = 0, SSSP = 1, SSWP = 2, WCC = 3 };
+
+constexpr uint32_t DEAD = 0x80000000u;
+constexpr uint32_t NBR_MASK = 0x7FFFFFFFu;
+constexpr uint32_t NILP = 0xFFFFFFFFu;
+constexpr uint32_t INDEX_THRESHOLD = 512;   // storage.hpp:304
+constexpr uint64_t HEMPTY = ~0ull;
+constexpr uint32_t LONG_LIST = 128;         // longer lists go to the chunk queue
+constexpr uint32_t CHUNK = 512;             // entries per chunk item
+constexpr int GROUP = 16;                   // lanes per vertex in vertex passes
+constexpr int WAVE = 64;
+
+struct Slots {
+    uint32_t *nbr;
+    uint32_t *wt;
+    uint32_t *cnt;
+    uint64_t *hkey;
+    uint32_t *hval;
+};
+
+struct List {
+    uint64_t *off;
+    uint32_t *len;
+    uint32_t *cap;
+    uint32_t *deg;
+    uint64_t *hoff;
+    uint32_t *hcap;
+    uint32_t *hcnt;
+    int32_t *head;
+};
+
+struct Pool {
+    unsigned long long slot_top, slot_cap;
+    unsigned long long hash_top, hash_cap;
+    unsigned int oom;
+    unsigned int pad;
+};
+
+// edge-op kinds (bitmask)
+constexpr uint8_t K_INS = 1;      // store insert (+1), seed if old==0 and live
+constexpr uint8_t K_DEL = 2;      // store delete (-1), root if old==1
+constexpr uint8_t K_FORCE = 4;    // engine-only edge from update_tree_add/del
+constexpr uint8_t K_UNDIR = 8;    // undirected op (both directions)
+constexpr uint8_t K_NOSTORE = 16; // do not touch the store
+
+struct EdgeOps {
+    uint32_t *src, *dst, *w;
+    uint8_t *kind;
+    uint32_t *old;      // reference add/del return value
+    uint32_t *slot;     // local index of the (src->dst) entry in src's out-list
+    uint32_t n;
+};
+
+// list-op flags
+constexpr uint8_t L_IN = 1;       // in-list (else out-list)
+constexpr uint8_t L_RET = 2;      // its old count is the edge op's return value
+constexpr uint8_t L_SLOT = 4;     // its local slot is the edge op's liveness slot
+constexpr uint8_t L_SELF = 8;     // undirected self-loop folded into one +-2 op
+
+struct ListOps {
+    uint32_t *v, *nbr, *w, *edge;
+    int8_t *delta;
+    uint8_t *flags;
+    int32_t *next;
+    uint32_t n;
+};
+
+enum Phase : uint32_t {
+    PH_DONE = 0,
+    PH_SEED = 1,
+    PH_PUSH = 2,
+    PH_DEL_ROOTS = 3,
+    PH_DEL_RESCUE = 4,
+    PH_DEL_EXPAND = 5,
+    PH_DEL_RESET = 6,
+    PH_DEL_PULL = 7,
+    PH_BUILD_INIT = 8,
+    PH_REBUILD = 9,
+};
+
+constexpr uint32_t F_NEED_REBUILD = 1;
+constexpr uint32_t F_QUEUE_OVERFLOW = 2;
+constexpr uint32_t F_ALL_MODIFIED = 4;
+
+struct Ctl {
+    uint32_t phase, stage, level, del_pending;
+    uint32_t n_cur, n_next, n_tree, n_chunk;
+    uint32_t n_mod, n_ops;
+    uint32_t sel_cur, sel_next, sel_tree;
+    uint32_t epoch, mepoch;
+    int32_t idx;
+    uint32_t flags;
+    uint32_t pad;
+    unsigned long long edges, relax_ok, levels, pad2;
+};
+
+struct Tuning {
+    uint32_t rescue_level;        // rescue when level <= this ...
+    uint32_t rescue_frontier;     // ... or frontier < this (graph.hpp:1324)
+    uint32_t recompute_level;     // fallback checked up to this level (graph.hpp:1252)
+    uint32_t recompute_abs;       // ... when subtree+frontier > this ...
+    float recompute_frac;         // ... and > frac * V
+    uint32_t walk_limit;          // has_parents step cap (graph.hpp:1900)
+};
+
+struct EngineArgs {
+    uint32_t V;
+    int algo;
+    int trace;                    // maintain the modified set
+    int undirected;               // algorithm sees edges in both directions
+    Slots slots;
+    List lists[2];
+    int npush, push_l[2];         // lists scanned by push / expand
+    int npull, pull_l[2];         // lists scanned by pull / rescue
+    uint64_t *val;
+    uint32_t *init;
+    uint32_t *init_active;        // nullptr: only vertex `root` starts active
+    uint32_t root;
+    int32_t *inv;
+    uint32_t *stamp, *mstamp;
+    uint32_t *q[3];
+    uint64_t *cq;
+    uint32_t cq_cap;
+    uint32_t *mlist;
+    Ctl *ctl;
+    EdgeOps ops;
+    Tuning tune;
+};
+
+struct StoreArgs {
+    uint32_t V;
+    int symmetric;
+    int weighted;
+    Slots slots;
+    List lists[2];
+    Pool *pool;
+    EdgeOps ops;
+    ListOps lops;
+    uint32_t *err;   // bad vertex ids etc.
+};
+
+// host-side launchers (store.hip / engine.hip)
+void launch_store_batch(const StoreArgs &a, hipStream_t s);
+void launch_store_reset_heads(const StoreArgs &a, hipStream_t s);
+void launch_engine_block(const EngineArgs &a, uint32_t block_max, hipStream_t s);
+void launch_engine_grid_step(const EngineArgs &a, const Ctl &host_ctl, hipStream_t s);
+void launch_engine_fill_init(const EngineArgs &a, hipStream_t s);
+void launch_gen_rmat(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, int scale,
+                     uint64_t seed, double a, double b, double c, hipStream_t s);
+void launch_gen_er(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, uint64_t V,
+                   uint64_t seed, hipStream_t s);
+void launch_gen_weights(uint32_t *w, uint64_t n, uint64_t start, uint64_t seed, hipStream_t s);
+void launch_classify(const EngineArgs &a, uint32_t n, uint8_t *unsafe, hipStream_t s);
+void launch_max_degree(const uint64_t *src, uint64_t n, uint64_t V, uint32_t *deg_tmp,
+                       unsigned long long *best, hipStream_t s);
+void launch_ops_from_stream(const uint64_t *src, const uint64_t *dst, const uint32_t *w,
+                            uint64_t first, uint32_t n, uint8_t kind, EdgeOps ops,
+                            uint32_t base, hipStream_t s);
+
+}  // namespace rg

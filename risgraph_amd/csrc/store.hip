@@ -1,0 +1,424 @@
+// store.hip -- HBM slab adjacency store: batched edge insert/delete.
+//
+// Replaces IndexedEdgeStorage::update_edge (reference core/storage.hpp:345-398)
+// called from Graph::add_edge / del_edge (core/graph.hpp:203-253).
+//
+// A batch of edge ops is expanded into list ops (out-list of src and in-list
+// of dst, or both endpoint lists of a symmetric graph).  Every list op is
+// pushed onto a per-(direction, vertex) chain with one atomicExch, so each
+// list is owned by exactly one wave in the apply kernel: no locks, no
+// cross-wave races, and the block of a list may be re-allocated (doubling,
+// from a bump pool) before the wave appends to it.  The owning wave applies
+// the chain's inserts first, then its deletes, so a batch behaves like the
+// reference batch driver (all adds, then all dels: bfs_inc_batch.cpp:142-178).
+// Lookups scan the list 64 entries per step (ballot on the match), or probe
+// the list's open-addressing index once it holds >= 512 entries.
+#include "rg_internal.h"
+#include "rg_gen.h"
+
+namespace rg {
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+__device__ __forceinline__ uint64_t hkey_of(uint32_t nbr, uint32_t w)
+{
+    return (uint64_t)(nbr & NBR_MASK) | ((uint64_t)w << 32);
+}
+
+__device__ __forceinline__ uint32_t next_pow2(uint32_t x)
+{
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// ---------------------------------------------------------------------
+// expand edge ops into list ops and link them into per-list chains
+// ---------------------------------------------------------------------
+__global__ void k_expand_link(StoreArgs a)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.ops.n) return;
+    uint32_t s = a.ops.src[i], d = a.ops.dst[i], w = a.weighted ? a.ops.w[i] : 0;
+    uint8_t kind = a.ops.kind[i];
+    a.ops.old[i] = 0;
+    a.ops.slot[i] = NILP;
+    uint32_t base = 4 * i;
+    for (int k = 0; k < 4; k++) a.lops.v[base + k] = NILP;
+    if (kind & K_NOSTORE) return;
+    if (s >= a.V || d >= a.V) {
+        atomicOr(a.err, 1u);
+        return;
+    }
+    int8_t sign = (kind & K_INS) ? 1 : -1;
+    bool undir = kind & K_UNDIR;
+    uint32_t nl = 0;
+    auto emit = [&](uint32_t v, uint32_t nbr, int8_t delta, uint8_t flags) {
+        uint32_t j = base + nl++;
+        a.lops.v[j] = v;
+        a.lops.nbr[j] = nbr;
+        a.lops.w[j] = w;
+        a.lops.edge[j] = i;
+        a.lops.delta[j] = delta;
+        a.lops.flags[j] = flags;
+        int L = (flags & L_IN) ? 1 : 0;
+        a.lops.next[j] = atomicExch(&a.lists[L].head[v], (int32_t)j);
+    };
+    // graph.hpp:203-253: a directed op touches out[src] (+ in[dst] unless
+    // symmetric); an undirected op is the directed op both ways and returns
+    // the old count of the second (d->s) direction, minus 1 for a self-loop
+    // on add.  A self-loop's two same-key updates fold into one +-2 op.
+    if (!undir) {
+        emit(s, d, sign, L_RET | L_SLOT);
+        if (!a.symmetric) emit(d, s, sign, L_IN);
+    } else if (s != d) {
+        emit(s, d, sign, L_SLOT);
+        if (!a.symmetric) emit(d, s, sign, L_IN);
+        emit(d, s, sign, L_RET);
+        if (!a.symmetric) emit(s, d, sign, L_IN);
+    } else {
+        emit(s, s, (int8_t)(2 * sign), L_RET | L_SLOT | L_SELF);
+        if (!a.symmetric) emit(s, s, (int8_t)(2 * sign), L_IN | L_SELF);
+    }
+}
+
+// ---------------------------------------------------------------------
+// apply: one wave owns one list (the chain's head op)
+// ---------------------------------------------------------------------
+struct ListState {
+    uint64_t off, hoff;
+    uint32_t len, cap, deg, hcap, hcnt;
+};
+
+__device__ uint64_t pool_alloc(Pool *p, bool hash, uint64_t n)
+{
+    unsigned long long *top = hash ? &p->hash_top : &p->slot_top;
+    unsigned long long cap = hash ? p->hash_cap : p->slot_cap;
+    unsigned long long o = atomicAdd(top, (unsigned long long)n);
+    if (o + n > cap) {
+        atomicOr(&p->oom, 1u);
+        return HEMPTY;
+    }
+    return o;
+}
+
+// wave-parallel (re)build of a list's hash index at capacity `hcap`
+__device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, int lane)
+{
+    uint64_t h0 = 0;
+    if (lane == 0) h0 = pool_alloc(a.pool, true, hcap);
+    h0 = __shfl(h0, 0);
+    if (h0 == HEMPTY) return false;
+    for (uint32_t b = lane; b < hcap; b += WAVE) a.slots.hkey[h0 + b] = HEMPTY;
+    __threadfence_block();
+    uint32_t mask = hcap - 1;
+    for (uint32_t i = lane; i < st.len; i += WAVE) {
+        uint32_t nb = a.slots.nbr[st.off + i];
+        uint32_t w = a.slots.wt ? a.slots.wt[st.off + i] : 0;
+        uint64_t key = hkey_of(nb, w);
+        uint32_t b = (uint32_t)mix64(key) & mask;
+        uint32_t probes = 0;
+        while (atomicCAS((unsigned long long *)&a.slots.hkey[h0 + b], (unsigned long long)HEMPTY,
+                         (unsigned long long)key) != HEMPTY) {
+            b = (b + 1) & mask;
+            if (++probes >= hcap) break;   // cannot happen below 50% load; never spin
+        }
+        if (probes >= hcap) atomicOr(&a.pool->oom, 2u);
+        else a.slots.hval[h0 + b] = i;
+    }
+    __threadfence_block();
+    st.hoff = h0;
+    st.hcap = hcap;
+    st.hcnt = st.len;
+    return true;
+}
+
+// find key in the list: local index or NILP.  With `ins_bucket`, returns the
+// empty hash bucket where the key would go (hashed lists only).
+__device__ uint32_t list_find(const StoreArgs &a, const ListState &st, uint32_t key_nbr, uint32_t key_w,
+                              int lane, uint32_t *ins_bucket)
+{
+    if (st.hoff != HEMPTY) {
+        uint64_t key = hkey_of(key_nbr, key_w);
+        uint32_t mask = st.hcap - 1;
+        uint32_t h = (uint32_t)mix64(key) & mask;
+        for (uint32_t probe = 0; probe < st.hcap; probe += WAVE) {
+            uint32_t b = (h + probe + lane) & mask;
+            uint64_t k = a.slots.hkey[st.hoff + b];
+            uint64_t hit = __ballot(k == key);
+            uint64_t emp = __ballot(k == HEMPTY);
+            uint64_t any = hit | emp;
+            if (any) {
+                int first = __ffsll((unsigned long long)any) - 1;
+                uint32_t fb = (h + probe + first) & mask;
+                if ((hit >> first) & 1ull) {
+                    return a.slots.hval[st.hoff + fb];
+                }
+                if (ins_bucket) *ins_bucket = fb;
+                return NILP;
+            }
+        }
+        if (ins_bucket) *ins_bucket = NILP;
+        return NILP;
+    }
+    for (uint32_t b = 0; b < st.len; b += WAVE) {
+        uint32_t i = b + lane;
+        bool m = false;
+        if (i < st.len) {
+            uint32_t nb = a.slots.nbr[st.off + i];
+            m = (nb & NBR_MASK) == key_nbr;
+            if (m && a.slots.wt) m = a.slots.wt[st.off + i] == key_w;
+        }
+        uint64_t hit = __ballot(m);
+        if (hit) return b + (uint32_t)(__ffsll((unsigned long long)hit) - 1);
+    }
+    return NILP;
+}
+
+__global__ void k_apply(StoreArgs a)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const uint32_t nw = (gridDim.x * blockDim.x) / WAVE;
+    const uint32_t nlops = 4 * a.ops.n;
+    for (uint32_t j = wid; j < nlops; j += nw) {
+        uint32_t v = a.lops.v[j];
+        if (v == NILP) continue;
+        int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
+        const List &li = a.lists[L];
+        if (li.head[v] != (int32_t)j) continue;   // not the chain head: its owner applies it
+
+        ListState st;
+        st.off = li.off[v];
+        st.len = li.len[v];
+        st.cap = li.cap[v];
+        st.deg = li.deg[v];
+        st.hoff = li.hoff[v];
+        st.hcap = li.hcap[v];
+        st.hcnt = li.hcnt[v];
+
+        // reserve room for every insert in the chain up front; a chain is at
+        // most nlops long (the walk bound guards against a corrupt link)
+        uint32_t n_ins = 0, clen = 0;
+        bool ok = true;
+        for (int32_t k = (int32_t)j; k >= 0 && (uint32_t)k < nlops; k = a.lops.next[k]) {
+            if (a.lops.delta[k] > 0) n_ins++;
+            if (++clen > nlops) { ok = false; break; }
+        }
+        if (!ok) atomicOr(a.err, 2u);
+        if (ok && st.len + n_ins > st.cap) {
+            uint32_t ncap = next_pow2(st.len + n_ins);
+            if (ncap < 4) ncap = 4;
+            uint64_t o = 0;
+            if (lane == 0) o = pool_alloc(a.pool, false, ncap);
+            o = __shfl(o, 0);
+            if (o == HEMPTY) {
+                ok = false;
+            } else {
+                for (uint32_t i = lane; i < st.len; i += WAVE) {
+                    a.slots.nbr[o + i] = a.slots.nbr[st.off + i];
+                    a.slots.cnt[o + i] = a.slots.cnt[st.off + i];
+                    if (a.slots.wt) a.slots.wt[o + i] = a.slots.wt[st.off + i];
+                }
+                __threadfence_block();
+                st.off = o;
+                st.cap = ncap;
+            }
+        }
+
+        for (int pass = 0; pass < 2 && ok; pass++) {
+            for (int32_t k = (int32_t)j; k >= 0 && (uint32_t)k < nlops; k = a.lops.next[k]) {
+                int8_t delta = a.lops.delta[k];
+                if ((pass == 0) != (delta > 0)) continue;
+                uint32_t key_nbr = a.lops.nbr[k];
+                uint32_t key_w = a.slots.wt ? a.lops.w[k] : 0;
+                uint32_t bucket = NILP;
+                uint32_t idx = list_find(a, st, key_nbr, key_w, lane, &bucket);
+                if (idx == NILP && delta > 0) {
+                    // append a new entry with count 0 (storage.hpp:371-374, 389-392)
+                    idx = st.len;
+                    if (lane == 0) {
+                        a.slots.nbr[st.off + idx] = key_nbr | DEAD;
+                        a.slots.cnt[st.off + idx] = 0;
+                        if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
+                    }
+                    st.len++;
+                    __threadfence_block();
+                    if (st.hoff != HEMPTY) {
+                        if (2 * (st.hcnt + 1) > st.hcap) {
+                            if (!build_index(a, st, st.hcap * 2, lane)) { ok = false; break; }
+                        } else {
+                            if (lane == 0) {
+                                a.slots.hkey[st.hoff + bucket] = hkey_of(key_nbr, key_w);
+                                a.slots.hval[st.hoff + bucket] = idx;
+                            }
+                            st.hcnt++;
+                            __threadfence_block();
+                        }
+                    } else if (st.len >= INDEX_THRESHOLD) {
+                        uint32_t hc = next_pow2(2 * st.len);
+                        if (hc < 1024) hc = 1024;
+                        if (!build_index(a, st, hc, lane)) { ok = false; break; }
+                    }
+                }
+                uint32_t c = idx == NILP ? 0 : a.slots.cnt[st.off + idx];
+                uint32_t nc, ret;
+                int32_t dchange;
+                switch (delta) {
+                case 1: nc = c + 1; ret = c; dchange = 1; break;
+                case -1: nc = c ? c - 1 : 0; ret = c; dchange = c ? -1 : 0; break;
+                case 2: nc = c + 2; ret = c; dchange = 2; break;
+                default: /* -2: undirected self-loop delete, two -1 steps */
+                    ret = c ? c - 1 : 0;
+                    nc = c >= 2 ? c - 2 : 0;
+                    dchange = -(int32_t)(c >= 2 ? 2 : c);
+                    break;
+                }
+                if (idx != NILP && lane == 0) {
+                    a.slots.cnt[st.off + idx] = nc;
+                    if ((c == 0) != (nc == 0))
+                        a.slots.nbr[st.off + idx] = key_nbr | (nc ? 0u : DEAD);
+                }
+                st.deg = (uint32_t)((int32_t)st.deg + dchange);
+                uint8_t fl = a.lops.flags[k];
+                if (lane == 0) {
+                    uint32_t e = a.lops.edge[k];
+                    if (fl & L_RET) a.ops.old[e] = ret;
+                    if (fl & L_SLOT) a.ops.slot[e] = idx;
+                }
+                __threadfence_block();
+            }
+        }
+        if (lane == 0) {
+            li.off[v] = st.off;
+            li.len[v] = st.len;
+            li.cap[v] = st.cap;
+            li.deg[v] = st.deg;
+            li.hoff[v] = st.hoff;
+            li.hcap[v] = st.hcap;
+            li.hcnt[v] = st.hcnt;
+            li.head[v] = -1;
+        }
+    }
+}
+
+void launch_store_batch(const StoreArgs &a, hipStream_t s)
+{
+    if (a.ops.n == 0) return;
+    uint32_t n = a.ops.n;
+    hipLaunchKernelGGL(k_expand_link, dim3((n + 255) / 256), dim3(256), 0, s, a);
+    uint64_t waves = 4ull * n;
+    uint64_t blocks = (waves * WAVE + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_apply, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------
+// synthetic streams (rg_gen.h) and stream -> edge-op conversion
+// ---------------------------------------------------------------------
+__global__ void k_gen_rmat(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, int scale,
+                           uint64_t key, RmatThresholds t)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        rmat_edge(key, start + i, scale, t, &src[i], &dst[i]);
+}
+
+__global__ void k_gen_er(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, uint64_t V, uint64_t key)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        er_edge(key, start + i, V, &src[i], &dst[i]);
+}
+
+__global__ void k_gen_w(uint32_t *w, uint64_t n, uint64_t start, uint64_t key)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        w[i] = stream_weight(key, start + i);
+}
+
+static dim3 gen_grid(uint64_t n)
+{
+    uint64_t b = (n + 255) / 256;
+    if (b > 16384) b = 16384;
+    if (b == 0) b = 1;
+    return dim3((uint32_t)b);
+}
+
+void launch_gen_rmat(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, int scale,
+                     uint64_t seed, double a, double b, double c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gen_rmat, gen_grid(n), dim3(256), 0, s, src, dst, n, start, scale,
+                       draw_key(seed), rmat_thresholds(a, b, c));
+}
+
+void launch_gen_er(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, uint64_t V,
+                   uint64_t seed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gen_er, gen_grid(n), dim3(256), 0, s, src, dst, n, start, V, draw_key(seed));
+}
+
+void launch_gen_weights(uint32_t *w, uint64_t n, uint64_t start, uint64_t seed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gen_w, gen_grid(n), dim3(256), 0, s, w, n, start, draw_key(seed));
+}
+
+// out-degree histogram of a stream prefix and its arg-max (ties -> lowest id),
+// the drivers' root choice in these workloads
+__global__ void k_hist(const uint64_t *src, uint64_t n, uint64_t V, uint32_t *deg)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        if (src[i] < V) atomicAdd(&deg[src[i]], 1u);
+}
+
+__global__ void k_argmax(const uint32_t *deg, uint64_t V, unsigned long long *best)
+{
+    unsigned long long b = 0;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        unsigned long long x = ((unsigned long long)deg[v] << 32) | (0xFFFFFFFFull - v);
+        if (x > b) b = x;
+    }
+    atomicMax(best, b);
+}
+
+void launch_max_degree(const uint64_t *src, uint64_t n, uint64_t V, uint32_t *deg_tmp,
+                       unsigned long long *best, hipStream_t s)
+{
+    (void)hipMemsetAsync(deg_tmp, 0, V * sizeof(uint32_t), s);
+    (void)hipMemsetAsync(best, 0, sizeof(unsigned long long), s);
+    hipLaunchKernelGGL(k_hist, gen_grid(n), dim3(256), 0, s, src, n, V, deg_tmp);
+    hipLaunchKernelGGL(k_argmax, gen_grid(V), dim3(256), 0, s, deg_tmp, V, best);
+}
+
+__global__ void k_ops_from_stream(const uint64_t *src, const uint64_t *dst, const uint32_t *w,
+                                  uint64_t first, uint32_t n, uint8_t kind, EdgeOps ops, uint32_t base)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ops.src[base + i] = (uint32_t)src[first + i];
+    ops.dst[base + i] = (uint32_t)dst[first + i];
+    ops.w[base + i] = w ? w[first + i] : 0;
+    ops.kind[base + i] = kind;
+}
+
+void launch_ops_from_stream(const uint64_t *src, const uint64_t *dst, const uint32_t *w,
+                            uint64_t first, uint32_t n, uint8_t kind, EdgeOps ops,
+                            uint32_t base, hipStream_t s)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_ops_from_stream, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, w, first, n,
+                       kind, ops, base);
+}
+
+}  // namespace rg

@@ -1,0 +1,333 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+After every update or batch the device labels must equal, bit for bit,
+(a) the cpu_ref restatement of the reference engine run on the same ops and
+(b) the static closed form on the live multigraph; device parent trees must
+be valid dependency trees (parents may differ from cpu_ref where ties allow).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import risgraph_amd as rg
+import streams
+from oracle.model import MultigraphModel
+
+pytestmark = pytest.mark.gpu
+
+ALGO_NAMES = ["bfs", "sssp", "sswp", "wcc"]
+
+
+def _weighted(algo):
+    return algo in (rg.SSSP, rg.SSWP)
+
+
+def _check_labels(L, algo, V, ref_g, root, init=None, ref_vals=None, undirected=None):
+    vals, par = L.read()
+    ls, ld, lw = ref_g.live_edges()
+    exp = oracle.static_values(algo, V, ls, ld, lw, root, init)
+    if ref_vals is not None:
+        assert np.array_equal(ref_vals, exp), "oracle disagreement (cpu_ref vs static)"
+    bad = np.nonzero(vals != exp)[0]
+    assert len(bad) == 0, f"{len(bad)} vertices differ, first {bad[:5]} got {vals[bad[:5]]} want {exp[bad[:5]]}"
+    t = oracle.check_tree(algo, V, ls, ld, lw, root, vals, par, init=init, undirected=undirected)
+    assert t == 0, f"invalid tree at vertex {t - 1}"
+
+
+def test_stream_generators_match_definition():
+    s = rg.Stream("rmat", 12, 5000, seed=7, wseed=9)
+    src, dst, w = s.read()
+    es, ed = streams.rmat(12, 5000, 7)
+    assert np.array_equal(src, es) and np.array_equal(dst, ed)
+    assert np.array_equal(w.astype(np.uint64), streams.weights(5000, 9))
+    e = rg.Stream("er", 1000, 3000, seed=5)
+    src, dst, _ = e.read()
+    es, ed = streams.erdos_renyi(1000, 3000, 5)
+    assert np.array_equal(src, es) and np.array_equal(dst, ed)
+    es, _ = streams.rmat(12, 5000, 7)
+    assert s.max_degree_vertex(5000, 4096) == streams.max_out_degree_root(es, 4096)
+
+
+@pytest.mark.parametrize("symmetric,weighted", [(False, False), (False, True), (True, False)])
+def test_store_matches_model(symmetric, weighted):
+    rng = np.random.default_rng(11)
+    V = 2000
+    g = rg.Graph(V, 20000, symmetric=symmetric, dual=not symmetric, weighted=weighted)
+    m = MultigraphModel(V, symmetric=symmetric, weighted=weighted)
+    # single ops, including a hub above the 512-entry index threshold
+    for k in range(3000):
+        a = 0 if rng.random() < 0.4 else int(rng.integers(0, V))
+        b = int(rng.integers(0, V))
+        w = int(rng.integers(1, 4))
+        directed = True if not symmetric else bool(rng.random() < 0.2)
+        if rng.random() < 0.7:
+            assert g.add_edge(a, b, w, directed) == m.add_edge(a, b, w, directed), k
+        else:
+            assert g.del_edge(a, b, w, directed) == m.del_edge(a, b, w, directed), k
+    # batches: the multiset of returned old counts is order-free
+    for rep in range(5):
+        n = 4000
+        a = np.where(rng.random(n) < 0.3, 0, rng.integers(0, V, n)).astype(np.uint64)
+        b = rng.integers(0, V, n).astype(np.uint64)
+        w = rng.integers(1, 4, n).astype(np.uint64)
+        delta = 1 if rep % 2 == 0 else -1
+        directed = not symmetric
+        got = g.update_edges(a, b, w, delta, directed)
+        fm = m.add_edge if delta > 0 else m.del_edge
+        want = [fm(int(x), int(y), int(z), directed) for x, y, z in zip(a, b, w)]
+        # per key, the returned values are a permutation of the sequential ones
+        from collections import defaultdict
+        gk, wk = defaultdict(list), defaultdict(list)
+        for i in range(n):
+            key = (int(a[i]), int(b[i]), int(w[i]) if weighted else 0)
+            if not directed:
+                key = (min(key[0], key[1]), max(key[0], key[1]), key[2])
+            gk[key].append(int(got[i]))
+            wk[key].append(want[i])
+        for key in wk:
+            assert sorted(gk[key]) == sorted(wk[key]), key
+    assert g.get_degree(0) == m.degree(0)
+    assert g.get_degree(0) > 512
+    for k in range(200):
+        a, b, w = int(rng.integers(0, V)), int(rng.integers(0, V)), int(rng.integers(1, 4))
+        assert g.get_edge_num(a, b, w) == m.count(a, b, w)
+
+
+def test_invalid_vertex_is_an_error():
+    g = rg.Graph(10, 10)
+    with pytest.raises(rg.RgError) as e:
+        g.add_edge(3, 10)
+    assert e.value.code == rg.RG_EINVAL_VERTEX
+    assert g.add_edge(3, 4) == 0
+
+
+def _setup(algo, kind, V, E, load_frac, seed=1, wseed=2, tuning=None):
+    """Device graph + labels and cpu_ref twin, both loaded with the same prefix."""
+    weighted = _weighted(algo)
+    sym = algo == rg.WCC
+    scale = int(np.log2(V))
+    s = rg.Stream(kind, scale if kind == "rmat" else V, E, seed=seed, wseed=wseed)
+    src, dst, w = s.read()
+    w64 = w.astype(np.uint64)
+    n0 = int(E * load_frac)
+    directed = not sym
+    g = rg.Graph(V, E, symmetric=sym, dual=not sym, weighted=weighted)
+    g.load_stream(s, 0, n0, directed)
+    ref = oracle.RefGraph(V, E, symmetric=sym, dual=not sym, weighted=weighted)
+    ref.add_edges(src[:n0], dst[:n0], w64[:n0] if weighted else None, directed)
+    root = s.max_degree_vertex(n0, V)
+    assert root == streams.max_out_degree_root(src[:n0], V)
+    L = g.alloc_vertex_tree_array(algo, root)
+    if tuning:
+        L.set_tuning(**tuning)
+    L.build_tree()
+    RL = oracle.RefLabels(ref, algo, root)
+    RL.build()
+    return dict(s=s, src=src, dst=dst, w=w64, n0=n0, g=g, L=L, ref=ref, RL=RL, root=root,
+                directed=directed, weighted=weighted)
+
+
+def _ref_step(t, i_add, i_del):
+    ref, RL, d = t["ref"], t["RL"], t["directed"]
+    src, dst, w = t["src"], t["dst"], t["w"]
+    wa = int(w[i_add]) if t["weighted"] else 0
+    if ref.add_edge(int(src[i_add]), int(dst[i_add]), wa, d) == 0:
+        RL.inc_add(int(src[i_add]), int(dst[i_add]), wa, d)
+    wd = int(w[i_del]) if t["weighted"] else 0
+    if ref.del_edge(int(src[i_del]), int(dst[i_del]), wd, d) == 1:
+        RL.inc_del(int(src[i_del]), int(dst[i_del]), wd, d)
+
+
+@pytest.mark.parametrize("name", ALGO_NAMES)
+@pytest.mark.parametrize("mode", ["auto", "grid"])
+def test_plain_stream_parity(name, mode):
+    algo = rg.ALGOS[name]
+    tuning = dict(block_max=0) if mode == "grid" else None
+    t = _setup(algo, "rmat", 1024, 1 << 14, 0.9, tuning=tuning)
+    V = 1024
+    _check_labels(t["L"], algo, V, t["ref"], t["root"], ref_vals=t["RL"].read()[0])
+    n_upd = 150 if mode == "grid" else 400
+    for k in range(n_upd):
+        i_add, i_del = t["n0"] + k, k
+        t["g"].stream_plain([t["L"]], t["s"], i_add, i_del, 1, t["directed"], latency=False)
+        _ref_step(t, i_add, i_del)
+        _check_labels(t["L"], algo, V, t["ref"], t["root"], ref_vals=t["RL"].read()[0])
+
+
+@pytest.mark.parametrize("name", ALGO_NAMES)
+@pytest.mark.parametrize("batch", [1, 10, 100, 1000])
+def test_batched_stream_parity(name, batch):
+    algo = rg.ALGOS[name]
+    V, E = 4096, 1 << 16
+    t = _setup(algo, "rmat", V, E, 0.8, seed=3)
+    nb = min(8, (E - t["n0"]) // batch)
+    for b in range(nb):
+        i0 = t["n0"] + b * batch
+        d0 = b * batch
+        t["g"].apply_stream([t["L"]], t["s"], i0, batch, d0, batch, t["directed"])
+        # reference batch driver: all adds, then all dels (bfs_inc_batch.cpp:142-178)
+        ref, RL, d = t["ref"], t["RL"], t["directed"]
+        src, dst, w = t["src"], t["dst"], t["w"] if t["weighted"] else np.zeros_like(t["w"])
+        idx = np.arange(i0, i0 + batch)
+        olds = np.array([ref.add_edge(int(src[i]), int(dst[i]), int(w[i]), d) for i in idx])
+        new = idx[olds == 0]
+        if len(new):
+            RL.inc_add_batch(src[new], dst[new], w[new], d)
+        didx = np.arange(d0, d0 + batch)
+        olds = np.array([ref.del_edge(int(src[i]), int(dst[i]), int(w[i]), d) for i in didx])
+        gone = didx[olds == 1]
+        if len(gone):
+            RL.inc_del_batch(src[gone], dst[gone], w[gone], d)
+        _check_labels(t["L"], algo, V, ref, t["root"], ref_vals=RL.read()[0])
+
+
+@pytest.mark.parametrize("name", ALGO_NAMES)
+def test_reference_driver_loop_through_facade_calls(name):
+    """add_edge -> update_tree_add / del_edge -> update_tree_del, as bfs_inc.cpp:160-189."""
+    algo = rg.ALGOS[name]
+    V = 1024
+    t = _setup(algo, "er", V, 8192, 0.9, seed=5)
+    g, L, d = t["g"], t["L"], t["directed"]
+    src, dst, w = t["src"], t["dst"], t["w"]
+    for k in range(200):
+        i, j = t["n0"] + k, k
+        wa = int(w[i]) if t["weighted"] else 0
+        if g.add_edge(int(src[i]), int(dst[i]), wa, d) == 0:
+            L.update_tree_add(int(src[i]), int(dst[i]), wa, d)
+        wd = int(w[j]) if t["weighted"] else 0
+        if g.del_edge(int(src[j]), int(dst[j]), wd, d) == 1:
+            L.update_tree_del(int(src[j]), int(dst[j]), wd, d)
+        _ref_step(t, i, j)
+        _check_labels(L, algo, V, t["ref"], t["root"], ref_vals=t["RL"].read()[0])
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(recompute_abs=0, recompute_frac=0.0),        # every deletion falls back to a full rebuild
+    dict(rescue_level=0, rescue_frontier=0),          # never rescue: reset + pull + push only
+    dict(walk_limit=1),                               # ancestor walks give up: rebuild path
+], ids=["always_recompute", "no_rescue", "tiny_walk"])
+def test_deletion_strategies_keep_values(tuning):
+    for name in ("bfs", "sssp"):
+        algo = rg.ALGOS[name]
+        V = 1024
+        t = _setup(algo, "rmat", V, 1 << 14, 0.9, seed=9, tuning=tuning)
+        for k in range(60):
+            i_add, i_del = t["n0"] + k, k
+            t["g"].stream_plain([t["L"]], t["s"], i_add, i_del, 1, t["directed"], latency=False)
+            _ref_step(t, i_add, i_del)
+            _check_labels(t["L"], algo, V, t["ref"], t["root"])
+
+
+def test_four_trees_on_one_store():
+    """BFS/SSSP/SSWP and WCC (undirected view) maintained together on one directed store."""
+    V, E = 4096, 1 << 16
+    s = rg.Stream("rmat", 12, E, seed=1, wseed=2)
+    src, dst, w = s.read()
+    w = w.astype(np.uint64)
+    n0 = int(0.9 * E)
+    g = rg.Graph(V, E, symmetric=False, dual=True, weighted=True)
+    g.load_stream(s, 0, n0, True)
+    root = s.max_degree_vertex(n0, V)
+    Ls = [g.alloc_vertex_tree_array(a, root) for a in (rg.BFS, rg.SSSP, rg.SSWP, rg.WCC)]
+    for L in Ls:
+        L.build_tree()
+    m = MultigraphModel(V, symmetric=False, weighted=True)
+    for i in range(n0):
+        m.add_edge(int(src[i]), int(dst[i]), int(w[i]))
+
+    def check():
+        ls, ld, lw = m.live_edges()
+        for L in Ls:
+            vals, par = L.read()
+            exp = oracle.static_values(L.algo, V, ls, ld, lw, root)
+            assert np.array_equal(vals, exp), L.algo
+            assert oracle.check_tree(L.algo, V, ls, ld, lw, root, vals, par,
+                                     undirected=(L.algo == rg.WCC)) == 0, L.algo
+
+    check()
+    pos = 0
+    for batch in (1, 1, 7, 64, 512, 3000):
+        g.apply_stream(Ls, s, n0 + pos, batch, pos, batch, True)
+        for i in range(n0 + pos, n0 + pos + batch):
+            m.add_edge(int(src[i]), int(dst[i]), int(w[i]))
+        for i in range(pos, pos + batch):
+            m.del_edge(int(src[i]), int(dst[i]), int(w[i]))
+        pos += batch
+        check()
+
+
+def test_wcc_custom_init_labels():
+    V, E = 2048, 6000
+    s = rg.Stream("er", V, E, seed=4)
+    src, dst, _ = s.read()
+    g = rg.Graph(V, E, symmetric=True, dual=False)
+    g.load_stream(s, 0, 5000, False)
+    perm = np.random.default_rng(3).permutation(V).astype(np.uint64)
+    L = g.alloc_vertex_tree_array(rg.WCC, 0, init=perm)
+    L.build_tree()
+    m = MultigraphModel(V, symmetric=True)
+    for i in range(5000):
+        m.add_edge(int(src[i]), int(dst[i]), 0, False)
+    for k in range(0, 1000, 100):
+        g.apply_stream([L], s, 5000 + k, 100, k, 100, False)
+        for i in range(5000 + k, 5100 + k):
+            m.add_edge(int(src[i]), int(dst[i]), 0, False)
+        for i in range(k, k + 100):
+            m.del_edge(int(src[i]), int(dst[i]), 0, False)
+        ls, ld, lw = m.live_edges()
+        vals, par = L.read()
+        assert np.array_equal(vals, oracle.static_values(rg.WCC, V, ls, ld, None, 0, perm))
+        assert oracle.check_tree(rg.WCC, V, ls, ld, None, 0, vals, par, init=perm) == 0
+
+
+def test_classifier_matches_cpu_ref():
+    algo = rg.SSSP
+    t = _setup(algo, "rmat", 1024, 1 << 14, 0.9)
+    src, dst, w = t["src"], t["dst"], t["w"]
+    idx = np.arange(t["n0"], t["n0"] + 500)
+    got = t["L"].classify(src[idx], dst[idx], w[idx], np.ones(500), True)
+    want = [t["RL"].need_add(int(src[i]), int(dst[i]), int(w[i])) for i in idx]
+    assert got.tolist() == want
+    didx = np.arange(0, 500)
+    got = t["L"].classify(src[didx], dst[didx], w[didx], -np.ones(500), True)
+    vals, par = t["L"].read()
+    want = [int(par[int(dst[i])]) == int(src[i]) for i in didx]
+    assert got.tolist() == want
+
+
+def test_modified_set_tracks_changes():
+    algo = rg.BFS
+    t = _setup(algo, "rmat", 1024, 1 << 14, 0.9)
+    L = t["L"]
+    L.clear_modified()
+    before = L.read()[0]
+    t["g"].apply_stream([L], t["s"], t["n0"], 50, 0, 50, True)
+    after = L.read()[0]
+    mods, allf = L.get_modified()
+    changed = set(np.nonzero(before != after)[0].tolist())
+    assert allf or changed <= set(mods.tolist())
+
+
+def test_empty_and_degenerate_batches():
+    V = 64
+    g = rg.Graph(V, 256, symmetric=False, dual=True, weighted=True)
+    s = rg.Stream("rmat", 6, 256, seed=2, wseed=3)
+    g.load_stream(s, 0, 200, True)
+    L = g.alloc_vertex_tree_array(rg.SSSP, 0)
+    L.build_tree()
+    g.apply_stream([L], s, 200, 0, 0, 0, True)        # nothing
+    g.apply_stream([L], s, 200, 0, 0, 10, True)       # deletes only
+    g.apply_stream([L], s, 200, 10, 0, 0, True)       # inserts only
+    # self-loop and absent-edge deletes
+    assert g.del_edge(5, 6, 999) == 0
+    assert g.add_edge(7, 7, 1001) == 0          # stream weights are <= 1000: a new key
+    assert g.del_edge(7, 7, 1001) == 1
+    L.update_tree_del(7, 7, 1001)
+    src, dst, w = s.read()
+    m = MultigraphModel(V, weighted=True)
+    for i in range(210):
+        m.add_edge(int(src[i]), int(dst[i]), int(w[i]))
+    for i in range(10):
+        m.del_edge(int(src[i]), int(dst[i]), int(w[i]))
+    ls, ld, lw = m.live_edges()
+    assert np.array_equal(L.read()[0], oracle.static_values(rg.SSSP, V, ls, ld, lw, 0))

@@ -220,6 +220,64 @@ def test_store_matches_model_with_hubs(symmetric, weighted):
     assert sorted(zip(s1.tolist(), d1.tolist(), w1.tolist())) == sorted(zip(s2.tolist(), d2.tolist(), w2.tolist()))
 
 
+@pytest.mark.parametrize("symmetric,weighted,directed", [(False, False, True), (False, True, True),
+                                                          (True, False, False), (False, True, False)])
+def test_bulk_load_equals_sequential_adds(symmetric, weighted, directed):
+    V, E = 700, 9000
+    s, d = streams.rmat(10, E, 5)
+    s, d = s % V, d % V
+    w = streams.weights(E, 6) % np.uint64(3) + np.uint64(1)
+    s[:800] = 0                                   # a hub above the 512-entry index threshold
+    a = oracle.RefGraph(V, E, symmetric, True, weighted)
+    b = oracle.RefGraph(V, E, symmetric, True, weighted)
+    a.add_edges(s, d, w, directed)
+    b.bulk_load(s, d, w, directed)
+    la, lb = a.live_edges(), b.live_edges()
+    ka = sorted(zip(*[x.tolist() for x in la]))
+    kb = sorted(zip(*[x.tolist() for x in lb]))
+    assert ka == kb
+    for v in range(V):
+        assert a.degree(v) == b.degree(v)
+        assert a.degree(v, True) == b.degree(v, True)
+    for i in range(0, E, 37):
+        assert a.edge_num(int(s[i]), int(d[i]), int(w[i])) == b.edge_num(int(s[i]), int(d[i]), int(w[i]))
+    with pytest.raises(ValueError):
+        b.bulk_load(s, d, w, directed)
+    # incremental updates behave the same afterwards
+    for i in range(300):
+        x, y, z = int(d[i]), int(s[i]), int(w[i])
+        assert a.add_edge(x, y, z, directed) == b.add_edge(x, y, z, directed)
+        assert a.del_edge(int(s[i]), int(d[i]), z, directed) == b.del_edge(int(s[i]), int(d[i]), z, directed)
+
+
+def test_labels_assign_matches_build():
+    V, E = 1024, 12000
+    s, d = streams.rmat(10, E, 8)
+    w = streams.weights(E, 9)
+    for algo in (BFS, SSSP, SSWP, WCC):
+        sym = algo == WCC
+        g = oracle.RefGraph(V, E, sym, not sym, algo in (SSSP, SSWP))
+        g.bulk_load(s, d, w, not sym)
+        root = streams.max_out_degree_root(s, V)
+        L1 = oracle.RefLabels(g, algo, root)
+        L1.build()
+        v1, p1, w1 = L1.read()
+        L2 = oracle.RefLabels(g, algo, root)
+        L2.assign(v1, p1)
+        v2, p2, _ = L2.read()
+        assert np.array_equal(v1, v2) and np.array_equal(p1, p2)
+        # a parent with no live edge into the vertex must be rejected
+        ls, ld, _ = g.live_edges()
+        edges = set(zip(ls.tolist(), ld.tolist()))
+        v = int(np.nonzero(p1 < V)[0][0])
+        p = next(x for x in range(V) if (x, v) not in edges and (v, x) not in edges)
+        bad = p1.copy()
+        bad[v] = p
+        L3 = oracle.RefLabels(g, algo, root)
+        with pytest.raises(ValueError):
+            L3.assign(v1, bad)
+
+
 def _window_stream(kind, V, E, load_frac, seed=1, wseed=2):
     if kind == "rmat":
         s, d = streams.rmat(int(np.log2(V)), E, seed)
