@@ -164,11 +164,15 @@ int rg_last_gpu_ms(rg_graph *g, double *ms);
  * the all_inc stream (all_inc.cpp:254-300), and arrive by a Poisson process
  * of `rate` updates/s from `clients` simulated clients (client u % clients).
  * Every loop iteration admits all arrived updates up to the current batch
- * cap, applies them as one device batch to the store and every labels
- * instance in L, and completes them together.  With target_ms > 0 the cap
- * adapts like the reference's num_changes_threshold (x0.9 when a batch's
- * oldest request waited past half the target, else +16 / x1.01, clamped to
- * [1, cap_max]); batch_cap = 1 is the plain one-update-at-a-time path.
+ * cap (the admission budget), applies them as one device batch to the store
+ * and every labels instance in L, and completes them together.  With
+ * target_ms > 0 the cap adapts in the spirit of the reference's
+ * num_changes_threshold (bfs_inc_rt.cpp:409-427), but on device service
+ * time: x0.7 when a batch took longer than target/4, x1.5+16 while the
+ * backlog exceeds the cap, clamped to [1, cap_max].  batch_cap = 1 with no
+ * target is the plain one-update-at-a-time path.  When the oldest waiting
+ * request is older than max(10 * target, 200 ms) the run is overloaded: the
+ * rest is applied untimed in cap_max batches and r->overloaded is set.
  * Latency = completion time - arrival time (no coordinated omission). */
 typedef struct rg_serve_params {
     uint64_t ins_first, del_first, n_updates;
@@ -185,6 +189,9 @@ typedef struct rg_serve_result {
     double p50_ms, p99_ms, p999_ms, max_ms, mean_ms;
     double mean_batch;
     uint64_t batches, max_batch, grid_steps, rebuilds;
+    uint64_t overloaded;      /* 1 when the run fell behind (see above) */
+    uint64_t served;          /* updates completed with a measured latency */
+    double gpu_ms;            /* summed device time of the batches */
 } rg_serve_result;
 int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_params *p,
              rg_serve_result *r, double *lat_ms);

@@ -65,6 +65,7 @@ def lib():
             "or_labels_read": (None, [vp, vp, vp, vp]),
             "or_recompute_count": (u64, [vp]),
             "or_stream_plain": (u64, [vp, vp, u64, vp, vp, vp, vp, vp, vp, ctypes.c_int, vp]),
+            "or_stream_mixed": (u64, [vp, vp, ctypes.c_int, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -228,3 +229,15 @@ class RefLabels:
         unsafe = lib().or_stream_plain(self.g.h, self.h, n, *[_p(x) for x in a], int(directed),
                                        _p(lat))
         return int(unsafe), lat
+
+
+def stream_mixed(gd, Ld, gu, Lu, n_updates, add_s, add_d, add_w, del_s, del_d, del_w, latency=True):
+    """all_inc-style alternation over BFS/SSSP/SSWP (Ld on gd) and WCC (Lu on gu).
+
+    Returns (#updates that ran incremental work, per-update wall ns or None)."""
+    arr = (vp * max(1, len(Ld)))(*[L.h for L in Ld])
+    a = [_u64(x) for x in (add_s, add_d, add_w, del_s, del_d, del_w)]
+    lat = np.empty(n_updates, np.float64) if latency else None
+    unsafe = lib().or_stream_mixed(gd.h if gd else None, arr, len(Ld), gu.h if gu else None,
+                                   Lu.h if Lu else None, n_updates, *[_p(x) for x in a], _p(lat))
+    return int(unsafe), lat

@@ -491,36 +491,58 @@ static int key_cmp(const void *a, const void *b)
     return 0;
 }
 
-static void bulk_store(or_store *st, uint64_t V, uint64_t m, const uint64_t *lv,
-                       const uint64_t *lnbr, const uint64_t *ldata)
+/* list ops of add_edge (graph.hpp:203-227) for one direction: pass 0 fills
+ * out-lists (src gets (dst, w)), pass 1 in-lists (dst gets (src, w)); an
+ * undirected add contributes both orientations. */
+static inline void bulk_op(uint64_t i, int k, int pass, const uint64_t *src, const uint64_t *dst,
+                           uint64_t *v, uint64_t *nbr)
 {
+    uint64_t a = src[i], b = dst[i];
+    if (k) { uint64_t t = a; a = b; b = t; }
+    if (pass) { uint64_t t = a; a = b; b = t; }
+    *v = a;
+    *nbr = b;
+}
+
+static void bulk_store(or_store *st, uint64_t V, uint64_t n, const uint64_t *src, const uint64_t *dst,
+                       const uint64_t *w, int weighted, int per, int pass)
+{
+    uint64_t m = n * per;
     uint64_t *off = calloc(V + 1, sizeof(uint64_t));
-    for (uint64_t i = 0; i < m; i++) off[lv[i] + 1]++;
+    for (uint64_t i = 0; i < n; i++)
+        for (int k = 0; k < per; k++) {
+            uint64_t v, nb;
+            bulk_op(i, k, pass, src, dst, &v, &nb);
+            off[v + 1]++;
+        }
     for (uint64_t v = 0; v < V; v++) off[v + 1] += off[v];
     uint64_t *pos = malloc((V + 1) * sizeof(uint64_t));
     memcpy(pos, off, (V + 1) * sizeof(uint64_t));
-    key_t_ *k = malloc((m ? m : 1) * sizeof(key_t_));
-    for (uint64_t i = 0; i < m; i++) {
-        uint64_t p = pos[lv[i]]++;
-        k[p].nbr = lnbr[i];
-        k[p].data = ldata[i];
-    }
+    key_t_ *kk = malloc((m ? m : 1) * sizeof(key_t_));
+    for (uint64_t i = 0; i < n; i++)
+        for (int k = 0; k < per; k++) {
+            uint64_t v, nb;
+            bulk_op(i, k, pass, src, dst, &v, &nb);
+            uint64_t p = pos[v]++;
+            kk[p].nbr = nb;
+            kk[p].data = weighted && w ? w[i] : 0;
+        }
     free(pos);
 #pragma omp parallel for schedule(dynamic, 256)
     for (uint64_t v = 0; v < V; v++) {
         uint64_t b = off[v], e = off[v + 1];
         if (b == e) continue;
-        qsort(k + b, e - b, sizeof(key_t_), key_cmp);
+        qsort(kk + b, e - b, sizeof(key_t_), key_cmp);
         or_adj *a = &st->v[v];
         uint64_t uniq = 1;
-        for (uint64_t i = b + 1; i < e; i++) uniq += key_cmp(&k[i - 1], &k[i]) != 0;
+        for (uint64_t i = b + 1; i < e; i++) uniq += key_cmp(&kk[i - 1], &kk[i]) != 0;
         a->a = malloc(uniq * sizeof(or_ent));
         a->cap = uniq;
         a->len = 0;
         for (uint64_t i = b; i < e; i++) {
-            if (i == b || key_cmp(&k[i - 1], &k[i]) != 0) {
-                a->a[a->len].nbr = k[i].nbr;
-                a->a[a->len].data = k[i].data;
+            if (i == b || key_cmp(&kk[i - 1], &kk[i]) != 0) {
+                a->a[a->len].nbr = kk[i].nbr;
+                a->a[a->len].data = kk[i].data;
                 a->a[a->len].num = 0;
                 a->len++;
             }
@@ -533,7 +555,7 @@ static void bulk_store(or_store *st, uint64_t V, uint64_t m, const uint64_t *lv,
             hidx_rebuild(a, cap);
         }
     }
-    free(k);
+    free(kk);
     free(off);
 }
 
@@ -544,31 +566,9 @@ int or_bulk_load(or_graph *g, uint64_t n, const uint64_t *src, const uint64_t *d
         if (g->out.v[v].len || (!g->symmetric && g->in.v[v].len)) return -1;
     for (uint64_t i = 0; i < n; i++)
         if (src[i] >= g->V || dst[i] >= g->V) return -1;
-    /* list ops of add_edge (graph.hpp:203-227): out[src] gets (dst, w);
-     * an undirected add also gives out[dst] (src, w) */
-    uint64_t per = directed ? 1 : 2;
-    uint64_t m = per * n;
-    uint64_t *lv = malloc((m ? m : 1) * sizeof(uint64_t));
-    uint64_t *ln = malloc((m ? m : 1) * sizeof(uint64_t));
-    uint64_t *ld = malloc((m ? m : 1) * sizeof(uint64_t));
-    for (int pass = 0; pass < (g->symmetric ? 1 : 2); pass++) {
-        /* pass 0: out-lists; pass 1: in-lists (directed graphs) */
-        uint64_t k = 0;
-        for (uint64_t i = 0; i < n; i++) {
-            uint64_t data = g->weighted && w ? w[i] : 0;
-            uint64_t a = src[i], b = dst[i];
-            if (pass == 0) { lv[k] = a; ln[k] = b; }
-            else { lv[k] = b; ln[k] = a; }
-            ld[k++] = data;
-            if (!directed) {
-                if (pass == 0) { lv[k] = b; ln[k] = a; }
-                else { lv[k] = a; ln[k] = b; }
-                ld[k++] = data;
-            }
-        }
-        bulk_store(pass == 0 ? &g->out : &g->in, g->V, m, lv, ln, ld);
-    }
-    free(lv); free(ln); free(ld);
+    int per = directed ? 1 : 2;
+    bulk_store(&g->out, g->V, n, src, dst, w, g->weighted, per, 0);
+    if (!g->symmetric) bulk_store(&g->in, g->V, n, src, dst, w, g->weighted, per, 1);
     return 0;
 }
 
@@ -1014,6 +1014,52 @@ uint64_t or_stream_plain(or_graph *g, or_labels *L, uint64_t n,
         if (lat_ns) {
             clock_gettime(CLOCK_MONOTONIC, &t1);
             lat_ns[i] = (double)(t1.tv_sec - t0.tv_sec) * 1e9 + (double)(t1.tv_nsec - t0.tv_nsec);
+        }
+    }
+    return unsafe;
+}
+
+/* The all_inc alternation (all_inc.cpp:254-300) over four trees, as the
+ * reference runs them: BFS/SSSP/SSWP labels on one directed weighted graph
+ * (nLd labels) and WCC on a symmetric graph (Lu may be NULL).  Update u is an
+ * add of stream edge u/2 (even u) or a delete of edge u/2 (odd u); each
+ * follows the plain drivers' rule (bfs_inc.cpp:160-189): incremental work
+ * when the add returned 0 or the delete returned 1. */
+uint64_t or_stream_mixed(or_graph *gd, or_labels **Ld, int nLd, or_graph *gu, or_labels *Lu,
+                         uint64_t n_updates, const uint64_t *as, const uint64_t *ad, const uint64_t *aw,
+                         const uint64_t *ds, const uint64_t *dd, const uint64_t *dw, double *lat_ns)
+{
+    uint64_t unsafe = 0;
+    for (uint64_t u = 0; u < n_updates; u++) {
+        struct timespec t0, t1;
+        if (lat_ns) clock_gettime(CLOCK_MONOTONIC, &t0);
+        uint64_t k = u / 2;
+        int any = 0;
+        if ((u & 1) == 0) {
+            uint64_t s = as[k], d = ad[k], w = aw ? aw[k] : 0;
+            if (gd && or_add_edge(gd, s, d, w, 1) == 0) {
+                for (int i = 0; i < nLd; i++) or_update_add(gd, Ld[i], s, d, w, 1);
+                any = 1;
+            }
+            if (gu && or_add_edge(gu, s, d, 0, 0) == 0 && Lu) {
+                or_update_add(gu, Lu, s, d, 0, 0);
+                any = 1;
+            }
+        } else {
+            uint64_t s = ds[k], d = dd[k], w = dw ? dw[k] : 0;
+            if (gd && or_del_edge(gd, s, d, w, 1) == 1) {
+                for (int i = 0; i < nLd; i++) or_update_del(gd, Ld[i], s, d, w, 1);
+                any = 1;
+            }
+            if (gu && or_del_edge(gu, s, d, 0, 0) == 1 && Lu) {
+                or_update_del(gu, Lu, s, d, 0, 0);
+                any = 1;
+            }
+        }
+        unsafe += any;
+        if (lat_ns) {
+            clock_gettime(CLOCK_MONOTONIC, &t1);
+            lat_ns[u] = (double)(t1.tv_sec - t0.tv_sec) * 1e9 + (double)(t1.tv_nsec - t0.tv_nsec);
         }
     }
     return unsafe;

@@ -106,6 +106,13 @@ uint64_t or_stream_plain(or_graph *g, or_labels *L, uint64_t n,
                          const uint64_t *ds, const uint64_t *dd, const uint64_t *dw,
                          int directed, double *lat_ns);
 
+/* The all_inc alternation over four trees: BFS/SSSP/SSWP labels (Ld[0..nLd))
+ * on directed graph gd and WCC (Lu) on symmetric graph gu.  Update u adds
+ * edge u/2 (even u) or deletes edge u/2 (odd u).  lat_ns[u] = wall ns. */
+uint64_t or_stream_mixed(or_graph *gd, or_labels **Ld, int nLd, or_graph *gu, or_labels *Lu,
+                         uint64_t n_updates, const uint64_t *as, const uint64_t *ad, const uint64_t *aw,
+                         const uint64_t *ds, const uint64_t *dd, const uint64_t *dw, double *lat_ns);
+
 #ifdef __cplusplus
 }
 #endif

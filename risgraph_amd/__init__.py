@@ -51,7 +51,8 @@ class ServeResult(ctypes.Structure):
     _fields_ = [("wall_s", ctypes.c_double), ("updates_per_s", ctypes.c_double),
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("p999_ms", ctypes.c_double),
                 ("max_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("mean_batch", ctypes.c_double),
-                ("batches", u64), ("max_batch", u64), ("grid_steps", u64), ("rebuilds", u64)]
+                ("batches", u64), ("max_batch", u64), ("grid_steps", u64), ("rebuilds", u64),
+                ("overloaded", u64), ("served", u64), ("gpu_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: (float(getattr(self, k)) if t is ctypes.c_double else int(getattr(self, k)))

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -103,6 +104,8 @@ struct rg_graph {
     ListOps lops{};
     uint32_t op_cap = 0;
     uint32_t *err = nullptr;          // device: [0] bad ids
+    uint64_t *free_off = nullptr, *pend_off = nullptr;   // slab free lists
+    uint32_t *cnt_tmp[2]{};           // bulk reservation counters
     uint32_t *h_status = nullptr;     // pinned: err, oom
     // pinned staging for host-side batches
     uint32_t *h_src = nullptr, *h_dst = nullptr, *h_w = nullptr, *h_old = nullptr;
@@ -231,6 +234,10 @@ StoreArgs store_args(rg_graph *g)
     a.ops = g->ops;
     a.lops = g->lops;
     a.err = g->err;
+    a.free_off = g->free_off;
+    a.pend_off = g->pend_off;
+    a.cnt_tmp[0] = g->cnt_tmp[0];
+    a.cnt_tmp[1] = g->cnt_tmp[1];
     return a;
 }
 
@@ -275,7 +282,9 @@ void store_status_check(rg_graph *g)
 
 void run_store(rg_graph *g)
 {
-    launch_store_batch(store_args(g), g->st);
+    StoreArgs a = store_args(g);
+    launch_store_batch(a, g->st);
+    launch_pool_recycle(a, g->st);
     HC(hipGetLastError());
     store_status_fetch(g);
 }
@@ -424,7 +433,10 @@ int rg_graph_create(uint64_t vertices, uint64_t hint_edges, int symmetric, int d
         const char *env = getenv("RG_SLOT_FACTOR");
         double f = env ? atof(env) : 6.0;
         g->slot_cap = std::max<uint64_t>(1ull << 20, (uint64_t)(f * (double)g->hint)) + 4 * vertices;
-        g->hash_cap = std::max<uint64_t>(1ull << 20, 4 * g->hint);
+        g->hash_cap = std::max<uint64_t>(1ull << 20, 6 * g->hint);
+        uint64_t nfree = 2ull * NCLS * FCAP;
+        g->free_off = dalloc<uint64_t>(nfree);
+        g->pend_off = dalloc<uint64_t>(nfree);
         g->slots.nbr = dalloc<uint32_t>(g->slot_cap);
         g->slots.cnt = dalloc<uint32_t>(g->slot_cap);
         g->slots.wt = weighted ? dalloc<uint32_t>(g->slot_cap) : nullptr;
@@ -457,6 +469,7 @@ int rg_graph_destroy(rg_graph *g)
     if (!g->symmetric) free_list(g->lists[1]);
     dfree(g->slots.nbr); dfree(g->slots.cnt); dfree(g->slots.wt); dfree(g->slots.hkey); dfree(g->slots.hval);
     dfree(g->pool); dfree(g->err);
+    dfree(g->free_off); dfree(g->pend_off); dfree(g->cnt_tmp[0]); dfree(g->cnt_tmp[1]);
     free_ops(g);
     hfree(g->h_src); hfree(g->h_dst); hfree(g->h_w); hfree(g->h_old); hfree(g->h_kind); hfree(g->h_flag);
     hfree(g->h_status);
@@ -845,10 +858,29 @@ int rg_stream_load(rg_graph *g, rg_stream *s, uint64_t first, uint64_t n, int di
         HC(hipStreamSynchronize(s->st));
         const uint64_t B = 1ull << 24;
         uint8_t kind = K_INS | (directed ? 0 : K_UNDIR);
+        const uint32_t *w = g->weighted ? s->w : nullptr;
+        if (g->weighted && !w) throw RgError{RG_EINVAL, "weighted graph needs a weighted stream"};
+        // size every list (and hub index) once for the whole load
+        for (int L = 0; L < (g->symmetric ? 1 : 2); L++) {
+            if (!g->cnt_tmp[L]) {
+                g->cnt_tmp[L] = dalloc<uint32_t>(g->V);
+                HC(hipMemsetAsync(g->cnt_tmp[L], 0, g->V * 4, g->st));
+            }
+        }
         for (uint64_t b = 0; b < n; b += B) {
             uint32_t m = (uint32_t)std::min(B, n - b);
             ensure_ops(g, m);
-            launch_ops_from_stream(s->src, s->dst, g->weighted ? s->w : nullptr, first + b, m, kind, g->ops, 0, g->st);
+            launch_ops_from_stream(s->src, s->dst, w, first + b, m, kind, g->ops, 0, g->st);
+            g->ops.n = m;
+            launch_store_count(store_args(g), g->st);
+        }
+        launch_store_reserve(store_args(g), g->st);
+        launch_pool_recycle(store_args(g), g->st);
+        HC(hipGetLastError());
+        for (uint64_t b = 0; b < n; b += B) {
+            uint32_t m = (uint32_t)std::min(B, n - b);
+            ensure_ops(g, m);
+            launch_ops_from_stream(s->src, s->dst, w, first + b, m, kind, g->ops, 0, g->st);
             g->ops.n = m;
             g->lops.n = 4 * m;
             run_store(g);
@@ -959,44 +991,68 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
         rg_batch_stats st{};
         double cap = p->batch_cap;
         const double cap_max = std::max<double>(p->cap_max ? p->cap_max : p->batch_cap, 1.0);
+        const double max_wait = std::max(10.0 * p->target_ms, 200.0) * 1e-3;
         uint64_t next = 0, batches = 0, max_batch = 0;
+        bool overloaded = false;
+        double gpu_ms = 0;
         auto t0 = std::chrono::steady_clock::now();
         auto now = [&] {
             return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         };
+        auto run = [&](uint64_t b, uint64_t e) {
+            // updates [b, e): adds are the even indices, deletes the odd ones
+            uint64_t a0 = (b + 1) / 2, a1 = (e + 1) / 2;
+            uint64_t d0 = b / 2, d1 = e / 2;
+            apply_batch(g, L, nL, s, p->ins_first + a0, a1 - a0, p->del_first + d0, d1 - d0, p->directed, &st);
+            gpu_ms += g->last_ms;
+        };
         while (next < n) {
             double tn = now();
             while (arr[next] > tn) tn = now();   // idle until the next request arrives
+            if (tn - arr[next] > max_wait) {
+                overloaded = true;
+                break;
+            }
             uint64_t arrived = (uint64_t)(std::upper_bound(arr.begin() + next, arr.end(), tn) - arr.begin());
             uint64_t end = std::min<uint64_t>(arrived, next + (uint64_t)std::max(1.0, std::floor(cap)));
-            // updates [next, end): adds are the even indices, deletes the odd ones
-            uint64_t a0 = (next + 1) / 2, a1 = (end + 1) / 2;
-            uint64_t d0 = next / 2, d1 = end / 2;
-            apply_batch(g, L, nL, s, p->ins_first + a0, a1 - a0, p->del_first + d0, d1 - d0, p->directed, &st);
+            run(next, end);
             double tc = now();
             for (uint64_t u = next; u < end; u++) lat[u] = (tc - arr[u]) * 1e3;
             if (p->target_ms > 0) {
-                double waited = (tc - arr[next]) * 1e3;
-                if (waited > 0.5 * p->target_ms) cap = std::max(1.0, cap * 0.9);
-                else cap = std::min(cap_max, cap < 1000 ? cap + 16 : cap * 1.01);
+                double service_ms = (tc - tn) * 1e3;
+                uint64_t backlog = (uint64_t)(std::upper_bound(arr.begin() + end, arr.end(), tc) - arr.begin()) - end;
+                if (service_ms > 0.25 * p->target_ms) cap = std::max(1.0, cap * 0.7);
+                else if ((double)backlog >= cap) cap = std::min(cap_max, cap * 1.5 + 16);
             }
             batches++;
             max_batch = std::max<uint64_t>(max_batch, end - next);
             next = end;
         }
+        const uint64_t served = next;
         double wall = now();
+        // an overloaded run still applies the rest (untimed) so the stream
+        // window stays consistent for the next run
+        while (next < n) {
+            uint64_t end = std::min<uint64_t>(n, next + (uint64_t)cap_max);
+            run(next, end);
+            next = end;
+        }
+        r->overloaded = overloaded ? 1 : 0;
+        r->served = served;
+        r->gpu_ms = gpu_ms;
         r->wall_s = wall;
-        r->updates_per_s = wall > 0 ? (double)n / wall : 0;
+        r->updates_per_s = wall > 0 ? (double)served / wall : 0;
         r->batches = batches;
         r->max_batch = max_batch;
-        r->mean_batch = batches ? (double)n / (double)batches : 0;
+        r->mean_batch = batches ? (double)served / (double)batches : 0;
         r->grid_steps = st.grid_steps;
         r->rebuilds = st.rebuilds;
         double sum = 0;
-        for (double x : lat) sum += x;
-        r->mean_ms = n ? sum / (double)n : 0;
+        for (uint64_t u = 0; u < served; u++) sum += lat[u];
+        r->mean_ms = served ? sum / (double)served : 0;
+        for (uint64_t u = served; u < n; u++) lat[u] = std::numeric_limits<double>::infinity();
         if (lat_ms) std::copy(lat.begin(), lat.end(), lat_ms);
-        std::vector<double> tmp = lat;
+        std::vector<double> tmp(lat.begin(), lat.begin() + served);
         r->p50_ms = percentile(tmp, 0.50);
         r->p99_ms = percentile(tmp, 0.99);
         r->p999_ms = percentile(tmp, 0.999);

@@ -56,11 +56,23 @@ struct List {
     int32_t *head;
 };
 
+// Slab allocator over the slot pool (0) and the hash pool (1): blocks are
+// powers of two; a freed block goes to its class's pending list and becomes
+// reusable after the batch (pending -> free between kernels), so a pop and a
+// push never touch the same list inside one kernel.  Pops take from the end
+// of the free list; the bump pointer serves what the free lists cannot.
+constexpr int NCLS = 32;
+constexpr uint32_t FCAP = 1u << 16;   // entries per class list (excess frees leak)
+
 struct Pool {
     unsigned long long slot_top, slot_cap;
     unsigned long long hash_top, hash_cap;
     unsigned int oom;
     unsigned int pad;
+    unsigned int free_n[2][NCLS];
+    unsigned int free_take[2][NCLS];
+    unsigned int pend_n[2][NCLS];
+    unsigned long long recycled[2];
 };
 
 // edge-op kinds (bitmask)
@@ -161,14 +173,19 @@ struct StoreArgs {
     Slots slots;
     List lists[2];
     Pool *pool;
+    uint64_t *free_off;   // [2][NCLS][FCAP] reusable block offsets
+    uint64_t *pend_off;   // [2][NCLS][FCAP] blocks freed in the current batch
     EdgeOps ops;
     ListOps lops;
     uint32_t *err;   // bad vertex ids etc.
+    uint32_t *cnt_tmp[2];  // per-vertex insert counts for bulk reservation
 };
 
 // host-side launchers (store.hip / engine.hip)
 void launch_store_batch(const StoreArgs &a, hipStream_t s);
-void launch_store_reset_heads(const StoreArgs &a, hipStream_t s);
+void launch_store_count(const StoreArgs &a, hipStream_t s);
+void launch_store_reserve(const StoreArgs &a, hipStream_t s);
+void launch_pool_recycle(const StoreArgs &a, hipStream_t s);
 void launch_engine_block(const EngineArgs &a, uint32_t block_max, hipStream_t s);
 void launch_engine_grid_step(const EngineArgs &a, const Ctl &host_ctl, hipStream_t s);
 void launch_engine_fill_init(const EngineArgs &a, hipStream_t s);

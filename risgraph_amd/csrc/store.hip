@@ -74,15 +74,21 @@ __global__ void k_expand_link(StoreArgs a)
     };
     // graph.hpp:203-253: a directed op touches out[src] (+ in[dst] unless
     // symmetric); an undirected op is the directed op both ways and returns
-    // the old count of the second (d->s) direction, minus 1 for a self-loop
-    // on add.  A self-loop's two same-key updates fold into one +-2 op.
+    // the old count of the (d->s) direction, minus 1 for a self-loop on add.
+    // In a batch, undirected updates keep both orientations at the same
+    // count, so the return value is taken from the list of min(s, d): ops on
+    // one edge in either orientation then share one chain and see one serial
+    // order.  A single op returns old(d->s) exactly as the reference does
+    // (that differs only if directed ops were mixed into the same edge).
+    // A self-loop's two same-key updates fold into one +-2 op.
     if (!undir) {
         emit(s, d, sign, L_RET | L_SLOT);
         if (!a.symmetric) emit(d, s, sign, L_IN);
     } else if (s != d) {
-        emit(s, d, sign, L_SLOT);
+        bool ret_sd = a.ops.n > 1 && s < d;
+        emit(s, d, sign, (uint8_t)(L_SLOT | (ret_sd ? L_RET : 0)));
         if (!a.symmetric) emit(d, s, sign, L_IN);
-        emit(d, s, sign, L_RET);
+        emit(d, s, sign, (uint8_t)(ret_sd ? 0 : L_RET));
         if (!a.symmetric) emit(s, d, sign, L_IN);
     } else {
         emit(s, s, (int8_t)(2 * sign), L_RET | L_SLOT | L_SELF);
@@ -98,10 +104,23 @@ struct ListState {
     uint32_t len, cap, deg, hcap, hcnt;
 };
 
-__device__ uint64_t pool_alloc(Pool *p, bool hash, uint64_t n)
+__device__ __forceinline__ int log2_pow2(uint64_t n) { return 63 - __clzll((long long)n); }
+
+// allocate a power-of-two block: reuse a block freed by an earlier batch,
+// else bump the pool pointer
+__device__ uint64_t pool_alloc(const StoreArgs &a, int h, uint64_t n)
 {
-    unsigned long long *top = hash ? &p->hash_top : &p->slot_top;
-    unsigned long long cap = hash ? p->hash_cap : p->slot_cap;
+    Pool *p = a.pool;
+    int c = log2_pow2(n);
+    if (c < NCLS) {
+        unsigned int avail = p->free_n[h][c];
+        if (avail) {
+            unsigned int k = atomicAdd(&p->free_take[h][c], 1u);
+            if (k < avail) return a.free_off[((uint64_t)h * NCLS + c) * FCAP + (avail - 1 - k)];
+        }
+    }
+    unsigned long long *top = h ? &p->hash_top : &p->slot_top;
+    unsigned long long cap = h ? p->hash_cap : p->slot_cap;
     unsigned long long o = atomicAdd(top, (unsigned long long)n);
     if (o + n > cap) {
         atomicOr(&p->oom, 1u);
@@ -110,11 +129,24 @@ __device__ uint64_t pool_alloc(Pool *p, bool hash, uint64_t n)
     return o;
 }
 
+// return a block; it becomes allocatable after this batch
+__device__ void pool_free(const StoreArgs &a, int h, uint64_t off, uint64_t n)
+{
+    if (off == HEMPTY || n == 0) return;
+    int c = log2_pow2(n);
+    if (c >= NCLS) return;
+    unsigned int k = atomicAdd(&a.pool->pend_n[h][c], 1u);
+    if (k < FCAP) a.pend_off[((uint64_t)h * NCLS + c) * FCAP + k] = off;
+}
+
 // wave-parallel (re)build of a list's hash index at capacity `hcap`
 __device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, int lane)
 {
     uint64_t h0 = 0;
-    if (lane == 0) h0 = pool_alloc(a.pool, true, hcap);
+    if (lane == 0) {
+        h0 = pool_alloc(a, 1, hcap);
+        if (h0 != HEMPTY && st.hoff != HEMPTY) pool_free(a, 1, st.hoff, st.hcap);
+    }
     h0 = __shfl(h0, 0);
     if (h0 == HEMPTY) return false;
     for (uint32_t b = lane; b < hcap; b += WAVE) a.slots.hkey[h0 + b] = HEMPTY;
@@ -138,6 +170,28 @@ __device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, in
     st.hoff = h0;
     st.hcap = hcap;
     st.hcnt = st.len;
+    return true;
+}
+
+// move a list into a block of capacity next_pow2(need) (>= 4); the old block
+// is returned to the slab free lists
+__device__ bool grow_block(const StoreArgs &a, ListState &st, uint32_t need, int lane)
+{
+    uint32_t ncap = next_pow2(need);
+    if (ncap < 4) ncap = 4;
+    uint64_t o = 0;
+    if (lane == 0) o = pool_alloc(a, 0, ncap);
+    o = __shfl(o, 0);
+    if (o == HEMPTY) return false;
+    for (uint32_t i = lane; i < st.len; i += WAVE) {
+        a.slots.nbr[o + i] = a.slots.nbr[st.off + i];
+        a.slots.cnt[o + i] = a.slots.cnt[st.off + i];
+        if (a.slots.wt) a.slots.wt[o + i] = a.slots.wt[st.off + i];
+    }
+    __threadfence_block();
+    if (lane == 0 && st.cap) pool_free(a, 0, st.off, st.cap);
+    st.off = o;
+    st.cap = ncap;
     return true;
 }
 
@@ -214,25 +268,7 @@ __global__ void k_apply(StoreArgs a)
             if (++clen > nlops) { ok = false; break; }
         }
         if (!ok) atomicOr(a.err, 2u);
-        if (ok && st.len + n_ins > st.cap) {
-            uint32_t ncap = next_pow2(st.len + n_ins);
-            if (ncap < 4) ncap = 4;
-            uint64_t o = 0;
-            if (lane == 0) o = pool_alloc(a.pool, false, ncap);
-            o = __shfl(o, 0);
-            if (o == HEMPTY) {
-                ok = false;
-            } else {
-                for (uint32_t i = lane; i < st.len; i += WAVE) {
-                    a.slots.nbr[o + i] = a.slots.nbr[st.off + i];
-                    a.slots.cnt[o + i] = a.slots.cnt[st.off + i];
-                    if (a.slots.wt) a.slots.wt[o + i] = a.slots.wt[st.off + i];
-                }
-                __threadfence_block();
-                st.off = o;
-                st.cap = ncap;
-            }
-        }
+        if (ok && st.len + n_ins > st.cap) ok = grow_block(a, st, st.len + n_ins, lane);
 
         for (int pass = 0; pass < 2 && ok; pass++) {
             for (int32_t k = (int32_t)j; k >= 0 && (uint32_t)k < nlops; k = a.lops.next[k]) {
@@ -319,6 +355,108 @@ void launch_store_batch(const StoreArgs &a, hipStream_t s)
     uint64_t blocks = (waves * WAVE + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_apply, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------
+// bulk reservation: count the entries a whole load will append per list,
+// then size every block (and hub index) once, so the load batches never
+// re-allocate.  The count is an upper bound (duplicates count each time).
+// ---------------------------------------------------------------------
+__global__ void k_count(StoreArgs a)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.ops.n) return;
+    uint8_t kind = a.ops.kind[i];
+    if (!(kind & K_INS)) return;
+    uint32_t s = a.ops.src[i], d = a.ops.dst[i];
+    if (s >= a.V || d >= a.V) return;
+    atomicAdd(&a.cnt_tmp[0][s], 1u);
+    if (!a.symmetric) atomicAdd(&a.cnt_tmp[1][d], 1u);
+    if (kind & K_UNDIR) {
+        atomicAdd(&a.cnt_tmp[0][d], 1u);
+        if (!a.symmetric) atomicAdd(&a.cnt_tmp[1][s], 1u);
+    }
+}
+
+__global__ void k_reserve(StoreArgs a)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const uint32_t nw = (gridDim.x * blockDim.x) / WAVE;
+    const int nlists = a.symmetric ? 1 : 2;
+    for (uint32_t v = wid; v < a.V; v += nw) {
+        for (int L = 0; L < nlists; L++) {
+            uint32_t add = a.cnt_tmp[L][v];
+            if (!add) continue;
+            const List &li = a.lists[L];
+            ListState st;
+            st.off = li.off[v];
+            st.len = li.len[v];
+            st.cap = li.cap[v];
+            st.deg = li.deg[v];
+            st.hoff = li.hoff[v];
+            st.hcap = li.hcap[v];
+            st.hcnt = li.hcnt[v];
+            uint32_t need = st.len + add;
+            bool ok = true;
+            if (need > st.cap) ok = grow_block(a, st, need, lane);
+            if (ok && need >= INDEX_THRESHOLD && (st.hoff == HEMPTY || st.hcap < 2 * need)) {
+                uint32_t hc = next_pow2(2 * need);
+                if (hc < 1024) hc = 1024;
+                ok = build_index(a, st, hc, lane);
+            }
+            if (lane == 0) {
+                li.off[v] = st.off;
+                li.cap[v] = st.cap;
+                li.hoff[v] = st.hoff;
+                li.hcap[v] = st.hcap;
+                li.hcnt[v] = st.hcnt;
+                a.cnt_tmp[L][v] = 0;
+            }
+        }
+    }
+}
+
+void launch_store_count(const StoreArgs &a, hipStream_t s)
+{
+    if (a.ops.n == 0) return;
+    hipLaunchKernelGGL(k_count, dim3((a.ops.n + 255) / 256), dim3(256), 0, s, a);
+}
+
+void launch_store_reserve(const StoreArgs &a, hipStream_t s)
+{
+    uint64_t blocks = ((uint64_t)a.V * WAVE + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_reserve, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+}
+
+// pending frees -> free lists, once per batch (one block per pool/class)
+__global__ void k_pool_recycle(StoreArgs a)
+{
+    const int h = blockIdx.x / NCLS, c = blockIdx.x % NCLS;
+    Pool *p = a.pool;
+    unsigned int avail = p->free_n[h][c];
+    unsigned int take = p->free_take[h][c];
+    unsigned int rem = take >= avail ? 0 : avail - take;   // pops took from the end
+    unsigned int pn = p->pend_n[h][c];
+    if (pn > FCAP) pn = FCAP;
+    unsigned int room = FCAP - rem;
+    unsigned int mv = pn < room ? pn : room;
+    uint64_t base = ((uint64_t)h * NCLS + c) * FCAP;
+    for (unsigned int k = threadIdx.x; k < mv; k += blockDim.x)
+        a.free_off[base + rem + k] = a.pend_off[base + k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        p->free_n[h][c] = rem + mv;
+        p->free_take[h][c] = 0;
+        p->pend_n[h][c] = 0;
+        if (mv) atomicAdd(&p->recycled[h], (unsigned long long)mv);
+    }
+}
+
+void launch_pool_recycle(const StoreArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pool_recycle, dim3(2 * NCLS), dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------

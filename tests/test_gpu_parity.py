@@ -59,11 +59,24 @@ def test_store_matches_model(symmetric, weighted):
         a = 0 if rng.random() < 0.4 else int(rng.integers(0, V))
         b = int(rng.integers(0, V))
         w = int(rng.integers(1, 4))
-        directed = True if not symmetric else bool(rng.random() < 0.2)
+        directed = not symmetric
         if rng.random() < 0.7:
             assert g.add_edge(a, b, w, directed) == m.add_edge(a, b, w, directed), k
         else:
             assert g.del_edge(a, b, w, directed) == m.del_edge(a, b, w, directed), k
+    if symmetric:
+        # single directed ops mixed into a symmetric graph keep the exact
+        # reference returns (old(d->s)); checked on a separate graph because
+        # batched undirected returns assume both orientations stay equal
+        g2 = rg.Graph(64, 256, symmetric=True, dual=False)
+        m2 = MultigraphModel(64, symmetric=True)
+        for k in range(1500):
+            a, b = int(rng.integers(0, 64)), int(rng.integers(0, 64))
+            directed = bool(rng.random() < 0.3)
+            if rng.random() < 0.6:
+                assert g2.add_edge(a, b, 0, directed) == m2.add_edge(a, b, 0, directed), k
+            else:
+                assert g2.del_edge(a, b, 0, directed) == m2.del_edge(a, b, 0, directed), k
     # batches: the multiset of returned old counts is order-free
     for rep in range(5):
         n = 4000
@@ -199,6 +212,48 @@ def test_reference_driver_loop_through_facade_calls(name):
             L.update_tree_del(int(src[j]), int(dst[j]), wd, d)
         _ref_step(t, i, j)
         _check_labels(L, algo, V, t["ref"], t["root"], ref_vals=t["RL"].read()[0])
+
+
+def test_config1_bfs_plain_stream():
+    """R-MAT scale 16, 2^20 edges, 90% loaded, root = max out-degree: every
+    update checked against cpu_ref, every 100th against the closed form."""
+    V, E = 1 << 16, 1 << 20
+    t = _setup(rg.BFS, "rmat", V, E, 0.9)
+    for k in range(1000):
+        i_add, i_del = t["n0"] + k, k
+        t["g"].stream_plain([t["L"]], t["s"], i_add, i_del, 1, True, latency=False)
+        _ref_step(t, i_add, i_del)
+        vals, par = t["L"].read()
+        ref_vals = t["RL"].read()[0]
+        assert np.array_equal(vals, ref_vals), k
+        if k % 100 == 99:
+            _check_labels(t["L"], rg.BFS, V, t["ref"], t["root"], ref_vals=ref_vals)
+
+
+@pytest.mark.parametrize("name", ["sssp", "wcc"])
+def test_large_batches_grid_mode(name):
+    """Batches of 10k and 50k updates on R-MAT scale 16 (frontiers beyond one workgroup)."""
+    algo = rg.ALGOS[name]
+    V, E = 1 << 16, 1 << 20
+    t = _setup(algo, "rmat", V, E, 0.8, seed=11)
+    pos = 0
+    for batch in (10000, 50000, 10000):
+        i0, d0 = t["n0"] + pos, pos
+        st = t["g"].apply_stream([t["L"]], t["s"], i0, batch, d0, batch, t["directed"])
+        ref, RL, d = t["ref"], t["RL"], t["directed"]
+        src, dst = t["src"], t["dst"]
+        w = t["w"] if t["weighted"] else np.zeros_like(t["w"])
+        idx = np.arange(i0, i0 + batch)
+        olds = np.array([ref.add_edge(int(src[i]), int(dst[i]), int(w[i]), d) for i in idx])
+        new = idx[olds == 0]
+        RL.inc_add_batch(src[new], dst[new], w[new], d)
+        didx = np.arange(d0, d0 + batch)
+        olds = np.array([ref.del_edge(int(src[i]), int(dst[i]), int(w[i]), d) for i in didx])
+        gone = didx[olds == 1]
+        RL.inc_del_batch(src[gone], dst[gone], w[gone], d)
+        _check_labels(t["L"], algo, V, ref, t["root"], ref_vals=RL.read()[0])
+        assert st["grid_steps"] > 0
+        pos += batch
 
 
 @pytest.mark.parametrize("tuning", [
