@@ -50,8 +50,8 @@ def parse():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--target-ms", type=float, default=20.0)
-    ap.add_argument("--step-seconds", type=float, default=2.0)
-    ap.add_argument("--calib-seconds", type=float, default=1.0)
+    ap.add_argument("--step-seconds", type=float, default=0.5)
+    ap.add_argument("--calib-seconds", type=float, default=0.3)
     ap.add_argument("--plain-updates", type=int, default=4000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -98,6 +98,7 @@ def main():
     t_setup = time.time()
     stream = rg.Stream("rmat", scale, E, seed=1, wseed=2)
     g = rg.Graph(V, E, symmetric=False, dual=True, weighted=True)
+    g.profile(1)          # HIP events around every store apply kernel of this process
     g.load_stream(stream, 0, n0, True)
     root = stream.max_degree_vertex(n0, V)
     names = ["bfs", "sssp", "sswp", "wcc"]
@@ -140,12 +141,12 @@ def main():
         plain = dict(res, offered=rate)
 
     # ---- batched serve: calibration ramp ----
-    rate = max(2.0 * plain_capacity, 20000.0)
+    rate = max(2.0 * plain_capacity, 100000.0)
     best = None
     for _ in range(30):
         n = int(max(20000, min(4_000_000, rate * args.calib_seconds)))
         # keep enough of the window for the warmup and timed steps at this rate
-        reserve = (args.warmup + args.steps) * int(max(20000, rate * args.step_seconds))
+        reserve = (args.warmup + args.steps + 2) * int(max(20000, rate * args.step_seconds))
         if cur.used + n + reserve > cur.limit:
             break
         res, _ = serve(g, labels, stream, cur, n, rate, 64, 1 << 20, args.target_ms)
@@ -184,6 +185,9 @@ def main():
     value = total / wall
     p = {q: float(np.percentile(lat_all, q)) for q in (50, 99, 99.9)}
     log(f"timed {args.steps} steps: {value:.0f} upd/s p50={p[50]:.2f} p99={p[99]:.2f} p99.9={p[99.9]:.2f}ms")
+
+    # ---- roofline: dominant kernel (store apply) timed live with HIP events ----
+    roof, batch_roof = roofline_sample(g, labels, stream, cur, rate, n_step, args, int(total / max(batches, 1)))
 
     # ---- correctness: incremental trees vs a fresh rebuild ----
     check = None
@@ -231,7 +235,8 @@ def main():
         "values_match_rebuild": check,
         "build_s": build_s,
         "store": info,
-        "roofline": None,
+        "roofline": roof,
+        "batch_roofline": batch_roof,
         "cpu_baseline": cpu.get("result"),
     }
     line = json.dumps(out)
@@ -239,6 +244,68 @@ def main():
     if args.json_out:
         with open(args.json_out, "w") as f:
             f.write(line + "\n")
+
+
+# ---------------------------------------------------------------------------
+# roofline
+# ---------------------------------------------------------------------------
+B_STORE = 32                      # bytes per directed edge op (out+in: key, weight, count RMW)
+B_VERTEX = 32                     # per changed vertex: label word r/w + out/in list headers
+B_EDGE = {"bfs": 12, "sssp": 16, "sswp": 16, "wcc": 12}   # nbr id (+ weight) + neighbour label word
+
+
+def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch):
+    """Store-apply kernel: algorithmic bytes (32 B per edge op) over its
+    event-timed duration, during a serving round at the operating rate.
+    Batch level: bytes from the changed-vertex set of whole batches over the
+    batch's device time (store + all four trees)."""
+    names = ["bfs", "sssp", "sswp", "wcc"]
+    eb0 = g.profile_read(1)
+    g.profile(3)
+    n = int(max(20000, rate * min(args.step_seconds, 0.5)))
+    res, _ = serve(g, labels, stream, cur, n, rate, 64, 1 << 20, args.target_ms)
+    eb1 = g.profile_read(1)
+    g.profile(1)
+    # every k_apply launch of this process (load batches included), the same
+    # set of dispatches rocprofv3 --stats averages for this command
+    ap = g.profile_read(0)
+    launch_ms = ap["ms"] / max(ap["launches"], 1)
+    bytes_per_launch = B_STORE * ap["units"] / max(ap["launches"], 1)
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
+    nb = eb1["launches"] - eb0["launches"]
+    roof = {
+        "kernel": "rg::k_apply (store insert/delete batch)",
+        "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+        "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
+        "avg_launch_ms": round(launch_ms, 5), "launches": ap["launches"],
+        "engine_block_launches": nb,
+        "engine_block_avg_ms": round((eb1["ms"] - eb0["ms"]) / max(nb, 1), 5),
+    }
+    # batch level, from the changed-vertex set (values before vs after)
+    out_deg, in_deg = g.degrees()
+    deg = out_deg.astype(np.int64) + in_deg.astype(np.int64)
+    tot_bytes, tot_ms, changed = 0, 0.0, 0
+    batch = max(2, mean_batch - mean_batch % 2)
+    for _ in range(3):
+        before = [L.read()[0] for L in labels]
+        ins_first, del_first, m = cur.take(batch)
+        g.apply_stream(labels, stream, ins_first, m // 2, del_first, m // 2, True)
+        tot_ms += g.last_gpu_ms()
+        b = B_STORE * m
+        for nm, L, v0 in zip(names, labels, before):
+            d = np.nonzero(L.read()[0] != v0)[0]
+            changed += len(d)
+            b += len(d) * B_VERTEX + int(deg[d].sum()) * B_EDGE[nm]
+        tot_bytes += b
+    ach = tot_bytes / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
+    batch_roof = {"batch_updates": batch, "algorithmic_bytes_per_batch": round(tot_bytes / 3, 1),
+                  "changed_vertices_per_batch": round(changed / 3, 1),
+                  "batch_gpu_ms": round(tot_ms / 3, 4), "achieved": round(ach, 3), "unit": "GB/s",
+                  "frac": round(ach / HBM_PEAK_GBS, 6)}
+    log(f"roofline: k_apply {achieved:.2f} GB/s ({launch_ms * 1e3:.1f} us/launch); "
+        f"batch {ach:.2f} GB/s over {tot_ms / 3:.3f} ms/batch")
+    return roof, batch_roof
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,436 @@
+// risgraph/graph.hpp -- Graph<EdgeData>: drop-in for the reference
+// core/graph.hpp interface, backed by the MI355X library through the C ABI
+// (include/risgraph_amd.h).
+//
+// The reference's engine takes C++ lambdas (update_func, equal_func,
+// init_label_func, ...) that cannot cross a C ABI and are not run on the
+// device.  The facade recognises which of the four reference algorithms the
+// lambdas implement (BFS s+1, SSSP s+w, SSWP min(s,w), WCC s; strict
+// improvement; bfs_inc.cpp:77-92, sssp_inc.cpp:82-97, sswp_inc.cpp:82-97,
+// wcc_inc.cpp:82-97) by probing them once, and throws std::runtime_error for
+// anything else -- there is no CPU fallback path.
+//
+// Labels: alloc_vertex_tree_array<VD>() returns a TreeArray whose operator[]
+// reads a host mirror refreshed from the device after compute calls, so
+// driver code like labels[v].data / labels[v].parent.nbr keeps working.
+// parent.data (the parent edge's weight) is not mirrored and reads 0.
+//
+// Thread safety follows the reference: add_edge / del_edge may be called from
+// many threads (the library serialises calls per graph); compute calls are
+// not reentrant (graph.hpp:1728-1730).
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../risgraph_amd.h"
+#include "atomic.hpp"
+#include "io.hpp"
+#include "type.hpp"
+
+namespace risgraph_detail {
+
+inline void check(int rc)
+{
+    if (rc != RG_OK) {
+        if (rc == RG_EINVAL_VERTEX) throw std::runtime_error("VertexId error.");
+        throw std::runtime_error(std::string("risgraph_amd: ") + rg_last_error());
+    }
+}
+
+// All vertices, for stream_vertices(fn, get_dense_active_all()).
+struct AllVertices {
+    uint64_t n;
+};
+
+// Sparse vertex set (get_modified(), graph.hpp:117-125).
+class VertexSet {
+public:
+    const std::vector<uint64_t> &get_sparse() const { return v_; }
+    uint64_t get_sparse_length() const { return v_.size(); }
+    bool is_dense() const { return false; }
+    std::vector<uint64_t> v_;
+};
+
+}  // namespace risgraph_detail
+
+template <typename EdgeData = void>
+class Graph {
+public:
+    using adjedge_type = AdjEdge<EdgeData>;
+    using edge_type = Edge<EdgeData>;
+    using Bitmap = risgraph_detail::AllVertices;
+    using ActiveSet = risgraph_detail::VertexSet;
+
+    template <typename VertexData>
+    struct VertexTree {
+        adjedge_type parent;
+        VertexData data;
+    };
+
+    // Labels handle + host mirror.  Copies share the device labels.
+    template <typename VertexData>
+    class TreeArray {
+    public:
+        TreeArray() = default;
+        const VertexTree<VertexData> &operator[](uint64_t v) const
+        {
+            st_->sync();
+            return st_->mirror[v];
+        }
+        uint64_t size() const { return st_->V; }
+
+    private:
+        friend class Graph;
+        struct State {
+            Graph *g = nullptr;
+            uint64_t V = 0;
+            rg_labels *L = nullptr;
+            int algo = -1;
+            bool dirty = false;
+            std::vector<VertexTree<VertexData>> mirror;
+            ~State()
+            {
+                if (L) rg_labels_destroy(L);
+            }
+            void sync()
+            {
+                if (!dirty) return;
+                std::vector<uint64_t> val(V), par(V);
+                if (L) risgraph_detail::check(rg_labels_read(L, 0, V, val.data(), par.data()));
+                for (uint64_t v = 0; v < V; v++) {
+                    mirror[v].data = (VertexData)val[v];
+                    mirror[v].parent.nbr = par[v];
+                }
+                dirty = false;
+            }
+        };
+        std::shared_ptr<State> st_;
+    };
+
+    Graph(uint64_t vertices, uint64_t hint_edges, bool symmetric, bool dual, bool trace_modified = false)
+        : vertices_(vertices), symmetric_(symmetric), trace_(trace_modified)
+    {
+        risgraph_detail::check(rg_graph_create(vertices, hint_edges, symmetric, dual,
+                                               !std::is_void<EdgeData>::value, &g_));
+    }
+    ~Graph()
+    {
+        if (g_) rg_graph_destroy(g_);
+    }
+    Graph(const Graph &) = delete;
+    Graph &operator=(const Graph &) = delete;
+
+    uint64_t get_vertices() const { return vertices_; }
+
+    // add_edge / del_edge (graph.hpp:203-253): old multiplicity
+    uint64_t add_edge(edge_type e, bool directed = true) { return update(e, 1, directed); }
+    uint64_t del_edge(edge_type e, bool directed = true) { return update(e, -1, directed); }
+
+    // The drivers' parallel add_edge / del_edge loops (bfs_inc.cpp:59-69,
+    // bfs_inc_batch.cpp:142-173) as one device batch per call; old[i] (may
+    // be null) receives each return value.
+    void add_edges(const edge_type *e, uint64_t n, bool directed = true, uint64_t *old = nullptr)
+    {
+        update_many(e, n, 1, directed, old);
+    }
+    void del_edges(const edge_type *e, uint64_t n, bool directed = true, uint64_t *old = nullptr)
+    {
+        update_many(e, n, -1, directed, old);
+    }
+
+    uint64_t get_outgoing_degree(uint64_t v)
+    {
+        uint64_t d = 0;
+        risgraph_detail::check(rg_degree(g_, v, 0, &d));
+        return d;
+    }
+    uint64_t get_incoming_degree(uint64_t v)
+    {
+        uint64_t d = 0;
+        risgraph_detail::check(rg_degree(g_, v, 1, &d));
+        return d;
+    }
+    uint64_t get_edge_num(edge_type e)
+    {
+        uint64_t n = 0;
+        risgraph_detail::check(rg_edge_num(g_, e.src, e.dst, (uint64_t)e.data, &n));
+        return n;
+    }
+
+    template <typename VertexData>
+    TreeArray<VertexData> alloc_vertex_tree_array()
+    {
+        TreeArray<VertexData> t;
+        t.st_ = std::make_shared<typename TreeArray<VertexData>::State>();
+        t.st_->g = this;
+        t.st_->V = vertices_;
+        t.st_->mirror.resize(vertices_);
+        for (auto &x : t.st_->mirror) {
+            x.parent.nbr = vertices_;   // empty parent (graph.hpp:90)
+            x.data = VertexData();
+        }
+        return t;
+    }
+
+    // build_tree (graph.hpp:873-985)
+    template <typename R, typename DataType, typename InitLabelFunc, typename ContinueReduceFunc,
+              typename UpdateFunc, typename ActiveResultFunc>
+    R build_tree(InitLabelFunc init_label_func, ContinueReduceFunc, UpdateFunc update_func, ActiveResultFunc,
+                 TreeArray<DataType> &labels)
+    {
+        bind(labels, init_label_func, update_func);
+        uint64_t r = 0;
+        risgraph_detail::check(rg_build(labels.st_->L, &r));
+        labels.st_->dirty = true;
+        return (R)r;
+    }
+
+    // need_update_tree_add (graph.hpp:987-1000), evaluated on the device labels
+    template <typename DataType, typename UpdateFunc>
+    bool need_update_tree_add(UpdateFunc, TreeArray<DataType> &labels, edge_type edge, bool directed = true)
+    {
+        return classify(labels, edge, 1, directed);
+    }
+
+    // need_update_tree_del (graph.hpp:1220-1233)
+    template <typename DataType>
+    bool need_update_tree_del(TreeArray<DataType> &labels, edge_type edge, bool directed = true)
+    {
+        return classify(labels, edge, -1, directed);
+    }
+
+    // update_tree_add single edge (graph.hpp:1124-1160)
+    template <typename R, typename DataType, typename ContinueReduceFunc, typename UpdateFunc,
+              typename ActiveResultFunc>
+    R update_tree_add(ContinueReduceFunc, UpdateFunc, ActiveResultFunc, TreeArray<DataType> &labels,
+                      edge_type edge, bool directed = true)
+    {
+        rg_edge e{edge.src, edge.dst, (uint64_t)edge.data};
+        uint64_t r = 0;
+        risgraph_detail::check(rg_inc_add(bound(labels), &e, 1, directed, &r));
+        labels.st_->dirty = true;
+        return (R)r;
+    }
+
+    // update_tree_add batch (graph.hpp:1162-1218)
+    template <typename R, typename DataType, typename ContinueReduceFunc, typename UpdateFunc,
+              typename ActiveResultFunc>
+    R update_tree_add(ContinueReduceFunc, UpdateFunc, ActiveResultFunc, TreeArray<DataType> &labels,
+                      const std::vector<edge_type> &edges, const uint64_t &length, bool directed = true)
+    {
+        std::vector<rg_edge> e = to_rg(edges, length);
+        uint64_t r = 0;
+        risgraph_detail::check(rg_inc_add(bound(labels), e.data(), length, directed, &r));
+        labels.st_->dirty = true;
+        return (R)r;
+    }
+
+    // update_tree_del single edge (graph.hpp:1622-1656)
+    template <typename R, typename DataType, typename InitLabelFunc, typename ContinueReduceFunc,
+              typename UpdateFunc, typename ActiveResultFunc, typename EqualFunc>
+    R update_tree_del(InitLabelFunc, ContinueReduceFunc, UpdateFunc, ActiveResultFunc, EqualFunc,
+                      TreeArray<DataType> &labels, edge_type edge, bool directed = true)
+    {
+        rg_edge e{edge.src, edge.dst, (uint64_t)edge.data};
+        uint64_t r = 0;
+        risgraph_detail::check(rg_inc_del(bound(labels), &e, 1, directed, &r));
+        labels.st_->dirty = true;
+        return (R)r;
+    }
+
+    // update_tree_del batch (graph.hpp:1658-1718)
+    template <typename R, typename DataType, typename InitLabelFunc, typename ContinueReduceFunc,
+              typename UpdateFunc, typename ActiveResultFunc, typename EqualFunc>
+    R update_tree_del(InitLabelFunc, ContinueReduceFunc, UpdateFunc, ActiveResultFunc, EqualFunc,
+                      TreeArray<DataType> &labels, const std::vector<edge_type> &edges, const uint64_t &length,
+                      bool directed = true)
+    {
+        std::vector<rg_edge> e = to_rg(edges, length);
+        uint64_t r = 0;
+        risgraph_detail::check(rg_inc_del(bound(labels), e.data(), length, directed, &r));
+        labels.st_->dirty = true;
+        return (R)r;
+    }
+
+    // stream_vertices over all vertices (graph.hpp:290-321, driver histograms)
+    Bitmap get_dense_active_all() const { return Bitmap{vertices_}; }
+
+    template <typename R>
+    R stream_vertices(std::function<R(uint64_t)> process, const Bitmap &all)
+    {
+        R r = 0;
+        for (uint64_t v = 0; v < all.n; v++) r += process(v);
+        return r;
+    }
+
+    // get_modified / clear_modified (graph.hpp:117-130): union over the
+    // labels bound on this graph (trace_modified graphs only)
+    ActiveSet &get_modified()
+    {
+        modified_.v_.clear();
+        if (!trace_) return modified_;
+        for (rg_labels *L : bound_) {
+            uint64_t n = 0;
+            int all = 0;
+            risgraph_detail::check(rg_modified_read(L, nullptr, 0, &n, &all));
+            if (all) {
+                modified_.v_.resize(vertices_);
+                for (uint64_t v = 0; v < vertices_; v++) modified_.v_[v] = v;
+                return modified_;
+            }
+            std::vector<uint64_t> tmp(n);
+            risgraph_detail::check(rg_modified_read(L, tmp.data(), n, &n, &all));
+            modified_.v_.insert(modified_.v_.end(), tmp.begin(), tmp.begin() + n);
+        }
+        std::sort(modified_.v_.begin(), modified_.v_.end());
+        modified_.v_.erase(std::unique(modified_.v_.begin(), modified_.v_.end()), modified_.v_.end());
+        return modified_;
+    }
+    uint64_t get_modified_length() { return get_modified().get_sparse_length(); }
+    void clear_modified()
+    {
+        for (rg_labels *L : bound_) risgraph_detail::check(rg_modified_clear(L));
+    }
+
+    template <typename R, typename Process>
+    R stream_vertices_hybrid(Process process, const ActiveSet &set)
+    {
+        R r = 0;
+        for (uint64_t v : set.get_sparse()) r += process(v);
+        return r;
+    }
+
+    rg_graph *handle() { return g_; }
+
+private:
+    uint64_t update(const edge_type &e, int delta, bool directed)
+    {
+        if (e.src >= vertices_ || e.dst >= vertices_) throw std::runtime_error("VertexId error.");
+        uint64_t old = 0;
+        risgraph_detail::check(rg_edge_update(g_, e.src, e.dst, (uint64_t)e.data, delta, directed, &old));
+        return old;
+    }
+
+    void update_many(const edge_type *e, uint64_t n, int delta, bool directed, uint64_t *old)
+    {
+        const uint64_t B = 1ull << 22;
+        std::vector<rg_edge> buf;
+        std::vector<uint64_t> ob;
+        for (uint64_t b = 0; b < n; b += B) {
+            uint64_t m = std::min(B, n - b);
+            buf.resize(m);
+            for (uint64_t i = 0; i < m; i++) buf[i] = rg_edge{e[b + i].src, e[b + i].dst, (uint64_t)e[b + i].data};
+            ob.resize(m);
+            risgraph_detail::check(rg_edge_update_batch(g_, buf.data(), m, delta, directed, ob.data()));
+            if (old) std::copy(ob.begin(), ob.end(), old + b);
+        }
+    }
+
+    static std::vector<rg_edge> to_rg(const std::vector<edge_type> &edges, uint64_t length)
+    {
+        std::vector<rg_edge> e(length);
+        for (uint64_t i = 0; i < length; i++) e[i] = rg_edge{edges[i].src, edges[i].dst, (uint64_t)edges[i].data};
+        return e;
+    }
+
+    template <typename DataType>
+    rg_labels *bound(TreeArray<DataType> &labels)
+    {
+        if (!labels.st_ || !labels.st_->L)
+            throw std::runtime_error("risgraph_amd: build_tree must run before incremental updates");
+        return labels.st_->L;
+    }
+
+    template <typename DataType>
+    bool classify(TreeArray<DataType> &labels, const edge_type &edge, int kind, bool directed)
+    {
+        rg_edge e{edge.src, edge.dst, (uint64_t)edge.data};
+        int8_t k = (int8_t)kind;
+        uint8_t unsafe = 0;
+        risgraph_detail::check(rg_classify(bound(labels), &e, &k, 1, directed, &unsafe));
+        return unsafe != 0;
+    }
+
+    // Recognise the algorithm from the lambdas (probe values chosen so the
+    // four reference update functions give four different answers).
+    template <typename UpdateFunc>
+    int identify(UpdateFunc upd)
+    {
+        adjedge_type ae{};
+        ae.nbr = 1;
+        set_weight(ae, 7);
+        auto a = upd(0, 1, (uint64_t)9, (uint64_t)1000, ae);   // BFS 10, SSSP 16, SSWP 7 (no), WCC 9
+        auto b = upd(0, 1, (uint64_t)9, (uint64_t)0, ae);      // SSWP: min(9,7)=7 > 0
+        auto c = upd(0, 1, (uint64_t)3, (uint64_t)5, ae);      // BFS 4<5, SSSP 10 (no), WCC 3<5
+        bool weighted = !std::is_void<EdgeData>::value;
+        if (a.first && (uint64_t)a.second == 10 && c.first && (uint64_t)c.second == 4) return RG_BFS;
+        if (weighted && a.first && (uint64_t)a.second == 16 && !c.first) return RG_SSSP;
+        if (!a.first && (uint64_t)a.second == 7 && b.first && (uint64_t)b.second == 7) return RG_SSWP;
+        if (a.first && (uint64_t)a.second == 9 && c.first && (uint64_t)c.second == 3) return RG_WCC;
+        throw std::runtime_error("risgraph_amd: update_func is not one of BFS / SSSP / SSWP / WCC");
+    }
+
+    template <typename T = EdgeData>
+    static typename std::enable_if<!std::is_void<T>::value>::type set_weight(adjedge_type &ae, uint64_t w)
+    {
+        ae.data = (T)w;
+    }
+    template <typename T = EdgeData>
+    static typename std::enable_if<std::is_void<T>::value>::type set_weight(adjedge_type &, uint64_t)
+    {
+    }
+
+    template <typename DataType, typename InitLabelFunc, typename UpdateFunc>
+    void bind(TreeArray<DataType> &labels, InitLabelFunc init, UpdateFunc upd)
+    {
+        auto &st = *labels.st_;
+        int algo = identify(upd);
+        if (st.L && st.algo == algo) return;
+        if (st.L) {
+            bound_.erase(std::remove(bound_.begin(), bound_.end(), st.L), bound_.end());
+            rg_labels_destroy(st.L);
+            st.L = nullptr;
+        }
+        uint64_t root = vertices_;
+        std::vector<uint64_t> initv;
+        if (algo == RG_WCC) {
+            initv.resize(vertices_);
+            for (uint64_t v = 0; v < vertices_; v++) {
+                auto p = init(v);
+                if (!p.second) throw std::runtime_error("risgraph_amd: WCC needs every vertex active at build");
+                initv[v] = (uint64_t)p.first;
+            }
+        } else {
+            // root-based init of the reference drivers: the one active vertex
+            for (uint64_t v = 0; v < vertices_; v++)
+                if (init(v).second) {
+                    if (root != vertices_) throw std::runtime_error("risgraph_amd: more than one root");
+                    root = v;
+                }
+            if (root == vertices_) throw std::runtime_error("risgraph_amd: no root in init_label_func");
+            uint64_t want_root = algo == RG_SSWP ? 134217728ull : 0;
+            uint64_t want_other = algo == RG_BFS ? 65536ull : algo == RG_SSSP ? 134217728ull : 0;
+            uint64_t other = root == 0 ? (vertices_ > 1 ? 1 : 0) : 0;
+            if ((uint64_t)init(root).first != want_root ||
+                (other != root && (uint64_t)init(other).first != want_other))
+                throw std::runtime_error("risgraph_amd: init labels differ from the reference sentinels");
+        }
+        risgraph_detail::check(rg_labels_create(g_, algo, root == vertices_ ? 0 : root,
+                                                initv.empty() ? nullptr : initv.data(), trace_, &st.L));
+        st.algo = algo;
+        bound_.push_back(st.L);
+    }
+
+    rg_graph *g_ = nullptr;
+    uint64_t vertices_;
+    bool symmetric_;
+    bool trace_;
+    std::vector<rg_labels *> bound_;
+    ActiveSet modified_;
+};

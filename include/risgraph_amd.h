@@ -158,6 +158,19 @@ int rg_stream_plain(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t i
  * engine kernel. */
 int rg_last_gpu_ms(rg_graph *g, double *ms);
 
+/* Kernel timing with HIP events on the streams the kernels run on.  Bit k of
+ * `mask` times kernels of kind k: the store apply kernel (kind 0) and the
+ * single-workgroup engine launches (kind 1).  rg_profile_read returns the
+ * launch count, summed device milliseconds and summed work units (edge ops
+ * for kind 0, launches for kind 1) accumulated since the graph was made. */
+int rg_profile_enable(rg_graph *g, int mask);
+int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint64_t *units);
+
+/* Live multiplicity degree of every vertex (out, and in for directed
+ * graphs; either pointer may be NULL), the `degree` of IndexedEdgeStorage
+ * (storage.hpp:396). */
+int rg_degrees(rg_graph *g, uint32_t *out_deg, uint32_t *in_deg);
+
 /* Serving loop: the device-side replacement of the latency-aware scheduler
  * of the *_inc_rt drivers (bfs_inc_rt.cpp:154-431).  Updates u = 0..n-1
  * alternate add (stream edge ins_first + u/2) and delete (del_first + u/2),

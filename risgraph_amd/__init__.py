@@ -97,6 +97,9 @@ SIGNATURES = {
     "rg_stream_plain": (ci, [vp, vp, ci, vp, u64, u64, u64, ci, vp, vp]),
     "rg_last_gpu_ms": (ci, [vp, ctypes.POINTER(ctypes.c_double)]),
     "rg_serve": (ci, [vp, vp, ci, vp, vp, vp, vp]),
+    "rg_profile_enable": (ci, [vp, ci]),
+    "rg_profile_read": (ci, [vp, ci, u64p, ctypes.POINTER(ctypes.c_double), u64p]),
+    "rg_degrees": (ci, [vp, vp, vp]),
 }
 
 _lib = None
@@ -268,6 +271,23 @@ class Graph:
         _check(load_library().rg_serve(self.h, arr, len(labels), stream.h, ctypes.byref(p),
                                        ctypes.byref(r), _ptr(lat)))
         return r.as_dict(), lat
+
+    def profile(self, mask=3):
+        """Time kernels with HIP events: bit 0 store apply, bit 1 engine block launches."""
+        _check(load_library().rg_profile_enable(self.h, int(mask)))
+
+    def profile_read(self, kind):
+        """kind 0 = store apply kernel, 1 = single-workgroup engine launches."""
+        n, units, ms = u64(), u64(), ctypes.c_double()
+        _check(load_library().rg_profile_read(self.h, kind, ctypes.byref(n), ctypes.byref(ms),
+                                              ctypes.byref(units)))
+        return {"launches": int(n.value), "ms": ms.value, "units": int(units.value)}
+
+    def degrees(self):
+        out = np.empty(self.V, np.uint32)
+        inc = np.empty(self.V, np.uint32)
+        _check(load_library().rg_degrees(self.h, _ptr(out), _ptr(inc)))
+        return out, inc
 
     def last_gpu_ms(self):
         ms = ctypes.c_double()

@@ -115,6 +115,17 @@ struct rg_graph {
     hipEvent_t ev[4]{};
     double last_ms = 0;
     std::recursive_mutex mu;
+    // kernel timing (rg_profile_enable)
+    struct ProfRec {
+        int kind;
+        hipEvent_t a, b;
+        uint64_t units;
+    };
+    int prof = 0;   // bit k: time kernels of kind k
+    std::vector<hipEvent_t> ev_free;
+    std::vector<ProfRec> prof_pending;
+    uint64_t p_launch[2]{}, p_units[2]{};
+    double p_ms[2]{};
 };
 
 struct rg_labels {
@@ -132,6 +143,8 @@ struct rg_labels {
     uint32_t *mlist = nullptr;
     Ctl *ctl = nullptr;
     Ctl *hc = nullptr;     // pinned mirror
+    hipStream_t st = nullptr;      // this tree's engine stream
+    hipEvent_t ev_done = nullptr;  // end of the last engine run
     Tuning tune{};
     uint32_t block_max = 4096;
     uint32_t *init_active = nullptr;
@@ -262,12 +275,6 @@ void upload_edges(rg_graph *g, const rg_edge *e, uint64_t n, uint8_t kind)
     g->lops.n = (uint32_t)(4 * n);
 }
 
-void store_status_fetch(rg_graph *g)
-{
-    HC(hipMemcpyAsync(&g->h_status[0], g->err, 4, hipMemcpyDeviceToHost, g->st));
-    HC(hipMemcpyAsync(&g->h_status[1], &g->pool->oom, 4, hipMemcpyDeviceToHost, g->st));
-}
-
 void store_status_check(rg_graph *g)
 {
     if (g->h_status[0]) {
@@ -280,13 +287,53 @@ void store_status_check(rg_graph *g)
     if (g->h_status[1]) throw RgError{RG_ENOMEM, "adjacency slot or hash pool exhausted"};
 }
 
+hipEvent_t prof_event(rg_graph *g)
+{
+    if (!g->ev_free.empty()) {
+        hipEvent_t e = g->ev_free.back();
+        g->ev_free.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HC(hipEventCreate(&e));
+    return e;
+}
+
+// bracket the next launch on stream s with events when profiling
+void prof_begin(rg_graph *g, int kind, uint64_t units, hipStream_t s, hipEvent_t *a, hipEvent_t *b)
+{
+    *a = *b = nullptr;
+    if (!((g->prof >> kind) & 1)) return;
+    *a = prof_event(g);
+    *b = prof_event(g);
+    g->prof_pending.push_back({kind, *a, *b, units});
+    (void)s;
+}
+
+// fold completed event pairs into the counters (call after the streams synced)
+void prof_collect(rg_graph *g)
+{
+    for (auto &r : g->prof_pending) {
+        float ms = 0;
+        HC(hipEventSynchronize(r.b));
+        HC(hipEventElapsedTime(&ms, r.a, r.b));
+        g->p_launch[r.kind]++;
+        g->p_ms[r.kind] += ms;
+        g->p_units[r.kind] += r.units;
+        g->ev_free.push_back(r.a);
+        g->ev_free.push_back(r.b);
+    }
+    g->prof_pending.clear();
+}
+
 void run_store(rg_graph *g)
 {
     StoreArgs a = store_args(g);
-    launch_store_batch(a, g->st);
-    launch_pool_recycle(a, g->st);
+    hipEvent_t ea, eb;
+    prof_begin(g, 0, g->ops.n, g->st, &ea, &eb);
+    launch_store_batch(a, g->st, ea, eb);
+    launch_pool_recycle(a, g->st);   // also publishes err/oom to g->h_status
     HC(hipGetLastError());
-    store_status_fetch(g);
 }
 
 EngineArgs engine_args(rg_labels *L)
@@ -325,6 +372,7 @@ EngineArgs engine_args(rg_labels *L)
     a.cq_cap = L->cq_cap;
     a.mlist = L->mlist;
     a.ctl = L->ctl;
+    a.hctl = L->hc;
     a.ops = g->ops;
     a.tune = L->tune;
     return a;
@@ -342,11 +390,8 @@ uint64_t host_work(const Ctl &h, uint64_t V)
     }
 }
 
-// Run one labels instance's engine from `phase0` to completion.
-void engine_run(rg_labels *L, uint32_t phase0, bool del_pending, uint32_t n_ops, bool new_idx,
-                rg_batch_stats *stats)
+void engine_init(rg_labels *L, uint32_t phase0, bool del_pending, uint32_t n_ops, bool new_idx)
 {
-    rg_graph *g = L->g;
     Ctl &h = *L->hc;
     h.phase = phase0;
     h.stage = 0;
@@ -361,38 +406,77 @@ void engine_run(rg_labels *L, uint32_t phase0, bool del_pending, uint32_t n_ops,
         if (h.idx >= 0x7FFFFFF0) h.idx = 1;  // marks are cleared after every run
     }
     h.flags &= F_ALL_MODIFIED;
-    HC(hipMemcpyAsync(L->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
-    EngineArgs a = engine_args(L);
+    h.levels = 0;
+}
+
+// Run the engines of labels L[0..n) from `phase0` to completion, each on its
+// own stream so the trees repair concurrently (different workgroups / CUs).
+// They start after the work already queued on the graph stream (the store
+// batch) and the host waits for all of them.
+void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool del_pending, uint32_t n_ops, bool new_idx,
+                      rg_batch_stats *stats)
+{
+    if (n <= 0) return;
+    rg_graph *g = L[0]->g;
+    HC(hipEventRecord(g->ev[3], g->st));
+    std::vector<EngineArgs> args(n);
+    std::vector<char> active(n, 1);
+    for (int i = 0; i < n; i++) {
+        HC(hipStreamWaitEvent(L[i]->st, g->ev[3], 0));
+        engine_init(L[i], phase0, del_pending, n_ops, new_idx);
+        HC(hipMemcpyAsync(L[i]->ctl, L[i]->hc, sizeof(Ctl), hipMemcpyHostToDevice, L[i]->st));
+        args[i] = engine_args(L[i]);
+    }
+    int left = n;
     uint64_t guard_iter = 0;
-    for (;;) {
-        uint64_t work = host_work(h, g->V);
-        if (h.phase != PH_BUILD_INIT && h.stage == 0 && work <= L->block_max) {
-            launch_engine_block(a, L->block_max, g->st);
-        } else {
-            launch_engine_grid_step(a, h, g->st);
-            if (stats) stats->grid_steps++;
+    while (left) {
+        for (int i = 0; i < n; i++) {
+            if (!active[i]) continue;
+            const Ctl &h = *L[i]->hc;
+            uint64_t work = host_work(h, g->V);
+            if (h.phase != PH_BUILD_INIT && h.stage == 0 && work <= L[i]->block_max) {
+                hipEvent_t ea, eb;
+                prof_begin(g, 1, 1, L[i]->st, &ea, &eb);
+                if (ea) HC(hipEventRecord(ea, L[i]->st));
+                launch_engine_block(args[i], L[i]->block_max, L[i]->st);
+                if (eb) HC(hipEventRecord(eb, L[i]->st));
+            } else {
+                launch_engine_grid_step(args[i], h, L[i]->st);
+                if (stats) stats->grid_steps++;
+            }
         }
         HC(hipGetLastError());
-        HC(hipMemcpyAsync(&h, L->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, g->st));
-        HC(hipStreamSynchronize(g->st));
-        if (h.flags & F_QUEUE_OVERFLOW) throw RgError{RG_EDEVICE, "engine chunk queue overflow"};
-        if (h.phase == PH_DONE) break;
-        if (h.phase == PH_REBUILD) {
-            // deletion fallback (graph.hpp:1252-1321): full recompute
-            h.phase = PH_BUILD_INIT;
-            h.stage = 0;
-            h.del_pending = 0;
-            h.n_cur = h.n_next = h.n_tree = h.n_chunk = 0;
-            h.sel_cur = 0; h.sel_next = 1; h.sel_tree = 2;
-            h.epoch += 1;
-            h.flags = (h.flags & ~F_NEED_REBUILD) | F_ALL_MODIFIED;
-            HC(hipMemcpyAsync(L->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
-            if (stats) stats->rebuilds++;
+        for (int i = 0; i < n; i++) {
+            if (!active[i]) continue;
+            HC(hipStreamSynchronize(L[i]->st));
+            Ctl &h = *L[i]->hc;   // published by the last kernel
+            if (h.flags & F_QUEUE_OVERFLOW) throw RgError{RG_EDEVICE, "engine queue overflow"};
+            if (h.phase == PH_DONE) {
+                active[i] = 0;
+                left--;
+                if (stats) stats->levels += h.levels;
+                HC(hipEventRecord(L[i]->ev_done, L[i]->st));
+            } else if (h.phase == PH_REBUILD) {
+                // deletion fallback (graph.hpp:1252-1321): full recompute
+                h.phase = PH_BUILD_INIT;
+                h.stage = 0;
+                h.del_pending = 0;
+                h.n_cur = h.n_next = h.n_tree = h.n_chunk = 0;
+                h.sel_cur = 0; h.sel_next = 1; h.sel_tree = 2;
+                h.epoch += 1;
+                h.flags = (h.flags & ~F_NEED_REBUILD) | F_ALL_MODIFIED;
+                HC(hipMemcpyAsync(L[i]->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, L[i]->st));
+                if (stats) stats->rebuilds++;
+            }
         }
         if (++guard_iter > (1ull << 26)) throw RgError{RG_EDEVICE, "engine did not converge"};
     }
-    if (stats) stats->levels += h.levels;
-    h.levels = 0;
+}
+
+void engine_run(rg_labels *L, uint32_t phase0, bool del_pending, uint32_t n_ops, bool new_idx,
+                rg_batch_stats *stats)
+{
+    engine_run_multi(&L, 1, phase0, del_pending, n_ops, new_idx, stats);
 }
 
 void labels_free(rg_labels *L)
@@ -401,6 +485,10 @@ void labels_free(rg_labels *L)
     dfree(L->q[0]); dfree(L->q[1]); dfree(L->q[2]); dfree(L->cq); dfree(L->mlist); dfree(L->ctl);
     dfree(L->init_active);
     hfree(L->hc);
+    if (L->ev_done) (void)hipEventDestroy(L->ev_done);
+    if (L->st) (void)hipStreamDestroy(L->st);
+    L->ev_done = nullptr;
+    L->st = nullptr;
 }
 
 }  // namespace
@@ -474,6 +562,11 @@ int rg_graph_destroy(rg_graph *g)
     hfree(g->h_src); hfree(g->h_dst); hfree(g->h_w); hfree(g->h_old); hfree(g->h_kind); hfree(g->h_flag);
     hfree(g->h_status);
     for (auto &e : g->ev) if (e) (void)hipEventDestroy(e);
+    for (auto &r : g->prof_pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto &e : g->ev_free) (void)hipEventDestroy(e);
     if (g->st) (void)hipStreamDestroy(g->st);
     delete g;
     return RG_OK;
@@ -592,6 +685,8 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->hc = halloc<Ctl>(1);
         memset(L->hc, 0, sizeof(Ctl));
         L->hc->mepoch = 1;
+        HC(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
+        HC(hipEventCreate(&L->ev_done));
         HC(hipMemsetAsync(L->inv, 0, V * 4, g->st));
         HC(hipMemsetAsync(L->stamp, 0, V * 4, g->st));
         HC(hipMemsetAsync(L->mstamp, 0, V * 4, g->st));
@@ -640,6 +735,7 @@ int rg_labels_destroy(rg_labels *L)
 {
     if (!L) return RG_OK;
     if (L->g && L->g->st) (void)hipStreamSynchronize(L->g->st);
+    if (L->st) (void)hipStreamSynchronize(L->st);
     labels_free(L);
     delete L;
     return RG_OK;
@@ -909,16 +1005,21 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
     HC(hipEventRecord(g->ev[0], g->st));
     run_store(g);
     HC(hipEventRecord(g->ev[1], g->st));
-    for (int i = 0; i < nL; i++) {
-        uint32_t ph = n_ins ? PH_SEED : PH_DEL_ROOTS;
-        engine_run(L[i], ph, n_ins && n_del, (uint32_t)n, n_del > 0, stats);
-    }
-    HC(hipEventRecord(g->ev[2], g->st));
-    HC(hipEventSynchronize(g->ev[2]));
+    uint32_t ph = n_ins ? PH_SEED : PH_DEL_ROOTS;
+    engine_run_multi(L, nL, ph, n_ins && n_del, (uint32_t)n, n_del > 0, stats);
+    HC(hipEventSynchronize(g->ev[1]));
     store_status_check(g);
+    // device time of the batch: store start to the last tree's last kernel
     float ms = 0;
-    HC(hipEventElapsedTime(&ms, g->ev[0], g->ev[2]));
+    HC(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
+    for (int i = 0; i < nL; i++) {
+        float m = 0;
+        HC(hipEventSynchronize(L[i]->ev_done));
+        HC(hipEventElapsedTime(&m, g->ev[0], L[i]->ev_done));
+        ms = std::max(ms, m);
+    }
     g->last_ms = ms;
+    if (g->prof) prof_collect(g);
     if (stats) {
         stats->seeds += n_ins;
         stats->roots += n_del;
@@ -957,6 +1058,42 @@ int rg_last_gpu_ms(rg_graph *g, double *ms)
     if (!g || !ms) return RG_EINVAL;
     *ms = g->last_ms;
     return RG_OK;
+}
+
+int rg_profile_enable(rg_graph *g, int on)
+{
+    if (!g) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        prof_collect(g);
+        g->prof = on & 3;
+    });
+}
+
+int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint64_t *units)
+{
+    if (!g || kind < 0 || kind > 1) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        prof_collect(g);
+        if (launches) *launches = g->p_launch[kind];
+        if (ms) *ms = g->p_ms[kind];
+        if (units) *units = g->p_units[kind];
+    });
+}
+
+int rg_degrees(rg_graph *g, uint32_t *out_deg, uint32_t *in_deg)
+{
+    if (!g) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (out_deg) HC(hipMemcpyAsync(out_deg, g->lists[0].deg, g->V * 4, hipMemcpyDeviceToHost, g->st));
+        if (in_deg) {
+            const List &l = g->symmetric ? g->lists[0] : g->lists[1];
+            HC(hipMemcpyAsync(in_deg, l.deg, g->V * 4, hipMemcpyDeviceToHost, g->st));
+        }
+        HC(hipStreamSynchronize(g->st));
+    });
 }
 
 static double percentile(std::vector<double> &v, double q)

@@ -521,6 +521,17 @@ __device__ __forceinline__ uint32_t phase_work(const EngineArgs &a, const Snap &
     }
 }
 
+// copy the control block to the pinned host mirror (one thread), so the
+// host reads the state after a stream sync without a copy launch
+__device__ void publish(const EngineArgs &a)
+{
+    if (!a.hctl) return;
+    const uint32_t *src = (const uint32_t *)a.ctl;
+    uint32_t *dst = (uint32_t *)a.hctl;
+    for (uint32_t i = 0; i < sizeof(Ctl) / 4; i++) dst[i] = ald(src + i);
+    __threadfence_system();
+}
+
 // ---------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------
@@ -560,6 +571,8 @@ __global__ __launch_bounds__(1024) void k_block(EngineArgs a, uint32_t block_max
         __syncthreads();
         if (tid == 0) advance(a);
     }
+    __syncthreads();
+    if (tid == 0) publish(a);
 }
 
 template <int A>
@@ -586,7 +599,11 @@ __global__ void k_grid_chunks(EngineArgs a)
     pass_chunks<A>(a, s, tid, nthr, s.n_chunk);
 }
 
-__global__ void k_advance(EngineArgs a) { advance(a); }
+__global__ void k_advance(EngineArgs a)
+{
+    advance(a);
+    publish(a);
+}
 
 template <int A>
 static void block_launch(const EngineArgs &a, uint32_t block_max, hipStream_t st)

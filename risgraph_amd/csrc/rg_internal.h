@@ -162,6 +162,7 @@ struct EngineArgs {
     uint32_t cq_cap;
     uint32_t *mlist;
     Ctl *ctl;
+    Ctl *hctl;                    // pinned host mirror, written at the end of each launch
     EdgeOps ops;
     Tuning tune;
 };
@@ -178,11 +179,14 @@ struct StoreArgs {
     EdgeOps ops;
     ListOps lops;
     uint32_t *err;   // bad vertex ids etc.
+    uint32_t *hstatus;  // pinned host words: err, oom (written after each batch)
     uint32_t *cnt_tmp[2];  // per-vertex insert counts for bulk reservation
 };
 
 // host-side launchers (store.hip / engine.hip)
-void launch_store_batch(const StoreArgs &a, hipStream_t s);
+// ev_a / ev_b (may be null) are recorded around the apply kernel
+void launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a = nullptr,
+                        hipEvent_t ev_b = nullptr);
 void launch_store_count(const StoreArgs &a, hipStream_t s);
 void launch_store_reserve(const StoreArgs &a, hipStream_t s);
 void launch_pool_recycle(const StoreArgs &a, hipStream_t s);

@@ -346,7 +346,7 @@ __global__ void k_apply(StoreArgs a)
     }
 }
 
-void launch_store_batch(const StoreArgs &a, hipStream_t s)
+void launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b)
 {
     if (a.ops.n == 0) return;
     uint32_t n = a.ops.n;
@@ -354,7 +354,9 @@ void launch_store_batch(const StoreArgs &a, hipStream_t s)
     uint64_t waves = 4ull * n;
     uint64_t blocks = (waves * WAVE + 255) / 256;
     if (blocks > 65536) blocks = 65536;
+    if (ev_a) (void)hipEventRecord(ev_a, s);
     hipLaunchKernelGGL(k_apply, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    if (ev_b) (void)hipEventRecord(ev_b, s);
 }
 
 // ---------------------------------------------------------------------
@@ -451,6 +453,12 @@ __global__ void k_pool_recycle(StoreArgs a)
         p->free_take[h][c] = 0;
         p->pend_n[h][c] = 0;
         if (mv) atomicAdd(&p->recycled[h], (unsigned long long)mv);
+    }
+    // publish the batch status (bad ids, pool exhaustion) to pinned host memory
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.hstatus) {
+        a.hstatus[0] = __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a.hstatus[1] = __hip_atomic_load(&p->oom, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence_system();
     }
 }
 
