@@ -166,6 +166,12 @@ int rg_last_gpu_ms(rg_graph *g, double *ms);
 int rg_profile_enable(rg_graph *g, int mask);
 int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint64_t *units);
 
+/* Per-phase device time of single-workgroup engine passes on one labels
+ * instance since its creation: us[p] microseconds and passes[p] for phase p
+ * (0 done, 1 seed, 2 push, 3 delete roots, 4 rescue, 5 expand, 6 reset,
+ * 7 pull, 8 build init, 9 rebuild); arrays of 10. */
+int rg_labels_phase_time(rg_labels *L, double *us, uint64_t *passes);
+
 /* Live multiplicity degree of every vertex (out, and in for directed
  * graphs; either pointer may be NULL), the `degree` of IndexedEdgeStorage
  * (storage.hpp:396). */

@@ -100,7 +100,10 @@ SIGNATURES = {
     "rg_profile_enable": (ci, [vp, ci]),
     "rg_profile_read": (ci, [vp, ci, u64p, ctypes.POINTER(ctypes.c_double), u64p]),
     "rg_degrees": (ci, [vp, vp, vp]),
+    "rg_labels_phase_time": (ci, [vp, vp, vp]),
 }
+
+PHASES = ["done", "seed", "push", "del_roots", "rescue", "expand", "reset", "pull", "build_init", "rebuild"]
 
 _lib = None
 
@@ -354,6 +357,13 @@ class Labels:
         p = np.empty(count, np.uint64)
         _check(load_library().rg_labels_read(self.h, first, count, _ptr(v), _ptr(p)))
         return v, p
+
+    def phase_time(self):
+        """Block-mode engine time per phase: {phase: (microseconds, passes)}."""
+        us = np.zeros(10, np.float64)
+        n = np.zeros(10, np.uint64)
+        _check(load_library().rg_labels_phase_time(self.h, _ptr(us), _ptr(n)))
+        return {PHASES[p]: (float(us[p]), int(n[p])) for p in range(10) if n[p]}
 
     def get_modified(self):
         n = u64()

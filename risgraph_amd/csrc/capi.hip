@@ -143,6 +143,7 @@ struct rg_labels {
     uint32_t *mlist = nullptr;
     Ctl *ctl = nullptr;
     Ctl *hc = nullptr;     // pinned mirror
+    unsigned long long *ticks = nullptr;   // per-phase block-mode time (device)
     hipStream_t st = nullptr;      // this tree's engine stream
     hipEvent_t ev_done = nullptr;  // end of the last engine run
     Tuning tune{};
@@ -373,6 +374,7 @@ EngineArgs engine_args(rg_labels *L)
     a.mlist = L->mlist;
     a.ctl = L->ctl;
     a.hctl = L->hc;
+    a.ticks = L->ticks;
     a.ops = g->ops;
     a.tune = L->tune;
     return a;
@@ -382,7 +384,7 @@ uint64_t host_work(const Ctl &h, uint64_t V)
 {
     switch (h.phase) {
     case PH_SEED:
-    case PH_DEL_ROOTS: return h.n_ops;
+    case PH_DEL_ROOTS: return h.n_ops / 16;   // same units as the device's phase_work
     case PH_DEL_RESET:
     case PH_DEL_PULL: return h.n_tree;
     case PH_BUILD_INIT: return V;
@@ -420,11 +422,15 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool del_pending, u
     rg_graph *g = L[0]->g;
     HC(hipEventRecord(g->ev[3], g->st));
     std::vector<EngineArgs> args(n);
-    std::vector<char> active(n, 1);
+    std::vector<char> active(n, 1), pending_init(n, 0);
     for (int i = 0; i < n; i++) {
         HC(hipStreamWaitEvent(L[i]->st, g->ev[3], 0));
         engine_init(L[i], phase0, del_pending, n_ops, new_idx);
-        HC(hipMemcpyAsync(L[i]->ctl, L[i]->hc, sizeof(Ctl), hipMemcpyHostToDevice, L[i]->st));
+        const Ctl &h = *L[i]->hc;
+        // a run that starts in block mode hands its initial control block to
+        // the kernel through the pinned mirror; otherwise copy it now
+        if (h.phase != PH_BUILD_INIT && host_work(h, g->V) <= L[i]->block_max) pending_init[i] = 1;
+        else HC(hipMemcpyAsync(L[i]->ctl, L[i]->hc, sizeof(Ctl), hipMemcpyHostToDevice, L[i]->st));
         args[i] = engine_args(L[i]);
     }
     int left = n;
@@ -438,7 +444,8 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool del_pending, u
                 hipEvent_t ea, eb;
                 prof_begin(g, 1, 1, L[i]->st, &ea, &eb);
                 if (ea) HC(hipEventRecord(ea, L[i]->st));
-                launch_engine_block(args[i], L[i]->block_max, L[i]->st);
+                launch_engine_block(args[i], L[i]->block_max, pending_init[i], L[i]->st);
+                pending_init[i] = 0;
                 if (eb) HC(hipEventRecord(eb, L[i]->st));
             } else {
                 launch_engine_grid_step(args[i], h, L[i]->st);
@@ -484,6 +491,7 @@ void labels_free(rg_labels *L)
     dfree(L->val); dfree(L->init); dfree(L->inv); dfree(L->stamp); dfree(L->mstamp);
     dfree(L->q[0]); dfree(L->q[1]); dfree(L->q[2]); dfree(L->cq); dfree(L->mlist); dfree(L->ctl);
     dfree(L->init_active);
+    dfree(L->ticks);
     hfree(L->hc);
     if (L->ev_done) (void)hipEventDestroy(L->ev_done);
     if (L->st) (void)hipStreamDestroy(L->st);
@@ -682,6 +690,8 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->cq = dalloc<uint64_t>(L->cq_cap);
         L->mlist = dalloc<uint32_t>(V);
         L->ctl = dalloc<Ctl>(1);
+        L->ticks = dalloc<unsigned long long>(2 * NPHASE);
+        HC(hipMemsetAsync(L->ticks, 0, 2 * NPHASE * 8, g->st));
         L->hc = halloc<Ctl>(1);
         memset(L->hc, 0, sizeof(Ctl));
         L->hc->mepoch = 1;
@@ -1079,6 +1089,27 @@ int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint6
         if (launches) *launches = g->p_launch[kind];
         if (ms) *ms = g->p_ms[kind];
         if (units) *units = g->p_units[kind];
+    });
+}
+
+int rg_labels_phase_time(rg_labels *L, double *us, uint64_t *passes)
+{
+    if (!L) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        unsigned long long t[2 * NPHASE];
+        HC(hipMemcpyAsync(t, L->ticks, sizeof(t), hipMemcpyDeviceToHost, g->st));
+        HC(hipStreamSynchronize(g->st));
+        int khz = 0;
+        int dev = 0;
+        HC(hipGetDevice(&dev));
+        HC(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        double per_us = khz > 0 ? khz / 1000.0 : 100.0;
+        for (int p = 0; p < NPHASE; p++) {
+            if (us) us[p] = (double)t[p] / per_us;
+            if (passes) passes[p] = t[NPHASE + p];
+        }
     });
 }
 

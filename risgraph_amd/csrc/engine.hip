@@ -75,11 +75,12 @@ struct Snap {
     uint32_t phase, stage, level, n_cur, n_tree, n_chunk, n_ops;
     uint32_t sel_cur, sel_next, sel_tree, epoch, mepoch;
     int32_t idx;
+    Ctl *c;   // live control block: LDS copy in block mode, global in grid mode
 };
 
-__device__ __forceinline__ void take_snap(const EngineArgs &a, Snap &s)
+__device__ __forceinline__ void take_snap(Ctl *c, Snap &s)
 {
-    Ctl *c = a.ctl;
+    s.c = c;
     s.phase = ald(&c->phase);
     s.stage = ald(&c->stage);
     s.level = ald(&c->level);
@@ -111,32 +112,32 @@ __device__ __forceinline__ bool relax(uint64_t *val, uint32_t dst, uint64_t c, u
 }
 
 // bounded append to a vertex queue (capacity V; dedupe keeps it in range)
-__device__ __forceinline__ void q_append(const EngineArgs &a, uint32_t *q, uint32_t *count, uint32_t v)
+__device__ __forceinline__ void q_append(const EngineArgs &a, Ctl *c, uint32_t *q, uint32_t *count, uint32_t v)
 {
     uint32_t pos = atomicAdd(count, 1u);
     if (pos < a.V) q[pos] = v;
-    else atomicOr(&a.ctl->flags, F_QUEUE_OVERFLOW);
+    else atomicOr(&c->flags, F_QUEUE_OVERFLOW);
 }
 
 __device__ __forceinline__ void push_next(const EngineArgs &a, const Snap &s, uint32_t v)
 {
-    if (atomicExch(&a.stamp[v], s.epoch) != s.epoch) q_append(a, a.q[s.sel_next], &a.ctl->n_next, v);
+    if (atomicExch(&a.stamp[v], s.epoch) != s.epoch) q_append(a, s.c, a.q[s.sel_next], &s.c->n_next, v);
 }
 
 __device__ __forceinline__ void mark_mod(const EngineArgs &a, const Snap &s, uint32_t v)
 {
     if (a.trace && atomicExch(&a.mstamp[v], s.mepoch) != s.mepoch) {
-        uint32_t pos = atomicAdd(&a.ctl->n_mod, 1u);
+        uint32_t pos = atomicAdd(&s.c->n_mod, 1u);
         if (pos < a.V) a.mlist[pos] = v;
     }
 }
 
-__device__ __forceinline__ void defer_chunks(const EngineArgs &a, uint32_t v, int L, uint32_t len)
+__device__ __forceinline__ void defer_chunks(const EngineArgs &a, const Snap &s, uint32_t v, int L, uint32_t len)
 {
     uint32_t nch = (len + CHUNK - 1) / CHUNK;
-    uint32_t pos = atomicAdd(&a.ctl->n_chunk, nch);
+    uint32_t pos = atomicAdd(&s.c->n_chunk, nch);
     if (pos + nch > a.cq_cap) {
-        atomicOr(&a.ctl->flags, F_QUEUE_OVERFLOW);
+        atomicOr(&s.c->flags, F_QUEUE_OVERFLOW);
         return;
     }
     for (uint32_t k = 0; k < nch; k++)
@@ -146,7 +147,7 @@ __device__ __forceinline__ void defer_chunks(const EngineArgs &a, uint32_t v, in
 // has_parents (graph.hpp:1889-1907): true when an ancestor of x (x
 // included) carries the current invalidation mark; a walk longer than the cap
 // asks for a full rebuild, like the reference's 0.1*V guard.
-__device__ bool has_parents(const EngineArgs &a, uint32_t x, int32_t idx)
+__device__ bool has_parents(const EngineArgs &a, const Snap &s, uint32_t x, int32_t idx)
 {
     uint32_t steps = 0;
     for (;;) {
@@ -156,7 +157,7 @@ __device__ bool has_parents(const EngineArgs &a, uint32_t x, int32_t idx)
         if (p == NILP) return false;
         x = p;
         if (++steps > a.tune.walk_limit) {
-            atomicOr(&a.ctl->flags, F_NEED_REBUILD);
+            atomicOr(&s.c->flags, F_NEED_REBUILD);
             return true;
         }
     }
@@ -201,7 +202,7 @@ __device__ void rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uin
         uint32_t w;
         entry_w<A>(a, off + i, w);
         if (cand<A>(vval(a.val[n]), w) != (uint64_t)dv) continue;
-        if (has_parents(a, n, s.idx)) continue;
+        if (has_parents(a, s, n, s.idx)) continue;
         if (atomicCAS(&a.inv[d], s.idx, -s.idx) == s.idx) {
             a.val[d] = pack(dv, n);
             break;
@@ -217,7 +218,7 @@ __device__ void expand_range(const EngineArgs &a, const Snap &s, uint32_t x, uin
         uint32_t c = a.slots.nbr[off + i];
         if (c & DEAD) continue;
         if (vpar(a.val[c]) != x) continue;
-        if (atomicCAS(&a.inv[c], 0, s.idx) == 0) q_append(a, qn, &a.ctl->n_next, c);
+        if (atomicCAS(&a.inv[c], 0, s.idx) == 0) q_append(a, s.c, qn, &s.c->n_next, c);
     }
 }
 
@@ -274,7 +275,7 @@ __device__ void seed_edge(const EngineArgs &a, const Snap &s, uint32_t src, uint
 __device__ __forceinline__ void mark_root(const EngineArgs &a, const Snap &s, uint32_t src, uint32_t dst)
 {
     if (vpar(a.val[dst]) != src) return;
-    if (atomicCAS(&a.inv[dst], 0, s.idx) == 0) q_append(a, a.q[s.sel_next], &a.ctl->n_next, dst);
+    if (atomicCAS(&a.inv[dst], 0, s.idx) == 0) q_append(a, s.c, a.q[s.sel_next], &s.c->n_next, dst);
 }
 
 template <int A>
@@ -339,7 +340,7 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
                 int L = a.push_l[li];
                 uint32_t len = a.lists[L].len[v];
                 if (len > LONG_LIST) {
-                    if (glane == 0) defer_chunks(a, v, L, len);
+                    if (glane == 0) defer_chunks(a, s, v, L, len);
                     continue;
                 }
                 push_range<A>(a, s, v, uv, a.lists[L].off[v], 0, len, glane, GROUP);
@@ -351,7 +352,7 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
                 int L = a.pull_l[li];
                 uint32_t len = a.lists[L].len[v];
                 if (len > LONG_LIST) {
-                    if (glane == 0) defer_chunks(a, v, L, len);
+                    if (glane == 0) defer_chunks(a, s, v, L, len);
                     continue;
                 }
                 rescue_range<A>(a, s, v, dv, a.lists[L].off[v], 0, len, glane, GROUP);
@@ -363,12 +364,12 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
                 continue;
             }
             if (iv != s.idx) continue;
-            if (glane == 0) q_append(a, a.q[s.sel_tree], &a.ctl->n_tree, v);
+            if (glane == 0) q_append(a, s.c, a.q[s.sel_tree], &s.c->n_tree, v);
             for (int li = 0; li < a.npush; li++) {
                 int L = a.push_l[li];
                 uint32_t len = a.lists[L].len[v];
                 if (len > LONG_LIST) {
-                    if (glane == 0) defer_chunks(a, v, L, len);
+                    if (glane == 0) defer_chunks(a, s, v, L, len);
                     continue;
                 }
                 expand_range(a, s, v, a.lists[L].off[v], 0, len, glane, GROUP);
@@ -379,7 +380,7 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
                 int L = a.pull_l[li];
                 uint32_t len = a.lists[L].len[v];
                 if (len > LONG_LIST) {
-                    if (glane == 0) defer_chunks(a, v, L, len);
+                    if (glane == 0) defer_chunks(a, s, v, L, len);
                     continue;
                 }
                 uint64_t b = pull_range<A>(a, a.lists[L].off[v], 0, len, glane, GROUP);
@@ -422,9 +423,8 @@ __device__ void pass_chunks(const EngineArgs &a, const Snap &s, uint32_t tid, ui
 }
 
 // phase transitions, run by one thread after a pass completed
-__device__ void advance(const EngineArgs &a)
+__device__ void advance(const EngineArgs &a, Ctl *c)
 {
-    Ctl *c = a.ctl;
     uint32_t ph = ald(&c->phase);
     auto swap_queues = [&]() {
         uint32_t cur = ald(&c->sel_cur), nxt = ald(&c->sel_next);
@@ -509,11 +509,13 @@ __device__ void advance(const EngineArgs &a)
     }
 }
 
+// work of the current pass, in units comparable with block_max; an op pass
+// (one thread per op, no list scans) counts 1/16 per op
 __device__ __forceinline__ uint32_t phase_work(const EngineArgs &a, const Snap &s)
 {
     switch (s.phase) {
     case PH_SEED:
-    case PH_DEL_ROOTS: return s.n_ops;
+    case PH_DEL_ROOTS: return s.n_ops / 16;
     case PH_DEL_RESET:
     case PH_DEL_PULL: return s.n_tree;
     case PH_BUILD_INIT: return a.V;
@@ -523,10 +525,10 @@ __device__ __forceinline__ uint32_t phase_work(const EngineArgs &a, const Snap &
 
 // copy the control block to the pinned host mirror (one thread), so the
 // host reads the state after a stream sync without a copy launch
-__device__ void publish(const EngineArgs &a)
+__device__ void publish(const EngineArgs &a, const Ctl *c)
 {
     if (!a.hctl) return;
-    const uint32_t *src = (const uint32_t *)a.ctl;
+    const uint32_t *src = (const uint32_t *)c;
     uint32_t *dst = (uint32_t *)a.hctl;
     for (uint32_t i = 0; i < sizeof(Ctl) / 4; i++) dst[i] = ald(src + i);
     __threadfence_system();
@@ -536,50 +538,74 @@ __device__ void publish(const EngineArgs &a)
 // kernels
 // ---------------------------------------------------------------------
 template <int A>
-__global__ __launch_bounds__(1024) void k_block(EngineArgs a, uint32_t block_max, uint32_t chunk_max)
+__global__ __launch_bounds__(1024) void k_block(EngineArgs a, uint32_t block_max, uint32_t chunk_max, int do_init)
 {
+    // the control block lives in LDS for the whole launch: queue counters and
+    // phase transitions are workgroup-local atomics instead of device-scope
+    // round trips; it goes back to HBM (and the pinned mirror) at exit
+    __shared__ Ctl sctl;
     __shared__ Snap snap;
     __shared__ uint32_t nch;
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
+    if (tid == 0) {
+        // a run's first launch takes its initial control block from the
+        // pinned host mirror (no separate copy launch)
+        const uint32_t *src = (const uint32_t *)(do_init ? a.hctl : a.ctl);
+        uint32_t *dst = (uint32_t *)&sctl;
+        for (uint32_t i = 0; i < sizeof(Ctl) / 4; i++) dst[i] = ald(src + i);
+    }
     // every pass either finishes the phase or hands it to grid mode; the cap
     // only bounds the launch if the state machine were ever corrupted
     for (uint32_t iter = 0; iter < (1u << 24); iter++) {
         __threadfence_block();
         __syncthreads();
-        if (tid == 0) take_snap(a, snap);
+        if (tid == 0) take_snap(&sctl, snap);
         __syncthreads();
         const Snap s = snap;
         if (s.phase == PH_DONE || s.phase == PH_REBUILD) break;
         if (phase_work(a, s) > block_max) break;
+        const uint64_t t_pass = wall_clock64();
         if (s.stage == 0) {
             if (s.phase == PH_SEED || s.phase == PH_DEL_ROOTS) pass_ops<A>(a, s, tid, nthr);
             else pass_vertices<A>(a, s, tid, nthr);
             __threadfence_block();
             __syncthreads();
-            if (tid == 0) nch = ald(&a.ctl->n_chunk);
+            if (tid == 0) nch = sctl.n_chunk;
             __syncthreads();
             if (nch > chunk_max) {
-                if (tid == 0) ast(&a.ctl->stage, 1u);
+                if (tid == 0) sctl.stage = 1u;
                 break;
             }
         } else {
-            if (tid == 0) nch = ald(&a.ctl->n_chunk);
+            if (tid == 0) nch = sctl.n_chunk;
             __syncthreads();
         }
         pass_chunks<A>(a, s, tid, nthr, nch);
         __threadfence_block();
         __syncthreads();
-        if (tid == 0) advance(a);
+        if (tid == 0) {
+            if (a.ticks && s.phase < NPHASE) {
+                a.ticks[s.phase] += wall_clock64() - t_pass;
+                a.ticks[NPHASE + s.phase] += 1;
+            }
+            advance(a, &sctl);
+        }
     }
+    __threadfence_block();
     __syncthreads();
-    if (tid == 0) publish(a);
+    if (tid == 0) {
+        const uint32_t *src = (const uint32_t *)&sctl;
+        uint32_t *dst = (uint32_t *)a.ctl;
+        for (uint32_t i = 0; i < sizeof(Ctl) / 4; i++) ast(dst + i, src[i]);
+        publish(a, &sctl);
+    }
 }
 
 template <int A>
 __global__ void k_grid_main(EngineArgs a)
 {
     __shared__ Snap snap;
-    if (threadIdx.x == 0) take_snap(a, snap);
+    if (threadIdx.x == 0) take_snap(a.ctl, snap);
     __syncthreads();
     const Snap s = snap;
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthr = gridDim.x * blockDim.x;
@@ -592,7 +618,7 @@ template <int A>
 __global__ void k_grid_chunks(EngineArgs a)
 {
     __shared__ Snap snap;
-    if (threadIdx.x == 0) take_snap(a, snap);
+    if (threadIdx.x == 0) take_snap(a.ctl, snap);
     __syncthreads();
     const Snap s = snap;
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthr = gridDim.x * blockDim.x;
@@ -601,23 +627,23 @@ __global__ void k_grid_chunks(EngineArgs a)
 
 __global__ void k_advance(EngineArgs a)
 {
-    advance(a);
-    publish(a);
+    advance(a, a.ctl);
+    publish(a, a.ctl);
 }
 
 template <int A>
-static void block_launch(const EngineArgs &a, uint32_t block_max, hipStream_t st)
+static void block_launch(const EngineArgs &a, uint32_t block_max, int do_init, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_block<A>, dim3(1), dim3(1024), 0, st, a, block_max, 64u);
+    hipLaunchKernelGGL(k_block<A>, dim3(1), dim3(1024), 0, st, a, block_max, 64u, do_init);
 }
 
-void launch_engine_block(const EngineArgs &a, uint32_t block_max, hipStream_t st)
+void launch_engine_block(const EngineArgs &a, uint32_t block_max, int do_init, hipStream_t st)
 {
     switch (a.algo) {
-    case BFS: block_launch<BFS>(a, block_max, st); break;
-    case SSSP: block_launch<SSSP>(a, block_max, st); break;
-    case SSWP: block_launch<SSWP>(a, block_max, st); break;
-    default: block_launch<WCC>(a, block_max, st); break;
+    case BFS: block_launch<BFS>(a, block_max, do_init, st); break;
+    case SSSP: block_launch<SSSP>(a, block_max, do_init, st); break;
+    case SSWP: block_launch<SSWP>(a, block_max, do_init, st); break;
+    default: block_launch<WCC>(a, block_max, do_init, st); break;
     }
 }
 

@@ -117,6 +117,8 @@ enum Phase : uint32_t {
     PH_REBUILD = 9,
 };
 
+constexpr int NPHASE = 10;
+
 constexpr uint32_t F_NEED_REBUILD = 1;
 constexpr uint32_t F_QUEUE_OVERFLOW = 2;
 constexpr uint32_t F_ALL_MODIFIED = 4;
@@ -163,6 +165,7 @@ struct EngineArgs {
     uint32_t *mlist;
     Ctl *ctl;
     Ctl *hctl;                    // pinned host mirror, written at the end of each launch
+    unsigned long long *ticks;    // [2*NPHASE] per-phase wall-clock ticks and passes (block mode)
     EdgeOps ops;
     Tuning tune;
 };
@@ -190,7 +193,8 @@ void launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a = nul
 void launch_store_count(const StoreArgs &a, hipStream_t s);
 void launch_store_reserve(const StoreArgs &a, hipStream_t s);
 void launch_pool_recycle(const StoreArgs &a, hipStream_t s);
-void launch_engine_block(const EngineArgs &a, uint32_t block_max, hipStream_t s);
+// do_init: the kernel first copies the control block from the pinned mirror
+void launch_engine_block(const EngineArgs &a, uint32_t block_max, int do_init, hipStream_t s);
 void launch_engine_grid_step(const EngineArgs &a, const Ctl &host_ctl, hipStream_t s);
 void launch_engine_fill_init(const EngineArgs &a, hipStream_t s);
 void launch_gen_rmat(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, int scale,

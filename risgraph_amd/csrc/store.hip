@@ -139,20 +139,43 @@ __device__ void pool_free(const StoreArgs &a, int h, uint64_t off, uint64_t n)
     if (k < FCAP) a.pend_off[((uint64_t)h * NCLS + c) * FCAP + k] = off;
 }
 
-// wave-parallel (re)build of a list's hash index at capacity `hcap`
-__device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, int lane)
+// A list is owned by a group of G lanes (G = 16 in the apply and reserve
+// kernels, so one wave serves four lists).  Group control flow is uniform;
+// group ballots are the group's bits of the wave ballot.
+struct Grp {
+    int lane;   // lane within the group
+    int base;   // first wave lane of the group
+};
+
+template <int G>
+__device__ __forceinline__ uint32_t gballot(bool p, const Grp &g)
+{
+    uint64_t b = __ballot(p);
+    if constexpr (G == 64) return (uint32_t)b;   // only used with G <= 32
+    else return (uint32_t)((b >> g.base) & ((1ull << G) - 1));
+}
+
+template <int G>
+__device__ __forceinline__ uint64_t gbcast(uint64_t x, const Grp &g)
+{
+    return __shfl((unsigned long long)x, 0, G);
+}
+
+// group-parallel (re)build of a list's hash index at capacity `hcap`
+template <int G>
+__device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, const Grp &g)
 {
     uint64_t h0 = 0;
-    if (lane == 0) {
+    if (g.lane == 0) {
         h0 = pool_alloc(a, 1, hcap);
         if (h0 != HEMPTY && st.hoff != HEMPTY) pool_free(a, 1, st.hoff, st.hcap);
     }
-    h0 = __shfl(h0, 0);
+    h0 = gbcast<G>(h0, g);
     if (h0 == HEMPTY) return false;
-    for (uint32_t b = lane; b < hcap; b += WAVE) a.slots.hkey[h0 + b] = HEMPTY;
+    for (uint32_t b = g.lane; b < hcap; b += G) a.slots.hkey[h0 + b] = HEMPTY;
     __threadfence_block();
     uint32_t mask = hcap - 1;
-    for (uint32_t i = lane; i < st.len; i += WAVE) {
+    for (uint32_t i = g.lane; i < st.len; i += G) {
         uint32_t nb = a.slots.nbr[st.off + i];
         uint32_t w = a.slots.wt ? a.slots.wt[st.off + i] : 0;
         uint64_t key = hkey_of(nb, w);
@@ -175,21 +198,22 @@ __device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, in
 
 // move a list into a block of capacity next_pow2(need) (>= 4); the old block
 // is returned to the slab free lists
-__device__ bool grow_block(const StoreArgs &a, ListState &st, uint32_t need, int lane)
+template <int G>
+__device__ bool grow_block(const StoreArgs &a, ListState &st, uint32_t need, const Grp &g)
 {
     uint32_t ncap = next_pow2(need);
     if (ncap < 4) ncap = 4;
     uint64_t o = 0;
-    if (lane == 0) o = pool_alloc(a, 0, ncap);
-    o = __shfl(o, 0);
+    if (g.lane == 0) o = pool_alloc(a, 0, ncap);
+    o = gbcast<G>(o, g);
     if (o == HEMPTY) return false;
-    for (uint32_t i = lane; i < st.len; i += WAVE) {
+    for (uint32_t i = g.lane; i < st.len; i += G) {
         a.slots.nbr[o + i] = a.slots.nbr[st.off + i];
         a.slots.cnt[o + i] = a.slots.cnt[st.off + i];
         if (a.slots.wt) a.slots.wt[o + i] = a.slots.wt[st.off + i];
     }
     __threadfence_block();
-    if (lane == 0 && st.cap) pool_free(a, 0, st.off, st.cap);
+    if (g.lane == 0 && st.cap) pool_free(a, 0, st.off, st.cap);
     st.off = o;
     st.cap = ncap;
     return true;
@@ -197,25 +221,24 @@ __device__ bool grow_block(const StoreArgs &a, ListState &st, uint32_t need, int
 
 // find key in the list: local index or NILP.  With `ins_bucket`, returns the
 // empty hash bucket where the key would go (hashed lists only).
+template <int G>
 __device__ uint32_t list_find(const StoreArgs &a, const ListState &st, uint32_t key_nbr, uint32_t key_w,
-                              int lane, uint32_t *ins_bucket)
+                              const Grp &g, uint32_t *ins_bucket)
 {
     if (st.hoff != HEMPTY) {
         uint64_t key = hkey_of(key_nbr, key_w);
         uint32_t mask = st.hcap - 1;
         uint32_t h = (uint32_t)mix64(key) & mask;
-        for (uint32_t probe = 0; probe < st.hcap; probe += WAVE) {
-            uint32_t b = (h + probe + lane) & mask;
+        for (uint32_t probe = 0; probe < st.hcap; probe += G) {
+            uint32_t b = (h + probe + g.lane) & mask;
             uint64_t k = a.slots.hkey[st.hoff + b];
-            uint64_t hit = __ballot(k == key);
-            uint64_t emp = __ballot(k == HEMPTY);
-            uint64_t any = hit | emp;
+            uint32_t hit = gballot<G>(k == key, g);
+            uint32_t emp = gballot<G>(k == HEMPTY, g);
+            uint32_t any = hit | emp;
             if (any) {
-                int first = __ffsll((unsigned long long)any) - 1;
+                int first = __ffs(any) - 1;
                 uint32_t fb = (h + probe + first) & mask;
-                if ((hit >> first) & 1ull) {
-                    return a.slots.hval[st.hoff + fb];
-                }
+                if ((hit >> first) & 1u) return a.slots.hval[st.hoff + fb];
                 if (ins_bucket) *ins_bucket = fb;
                 return NILP;
             }
@@ -223,27 +246,32 @@ __device__ uint32_t list_find(const StoreArgs &a, const ListState &st, uint32_t 
         if (ins_bucket) *ins_bucket = NILP;
         return NILP;
     }
-    for (uint32_t b = 0; b < st.len; b += WAVE) {
-        uint32_t i = b + lane;
+    for (uint32_t b = 0; b < st.len; b += G) {
+        uint32_t i = b + g.lane;
         bool m = false;
         if (i < st.len) {
             uint32_t nb = a.slots.nbr[st.off + i];
             m = (nb & NBR_MASK) == key_nbr;
             if (m && a.slots.wt) m = a.slots.wt[st.off + i] == key_w;
         }
-        uint64_t hit = __ballot(m);
-        if (hit) return b + (uint32_t)(__ffsll((unsigned long long)hit) - 1);
+        uint32_t hit = gballot<G>(m, g);
+        if (hit) return b + (uint32_t)(__ffs(hit) - 1);
     }
     return NILP;
 }
 
+constexpr int AG = 16;   // lanes per list in the apply / reserve kernels
+
 __global__ void k_apply(StoreArgs a)
 {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    const uint32_t nw = (gridDim.x * blockDim.x) / WAVE;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    Grp g;
+    g.lane = threadIdx.x % AG;
+    g.base = (threadIdx.x % WAVE) - g.lane;
+    const uint32_t gid = t / AG;
+    const uint32_t ng = (gridDim.x * blockDim.x) / AG;
     const uint32_t nlops = 4 * a.ops.n;
-    for (uint32_t j = wid; j < nlops; j += nw) {
+    for (uint32_t j = gid; j < nlops; j += ng) {
         uint32_t v = a.lops.v[j];
         if (v == NILP) continue;
         int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
@@ -267,8 +295,8 @@ __global__ void k_apply(StoreArgs a)
             if (a.lops.delta[k] > 0) n_ins++;
             if (++clen > nlops) { ok = false; break; }
         }
-        if (!ok) atomicOr(a.err, 2u);
-        if (ok && st.len + n_ins > st.cap) ok = grow_block(a, st, st.len + n_ins, lane);
+        if (!ok && g.lane == 0) atomicOr(a.err, 2u);
+        if (ok && st.len + n_ins > st.cap) ok = grow_block<AG>(a, st, st.len + n_ins, g);
 
         for (int pass = 0; pass < 2 && ok; pass++) {
             for (int32_t k = (int32_t)j; k >= 0 && (uint32_t)k < nlops; k = a.lops.next[k]) {
@@ -277,11 +305,11 @@ __global__ void k_apply(StoreArgs a)
                 uint32_t key_nbr = a.lops.nbr[k];
                 uint32_t key_w = a.slots.wt ? a.lops.w[k] : 0;
                 uint32_t bucket = NILP;
-                uint32_t idx = list_find(a, st, key_nbr, key_w, lane, &bucket);
+                uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, &bucket);
                 if (idx == NILP && delta > 0) {
                     // append a new entry with count 0 (storage.hpp:371-374, 389-392)
                     idx = st.len;
-                    if (lane == 0) {
+                    if (g.lane == 0) {
                         a.slots.nbr[st.off + idx] = key_nbr | DEAD;
                         a.slots.cnt[st.off + idx] = 0;
                         if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
@@ -290,9 +318,9 @@ __global__ void k_apply(StoreArgs a)
                     __threadfence_block();
                     if (st.hoff != HEMPTY) {
                         if (2 * (st.hcnt + 1) > st.hcap) {
-                            if (!build_index(a, st, st.hcap * 2, lane)) { ok = false; break; }
+                            if (!build_index<AG>(a, st, st.hcap * 2, g)) { ok = false; break; }
                         } else {
-                            if (lane == 0) {
+                            if (g.lane == 0) {
                                 a.slots.hkey[st.hoff + bucket] = hkey_of(key_nbr, key_w);
                                 a.slots.hval[st.hoff + bucket] = idx;
                             }
@@ -302,7 +330,7 @@ __global__ void k_apply(StoreArgs a)
                     } else if (st.len >= INDEX_THRESHOLD) {
                         uint32_t hc = next_pow2(2 * st.len);
                         if (hc < 1024) hc = 1024;
-                        if (!build_index(a, st, hc, lane)) { ok = false; break; }
+                        if (!build_index<AG>(a, st, hc, g)) { ok = false; break; }
                     }
                 }
                 uint32_t c = idx == NILP ? 0 : a.slots.cnt[st.off + idx];
@@ -318,14 +346,14 @@ __global__ void k_apply(StoreArgs a)
                     dchange = -(int32_t)(c >= 2 ? 2 : c);
                     break;
                 }
-                if (idx != NILP && lane == 0) {
+                if (idx != NILP && g.lane == 0) {
                     a.slots.cnt[st.off + idx] = nc;
                     if ((c == 0) != (nc == 0))
                         a.slots.nbr[st.off + idx] = key_nbr | (nc ? 0u : DEAD);
                 }
                 st.deg = (uint32_t)((int32_t)st.deg + dchange);
                 uint8_t fl = a.lops.flags[k];
-                if (lane == 0) {
+                if (g.lane == 0) {
                     uint32_t e = a.lops.edge[k];
                     if (fl & L_RET) a.ops.old[e] = ret;
                     if (fl & L_SLOT) a.ops.slot[e] = idx;
@@ -333,7 +361,7 @@ __global__ void k_apply(StoreArgs a)
                 __threadfence_block();
             }
         }
-        if (lane == 0) {
+        if (g.lane == 0) {
             li.off[v] = st.off;
             li.len[v] = st.len;
             li.cap[v] = st.cap;
@@ -351,8 +379,8 @@ void launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipE
     if (a.ops.n == 0) return;
     uint32_t n = a.ops.n;
     hipLaunchKernelGGL(k_expand_link, dim3((n + 255) / 256), dim3(256), 0, s, a);
-    uint64_t waves = 4ull * n;
-    uint64_t blocks = (waves * WAVE + 255) / 256;
+    uint64_t groups = 4ull * n;
+    uint64_t blocks = (groups * AG + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (ev_a) (void)hipEventRecord(ev_a, s);
     hipLaunchKernelGGL(k_apply, dim3((uint32_t)blocks), dim3(256), 0, s, a);
@@ -382,9 +410,12 @@ __global__ void k_count(StoreArgs a)
 
 __global__ void k_reserve(StoreArgs a)
 {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-    const uint32_t nw = (gridDim.x * blockDim.x) / WAVE;
+    Grp g;
+    g.lane = threadIdx.x % AG;
+    g.base = (threadIdx.x % WAVE) - g.lane;
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / AG;
+    const uint32_t nw = (gridDim.x * blockDim.x) / AG;
+    const int lane = g.lane;
     const int nlists = a.symmetric ? 1 : 2;
     for (uint32_t v = wid; v < a.V; v += nw) {
         for (int L = 0; L < nlists; L++) {
@@ -401,11 +432,11 @@ __global__ void k_reserve(StoreArgs a)
             st.hcnt = li.hcnt[v];
             uint32_t need = st.len + add;
             bool ok = true;
-            if (need > st.cap) ok = grow_block(a, st, need, lane);
+            if (need > st.cap) ok = grow_block<AG>(a, st, need, g);
             if (ok && need >= INDEX_THRESHOLD && (st.hoff == HEMPTY || st.hcap < 2 * need)) {
                 uint32_t hc = next_pow2(2 * need);
                 if (hc < 1024) hc = 1024;
-                ok = build_index(a, st, hc, lane);
+                ok = build_index<AG>(a, st, hc, g);
             }
             if (lane == 0) {
                 li.off[v] = st.off;
@@ -427,7 +458,7 @@ void launch_store_count(const StoreArgs &a, hipStream_t s)
 
 void launch_store_reserve(const StoreArgs &a, hipStream_t s)
 {
-    uint64_t blocks = ((uint64_t)a.V * WAVE + 255) / 256;
+    uint64_t blocks = ((uint64_t)a.V * AG + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_reserve, dim3((uint32_t)blocks), dim3(256), 0, s, a);
 }

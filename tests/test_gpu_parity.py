@@ -231,11 +231,14 @@ def test_config1_bfs_plain_stream():
 
 
 @pytest.mark.parametrize("name", ["sssp", "wcc"])
-def test_large_batches_grid_mode(name):
-    """Batches of 10k and 50k updates on R-MAT scale 16 (frontiers beyond one workgroup)."""
+@pytest.mark.parametrize("mode", ["auto", "grid"])
+def test_large_batches(name, mode):
+    """Batches of 10k and 50k updates on R-MAT scale 16, in the automatic
+    block/grid choice and with every pass forced grid-wide."""
     algo = rg.ALGOS[name]
     V, E = 1 << 16, 1 << 20
-    t = _setup(algo, "rmat", V, E, 0.8, seed=11)
+    tuning = dict(block_max=0) if mode == "grid" else None
+    t = _setup(algo, "rmat", V, E, 0.8, seed=11, tuning=tuning)
     pos = 0
     for batch in (10000, 50000, 10000):
         i0, d0 = t["n0"] + pos, pos
@@ -252,7 +255,8 @@ def test_large_batches_grid_mode(name):
         gone = didx[olds == 1]
         RL.inc_del_batch(src[gone], dst[gone], w[gone], d)
         _check_labels(t["L"], algo, V, ref, t["root"], ref_vals=RL.read()[0])
-        assert st["grid_steps"] > 0
+        if mode == "grid":
+            assert st["grid_steps"] > 0
         pos += batch
 
 
