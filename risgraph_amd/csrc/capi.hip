@@ -144,10 +144,12 @@ struct rg_labels {
     Ctl *ctl = nullptr;
     Ctl *hc = nullptr;     // pinned mirror
     unsigned long long *ticks = nullptr;   // per-phase block-mode time (device)
-    hipStream_t st = nullptr;      // this tree's engine stream
     hipEvent_t ev_done = nullptr;  // end of the last engine run
+    EngineArgs *dargs = nullptr;   // device copy of the engine arguments (team launches)
+    EngineArgs *hargs = nullptr;   // pinned staging of the same
     Tuning tune{};
-    uint32_t block_max = 4096;
+    uint32_t team_w = 8;           // workgroups per tree in the persistent team launch
+    uint32_t block_max = 8192;     // largest pass (vertices, or ops/16) run by the team
     uint32_t *init_active = nullptr;
 };
 
@@ -408,61 +410,92 @@ void engine_init(rg_labels *L, uint32_t phase0, bool del_pending, uint32_t n_ops
         if (h.idx >= 0x7FFFFFF0) h.idx = 1;  // marks are cleared after every run
     }
     h.flags &= F_ALL_MODIFIED;
+    h.bar_count = h.bar_gen = h.bar_exit = 0;
     h.levels = 0;
 }
 
-// Run the engines of labels L[0..n) from `phase0` to completion, each on its
-// own stream so the trees repair concurrently (different workgroups / CUs).
-// They start after the work already queued on the graph stream (the store
-// batch) and the host waits for all of them.
+// A tree's next pass runs in the persistent team launch when it is not a
+// full rebuild and fits the team's work cap; otherwise it is one grid step.
+bool team_ok(const rg_labels *L, const Ctl &h)
+{
+    return h.phase != PH_BUILD_INIT && h.stage == 0 && host_work(h, L->g->V) <= L->block_max;
+}
+
+// Run the engines of labels L[0..n) from `phase0` to completion on the graph
+// stream, after the store batch already queued there.  Trees whose passes fit
+// the team cap share one persistent launch (W workgroups per tree, so the
+// trees repair concurrently on different CUs); larger passes go grid-wide.
 void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool del_pending, uint32_t n_ops, bool new_idx,
                       rg_batch_stats *stats)
 {
     if (n <= 0) return;
     rg_graph *g = L[0]->g;
-    HC(hipEventRecord(g->ev[3], g->st));
+    hipStream_t st = g->st;
     std::vector<EngineArgs> args(n);
     std::vector<char> active(n, 1), pending_init(n, 0);
     for (int i = 0; i < n; i++) {
-        HC(hipStreamWaitEvent(L[i]->st, g->ev[3], 0));
         engine_init(L[i], phase0, del_pending, n_ops, new_idx);
-        const Ctl &h = *L[i]->hc;
-        // a run that starts in block mode hands its initial control block to
+        // a run that starts in the team hands its initial control block to
         // the kernel through the pinned mirror; otherwise copy it now
-        if (h.phase != PH_BUILD_INIT && host_work(h, g->V) <= L[i]->block_max) pending_init[i] = 1;
-        else HC(hipMemcpyAsync(L[i]->ctl, L[i]->hc, sizeof(Ctl), hipMemcpyHostToDevice, L[i]->st));
+        if (team_ok(L[i], *L[i]->hc)) pending_init[i] = 1;
+        else HC(hipMemcpyAsync(L[i]->ctl, L[i]->hc, sizeof(Ctl), hipMemcpyHostToDevice, st));
         args[i] = engine_args(L[i]);
+        // the team kernel reads the arguments from device memory; refresh
+        // them when they changed (op buffers regrown, tuning)
+        if (memcmp(L[i]->hargs, &args[i], sizeof(EngineArgs)) != 0) {
+            memcpy(L[i]->hargs, &args[i], sizeof(EngineArgs));
+            HC(hipMemcpyAsync(L[i]->dargs, L[i]->hargs, sizeof(EngineArgs), hipMemcpyHostToDevice, st));
+        }
     }
     int left = n;
     uint64_t guard_iter = 0;
+    std::vector<int> team;
     while (left) {
+        team.clear();
         for (int i = 0; i < n; i++) {
             if (!active[i]) continue;
             const Ctl &h = *L[i]->hc;
-            uint64_t work = host_work(h, g->V);
-            if (h.phase != PH_BUILD_INIT && h.stage == 0 && work <= L[i]->block_max) {
-                hipEvent_t ea, eb;
-                prof_begin(g, 1, 1, L[i]->st, &ea, &eb);
-                if (ea) HC(hipEventRecord(ea, L[i]->st));
-                launch_engine_block(args[i], L[i]->block_max, pending_init[i], L[i]->st);
-                pending_init[i] = 0;
-                if (eb) HC(hipEventRecord(eb, L[i]->st));
+            if (team_ok(L[i], h)) {
+                team.push_back(i);
             } else {
-                launch_engine_grid_step(args[i], h, L[i]->st);
+                launch_engine_grid_step(args[i], h, st);
                 if (stats) stats->grid_steps++;
             }
         }
+        // one launch per group of up to MAXT trees with the same team width
+        // and work cap
+        while (!team.empty()) {
+            const uint32_t W = L[team[0]]->team_w, cap = L[team[0]]->block_max;
+            const EngineArgs *ta[MAXT];
+            uint32_t mask = 0;
+            int m = 0;
+            for (size_t k = 0; k < team.size() && m < MAXT;) {
+                int i = team[k];
+                if (L[i]->team_w != W || L[i]->block_max != cap) { k++; continue; }
+                ta[m] = L[i]->dargs;
+                if (pending_init[i]) mask |= 1u << m;
+                pending_init[i] = 0;
+                m++;
+                team.erase(team.begin() + (long)k);
+            }
+            hipEvent_t ea, eb;
+            prof_begin(g, 1, (uint64_t)m, st, &ea, &eb);
+            if (ea) HC(hipEventRecord(ea, st));
+            launch_engine_team(ta, m, W, cap, mask, st);
+            if (eb) HC(hipEventRecord(eb, st));
+        }
         HC(hipGetLastError());
+        HC(hipStreamSynchronize(st));
         for (int i = 0; i < n; i++) {
             if (!active[i]) continue;
-            HC(hipStreamSynchronize(L[i]->st));
             Ctl &h = *L[i]->hc;   // published by the last kernel
             if (h.flags & F_QUEUE_OVERFLOW) throw RgError{RG_EDEVICE, "engine queue overflow"};
+            if (h.flags & F_ABORT) throw RgError{RG_EDEVICE, "engine team barrier timed out"};
             if (h.phase == PH_DONE) {
                 active[i] = 0;
                 left--;
                 if (stats) stats->levels += h.levels;
-                HC(hipEventRecord(L[i]->ev_done, L[i]->st));
+                HC(hipEventRecord(L[i]->ev_done, st));
             } else if (h.phase == PH_REBUILD) {
                 // deletion fallback (graph.hpp:1252-1321): full recompute
                 h.phase = PH_BUILD_INIT;
@@ -472,7 +505,7 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool del_pending, u
                 h.sel_cur = 0; h.sel_next = 1; h.sel_tree = 2;
                 h.epoch += 1;
                 h.flags = (h.flags & ~F_NEED_REBUILD) | F_ALL_MODIFIED;
-                HC(hipMemcpyAsync(L[i]->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, L[i]->st));
+                HC(hipMemcpyAsync(L[i]->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, st));
                 if (stats) stats->rebuilds++;
             }
         }
@@ -492,11 +525,11 @@ void labels_free(rg_labels *L)
     dfree(L->q[0]); dfree(L->q[1]); dfree(L->q[2]); dfree(L->cq); dfree(L->mlist); dfree(L->ctl);
     dfree(L->init_active);
     dfree(L->ticks);
+    dfree(L->dargs);
+    hfree(L->hargs);
     hfree(L->hc);
     if (L->ev_done) (void)hipEventDestroy(L->ev_done);
-    if (L->st) (void)hipStreamDestroy(L->st);
     L->ev_done = nullptr;
-    L->st = nullptr;
 }
 
 }  // namespace
@@ -694,8 +727,10 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         HC(hipMemsetAsync(L->ticks, 0, 2 * NPHASE * 8, g->st));
         L->hc = halloc<Ctl>(1);
         memset(L->hc, 0, sizeof(Ctl));
+        L->dargs = dalloc<EngineArgs>(1);
+        L->hargs = halloc<EngineArgs>(1);
+        memset(L->hargs, 0, sizeof(EngineArgs));
         L->hc->mepoch = 1;
-        HC(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
         HC(hipEventCreate(&L->ev_done));
         HC(hipMemsetAsync(L->inv, 0, V * 4, g->st));
         HC(hipMemsetAsync(L->stamp, 0, V * 4, g->st));
@@ -728,6 +763,8 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->tune.recompute_abs = 2000000;
         L->tune.recompute_frac = 0.05f;
         L->tune.walk_limit = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(V / 10, 1u << 16));
+        if (const char *tw = getenv("RG_TEAM_W")) L->team_w = (uint32_t)std::max(1, std::min(64, atoi(tw)));
+        L->block_max = std::max<uint32_t>(4096, 1024 * L->team_w);
         if (const char *bm = getenv("RG_BLOCK_MAX")) L->block_max = (uint32_t)atoi(bm);
         HC(hipMemcpyAsync(L->ctl, L->hc, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
         HC(hipStreamSynchronize(g->st));
@@ -745,7 +782,6 @@ int rg_labels_destroy(rg_labels *L)
 {
     if (!L) return RG_OK;
     if (L->g && L->g->st) (void)hipStreamSynchronize(L->g->st);
-    if (L->st) (void)hipStreamSynchronize(L->st);
     labels_free(L);
     delete L;
     return RG_OK;

@@ -111,10 +111,24 @@ __device__ __forceinline__ bool relax(uint64_t *val, uint32_t dst, uint64_t c, u
     return false;
 }
 
+// one counter atomic per wave: the active lanes (all at this call site, so all
+// appending to the same queue) take consecutive positions
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t *count)
+{
+    const uint64_t m = __ballot(1);
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader);
+    return base + rank;
+}
+
 // bounded append to a vertex queue (capacity V; dedupe keeps it in range)
 __device__ __forceinline__ void q_append(const EngineArgs &a, Ctl *c, uint32_t *q, uint32_t *count, uint32_t v)
 {
-    uint32_t pos = atomicAdd(count, 1u);
+    uint32_t pos = wave_reserve(count);
     if (pos < a.V) q[pos] = v;
     else atomicOr(&c->flags, F_QUEUE_OVERFLOW);
 }
@@ -127,7 +141,7 @@ __device__ __forceinline__ void push_next(const EngineArgs &a, const Snap &s, ui
 __device__ __forceinline__ void mark_mod(const EngineArgs &a, const Snap &s, uint32_t v)
 {
     if (a.trace && atomicExch(&a.mstamp[v], s.mepoch) != s.mepoch) {
-        uint32_t pos = atomicAdd(&s.c->n_mod, 1u);
+        uint32_t pos = wave_reserve(&s.c->n_mod);
         if (pos < a.V) a.mlist[pos] = v;
     }
 }
@@ -537,67 +551,158 @@ __device__ void publish(const EngineArgs &a, const Ctl *c)
 // ---------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------
-template <int A>
-__global__ __launch_bounds__(1024) void k_block(EngineArgs a, uint32_t block_max, uint32_t chunk_max, int do_init)
+// Team barrier over the W workgroups of one tree (thread 0 of each
+// workgroup arrives).  Arrivals count up monotonically within a run, so
+// episode e is complete when the count reaches e*W; the last arriver runs
+// `last` (the pass's serial step) and then releases the episode.  Waiting is
+// bounded in wall-clock time: a timeout sets F_ABORT, which every waiter of
+// the tree sees, so all waves leave the launch and the host reports an error.
+constexpr uint64_t TEAM_SPIN_TICKS = 200000000ull;   // 2 s at the 100 MHz wall clock
+
+template <class F>
+__device__ __forceinline__ void team_sync(Ctl *c, uint32_t W, F &&last)
 {
-    // the control block lives in LDS for the whole launch: queue counters and
-    // phase transitions are workgroup-local atomics instead of device-scope
-    // round trips; it goes back to HBM (and the pinned mirror) at exit
-    __shared__ Ctl sctl;
-    __shared__ Snap snap;
-    __shared__ uint32_t nch;
-    const uint32_t tid = threadIdx.x, nthr = blockDim.x;
-    if (tid == 0) {
-        // a run's first launch takes its initial control block from the
-        // pinned host mirror (no separate copy launch)
-        const uint32_t *src = (const uint32_t *)(do_init ? a.hctl : a.ctl);
-        uint32_t *dst = (uint32_t *)&sctl;
-        for (uint32_t i = 0; i < sizeof(Ctl) / 4; i++) dst[i] = ald(src + i);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (W == 1) {
+            last();
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            const uint32_t old = atomicAdd(&c->bar_count, 1u);
+            const uint32_t ep = old / W + 1;
+            if (old % W == W - 1) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                last();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_store(&c->bar_gen, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                const uint64_t t0 = wall_clock64();
+                while ((int32_t)(ald(&c->bar_gen) - ep) < 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (ald(&c->flags) & F_ABORT) break;
+                    if (wall_clock64() - t0 > TEAM_SPIN_TICKS) {
+                        atomicOr(&c->flags, F_ABORT);
+                        break;
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
     }
+    __syncthreads();
+}
+
+struct TeamParams {
+    const EngineArgs *a[MAXT];   // device copies of the trees' arguments
+    uint32_t ntrees, W;
+    uint32_t work_max;     // a larger pass leaves the team for grid mode
+    uint32_t chunk_max;    // more chunk items than this leave for grid mode
+    uint32_t init_mask;
+};
+
+// The level-synchronous loop of one tree on its W workgroups.  With W == 1
+// the control block lives in LDS for the whole launch (workgroup-local
+// atomics, no barrier traffic) and goes back to HBM at exit; with W > 1 it
+// stays in HBM and passes are separated by the team barrier.
+template <int A>
+__device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32_t work_max, uint32_t chunk_max,
+                                         uint32_t wg, bool init, Ctl *lds_ctl, Snap &snap, uint32_t *sh)
+{
+    const uint32_t tid = wg * blockDim.x + threadIdx.x, nthr = W * blockDim.x;
+    Ctl *c = (W == 1) ? lds_ctl : a.ctl;
+    if (threadIdx.x == 0 && (W == 1 || (init && wg == 0))) {
+        // a run's first launch takes its initial control block from the
+        // pinned host mirror (no separate copy command); the barrier words
+        // are left alone, other workgroups may already be arriving
+        const uint32_t *src = (const uint32_t *)(init ? a.hctl : a.ctl);
+        uint32_t *dst = (uint32_t *)c;
+        const uint32_t b0 = offsetof(Ctl, bar_count) / 4, b2 = offsetof(Ctl, bar_exit) / 4;
+        for (uint32_t i = 0; i < sizeof(Ctl) / 4; i++) {
+            if (W == 1) dst[i] = ald(src + i);
+            else if (i < b0 || i > b2) ast(dst + i, src[i]);
+        }
+    }
+    if (W > 1 && init) team_sync(c, W, [] {});
     // every pass either finishes the phase or hands it to grid mode; the cap
     // only bounds the launch if the state machine were ever corrupted
     for (uint32_t iter = 0; iter < (1u << 24); iter++) {
-        __threadfence_block();
         __syncthreads();
-        if (tid == 0) take_snap(&sctl, snap);
+        if (threadIdx.x == 0) take_snap(c, snap);
         __syncthreads();
         const Snap s = snap;
-        if (s.phase == PH_DONE || s.phase == PH_REBUILD) break;
-        if (phase_work(a, s) > block_max) break;
+        if (s.phase == PH_DONE || s.phase == PH_REBUILD || s.phase == PH_BUILD_INIT) break;
+        if (ald(&c->flags) & F_ABORT) break;
+        if (phase_work(a, s) > work_max) break;
         const uint64_t t_pass = wall_clock64();
         if (s.stage == 0) {
             if (s.phase == PH_SEED || s.phase == PH_DEL_ROOTS) pass_ops<A>(a, s, tid, nthr);
             else pass_vertices<A>(a, s, tid, nthr);
-            __threadfence_block();
-            __syncthreads();
-            if (tid == 0) nch = sctl.n_chunk;
-            __syncthreads();
-            if (nch > chunk_max) {
-                if (tid == 0) sctl.stage = 1u;
-                break;
+            // the chunk queue is complete once every workgroup finished
+            team_sync(c, W, [&] {
+                if (ald(&c->n_chunk) > chunk_max) ast(&c->stage, 1u);
+            });
+            if (threadIdx.x == 0) {
+                sh[0] = ald(&c->n_chunk);
+                sh[1] = ald(&c->stage);
             }
+            __syncthreads();
+            if (sh[1] != 0) break;   // uniform: every workgroup read the same block
         } else {
-            if (tid == 0) nch = sctl.n_chunk;
+            if (threadIdx.x == 0) sh[0] = ald(&c->n_chunk);
             __syncthreads();
         }
-        pass_chunks<A>(a, s, tid, nthr, nch);
-        __threadfence_block();
-        __syncthreads();
-        if (tid == 0) {
+        pass_chunks<A>(a, s, tid, nthr, sh[0]);
+        team_sync(c, W, [&] {
             if (a.ticks && s.phase < NPHASE) {
                 a.ticks[s.phase] += wall_clock64() - t_pass;
                 a.ticks[NPHASE + s.phase] += 1;
             }
-            advance(a, &sctl);
+            advance(a, c);
+        });
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (W == 1) {
+            const uint32_t *src = (const uint32_t *)c;
+            uint32_t *dst = (uint32_t *)a.ctl;
+            for (uint32_t i = 0; i < sizeof(Ctl) / 4; i++) ast(dst + i, src[i]);
+            publish(a, a.ctl);
+        } else {
+            // the last workgroup out restarts the barrier for the next launch
+            // (nobody is waiting any more) and publishes the final block
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (atomicAdd(&c->bar_exit, 1u) == W - 1) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                ast(&c->bar_count, 0u);
+                ast(&c->bar_gen, 0u);
+                ast(&c->bar_exit, 0u);
+                publish(a, c);
+            }
         }
     }
-    __threadfence_block();
+}
+
+// Workgroup b serves tree b % ntrees: with 4 trees, a tree's workgroups sit
+// on two of the eight XCDs (workgroups are dealt to XCDs round-robin).
+__global__ __launch_bounds__(1024) void k_team(TeamParams P)
+{
+    __shared__ EngineArgs sa;
+    __shared__ Ctl lds_ctl;
+    __shared__ Snap snap;
+    __shared__ uint32_t sh[2];
+    const uint32_t t = blockIdx.x % P.ntrees, wg = blockIdx.x / P.ntrees;
+    static_assert(MAXT == 4, "tree selection below");
+    static_assert(sizeof(EngineArgs) % 4 == 0, "word copy below");
+    // the tree's arguments are staged in LDS: every pass reads them often
+    const uint32_t *src = (const uint32_t *)(t == 0 ? P.a[0] : t == 1 ? P.a[1] : t == 2 ? P.a[2] : P.a[3]);
+    for (uint32_t i = threadIdx.x; i < sizeof(EngineArgs) / 4; i += blockDim.x) ((uint32_t *)&sa)[i] = src[i];
     __syncthreads();
-    if (tid == 0) {
-        const uint32_t *src = (const uint32_t *)&sctl;
-        uint32_t *dst = (uint32_t *)a.ctl;
-        for (uint32_t i = 0; i < sizeof(Ctl) / 4; i++) ast(dst + i, src[i]);
-        publish(a, &sctl);
+    const bool init = (P.init_mask >> t) & 1u;
+    switch (sa.algo) {
+    case BFS: team_run<BFS>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh); break;
+    case SSSP: team_run<SSSP>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh); break;
+    case SSWP: team_run<SSWP>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh); break;
+    default: team_run<WCC>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh); break;
     }
 }
 
@@ -631,20 +736,18 @@ __global__ void k_advance(EngineArgs a)
     publish(a, a.ctl);
 }
 
-template <int A>
-static void block_launch(const EngineArgs &a, uint32_t block_max, int do_init, hipStream_t st)
+void launch_engine_team(const EngineArgs *const *a, int n, uint32_t W, uint32_t work_max, uint32_t init_mask,
+                        hipStream_t st)
 {
-    hipLaunchKernelGGL(k_block<A>, dim3(1), dim3(1024), 0, st, a, block_max, 64u, do_init);
-}
-
-void launch_engine_block(const EngineArgs &a, uint32_t block_max, int do_init, hipStream_t st)
-{
-    switch (a.algo) {
-    case BFS: block_launch<BFS>(a, block_max, do_init, st); break;
-    case SSSP: block_launch<SSSP>(a, block_max, do_init, st); break;
-    case SSWP: block_launch<SSWP>(a, block_max, do_init, st); break;
-    default: block_launch<WCC>(a, block_max, do_init, st); break;
-    }
+    if (n <= 0 || n > MAXT || W == 0) return;
+    TeamParams P{};
+    for (int i = 0; i < MAXT; i++) P.a[i] = a[i < n ? i : 0];
+    P.ntrees = (uint32_t)n;
+    P.W = W;
+    P.work_max = work_max;
+    P.chunk_max = 64u * W;
+    P.init_mask = init_mask;
+    hipLaunchKernelGGL(k_team, dim3(n * W), dim3(1024), 0, st, P);
 }
 
 template <int A>

@@ -122,6 +122,7 @@ constexpr int NPHASE = 10;
 constexpr uint32_t F_NEED_REBUILD = 1;
 constexpr uint32_t F_QUEUE_OVERFLOW = 2;
 constexpr uint32_t F_ALL_MODIFIED = 4;
+constexpr uint32_t F_ABORT = 8;        // a team barrier timed out
 
 struct Ctl {
     uint32_t phase, stage, level, del_pending;
@@ -131,9 +132,15 @@ struct Ctl {
     uint32_t epoch, mepoch;
     int32_t idx;
     uint32_t flags;
-    uint32_t pad;
+    uint32_t bar_count;    // team barrier: arrivals (monotonic within a launch)
+    uint32_t bar_gen;      // team barrier: last released episode
+    uint32_t bar_exit;     // team workgroups that left the launch
     unsigned long long edges, relax_ok, levels, pad2;
 };
+
+// One launch repairs up to MAXT trees; each tree gets W workgroups that
+// synchronise through a barrier in the tree's control block.
+constexpr int MAXT = 4;
 
 struct Tuning {
     uint32_t rescue_level;        // rescue when level <= this ...
@@ -165,7 +172,7 @@ struct EngineArgs {
     uint32_t *mlist;
     Ctl *ctl;
     Ctl *hctl;                    // pinned host mirror, written at the end of each launch
-    unsigned long long *ticks;    // [2*NPHASE] per-phase wall-clock ticks and passes (block mode)
+    unsigned long long *ticks;    // [2*NPHASE] per-phase wall-clock ticks and passes (team mode)
     EdgeOps ops;
     Tuning tune;
 };
@@ -193,8 +200,13 @@ void launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a = nul
 void launch_store_count(const StoreArgs &a, hipStream_t s);
 void launch_store_reserve(const StoreArgs &a, hipStream_t s);
 void launch_pool_recycle(const StoreArgs &a, hipStream_t s);
-// do_init: the kernel first copies the control block from the pinned mirror
-void launch_engine_block(const EngineArgs &a, uint32_t block_max, int do_init, hipStream_t s);
+// Persistent team launch over trees *a[0..n) (n <= MAXT; device copies of
+// their arguments), W workgroups each.
+// A tree leaves the launch when its next pass exceeds work_max (grid mode
+// takes over) or its phase is done.  Bit t of init_mask: tree t's control
+// block is taken from its pinned mirror (first launch of a run).
+void launch_engine_team(const EngineArgs *const *a, int n, uint32_t W, uint32_t work_max, uint32_t init_mask,
+                        hipStream_t s);
 void launch_engine_grid_step(const EngineArgs &a, const Ctl &host_ctl, hipStream_t s);
 void launch_engine_fill_init(const EngineArgs &a, hipStream_t s);
 void launch_gen_rmat(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, int scale,
