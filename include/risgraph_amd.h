@@ -132,8 +132,13 @@ int rg_stream_load(rg_graph *g, rg_stream *s, uint64_t first, uint64_t n, int di
 /* One sliding-window batch on device-resident data, the fused form of a
  * batch driver round (bfs_inc_batch.cpp:142-178): add stream edges
  * [ins_first, ins_first+n_ins), delete [del_first, del_first+n_del), then
- * repair every labels instance in L[0..nL) (inserts propagate first).
- * n_ins = n_del = 1 is one plain update (bfs_inc.cpp:160-189). */
+ * repair every labels instance in L[0..nL).  Ranges wrap around the end of
+ * the stream (positions are taken mod the stream length), so a sliding
+ * window can run past it: update k inserts e[(n0+k) mod n] and deletes
+ * e[k mod n].  n_ins = n_del = 1 is one plain update (bfs_inc.cpp:160-189).
+ * The repair of a mixed batch handles the deletions first and seeds the
+ * inserts in the same pass as the deletion pull; values are the unique
+ * fixpoint either way. */
 typedef struct rg_batch_stats {
     uint64_t seeds;          /* insert edges that reached the engine */
     uint64_t roots;          /* delete edges whose multiplicity hit 0 */
@@ -146,7 +151,8 @@ int rg_stream_apply(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t i
                     rg_batch_stats *stats);
 
 /* Run `n_steps` plain updates back to back (step k: add ins_first+k, then
- * delete del_first+k, each followed by its repair), the per-update GPU path
+ * delete del_first+k, positions mod the stream length, each followed by its
+ * repair), the per-update GPU path
  * of bfs_inc.cpp:160-189.  lat_ns (may be NULL) receives each step's wall
  * time in ns. */
 int rg_stream_plain(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first,
@@ -160,17 +166,25 @@ int rg_last_gpu_ms(rg_graph *g, double *ms);
 
 /* Kernel timing with HIP events on the streams the kernels run on.  Bit k of
  * `mask` times kernels of kind k: the store apply kernel (kind 0) and the
- * single-workgroup engine launches (kind 1).  rg_profile_read returns the
+ * persistent team engine launches (kind 1).  rg_profile_read returns the
  * launch count, summed device milliseconds and summed work units (edge ops
  * for kind 0, launches for kind 1) accumulated since the graph was made. */
 int rg_profile_enable(rg_graph *g, int mask);
 int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint64_t *units);
 
-/* Per-phase device time of single-workgroup engine passes on one labels
- * instance since its creation: us[p] microseconds and passes[p] for phase p
- * (0 done, 1 seed, 2 push, 3 delete roots, 4 rescue, 5 expand, 6 reset,
- * 7 pull, 8 build init, 9 rebuild); arrays of 10. */
+/* Per-phase device time of the persistent team's engine passes on one
+ * labels instance since its creation: us[p] microseconds and passes[p] for
+ * phase p (0 done, 1 seed, 2 push, 3 delete roots, 4 rescue + expand level,
+ * 5 expand-only level, 6 unused, 7 reset + pull (+ seeds of a mixed batch),
+ * 8 build init, 9 rebuild); arrays of 10. */
 int rg_labels_phase_time(rg_labels *L, double *us, uint64_t *passes);
+
+/* Where the persistent team's passes spend their time, per phase p (arrays
+ * as above): stage_us[k*10 + p] for k = 0 vertex stage, 1 first barrier wait,
+ * 2 chunk stage, 3 second barrier wait (as seen by the tree's first
+ * workgroup); work[p] summed pass work (queued vertices, or ops/16);
+ * chunks[p] summed chunk items.  Any pointer may be NULL. */
+int rg_labels_phase_detail(rg_labels *L, double *stage_us, uint64_t *work, uint64_t *chunks);
 
 /* Live multiplicity degree of every vertex (out, and in for directed
  * graphs; either pointer may be NULL), the `degree` of IndexedEdgeStorage
@@ -180,6 +194,7 @@ int rg_degrees(rg_graph *g, uint32_t *out_deg, uint32_t *in_deg);
 /* Serving loop: the device-side replacement of the latency-aware scheduler
  * of the *_inc_rt drivers (bfs_inc_rt.cpp:154-431).  Updates u = 0..n-1
  * alternate add (stream edge ins_first + u/2) and delete (del_first + u/2),
+ * positions mod the stream length,
  * the all_inc stream (all_inc.cpp:254-300), and arrive by a Poisson process
  * of `rate` updates/s from `clients` simulated clients (client u % clients).
  * Every loop iteration admits all arrived updates up to the current batch

@@ -101,6 +101,7 @@ SIGNATURES = {
     "rg_profile_read": (ci, [vp, ci, u64p, ctypes.POINTER(ctypes.c_double), u64p]),
     "rg_degrees": (ci, [vp, vp, vp]),
     "rg_labels_phase_time": (ci, [vp, vp, vp]),
+    "rg_labels_phase_detail": (ci, [vp, vp, vp, vp]),
 }
 
 PHASES = ["done", "seed", "push", "del_roots", "rescue", "expand", "reset", "pull", "build_init", "rebuild"]
@@ -364,6 +365,23 @@ class Labels:
         n = np.zeros(10, np.uint64)
         _check(load_library().rg_labels_phase_time(self.h, _ptr(us), _ptr(n)))
         return {PHASES[p]: (float(us[p]), int(n[p])) for p in range(10) if n[p]}
+
+    def phase_detail(self):
+        """Team pass breakdown per phase: {phase: dict(vertex_us, barrier1_us,
+        chunk_us, barrier2_us, work, chunks)} (first workgroup's clocks)."""
+        st = np.zeros(40, np.float64)
+        work = np.zeros(10, np.uint64)
+        ch = np.zeros(10, np.uint64)
+        _check(load_library().rg_labels_phase_detail(self.h, _ptr(st), _ptr(work), _ptr(ch)))
+        keys = ("vertex_us", "barrier1_us", "chunk_us", "barrier2_us")
+        out = {}
+        for p in range(10):
+            d = {k: float(st[i * 10 + p]) for i, k in enumerate(keys)}
+            d["work"] = int(work[p])
+            d["chunks"] = int(ch[p])
+            if any(d.values()):
+                out[PHASES[p]] = d
+        return out
 
     def get_modified(self):
         n = u64()

@@ -171,7 +171,8 @@ void alloc_list(List &l, uint64_t V, hipStream_t st)
     l.hoff = dalloc<uint64_t>(V);
     l.hcap = dalloc<uint32_t>(V);
     l.hcnt = dalloc<uint32_t>(V);
-    l.head = dalloc<int32_t>(V);
+    l.nops = dalloc<uint32_t>(V);
+    l.seg = dalloc<uint32_t>(V);
     HC(hipMemsetAsync(l.off, 0, V * 8, st));
     HC(hipMemsetAsync(l.len, 0, V * 4, st));
     HC(hipMemsetAsync(l.cap, 0, V * 4, st));
@@ -179,21 +180,22 @@ void alloc_list(List &l, uint64_t V, hipStream_t st)
     HC(hipMemsetAsync(l.hoff, 0xFF, V * 8, st));
     HC(hipMemsetAsync(l.hcap, 0, V * 4, st));
     HC(hipMemsetAsync(l.hcnt, 0, V * 4, st));
-    HC(hipMemsetAsync(l.head, 0xFF, V * 4, st));
+    HC(hipMemsetAsync(l.nops, 0, V * 4, st));
 }
 
 void free_list(List &l)
 {
     dfree(l.off); dfree(l.len); dfree(l.cap); dfree(l.deg);
-    dfree(l.hoff); dfree(l.hcap); dfree(l.hcnt); dfree(l.head);
+    dfree(l.hoff); dfree(l.hcap); dfree(l.hcnt); dfree(l.nops); dfree(l.seg);
 }
 
 void free_ops(rg_graph *g)
 {
     dfree(g->ops.src); dfree(g->ops.dst); dfree(g->ops.w); dfree(g->ops.kind);
-    dfree(g->ops.old); dfree(g->ops.slot);
+    dfree(g->ops.old); dfree(g->ops.live);
     dfree(g->lops.v); dfree(g->lops.nbr); dfree(g->lops.w); dfree(g->lops.edge);
-    dfree(g->lops.delta); dfree(g->lops.flags); dfree(g->lops.next);
+    dfree(g->lops.delta); dfree(g->lops.flags); dfree(g->lops.rank); dfree(g->lops.segbuf);
+    dfree(g->lops.slot);
     dfree(g->d_flag);
     g->op_cap = 0;
 }
@@ -210,7 +212,7 @@ void ensure_ops(rg_graph *g, uint64_t n)
     g->ops.w = dalloc<uint32_t>(cap);
     g->ops.kind = dalloc<uint8_t>(cap);
     g->ops.old = dalloc<uint32_t>(cap);
-    g->ops.slot = dalloc<uint32_t>(cap);
+    g->ops.live = dalloc<uint32_t>(cap);
     uint64_t lc = 4ull * cap;
     g->lops.v = dalloc<uint32_t>(lc);
     g->lops.nbr = dalloc<uint32_t>(lc);
@@ -218,7 +220,9 @@ void ensure_ops(rg_graph *g, uint64_t n)
     g->lops.edge = dalloc<uint32_t>(lc);
     g->lops.delta = dalloc<int8_t>(lc);
     g->lops.flags = dalloc<uint8_t>(lc);
-    g->lops.next = dalloc<int32_t>(lc);
+    g->lops.rank = dalloc<uint32_t>(lc);
+    g->lops.segbuf = dalloc<uint32_t>(lc);
+    g->lops.slot = dalloc<uint32_t>(lc);
     g->d_flag = dalloc<uint8_t>(cap);
     g->op_cap = cap;
 }
@@ -387,30 +391,37 @@ uint64_t host_work(const Ctl &h, uint64_t V)
     switch (h.phase) {
     case PH_SEED:
     case PH_DEL_ROOTS: return h.n_ops / 16;   // same units as the device's phase_work
-    case PH_DEL_RESET:
-    case PH_DEL_PULL: return h.n_tree;
+    case PH_DEL_PULL: return h.n_tree + (h.seed ? h.n_ops / 16 : 0);
     case PH_BUILD_INIT: return V;
     default: return h.n_cur;
     }
 }
 
-void engine_init(rg_labels *L, uint32_t phase0, bool del_pending, uint32_t n_ops, bool new_idx)
+// seed: a mixed batch starts with its deletions and seeds its inserts in the
+// pull pass (or right after the roots pass when no tree edge was deleted)
+void engine_init(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool new_idx)
 {
     Ctl &h = *L->hc;
     h.phase = phase0;
     h.stage = 0;
     h.level = 0;
-    h.del_pending = del_pending ? 1u : 0u;
+    h.seed = seed ? 1u : 0u;
     h.n_cur = h.n_next = h.n_tree = h.n_chunk = 0;
     h.n_ops = n_ops;
     h.sel_cur = 0; h.sel_next = 1; h.sel_tree = 2;
     h.epoch += 1;
-    if (new_idx || del_pending) {
+    if (new_idx || phase0 == PH_DEL_ROOTS) {
+        // invalidation marks of earlier runs stay in place and never match a
+        // fresh index; when the index wraps the marks are cleared
         h.idx += 1;
-        if (h.idx >= 0x7FFFFFF0) h.idx = 1;  // marks are cleared after every run
+        if (h.idx >= 0x7FFFFFF0) {
+            h.idx = 1;
+            HC(hipMemsetAsync(L->inv, 0, L->g->V * 4, L->g->st));
+        }
     }
     h.flags &= F_ALL_MODIFIED;
     h.bar_count = h.bar_gen = h.bar_exit = 0;
+    h.rbound = L->algo == SSWP ? 0u : 0xFFFFFFFFu;
     h.levels = 0;
 }
 
@@ -425,7 +436,7 @@ bool team_ok(const rg_labels *L, const Ctl &h)
 // stream, after the store batch already queued there.  Trees whose passes fit
 // the team cap share one persistent launch (W workgroups per tree, so the
 // trees repair concurrently on different CUs); larger passes go grid-wide.
-void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool del_pending, uint32_t n_ops, bool new_idx,
+void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t n_ops, bool new_idx,
                       rg_batch_stats *stats)
 {
     if (n <= 0) return;
@@ -434,7 +445,7 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool del_pending, u
     std::vector<EngineArgs> args(n);
     std::vector<char> active(n, 1), pending_init(n, 0);
     for (int i = 0; i < n; i++) {
-        engine_init(L[i], phase0, del_pending, n_ops, new_idx);
+        engine_init(L[i], phase0, seed, n_ops, new_idx);
         // a run that starts in the team hands its initial control block to
         // the kernel through the pinned mirror; otherwise copy it now
         if (team_ok(L[i], *L[i]->hc)) pending_init[i] = 1;
@@ -500,7 +511,7 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool del_pending, u
                 // deletion fallback (graph.hpp:1252-1321): full recompute
                 h.phase = PH_BUILD_INIT;
                 h.stage = 0;
-                h.del_pending = 0;
+                h.seed = 0;
                 h.n_cur = h.n_next = h.n_tree = h.n_chunk = 0;
                 h.sel_cur = 0; h.sel_next = 1; h.sel_tree = 2;
                 h.epoch += 1;
@@ -513,10 +524,10 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool del_pending, u
     }
 }
 
-void engine_run(rg_labels *L, uint32_t phase0, bool del_pending, uint32_t n_ops, bool new_idx,
+void engine_run(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool new_idx,
                 rg_batch_stats *stats)
 {
-    engine_run_multi(&L, 1, phase0, del_pending, n_ops, new_idx, stats);
+    engine_run_multi(&L, 1, phase0, seed, n_ops, new_idx, stats);
 }
 
 void labels_free(rg_labels *L)
@@ -723,8 +734,8 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->cq = dalloc<uint64_t>(L->cq_cap);
         L->mlist = dalloc<uint32_t>(V);
         L->ctl = dalloc<Ctl>(1);
-        L->ticks = dalloc<unsigned long long>(2 * NPHASE);
-        HC(hipMemsetAsync(L->ticks, 0, 2 * NPHASE * 8, g->st));
+        L->ticks = dalloc<unsigned long long>(NTICK * NPHASE);
+        HC(hipMemsetAsync(L->ticks, 0, NTICK * NPHASE * 8, g->st));
         L->hc = halloc<Ctl>(1);
         memset(L->hc, 0, sizeof(Ctl));
         L->dargs = dalloc<EngineArgs>(1);
@@ -1035,7 +1046,10 @@ int rg_stream_load(rg_graph *g, rg_stream *s, uint64_t first, uint64_t n, int di
 static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first, uint64_t n_ins,
                         uint64_t del_first, uint64_t n_del, int directed, rg_batch_stats *stats)
 {
-    if (ins_first + n_ins > s->n || del_first + n_del > s->n) throw RgError{RG_EINVAL, "range out of bounds"};
+    // ranges wrap around the end of the stream (a sliding window over a
+    // circular stream: update k inserts e[(n0 + k) mod n], deletes e[k mod n])
+    if (ins_first >= s->n || del_first >= s->n || n_ins > s->n || n_del > s->n)
+        throw RgError{RG_EINVAL, "range out of bounds"};
     for (int i = 0; i < nL; i++)
         if (!L[i] || L[i]->g != g) throw RgError{RG_EINVAL, "labels belong to another graph"};
     uint64_t n = n_ins + n_del;
@@ -1044,14 +1058,19 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
     uint8_t ud = directed ? 0 : K_UNDIR;
     const uint32_t *w = g->weighted ? s->w : nullptr;
     if (g->weighted && !w) throw RgError{RG_EINVAL, "weighted graph needs a weighted stream"};
-    launch_ops_from_stream(s->src, s->dst, w, ins_first, (uint32_t)n_ins, K_INS | ud, g->ops, 0, g->st);
-    launch_ops_from_stream(s->src, s->dst, w, del_first, (uint32_t)n_del, K_DEL | ud, g->ops, (uint32_t)n_ins, g->st);
+    auto ops_range = [&](uint64_t first, uint64_t cnt, uint8_t kind, uint32_t base) {
+        uint64_t a = std::min<uint64_t>(cnt, s->n - first);
+        launch_ops_from_stream(s->src, s->dst, w, first, (uint32_t)a, kind, g->ops, base, g->st);
+        if (a < cnt) launch_ops_from_stream(s->src, s->dst, w, 0, (uint32_t)(cnt - a), kind, g->ops, base + (uint32_t)a, g->st);
+    };
+    ops_range(ins_first, n_ins, K_INS | ud, 0);
+    ops_range(del_first, n_del, K_DEL | ud, (uint32_t)n_ins);
     g->ops.n = (uint32_t)n;
     g->lops.n = (uint32_t)(4 * n);
     HC(hipEventRecord(g->ev[0], g->st));
     run_store(g);
     HC(hipEventRecord(g->ev[1], g->st));
-    uint32_t ph = n_ins ? PH_SEED : PH_DEL_ROOTS;
+    uint32_t ph = n_del ? PH_DEL_ROOTS : PH_SEED;
     engine_run_multi(L, nL, ph, n_ins && n_del, (uint32_t)n, n_del > 0, stats);
     HC(hipEventSynchronize(g->ev[1]));
     store_status_check(g);
@@ -1092,7 +1111,7 @@ int rg_stream_plain(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t i
         HC(hipStreamSynchronize(s->st));
         for (uint64_t k = 0; k < n_steps; k++) {
             auto t0 = std::chrono::steady_clock::now();
-            apply_batch(g, L, nL, s, ins_first + k, 1, del_first + k, 1, directed, stats);
+            apply_batch(g, L, nL, s, (ins_first + k) % s->n, 1, (del_first + k) % s->n, 1, directed, stats);
             auto t1 = std::chrono::steady_clock::now();
             if (lat_ns) lat_ns[k] = (double)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
         }
@@ -1134,7 +1153,7 @@ int rg_labels_phase_time(rg_labels *L, double *us, uint64_t *passes)
     return guard([&] {
         rg_graph *g = L->g;
         std::lock_guard<std::recursive_mutex> lk(g->mu);
-        unsigned long long t[2 * NPHASE];
+        unsigned long long t[NTICK * NPHASE];
         HC(hipMemcpyAsync(t, L->ticks, sizeof(t), hipMemcpyDeviceToHost, g->st));
         HC(hipStreamSynchronize(g->st));
         int khz = 0;
@@ -1145,6 +1164,29 @@ int rg_labels_phase_time(rg_labels *L, double *us, uint64_t *passes)
         for (int p = 0; p < NPHASE; p++) {
             if (us) us[p] = (double)t[p] / per_us;
             if (passes) passes[p] = t[NPHASE + p];
+        }
+    });
+}
+
+int rg_labels_phase_detail(rg_labels *L, double *stage_us, uint64_t *work, uint64_t *chunks)
+{
+    if (!L) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        unsigned long long t[NTICK * NPHASE];
+        HC(hipMemcpyAsync(t, L->ticks, sizeof(t), hipMemcpyDeviceToHost, g->st));
+        HC(hipStreamSynchronize(g->st));
+        int khz = 0;
+        int dev = 0;
+        HC(hipGetDevice(&dev));
+        HC(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        double per_us = khz > 0 ? khz / 1000.0 : 100.0;
+        for (int p = 0; p < NPHASE; p++) {
+            for (int k = 0; k < 4; k++)
+                if (stage_us) stage_us[k * NPHASE + p] = (double)t[(2 + k) * NPHASE + p] / per_us;
+            if (work) work[p] = t[6 * NPHASE + p];
+            if (chunks) chunks[p] = t[7 * NPHASE + p];
         }
     });
 }
@@ -1207,7 +1249,8 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
             // updates [b, e): adds are the even indices, deletes the odd ones
             uint64_t a0 = (b + 1) / 2, a1 = (e + 1) / 2;
             uint64_t d0 = b / 2, d1 = e / 2;
-            apply_batch(g, L, nL, s, p->ins_first + a0, a1 - a0, p->del_first + d0, d1 - d0, p->directed, &st);
+            apply_batch(g, L, nL, s, (p->ins_first + a0) % s->n, a1 - a0, (p->del_first + d0) % s->n, d1 - d0,
+                        p->directed, &st);
             gpu_ms += g->last_ms;
         };
         while (next < n) {

@@ -27,6 +27,8 @@
 // hand over at any pass boundary.
 #include "rg_internal.h"
 
+#include <algorithm>
+
 namespace rg {
 
 // ---------------------------------------------------------------------
@@ -73,7 +75,7 @@ __device__ __forceinline__ void ast(int32_t *p, int32_t v)
 // snapshot of the control block taken at a pass boundary
 struct Snap {
     uint32_t phase, stage, level, n_cur, n_tree, n_chunk, n_ops;
-    uint32_t sel_cur, sel_next, sel_tree, epoch, mepoch;
+    uint32_t sel_cur, sel_next, sel_tree, epoch, mepoch, seed, rbound;
     int32_t idx;
     Ctl *c;   // live control block: LDS copy in block mode, global in grid mode
 };
@@ -93,6 +95,8 @@ __device__ __forceinline__ void take_snap(Ctl *c, Snap &s)
     s.sel_tree = ald(&c->sel_tree);
     s.epoch = ald(&c->epoch);
     s.mepoch = ald(&c->mepoch);
+    s.seed = ald(&c->seed);
+    s.rbound = ald(&c->rbound);
     s.idx = ald(&c->idx);
 }
 
@@ -146,7 +150,11 @@ __device__ __forceinline__ void mark_mod(const EngineArgs &a, const Snap &s, uin
     }
 }
 
-__device__ __forceinline__ void defer_chunks(const EngineArgs &a, const Snap &s, uint32_t v, int L, uint32_t len)
+// chunk items: vertex | list << 32 | op << 33 | chunk << 35
+enum ChunkOp : uint32_t { CO_PUSH = 0, CO_RESCUE = 1, CO_EXPAND = 2, CO_PULL = 3 };
+
+__device__ __forceinline__ void defer_chunks(const EngineArgs &a, const Snap &s, uint32_t v, int L, uint32_t len,
+                                             ChunkOp op)
 {
     uint32_t nch = (len + CHUNK - 1) / CHUNK;
     uint32_t pos = atomicAdd(&s.c->n_chunk, nch);
@@ -155,19 +163,48 @@ __device__ __forceinline__ void defer_chunks(const EngineArgs &a, const Snap &s,
         return;
     }
     for (uint32_t k = 0; k < nch; k++)
-        a.cq[pos + k] = (uint64_t)v | ((uint64_t)L << 32) | ((uint64_t)k << 33);
+        a.cq[pos + k] = (uint64_t)v | ((uint64_t)L << 32) | ((uint64_t)op << 33) | ((uint64_t)k << 35);
+}
+
+// Invalidation marks (graph.hpp:1729) carry the deletion run's index: idx =
+// invalid in this run, -idx = rescued in this run.  Marks of earlier runs are
+// left in place and read as "not marked" (runs get fresh indices; the host
+// clears the array when the index wraps).
+__device__ __forceinline__ bool marked(int32_t iv, int32_t idx) { return iv == idx || iv == -idx; }
+
+// mark x invalid unless this run already marked (or rescued) it
+__device__ __forceinline__ bool mark_inv(const EngineArgs &a, const Snap &s, uint32_t x)
+{
+    int32_t old = a.inv[x];
+    while (!marked(old, s.idx)) {
+        int32_t prev = atomicCAS(&a.inv[x], old, s.idx);
+        if (prev == old) return true;
+        old = prev;
+    }
+    return false;
 }
 
 // has_parents (graph.hpp:1889-1907): true when an ancestor of x (x
-// included) carries the current invalidation mark; a walk longer than the cap
-// asks for a full rebuild, like the reference's 0.1*V guard.
+// included) carries a mark of this run; a walk longer than the cap asks for a
+// full rebuild, like the reference's 0.1*V guard.  Values are monotone along
+// tree paths (BFS/SSSP non-decreasing, SSWP non-increasing away from the
+// root), and every marked vertex descends from a deletion root, so an
+// ancestor whose value is below the smallest root value (above the largest,
+// for SSWP) has no marked ancestor either and the walk stops there.
+template <int A>
 __device__ bool has_parents(const EngineArgs &a, const Snap &s, uint32_t x, int32_t idx)
 {
     uint32_t steps = 0;
     for (;;) {
-        int32_t iv = a.inv[x];
-        if (iv == idx || iv == -idx) return true;
-        uint32_t p = vpar(a.val[x]);
+        const uint64_t w = a.val[x];
+        const int32_t iv = a.inv[x];
+        if constexpr (A == SSWP) {
+            if (vval(w) > s.rbound) return false;
+        } else if constexpr (A != WCC) {
+            if (vval(w) < s.rbound) return false;
+        }
+        if (marked(iv, idx)) return true;
+        uint32_t p = vpar(w);
         if (p == NILP) return false;
         x = p;
         if (++steps > a.tune.walk_limit) {
@@ -205,8 +242,13 @@ __device__ void push_range(const EngineArgs &a, const Snap &s, uint32_t u, uint3
     }
 }
 
+// rescue (graph.hpp:1323-1361): re-parent d onto a live in-edge whose source
+// gives d's current value and lies outside the invalidated subtree.  The
+// winning lane returns true.  The mark goes idx -> -idx -> 0: once d has its
+// new parent it is a valid ancestor again for later rescues (a reader that
+// still sees -idx, or 0 with the old parent, backs off, which is safe).
 template <int A>
-__device__ void rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uint32_t dv, uint64_t off,
+__device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uint32_t dv, uint64_t off,
                              uint32_t b, uint32_t e, int lane, int stride)
 {
     for (uint32_t i = b + lane; i < e; i += stride) {
@@ -216,14 +258,18 @@ __device__ void rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uin
         uint32_t w;
         entry_w<A>(a, off + i, w);
         if (cand<A>(vval(a.val[n]), w) != (uint64_t)dv) continue;
-        if (has_parents(a, s, n, s.idx)) continue;
+        if (has_parents<A>(a, s, n, s.idx)) continue;
         if (atomicCAS(&a.inv[d], s.idx, -s.idx) == s.idx) {
             a.val[d] = pack(dv, n);
-            break;
+            a.inv[d] = 0;
+            return true;
         }
     }
+    return false;
 }
 
+// children of x through live out-edges whose target still names x as parent
+// join the next level (graph.hpp:1382-1417)
 __device__ void expand_range(const EngineArgs &a, const Snap &s, uint32_t x, uint64_t off, uint32_t b,
                              uint32_t e, int lane, int stride)
 {
@@ -232,19 +278,22 @@ __device__ void expand_range(const EngineArgs &a, const Snap &s, uint32_t x, uin
         uint32_t c = a.slots.nbr[off + i];
         if (c & DEAD) continue;
         if (vpar(a.val[c]) != x) continue;
-        if (atomicCAS(&a.inv[c], 0, s.idx) == 0) q_append(a, s.c, qn, &s.c->n_next, c);
+        if (mark_inv(a, s, c)) q_append(a, s.c, qn, &s.c->n_next, c);
     }
 }
 
-// best candidate word over a range (min or max of value<<32|parent)
+// best candidate word over a range (min or max of value<<32|parent); in-
+// neighbours inside the invalidated subtree are skipped, their values are
+// being reset in the same pass
 template <int A>
-__device__ uint64_t pull_range(const EngineArgs &a, uint64_t off, uint32_t b, uint32_t e, int lane,
-                               int stride)
+__device__ uint64_t pull_range(const EngineArgs &a, const Snap &s, uint64_t off, uint32_t b, uint32_t e,
+                               int lane, int stride)
 {
     uint64_t best = (A == SSWP) ? 0ull : ~0ull;
     for (uint32_t i = b + lane; i < e; i += stride) {
         uint32_t n = a.slots.nbr[off + i];
         if (n & DEAD) continue;
+        if (a.inv[n] == s.idx) continue;
         uint32_t w;
         entry_w<A>(a, off + i, w);
         uint64_t c = cand<A>(vval(a.val[n]), w);
@@ -272,6 +321,14 @@ __device__ __forceinline__ void pull_apply(const EngineArgs &a, const Snap &s, u
     if (relax<A>(a.val, x, vval(best), vpar(best))) mark_mod(a, s, x);
 }
 
+// any lane of the calling lane's 16-lane group
+__device__ __forceinline__ bool group_any(bool p)
+{
+    const uint64_t b = __ballot(p);
+    const uint32_t base = __lane_id() & ~(uint32_t)(GROUP - 1);
+    return ((b >> base) & ((1ull << GROUP) - 1)) != 0;
+}
+
 // ---------------------------------------------------------------------
 // passes
 // ---------------------------------------------------------------------
@@ -286,38 +343,55 @@ __device__ void seed_edge(const EngineArgs &a, const Snap &s, uint32_t src, uint
     }
 }
 
+// a root also widens the value band of the invalidated subtree (has_parents)
+template <int A>
 __device__ __forceinline__ void mark_root(const EngineArgs &a, const Snap &s, uint32_t src, uint32_t dst)
 {
-    if (vpar(a.val[dst]) != src) return;
-    if (atomicCAS(&a.inv[dst], 0, s.idx) == 0) q_append(a, s.c, a.q[s.sel_next], &s.c->n_next, dst);
+    const uint64_t w = a.val[dst];
+    if (vpar(w) != src) return;
+    if (mark_inv(a, s, dst)) {
+        if constexpr (A == SSWP) atomicMax(&s.c->rbound, vval(w));
+        else if constexpr (A != WCC) atomicMin(&s.c->rbound, vval(w));
+        q_append(a, s.c, a.q[s.sel_next], &s.c->n_next, dst);
+    }
 }
 
+// One thread per edge op.  SEED: relax the inserted edges that created a live
+// entry (update_tree_add batch seeding, graph.hpp:1162-1218).  DEL_ROOTS: mark
+// targets of deleted tree edges (update_tree_del batch, graph.hpp:1658-1714).
+// DEL_PULL: seed the inserted edges of a mixed batch whose endpoints both lie
+// outside the invalidated subtree (an edge touching the subtree is covered by
+// the pull into its target or the push from its source).
 template <int A>
 __device__ void pass_ops(const EngineArgs &a, const Snap &s, uint32_t tid, uint32_t nthr)
 {
+    const bool pull = s.phase == PH_DEL_PULL;
     for (uint32_t i = tid; i < s.n_ops; i += nthr) {
         uint8_t k = a.ops.kind[i];
         uint32_t src = a.ops.src[i], dst = a.ops.dst[i];
         uint32_t w = (A == SSSP || A == SSWP) ? a.ops.w[i] : 0;
         bool both = a.undirected || (k & K_UNDIR);
-        if (s.phase == PH_SEED) {
+        if (s.phase == PH_SEED || pull) {
             if (!(k & K_INS)) continue;
-            bool seed = k & K_FORCE;
-            if (!seed && a.ops.old[i] == 0 && a.ops.slot[i] != NILP) {
-                uint64_t off = a.lists[0].off[src];
-                seed = a.slots.cnt[off + a.ops.slot[i]] > 0;
-            }
+            // an insert reaches the engine when it created a live entry
+            // (bfs_inc.cpp:162-165: add_edge returned 0 and the batch left it)
+            const bool seed = (k & K_FORCE) || (a.ops.old[i] == 0 && a.ops.live[i]);
             if (!seed) continue;
+            if (pull && (a.inv[src] == s.idx || a.inv[dst] == s.idx)) continue;
             seed_edge<A>(a, s, src, dst, w);
             if (both) seed_edge<A>(a, s, dst, src, w);
         } else {
             if (!(k & K_DEL)) continue;
             if (!((k & K_FORCE) || a.ops.old[i] == 1)) continue;
-            mark_root(a, s, src, dst);
-            if (both) mark_root(a, s, dst, src);
+            mark_root<A>(a, s, src, dst);
+            if (both) mark_root<A>(a, s, dst, src);
         }
     }
 }
+
+// queue entries of the deletion levels: bit 31 = rescue already attempted
+// (its in-list scan was deferred to chunk items), only expand
+constexpr uint32_t Q_EXPAND_ONLY = 0x80000000u;
 
 template <int A>
 __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, uint32_t nthr)
@@ -332,78 +406,97 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
         }
         return;
     }
-    if (ph == PH_DEL_RESET) {
-        const uint32_t *qt = a.q[s.sel_tree];
-        for (uint32_t i = tid; i < s.n_tree; i += nthr) {
-            uint32_t x = qt[i];
-            a.val[x] = pack(a.init[x], NILP);
-            a.inv[x] = 0;
-            mark_mod(a, s, x);
-        }
-        return;
-    }
     const int glane = tid % GROUP;
     const uint32_t gid = tid / GROUP, ngr = nthr / GROUP;
     const uint32_t n = (ph == PH_DEL_PULL) ? s.n_tree : s.n_cur;
     const uint32_t *qv = (ph == PH_DEL_PULL) ? a.q[s.sel_tree] : a.q[s.sel_cur];
     for (uint32_t gi = gid; gi < n; gi += ngr) {
-        uint32_t v = qv[gi];
+        const uint32_t raw = qv[gi];
+        const uint32_t v = raw & ~Q_EXPAND_ONLY;
         if (ph == PH_PUSH) {
             uint32_t uv = vval(a.val[v]);
             for (int li = 0; li < a.npush; li++) {
                 int L = a.push_l[li];
                 uint32_t len = a.lists[L].len[v];
                 if (len > LONG_LIST) {
-                    if (glane == 0) defer_chunks(a, s, v, L, len);
+                    if (glane == 0) defer_chunks(a, s, v, L, len, CO_PUSH);
                     continue;
                 }
                 push_range<A>(a, s, v, uv, a.lists[L].off[v], 0, len, glane, GROUP);
             }
-        } else if (ph == PH_DEL_RESCUE) {
-            if (a.inv[v] != s.idx) continue;
-            uint32_t dv = vval(a.val[v]);
-            for (int li = 0; li < a.npull; li++) {
-                int L = a.pull_l[li];
-                uint32_t len = a.lists[L].len[v];
-                if (len > LONG_LIST) {
-                    if (glane == 0) defer_chunks(a, s, v, L, len);
+        } else if (ph == PH_DEL_RESCUE || ph == PH_DEL_EXPAND) {
+            // one deletion level: try to rescue v, else keep it invalid and
+            // mark its children (rescue then expand per vertex, one pass)
+            if (a.inv[v] != s.idx) continue;   // rescued
+            if (ph == PH_DEL_RESCUE && !(raw & Q_EXPAND_ONLY)) {
+                const uint32_t dv = vval(a.val[v]);
+                bool won = false, deferred = false;
+                for (int li = 0; li < a.npull; li++) {
+                    int L = a.pull_l[li];
+                    uint32_t len = a.lists[L].len[v];
+                    if (len > LONG_LIST) {
+                        if (glane == 0) defer_chunks(a, s, v, L, len, CO_RESCUE);
+                        deferred = true;
+                        continue;
+                    }
+                    won |= rescue_range<A>(a, s, v, dv, a.lists[L].off[v], 0, len, glane, GROUP);
+                }
+                if (group_any(won)) continue;
+                if (deferred) {
+                    // the long in-list is scanned in this pass's chunk stage;
+                    // v is expanded at the next level unless that rescues it
+                    if (glane == 0) q_append(a, s.c, a.q[s.sel_next], &s.c->n_next, v | Q_EXPAND_ONLY);
                     continue;
                 }
-                rescue_range<A>(a, s, v, dv, a.lists[L].off[v], 0, len, glane, GROUP);
             }
-        } else if (ph == PH_DEL_EXPAND) {
-            int32_t iv = a.inv[v];
-            if (iv == -s.idx) {
-                if (glane == 0) a.inv[v] = 0;
-                continue;
-            }
-            if (iv != s.idx) continue;
             if (glane == 0) q_append(a, s.c, a.q[s.sel_tree], &s.c->n_tree, v);
             for (int li = 0; li < a.npush; li++) {
                 int L = a.push_l[li];
                 uint32_t len = a.lists[L].len[v];
                 if (len > LONG_LIST) {
-                    if (glane == 0) defer_chunks(a, s, v, L, len);
+                    if (glane == 0) defer_chunks(a, s, v, L, len, CO_EXPAND);
                     continue;
                 }
                 expand_range(a, s, v, a.lists[L].off[v], 0, len, glane, GROUP);
             }
         } else if (ph == PH_DEL_PULL) {
+            // reset v and pull its best candidate from in-neighbours outside
+            // the subtree (graph.hpp:1420-1510); long lists finish in chunks
             uint64_t best = (A == SSWP) ? 0ull : ~0ull;
             for (int li = 0; li < a.npull; li++) {
                 int L = a.pull_l[li];
                 uint32_t len = a.lists[L].len[v];
                 if (len > LONG_LIST) {
-                    if (glane == 0) defer_chunks(a, s, v, L, len);
+                    if (glane == 0) defer_chunks(a, s, v, L, len, CO_PULL);
                     continue;
                 }
-                uint64_t b = pull_range<A>(a, a.lists[L].off[v], 0, len, glane, GROUP);
+                uint64_t b = pull_range<A>(a, s, a.lists[L].off[v], 0, len, glane, GROUP);
                 if (A == SSWP ? b > best : b < best) best = b;
             }
             best = reduce_best<A>(best, GROUP);
-            if (glane == 0) pull_apply<A>(a, s, v, best);
+            if (glane == 0) {
+                const uint32_t iv = a.init[v];
+                const bool take = (A == SSWP) ? (best != 0ull && vval(best) > iv)
+                                              : (best != ~0ull && vval(best) < iv);
+                a.val[v] = take ? best : pack(iv, NILP);
+                push_next(a, s, v);
+                mark_mod(a, s, v);
+            }
         }
     }
+}
+
+// The vertex stage of a pass: op passes, vertex passes, and the mixed
+// pull pass (tree vertices plus the seeds of the batch's inserts).
+template <int A>
+__device__ __forceinline__ void pass_main(const EngineArgs &a, const Snap &s, uint32_t tid, uint32_t nthr)
+{
+    if (s.phase == PH_SEED || s.phase == PH_DEL_ROOTS) {
+        pass_ops<A>(a, s, tid, nthr);
+        return;
+    }
+    pass_vertices<A>(a, s, tid, nthr);
+    if (s.phase == PH_DEL_PULL && s.seed) pass_ops<A>(a, s, tid, nthr);
 }
 
 template <int A>
@@ -417,26 +510,30 @@ __device__ void pass_chunks(const EngineArgs &a, const Snap &s, uint32_t tid, ui
         uint64_t item = a.cq[ci];
         uint32_t v = (uint32_t)item;
         int L = (int)((item >> 32) & 1);
-        uint32_t k = (uint32_t)(item >> 33);
+        uint32_t op = (uint32_t)((item >> 33) & 3);
+        uint32_t k = (uint32_t)(item >> 35);
         uint32_t len = a.lists[L].len[v];
         uint64_t off = a.lists[L].off[v];
         uint32_t b = k * CHUNK, e = b + CHUNK < len ? b + CHUNK : len;
-        if (ph == PH_PUSH) {
+        if (op == CO_PUSH) {
             push_range<A>(a, s, v, vval(a.val[v]), off, b, e, lane, WAVE);
-        } else if (ph == PH_DEL_RESCUE) {
+        } else if (op == CO_RESCUE) {
             if (a.inv[v] != s.idx) continue;
             rescue_range<A>(a, s, v, vval(a.val[v]), off, b, e, lane, WAVE);
-        } else if (ph == PH_DEL_EXPAND) {
+        } else if (op == CO_EXPAND) {
             expand_range(a, s, v, off, b, e, lane, WAVE);
         } else {
-            uint64_t best = pull_range<A>(a, off, b, e, lane, WAVE);
+            uint64_t best = pull_range<A>(a, s, off, b, e, lane, WAVE);
             best = reduce_best<A>(best, WAVE);
             if (lane == 0) pull_apply<A>(a, s, v, best);
         }
     }
 }
 
-// phase transitions, run by one thread after a pass completed
+// phase transitions, run by one thread after a pass completed.  A batch with
+// deletions runs them first (roots, levels, pull), and its inserts are seeded
+// in the pull pass, so one push phase settles both; values are the unique
+// fixpoint either way, as in the reference's add-then-delete batch order.
 __device__ void advance(const EngineArgs &a, Ctl *c)
 {
     uint32_t ph = ald(&c->phase);
@@ -449,14 +546,6 @@ __device__ void advance(const EngineArgs &a, Ctl *c)
         ast(&c->epoch, ald(&c->epoch) + 1);
         c->levels += 1;
     };
-    auto after_push = [&]() {
-        if (ald(&c->del_pending)) {
-            ast(&c->del_pending, 0u);
-            ast(&c->phase, (uint32_t)PH_DEL_ROOTS);
-        } else {
-            ast(&c->phase, (uint32_t)PH_DONE);
-        }
-    };
     auto rescue_on = [&](uint32_t level, uint32_t n) {
         return level <= a.tune.rescue_level || n < a.tune.rescue_frontier;
     };
@@ -465,54 +554,37 @@ __device__ void advance(const EngineArgs &a, Ctl *c)
         return level <= a.tune.recompute_level && (double)tot > (double)a.tune.recompute_frac * a.V &&
                tot > a.tune.recompute_abs;
     };
+    auto level_next = [&](uint32_t level) {
+        uint32_t n = ald(&c->n_cur);
+        if (ald(&c->flags) & F_NEED_REBUILD) ast(&c->phase, (uint32_t)PH_REBUILD);
+        else if (n == 0) ast(&c->phase, (uint32_t)PH_DEL_PULL);
+        else if (recompute(level, n)) ast(&c->phase, (uint32_t)PH_REBUILD);
+        else ast(&c->phase, rescue_on(level, n) ? (uint32_t)PH_DEL_RESCUE : (uint32_t)PH_DEL_EXPAND);
+    };
     ast(&c->n_chunk, 0u);
     ast(&c->stage, 0u);
     switch (ph) {
     case PH_SEED:
     case PH_BUILD_INIT:
     case PH_PUSH:
+    case PH_DEL_PULL:
         swap_queues();
-        if (ald(&c->n_cur) == 0) after_push();
-        else ast(&c->phase, (uint32_t)PH_PUSH);
+        ast(&c->seed, 0u);
+        ast(&c->phase, ald(&c->n_cur) ? (uint32_t)PH_PUSH : (uint32_t)PH_DONE);
         break;
-    case PH_DEL_ROOTS: {
+    case PH_DEL_ROOTS:
         swap_queues();
         ast(&c->level, 0u);
         ast(&c->n_tree, 0u);
-        uint32_t n = ald(&c->n_cur);
-        if (n == 0) ast(&c->phase, (uint32_t)PH_DONE);
-        else if (recompute(0, n)) ast(&c->phase, (uint32_t)PH_REBUILD);
-        else ast(&c->phase, rescue_on(0, n) ? (uint32_t)PH_DEL_RESCUE : (uint32_t)PH_DEL_EXPAND);
+        if (ald(&c->n_cur) == 0) ast(&c->phase, ald(&c->seed) ? (uint32_t)PH_SEED : (uint32_t)PH_DONE);
+        else level_next(0);
         break;
-    }
     case PH_DEL_RESCUE:
-        if (ald(&c->flags) & F_NEED_REBUILD) ast(&c->phase, (uint32_t)PH_REBUILD);
-        else ast(&c->phase, (uint32_t)PH_DEL_EXPAND);
-        break;
     case PH_DEL_EXPAND: {
         swap_queues();
         uint32_t level = ald(&c->level) + 1;
         ast(&c->level, level);
-        uint32_t n = ald(&c->n_cur);
-        if (n == 0) ast(&c->phase, (uint32_t)PH_DEL_RESET);
-        else if (recompute(level, n)) ast(&c->phase, (uint32_t)PH_REBUILD);
-        else ast(&c->phase, rescue_on(level, n) ? (uint32_t)PH_DEL_RESCUE : (uint32_t)PH_DEL_EXPAND);
-        break;
-    }
-    case PH_DEL_RESET:
-        ast(&c->phase, (uint32_t)PH_DEL_PULL);
-        break;
-    case PH_DEL_PULL: {
-        // push to the fixpoint starting from the whole reset subtree
-        uint32_t cur = ald(&c->sel_cur), nxt = ald(&c->sel_next), tree = ald(&c->sel_tree);
-        ast(&c->sel_cur, tree);
-        ast(&c->sel_next, cur);
-        ast(&c->sel_tree, nxt);
-        ast(&c->n_cur, ald(&c->n_tree));
-        ast(&c->n_next, 0u);
-        ast(&c->n_tree, 0u);
-        ast(&c->epoch, ald(&c->epoch) + 1);
-        ast(&c->phase, ald(&c->n_cur) ? (uint32_t)PH_PUSH : (uint32_t)PH_DONE);
+        level_next(level);
         break;
     }
     default:
@@ -523,15 +595,14 @@ __device__ void advance(const EngineArgs &a, Ctl *c)
     }
 }
 
-// work of the current pass, in units comparable with block_max; an op pass
-// (one thread per op, no list scans) counts 1/16 per op
+// work of the current pass, in units comparable with the team cap; an op
+// pass (one thread per op, no list scans) counts 1/16 per op
 __device__ __forceinline__ uint32_t phase_work(const EngineArgs &a, const Snap &s)
 {
     switch (s.phase) {
     case PH_SEED:
     case PH_DEL_ROOTS: return s.n_ops / 16;
-    case PH_DEL_RESET:
-    case PH_DEL_PULL: return s.n_tree;
+    case PH_DEL_PULL: return s.n_tree + (s.seed ? s.n_ops / 16 : 0);
     case PH_BUILD_INIT: return a.V;
     default: return s.n_cur;
     }
@@ -633,14 +704,20 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
         if (s.phase == PH_DONE || s.phase == PH_REBUILD || s.phase == PH_BUILD_INIT) break;
         if (ald(&c->flags) & F_ABORT) break;
         if (phase_work(a, s) > work_max) break;
-        const uint64_t t_pass = wall_clock64();
+        // stage clocks of this workgroup: start, vertex stage done, first
+        // barrier passed, chunk stage done
+        uint64_t tk[4];
+        tk[0] = wall_clock64();
+        tk[1] = tk[2] = tk[0];
         if (s.stage == 0) {
-            if (s.phase == PH_SEED || s.phase == PH_DEL_ROOTS) pass_ops<A>(a, s, tid, nthr);
-            else pass_vertices<A>(a, s, tid, nthr);
+            pass_main<A>(a, s, tid, nthr);
+            __syncthreads();
+            tk[1] = wall_clock64();
             // the chunk queue is complete once every workgroup finished
             team_sync(c, W, [&] {
                 if (ald(&c->n_chunk) > chunk_max) ast(&c->stage, 1u);
             });
+            tk[2] = wall_clock64();
             if (threadIdx.x == 0) {
                 sh[0] = ald(&c->n_chunk);
                 sh[1] = ald(&c->stage);
@@ -652,13 +729,26 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
             __syncthreads();
         }
         pass_chunks<A>(a, s, tid, nthr, sh[0]);
+        __syncthreads();
+        tk[3] = wall_clock64();
         team_sync(c, W, [&] {
             if (a.ticks && s.phase < NPHASE) {
-                a.ticks[s.phase] += wall_clock64() - t_pass;
+                a.ticks[s.phase] += wall_clock64() - tk[0];
                 a.ticks[NPHASE + s.phase] += 1;
             }
             advance(a, c);
         });
+        if (a.ticks && wg == 0 && threadIdx.x == 0 && s.phase < NPHASE) {
+            // workgroup 0's view: vertex stage, first barrier wait, chunk
+            // stage, second barrier wait; pass work and chunk items
+            unsigned long long *d = a.ticks + 2 * NPHASE + s.phase;
+            d[0] += tk[1] - tk[0];
+            d[NPHASE] += tk[2] - tk[1];
+            d[2 * NPHASE] += tk[3] - tk[2];
+            d[3 * NPHASE] += wall_clock64() - tk[3];
+            d[4 * NPHASE] += phase_work(a, s);
+            d[5 * NPHASE] += sh[0];
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -715,8 +805,7 @@ __global__ void k_grid_main(EngineArgs a)
     const Snap s = snap;
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthr = gridDim.x * blockDim.x;
     if (s.stage != 0) return;
-    if (s.phase == PH_SEED || s.phase == PH_DEL_ROOTS) pass_ops<A>(a, s, tid, nthr);
-    else pass_vertices<A>(a, s, tid, nthr);
+    pass_main<A>(a, s, tid, nthr);
 }
 
 template <int A>
@@ -758,8 +847,7 @@ static void grid_launch(const EngineArgs &a, const Ctl &h, hipStream_t st)
     case PH_SEED:
     case PH_DEL_ROOTS: threads = h.n_ops; break;
     case PH_BUILD_INIT: threads = a.V; break;
-    case PH_DEL_RESET: threads = h.n_tree; break;
-    case PH_DEL_PULL: threads = (uint64_t)h.n_tree * GROUP; break;
+    case PH_DEL_PULL: threads = std::max<uint64_t>((uint64_t)h.n_tree * GROUP, h.seed ? h.n_ops : 0); break;
     default: threads = (uint64_t)h.n_cur * GROUP; break;
     }
     uint64_t blocks = (threads + 255) / 256;

@@ -10,7 +10,8 @@
 //               local slot) -- per-list open-addressing index built once a
 //               list reaches 512 entries (storage.hpp:304, 358-367).
 //   headers     per direction (out, in): off, len, cap, deg, hoff, hcap,
-//               hcnt, head (batch op chains).
+//               hcnt, nops / seg (the current batch's ops on the list and
+//               their segment in the batch's op-index buffer).
 // One labels instance (one algorithm's dependency tree):
 //   val[V]      u64 packed word value << 32 | parent (parent 0xFFFFFFFF =
 //               empty), the device form of VertexTree (graph.hpp:54-59),
@@ -53,7 +54,8 @@ struct List {
     uint64_t *hoff;
     uint32_t *hcap;
     uint32_t *hcnt;
-    int32_t *head;
+    uint32_t *nops;   // list ops of the current batch (0 between batches)
+    uint32_t *seg;    // their segment start in ListOps::segbuf
 };
 
 // Slab allocator over the slot pool (0) and the hash pool (1): blocks are
@@ -68,7 +70,7 @@ struct Pool {
     unsigned long long slot_top, slot_cap;
     unsigned long long hash_top, hash_cap;
     unsigned int oom;
-    unsigned int pad;
+    unsigned int seg_top;   // segment allocation of the current batch
     unsigned int free_n[2][NCLS];
     unsigned int free_take[2][NCLS];
     unsigned int pend_n[2][NCLS];
@@ -86,21 +88,23 @@ struct EdgeOps {
     uint32_t *src, *dst, *w;
     uint8_t *kind;
     uint32_t *old;      // reference add/del return value
-    uint32_t *slot;     // local index of the (src->dst) entry in src's out-list
+    uint32_t *live;     // the (src->dst) entry of src's out-list is live after the batch
     uint32_t n;
 };
 
 // list-op flags
 constexpr uint8_t L_IN = 1;       // in-list (else out-list)
 constexpr uint8_t L_RET = 2;      // its old count is the edge op's return value
-constexpr uint8_t L_SLOT = 4;     // its local slot is the edge op's liveness slot
+constexpr uint8_t L_SLOT = 4;     // its entry's final count gives the edge op's liveness
 constexpr uint8_t L_SELF = 8;     // undirected self-loop folded into one +-2 op
 
 struct ListOps {
     uint32_t *v, *nbr, *w, *edge;
     int8_t *delta;
     uint8_t *flags;
-    int32_t *next;
+    uint32_t *rank;     // position among the batch's ops on the same list
+    uint32_t *segbuf;   // list-op indices grouped by list (segments)
+    uint32_t *slot;     // local index the op resolved to (NILP: none)
     uint32_t n;
 };
 
@@ -118,6 +122,10 @@ enum Phase : uint32_t {
 };
 
 constexpr int NPHASE = 10;
+// per-phase team statistics, NPHASE words each: wall-clock ticks, passes,
+// then workgroup 0's vertex stage, first barrier, chunk stage and second
+// barrier ticks, summed pass work and summed chunk items
+constexpr int NTICK = 8;
 
 constexpr uint32_t F_NEED_REBUILD = 1;
 constexpr uint32_t F_QUEUE_OVERFLOW = 2;
@@ -125,7 +133,7 @@ constexpr uint32_t F_ALL_MODIFIED = 4;
 constexpr uint32_t F_ABORT = 8;        // a team barrier timed out
 
 struct Ctl {
-    uint32_t phase, stage, level, del_pending;
+    uint32_t phase, stage, level, seed;   // seed: inserts still to seed (mixed batch)
     uint32_t n_cur, n_next, n_tree, n_chunk;
     uint32_t n_mod, n_ops;
     uint32_t sel_cur, sel_next, sel_tree;
@@ -135,6 +143,8 @@ struct Ctl {
     uint32_t bar_count;    // team barrier: arrivals (monotonic within a launch)
     uint32_t bar_gen;      // team barrier: last released episode
     uint32_t bar_exit;     // team workgroups that left the launch
+    uint32_t rbound;       // smallest deletion-root value (largest for SSWP)
+    uint32_t pad;
     unsigned long long edges, relax_ok, levels, pad2;
 };
 
@@ -172,7 +182,7 @@ struct EngineArgs {
     uint32_t *mlist;
     Ctl *ctl;
     Ctl *hctl;                    // pinned host mirror, written at the end of each launch
-    unsigned long long *ticks;    // [2*NPHASE] per-phase wall-clock ticks and passes (team mode)
+    unsigned long long *ticks;    // [NTICK*NPHASE] per-phase pass statistics (team mode)
     EdgeOps ops;
     Tuning tune;
 };

@@ -4,15 +4,19 @@
 // called from Graph::add_edge / del_edge (core/graph.hpp:203-253).
 //
 // A batch of edge ops is expanded into list ops (out-list of src and in-list
-// of dst, or both endpoint lists of a symmetric graph).  Every list op is
-// pushed onto a per-(direction, vertex) chain with one atomicExch, so each
-// list is owned by exactly one wave in the apply kernel: no locks, no
-// cross-wave races, and the block of a list may be re-allocated (doubling,
-// from a bump pool) before the wave appends to it.  The owning wave applies
-// the chain's inserts first, then its deletes, so a batch behaves like the
-// reference batch driver (all adds, then all dels: bfs_inc_batch.cpp:142-178).
-// Lookups scan the list 64 entries per step (ballot on the match), or probe
-// the list's open-addressing index once it holds >= 512 entries.
+// of dst, or both endpoint lists of a symmetric graph).  Each list op takes
+// a rank on its list with one atomicAdd; the first op of every list reserves
+// the list's segment of an op-index buffer and every op drops its index into
+// place, so each list is owned by exactly one 16-lane group in the apply
+// kernel: no locks, no cross-group races, and the block of a list may be
+// re-allocated (doubling, from the slab pool) before the group appends to it.
+// The owner applies the list's inserts first, then its deletes, so a batch
+// behaves like the reference batch driver (all adds, then all dels:
+// bfs_inc_batch.cpp:142-178).  Short segments are applied one op at a time
+// (lookups scan the list 16 entries per step, or probe its open-addressing
+// index once it holds >= 512 entries); hub lists with many ops in one batch
+// are indexed and applied 16 ops per round, one op per lane, with same-key
+// ops of a round folded into one count update in lane order.
 #include "rg_internal.h"
 #include "rg_gen.h"
 
@@ -50,7 +54,7 @@ __global__ void k_expand_link(StoreArgs a)
     uint32_t s = a.ops.src[i], d = a.ops.dst[i], w = a.weighted ? a.ops.w[i] : 0;
     uint8_t kind = a.ops.kind[i];
     a.ops.old[i] = 0;
-    a.ops.slot[i] = NILP;
+    a.ops.live[i] = 0;
     uint32_t base = 4 * i;
     for (int k = 0; k < 4; k++) a.lops.v[base + k] = NILP;
     if (kind & K_NOSTORE) return;
@@ -70,15 +74,15 @@ __global__ void k_expand_link(StoreArgs a)
         a.lops.delta[j] = delta;
         a.lops.flags[j] = flags;
         int L = (flags & L_IN) ? 1 : 0;
-        a.lops.next[j] = atomicExch(&a.lists[L].head[v], (int32_t)j);
+        a.lops.rank[j] = atomicAdd(&a.lists[L].nops[v], 1u);
     };
     // graph.hpp:203-253: a directed op touches out[src] (+ in[dst] unless
     // symmetric); an undirected op is the directed op both ways and returns
     // the old count of the (d->s) direction, minus 1 for a self-loop on add.
     // In a batch, undirected updates keep both orientations at the same
     // count, so the return value is taken from the list of min(s, d): ops on
-    // one edge in either orientation then share one chain and see one serial
-    // order.  A single op returns old(d->s) exactly as the reference does
+    // one edge in either orientation then share one list owner and see one
+    // serial order.  A single op returns old(d->s) exactly as the reference does
     // (that differs only if directed ops were mixed into the same edge).
     // A self-loop's two same-key updates fold into one +-2 op.
     if (!undir) {
@@ -97,7 +101,48 @@ __global__ void k_expand_link(StoreArgs a)
 }
 
 // ---------------------------------------------------------------------
-// apply: one wave owns one list (the chain's head op)
+// segments: group the batch's list ops by list
+// ---------------------------------------------------------------------
+// the first op of every list reserves the list's segment (one counter
+// atomic per wave, from a wave-wide prefix sum of the segment sizes)
+__global__ void k_segment(StoreArgs a)
+{
+    const uint32_t nlops = 4 * a.ops.n;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t cnt = 0, v = NILP;
+    int L = 0;
+    if (j < nlops) {
+        v = a.lops.v[j];
+        if (v != NILP && a.lops.rank[j] == 0) {
+            L = (a.lops.flags[j] & L_IN) ? 1 : 0;
+            cnt = a.lists[L].nops[v];
+        }
+    }
+    const uint32_t lane = __lane_id();
+    uint32_t incl = cnt;
+    for (int d = 1; d < WAVE; d <<= 1) {
+        uint32_t y = __shfl_up(incl, d);
+        if (lane >= (uint32_t)d) incl += y;
+    }
+    const uint32_t total = __shfl(incl, WAVE - 1);
+    uint32_t base = 0;
+    if (lane == WAVE - 1 && total) base = atomicAdd(&a.pool->seg_top, total);
+    base = __shfl(base, WAVE - 1);
+    if (cnt) a.lists[L].seg[v] = base + incl - cnt;
+}
+
+__global__ void k_scatter(StoreArgs a)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= 4 * a.ops.n) return;
+    const uint32_t v = a.lops.v[j];
+    if (v == NILP) return;
+    const int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
+    a.lops.segbuf[a.lists[L].seg[v] + a.lops.rank[j]] = j;
+}
+
+// ---------------------------------------------------------------------
+// apply: one 16-lane group owns one list (its rank-0 op)
 // ---------------------------------------------------------------------
 struct ListState {
     uint64_t off, hoff;
@@ -260,7 +305,180 @@ __device__ uint32_t list_find(const StoreArgs &a, const ListState &st, uint32_t 
     return NILP;
 }
 
-constexpr int AG = 16;   // lanes per list in the apply / reserve kernels
+constexpr int AG = 16;        // lanes per list in the apply / reserve kernels
+constexpr uint32_t PAR_MIN = 8;   // lists with this many ops in a batch go 16 ops per round
+
+template <int G>
+__device__ __forceinline__ int32_t gsum(int32_t x)
+{
+    for (int m = G / 2; m > 0; m >>= 1) x += __shfl_xor(x, m, G);
+    return x;
+}
+
+// one op at a time, in segment order (short segments)
+__device__ bool apply_serial(const StoreArgs &a, ListState &st, const uint32_t *ops, uint32_t n, const Grp &g)
+{
+    for (int pass = 0; pass < 2; pass++) {
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t k = ops[i];
+            int8_t delta = a.lops.delta[k];
+            if ((pass == 0) != (delta > 0)) continue;
+            uint32_t key_nbr = a.lops.nbr[k];
+            uint32_t key_w = a.slots.wt ? a.lops.w[k] : 0;
+            uint32_t bucket = NILP;
+            uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, &bucket);
+            if (idx == NILP && delta > 0) {
+                // append a new entry with count 0 (storage.hpp:371-374, 389-392)
+                idx = st.len;
+                if (g.lane == 0) {
+                    a.slots.nbr[st.off + idx] = key_nbr | DEAD;
+                    a.slots.cnt[st.off + idx] = 0;
+                    if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
+                }
+                st.len++;
+                __threadfence_block();
+                if (st.hoff != HEMPTY) {
+                    if (2 * (st.hcnt + 1) > st.hcap) {
+                        if (!build_index<AG>(a, st, st.hcap * 2, g)) return false;
+                    } else {
+                        if (g.lane == 0) {
+                            a.slots.hkey[st.hoff + bucket] = hkey_of(key_nbr, key_w);
+                            a.slots.hval[st.hoff + bucket] = idx;
+                        }
+                        st.hcnt++;
+                        __threadfence_block();
+                    }
+                } else if (st.len >= INDEX_THRESHOLD) {
+                    uint32_t hc = next_pow2(2 * st.len);
+                    if (hc < 1024) hc = 1024;
+                    if (!build_index<AG>(a, st, hc, g)) return false;
+                }
+            }
+            uint32_t c = idx == NILP ? 0 : a.slots.cnt[st.off + idx];
+            uint32_t nc, ret;
+            int32_t dchange;
+            switch (delta) {
+            case 1: nc = c + 1; ret = c; dchange = 1; break;
+            case -1: nc = c ? c - 1 : 0; ret = c; dchange = c ? -1 : 0; break;
+            case 2: nc = c + 2; ret = c; dchange = 2; break;
+            default: /* -2: undirected self-loop delete, two -1 steps */
+                ret = c ? c - 1 : 0;
+                nc = c >= 2 ? c - 2 : 0;
+                dchange = -(int32_t)(c >= 2 ? 2 : c);
+                break;
+            }
+            if (idx != NILP && g.lane == 0) {
+                a.slots.cnt[st.off + idx] = nc;
+                if ((c == 0) != (nc == 0))
+                    a.slots.nbr[st.off + idx] = key_nbr | (nc ? 0u : DEAD);
+            }
+            st.deg = (uint32_t)((int32_t)st.deg + dchange);
+            if (g.lane == 0) {
+                if (a.lops.flags[k] & L_RET) a.ops.old[a.lops.edge[k]] = ret;
+                a.lops.slot[k] = idx;
+            }
+            __threadfence_block();
+        }
+    }
+    return true;
+}
+
+// 16 ops per round, one per lane (hub lists).  The list is indexed first with
+// room for every insert, so lanes probe independently; ops of a round with
+// the same key are folded: the first such lane (the leader) appends a new
+// entry if needed and applies the summed change to the count, and lane r
+// reports the count it would have seen had the ops run one by one in lane
+// order (clamped at 0 for deletes, as storage.hpp:378/393 do).
+__device__ bool apply_parallel(const StoreArgs &a, ListState &st, const uint32_t *ops, uint32_t n, uint32_t n_ins,
+                               const Grp &g)
+{
+    uint32_t hc = next_pow2(2 * (st.len + n_ins));
+    if (hc < 1024) hc = 1024;
+    if (st.hoff == HEMPTY || st.hcap < hc) {
+        if (!build_index<AG>(a, st, hc, g)) return false;
+    }
+    const uint32_t mask = st.hcap - 1;
+    for (int pass = 0; pass < 2; pass++) {
+        for (uint32_t r0 = 0; r0 < n; r0 += AG) {
+            const uint32_t i = r0 + g.lane;
+            const uint32_t k = i < n ? ops[i] : 0;
+            const int8_t delta = i < n ? a.lops.delta[k] : (int8_t)0;
+            const bool act = i < n && ((pass == 0) == (delta > 0));
+            const uint32_t key_nbr = act ? a.lops.nbr[k] : 0;
+            const uint32_t key_w = (act && a.slots.wt) ? a.lops.w[k] : 0;
+            const uint64_t key = act ? hkey_of(key_nbr, key_w) : HEMPTY;
+            const uint32_t ad = act ? (uint32_t)(delta < 0 ? -delta : delta) : 0;
+            uint32_t idx = NILP, bucket = NILP;
+            if (act) {
+                uint32_t b = (uint32_t)mix64(key) & mask;
+                for (uint32_t p = 0; p < st.hcap; p++, b = (b + 1) & mask) {
+                    const uint64_t kk = a.slots.hkey[st.hoff + b];
+                    if (kk == key) {
+                        idx = a.slots.hval[st.hoff + b];
+                        break;
+                    }
+                    if (kk == HEMPTY) {
+                        bucket = b;
+                        break;
+                    }
+                }
+            }
+            // same-key lanes of this round
+            uint32_t leader = g.lane, pre = 0, tot = 0;
+            bool first = true;
+            for (int q = 0; q < AG; q++) {
+                const uint64_t kq = __shfl((unsigned long long)key, g.base + q);
+                const uint32_t aq = __shfl(ad, g.base + q);
+                if (act && aq && kq == key) {
+                    if (first) {
+                        leader = (uint32_t)q;
+                        first = false;
+                    }
+                    if (q < g.lane) pre += aq;
+                    tot += aq;
+                }
+            }
+            const bool lead = act && leader == (uint32_t)g.lane;
+            // keys new to the list: each leader appends an entry with count 0
+            const bool fresh = pass == 0 && lead && idx == NILP;
+            const uint32_t fb = gballot<AG>(fresh, g);
+            if (fresh) {
+                idx = st.len + (uint32_t)__popc(fb & ((1u << g.lane) - 1));
+                a.slots.nbr[st.off + idx] = key_nbr | DEAD;
+                a.slots.cnt[st.off + idx] = 0;
+                if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
+                uint32_t b = bucket;
+                for (uint32_t p = 0; p < st.hcap; p++, b = (b + 1) & mask) {
+                    if (atomicCAS((unsigned long long *)&a.slots.hkey[st.hoff + b], (unsigned long long)HEMPTY,
+                                  (unsigned long long)key) == HEMPTY) {
+                        a.slots.hval[st.hoff + b] = idx;
+                        break;
+                    }
+                }
+            }
+            st.len += (uint32_t)__popc(fb);
+            st.hcnt += (uint32_t)__popc(fb);
+            idx = __shfl(idx, g.base + (int)leader);
+            uint32_t c = 0, cf = 0;
+            if (lead && idx != NILP) {
+                c = a.slots.cnt[st.off + idx];
+                cf = pass == 0 ? c + tot : (c > tot ? c - tot : 0);
+                a.slots.cnt[st.off + idx] = cf;
+                if ((c == 0) != (cf == 0)) a.slots.nbr[st.off + idx] = key_nbr | (cf ? 0u : DEAD);
+            }
+            st.deg = (uint32_t)((int32_t)st.deg + gsum<AG>(lead ? (int32_t)cf - (int32_t)c : 0));
+            c = __shfl(c, g.base + (int)leader);
+            if (act) {
+                const uint32_t cb = pass == 0 ? c + pre : (c > pre ? c - pre : 0);
+                const uint32_t ret = delta == -2 ? (cb ? cb - 1 : 0) : cb;
+                if (a.lops.flags[k] & L_RET) a.ops.old[a.lops.edge[k]] = ret;
+                a.lops.slot[k] = idx;
+            }
+            __threadfence_block();
+        }
+    }
+    return true;
+}
 
 __global__ void k_apply(StoreArgs a)
 {
@@ -272,11 +490,13 @@ __global__ void k_apply(StoreArgs a)
     const uint32_t ng = (gridDim.x * blockDim.x) / AG;
     const uint32_t nlops = 4 * a.ops.n;
     for (uint32_t j = gid; j < nlops; j += ng) {
-        uint32_t v = a.lops.v[j];
+        const uint32_t v = a.lops.v[j];
         if (v == NILP) continue;
-        int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
+        if (a.lops.rank[j] != 0) continue;   // the list's rank-0 op owns it
+        const int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
         const List &li = a.lists[L];
-        if (li.head[v] != (int32_t)j) continue;   // not the chain head: its owner applies it
+        const uint32_t n = li.nops[v];
+        const uint32_t *ops = a.lops.segbuf + li.seg[v];
 
         ListState st;
         st.off = li.off[v];
@@ -287,78 +507,28 @@ __global__ void k_apply(StoreArgs a)
         st.hcap = li.hcap[v];
         st.hcnt = li.hcnt[v];
 
-        // reserve room for every insert in the chain up front; a chain is at
-        // most nlops long (the walk bound guards against a corrupt link)
-        uint32_t n_ins = 0, clen = 0;
+        // room for every insert of the segment up front
+        int32_t n_ins = 0;
+        for (uint32_t i = g.lane; i < n; i += AG) n_ins += a.lops.delta[ops[i]] > 0 ? 1 : 0;
+        n_ins = gsum<AG>(n_ins);
         bool ok = true;
-        for (int32_t k = (int32_t)j; k >= 0 && (uint32_t)k < nlops; k = a.lops.next[k]) {
-            if (a.lops.delta[k] > 0) n_ins++;
-            if (++clen > nlops) { ok = false; break; }
+        if (st.len + (uint32_t)n_ins > st.cap) ok = grow_block<AG>(a, st, st.len + (uint32_t)n_ins, g);
+        if (ok) {
+            // lists below the index threshold stay unindexed (hash memory is
+            // spent only where the reference indexes too)
+            const bool par = n >= PAR_MIN && (st.hoff != HEMPTY || st.len + (uint32_t)n_ins >= INDEX_THRESHOLD);
+            if (par) ok = apply_parallel(a, st, ops, n, (uint32_t)n_ins, g);
+            else ok = apply_serial(a, st, ops, n, g);
         }
-        if (!ok && g.lane == 0) atomicOr(a.err, 2u);
-        if (ok && st.len + n_ins > st.cap) ok = grow_block<AG>(a, st, st.len + n_ins, g);
-
-        for (int pass = 0; pass < 2 && ok; pass++) {
-            for (int32_t k = (int32_t)j; k >= 0 && (uint32_t)k < nlops; k = a.lops.next[k]) {
-                int8_t delta = a.lops.delta[k];
-                if ((pass == 0) != (delta > 0)) continue;
-                uint32_t key_nbr = a.lops.nbr[k];
-                uint32_t key_w = a.slots.wt ? a.lops.w[k] : 0;
-                uint32_t bucket = NILP;
-                uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, &bucket);
-                if (idx == NILP && delta > 0) {
-                    // append a new entry with count 0 (storage.hpp:371-374, 389-392)
-                    idx = st.len;
-                    if (g.lane == 0) {
-                        a.slots.nbr[st.off + idx] = key_nbr | DEAD;
-                        a.slots.cnt[st.off + idx] = 0;
-                        if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
-                    }
-                    st.len++;
-                    __threadfence_block();
-                    if (st.hoff != HEMPTY) {
-                        if (2 * (st.hcnt + 1) > st.hcap) {
-                            if (!build_index<AG>(a, st, st.hcap * 2, g)) { ok = false; break; }
-                        } else {
-                            if (g.lane == 0) {
-                                a.slots.hkey[st.hoff + bucket] = hkey_of(key_nbr, key_w);
-                                a.slots.hval[st.hoff + bucket] = idx;
-                            }
-                            st.hcnt++;
-                            __threadfence_block();
-                        }
-                    } else if (st.len >= INDEX_THRESHOLD) {
-                        uint32_t hc = next_pow2(2 * st.len);
-                        if (hc < 1024) hc = 1024;
-                        if (!build_index<AG>(a, st, hc, g)) { ok = false; break; }
-                    }
-                }
-                uint32_t c = idx == NILP ? 0 : a.slots.cnt[st.off + idx];
-                uint32_t nc, ret;
-                int32_t dchange;
-                switch (delta) {
-                case 1: nc = c + 1; ret = c; dchange = 1; break;
-                case -1: nc = c ? c - 1 : 0; ret = c; dchange = c ? -1 : 0; break;
-                case 2: nc = c + 2; ret = c; dchange = 2; break;
-                default: /* -2: undirected self-loop delete, two -1 steps */
-                    ret = c ? c - 1 : 0;
-                    nc = c >= 2 ? c - 2 : 0;
-                    dchange = -(int32_t)(c >= 2 ? 2 : c);
-                    break;
-                }
-                if (idx != NILP && g.lane == 0) {
-                    a.slots.cnt[st.off + idx] = nc;
-                    if ((c == 0) != (nc == 0))
-                        a.slots.nbr[st.off + idx] = key_nbr | (nc ? 0u : DEAD);
-                }
-                st.deg = (uint32_t)((int32_t)st.deg + dchange);
-                uint8_t fl = a.lops.flags[k];
-                if (g.lane == 0) {
-                    uint32_t e = a.lops.edge[k];
-                    if (fl & L_RET) a.ops.old[e] = ret;
-                    if (fl & L_SLOT) a.ops.slot[e] = idx;
-                }
-                __threadfence_block();
+        __threadfence_block();
+        // liveness of the edge ops whose (src -> dst) entry is on this list,
+        // from the count after the whole batch
+        if (ok) {
+            for (uint32_t i = g.lane; i < n; i += AG) {
+                const uint32_t k = ops[i];
+                if (!(a.lops.flags[k] & L_SLOT)) continue;
+                const uint32_t idx = a.lops.slot[k];
+                a.ops.live[a.lops.edge[k]] = (idx != NILP && a.slots.cnt[st.off + idx] > 0) ? 1u : 0u;
             }
         }
         if (g.lane == 0) {
@@ -369,7 +539,7 @@ __global__ void k_apply(StoreArgs a)
             li.hoff[v] = st.hoff;
             li.hcap[v] = st.hcap;
             li.hcnt[v] = st.hcnt;
-            li.head[v] = -1;
+            li.nops[v] = 0;
         }
     }
 }
@@ -377,9 +547,12 @@ __global__ void k_apply(StoreArgs a)
 void launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b)
 {
     if (a.ops.n == 0) return;
-    uint32_t n = a.ops.n;
+    const uint32_t n = a.ops.n;
+    const uint32_t nl = 4 * n;
     hipLaunchKernelGGL(k_expand_link, dim3((n + 255) / 256), dim3(256), 0, s, a);
-    uint64_t groups = 4ull * n;
+    hipLaunchKernelGGL(k_segment, dim3((nl + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_scatter, dim3((nl + 255) / 256), dim3(256), 0, s, a);
+    uint64_t groups = (uint64_t)nl;
     uint64_t blocks = (groups * AG + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (ev_a) (void)hipEventRecord(ev_a, s);
@@ -484,6 +657,7 @@ __global__ void k_pool_recycle(StoreArgs a)
         p->free_take[h][c] = 0;
         p->pend_n[h][c] = 0;
         if (mv) atomicAdd(&p->recycled[h], (unsigned long long)mv);
+        if (blockIdx.x == 0) p->seg_top = 0;   // segments of the next batch
     }
     // publish the batch status (bad ids, pool exhaustion) to pinned host memory
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.hstatus) {
