@@ -390,3 +390,95 @@ def test_empty_and_degenerate_batches():
         m.del_edge(int(src[i]), int(dst[i]), int(w[i]))
     ls, ld, lw = m.live_edges()
     assert np.array_equal(L.read()[0], oracle.static_values(rg.SSSP, V, ls, ld, lw, 0))
+
+
+def _model_degrees(m, V):
+    out = np.zeros(V, np.int64)
+    inc = np.zeros(V, np.int64)
+    for (a, _, _), c in m.out.items():
+        out[a] += c
+    for (a, _, _), c in m.inc.items():
+        inc[a] += c
+    return out, inc
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_mixed_batches_store_state_and_wraparound(weighted):
+    """Mixed insert+delete batches on a small R-MAT multigraph whose hub lists
+    get dozens of same-key ops per batch (the 16-ops-per-round path), with the
+    sliding window running twice around the circular stream.  After every
+    batch the store (degrees, every key's count) equals the brute-force model
+    applied as the reference batch driver does (all adds, then all dels), and
+    the BFS / SSSP labels equal the static closed form."""
+    V, E = 1024, 1 << 15
+    s = rg.Stream("rmat", 10, E, seed=5, wseed=6)
+    src, dst, w = s.read()
+    w = w.astype(np.uint64) if weighted else np.zeros(E, np.uint64)
+    n0 = E // 2
+    g = rg.Graph(V, E, symmetric=False, dual=True, weighted=weighted)
+    g.load_stream(s, 0, n0, True)
+    m = MultigraphModel(V, symmetric=False, weighted=weighted)
+    for i in range(n0):
+        m.add_edge(int(src[i]), int(dst[i]), int(w[i]))
+    root = s.max_degree_vertex(n0, V)
+    algo = rg.SSSP if weighted else rg.BFS
+    L = g.alloc_vertex_tree_array(algo, root)
+    L.build_tree()
+    assert g.info()["hash_used"] > 0   # some hub list is indexed
+    pos, B = 0, 2000
+    while pos < 2 * E + 3 * B:        # past the stream end twice
+        g.apply_stream([L], s, (n0 + pos) % E, B, pos % E, B, True)
+        for i in range(B):
+            k = (n0 + pos + i) % E
+            m.add_edge(int(src[k]), int(dst[k]), int(w[k]))
+        for i in range(B):
+            k = (pos + i) % E
+            m.del_edge(int(src[k]), int(dst[k]), int(w[k]))
+        pos += B
+        out, inc = g.degrees()
+        mo, mi = _model_degrees(m, V)
+        assert np.array_equal(out.astype(np.int64), mo), pos
+        assert np.array_equal(inc.astype(np.int64), mi), pos
+        ls, ld, lw = m.live_edges()
+        vals, par = L.read()
+        exp = oracle.static_values(algo, V, ls, ld, lw, root)
+        assert np.array_equal(vals, exp), pos
+        assert oracle.check_tree(algo, V, ls, ld, lw, root, vals, par) == 0, pos
+    keys = list(m.out.items())
+    for (a, b, c), cnt in keys[:: max(1, len(keys) // 300)]:
+        assert g.get_edge_num(a, b, c) == cnt, (a, b, c)
+
+
+def test_serving_results_same_alone_and_in_the_mix():
+    """The same update sequence applied alone (one update per device batch)
+    and mixed into uneven batches gives identical labels for all four trees
+    at every batch boundary, and the serving loop's final state matches too."""
+    V, E = 4096, 1 << 16
+    s = rg.Stream("rmat", 12, E, seed=7, wseed=8)
+    n0 = int(0.9 * E)
+
+    def make():
+        g = rg.Graph(V, E, symmetric=False, dual=True, weighted=True)
+        g.load_stream(s, 0, n0, True)
+        root = s.max_degree_vertex(n0, V)
+        Ls = [g.alloc_vertex_tree_array(a, root) for a in (rg.BFS, rg.SSSP, rg.SSWP, rg.WCC)]
+        for L in Ls:
+            L.build_tree()
+        return g, Ls
+
+    ga, La = make()   # alone
+    gb, Lb = make()   # mixed batches
+    gc, Lc = make()   # serving loop
+    rng = np.random.default_rng(9)
+    pos = 0
+    while pos < 1500:
+        b = int(min(1500 - pos, max(1, np.exp(rng.uniform(0, np.log(400))))))
+        ga.stream_plain(La, s, n0 + pos, pos, b, True, latency=False)
+        gb.apply_stream(Lb, s, n0 + pos, b, pos, b, True)
+        pos += b
+        for x, y in zip(La, Lb):
+            assert np.array_equal(x.read()[0], y.read()[0]), (pos, x.algo)
+    res, _ = gc.serve(Lc, s, n0, 0, 2 * pos, 2e6, batch_cap=16, cap_max=2048, target_ms=5.0)
+    assert res["served"] == 2 * pos and res["batches"] > 1
+    for x, z in zip(La, Lc):
+        assert np.array_equal(x.read()[0], z.read()[0]), x.algo
