@@ -82,11 +82,15 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
                      int trace_modified, rg_labels **out);
 int rg_labels_destroy(rg_labels *L);
 /* Strategy knobs of do_update_tree_del (graph.hpp:1252, 1324, 1900) plus the
- * work size below which a whole update runs in one persistent workgroup.
- * Defaults: rescue while level <= 5 or frontier < 16384; full recompute when
- * level <= 10 and subtree+frontier > max(0.05*V, 2000000); ancestor walks
- * capped at walk_limit steps (0 = min(V/10, 65536)); block_max = 4096.
- * Values never depend on these; only the work done does. */
+ * largest pass (queued vertices, or edge ops / 16) that runs inside the
+ * persistent team launch; larger passes run grid-wide.  Defaults: rescue
+ * while level <= 5 or frontier < 16384; full recompute when level <= 10 and
+ * subtree+frontier > max(0.05*V, 2000000); ancestor walks capped at
+ * walk_limit steps (0 = min(V/10, 65536)); block_max = RG_AUTO (1024 per
+ * team workgroup; the team width grows with the batch, 8 to 32 workgroups
+ * per tree).  block_max = 0 runs every pass grid-wide.  Values never depend
+ * on these; only the work done does. */
+#define RG_AUTO 0xFFFFFFFFu
 int rg_labels_set_tuning(rg_labels *L, uint32_t rescue_level, uint32_t rescue_frontier,
                          uint32_t recompute_level, uint32_t recompute_abs, double recompute_frac,
                          uint32_t walk_limit, uint32_t block_max);
@@ -165,10 +169,11 @@ int rg_stream_plain(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t i
 int rg_last_gpu_ms(rg_graph *g, double *ms);
 
 /* Kernel timing with HIP events on the streams the kernels run on.  Bit k of
- * `mask` times kernels of kind k: the store apply kernel (kind 0) and the
- * persistent team engine launches (kind 1).  rg_profile_read returns the
- * launch count, summed device milliseconds and summed work units (edge ops
- * for kind 0, launches for kind 1) accumulated since the graph was made. */
+ * `mask` times kernels of kind k: the store apply kernel (kind 0), the
+ * persistent team engine launches (kind 1) and grid-wide engine passes
+ * (kind 2).  rg_profile_read returns the launch count, summed device
+ * milliseconds and summed work units (edge ops for kind 0, trees for kind 1,
+ * pass work for kind 2) accumulated since the graph was made. */
 int rg_profile_enable(rg_graph *g, int mask);
 int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint64_t *units);
 

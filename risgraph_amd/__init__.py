@@ -277,11 +277,13 @@ class Graph:
         return r.as_dict(), lat
 
     def profile(self, mask=3):
-        """Time kernels with HIP events: bit 0 store apply, bit 1 engine block launches."""
+        """Time kernels with HIP events: bit 0 store apply, bit 1 team engine
+        launches, bit 2 grid-wide engine passes."""
         _check(load_library().rg_profile_enable(self.h, int(mask)))
 
     def profile_read(self, kind):
-        """kind 0 = store apply kernel, 1 = single-workgroup engine launches."""
+        """kind 0 = store apply kernel, 1 = persistent team engine launches,
+        2 = grid-wide engine passes."""
         n, units, ms = u64(), u64(), ctypes.c_double()
         _check(load_library().rg_profile_read(self.h, kind, ctypes.byref(n), ctypes.byref(ms),
                                               ctypes.byref(units)))
@@ -318,9 +320,11 @@ class Labels:
             self.h = None
 
     def set_tuning(self, rescue_level=5, rescue_frontier=16384, recompute_level=10,
-                   recompute_abs=2000000, recompute_frac=0.05, walk_limit=0, block_max=4096):
+                   recompute_abs=2000000, recompute_frac=0.05, walk_limit=0, block_max=None):
+        """Deletion strategy knobs (rg_labels_set_tuning); block_max None = automatic."""
+        bm = 0xFFFFFFFF if block_max is None else int(block_max)
         _check(load_library().rg_labels_set_tuning(self.h, rescue_level, rescue_frontier, recompute_level,
-                                                   recompute_abs, recompute_frac, walk_limit, block_max))
+                                                   recompute_abs, recompute_frac, walk_limit, bm))
 
     def build_tree(self):
         r = u64()

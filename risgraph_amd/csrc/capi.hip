@@ -124,8 +124,8 @@ struct rg_graph {
     int prof = 0;   // bit k: time kernels of kind k
     std::vector<hipEvent_t> ev_free;
     std::vector<ProfRec> prof_pending;
-    uint64_t p_launch[2]{}, p_units[2]{};
-    double p_ms[2]{};
+    uint64_t p_launch[3]{}, p_units[3]{};
+    double p_ms[3]{};
 };
 
 struct rg_labels {
@@ -148,8 +148,9 @@ struct rg_labels {
     EngineArgs *dargs = nullptr;   // device copy of the engine arguments (team launches)
     EngineArgs *hargs = nullptr;   // pinned staging of the same
     Tuning tune{};
-    uint32_t team_w = 8;           // workgroups per tree in the persistent team launch
-    uint32_t block_max = 8192;     // largest pass (vertices, or ops/16) run by the team
+    uint32_t team_w = 0;           // workgroups per tree in the team launch (0: by batch size)
+    uint32_t block_max = RG_AUTO;  // largest pass (vertices, or ops/16) run by the team
+    uint32_t run_w = 8;            // team width of the current run
     uint32_t *init_active = nullptr;
 };
 
@@ -425,11 +426,24 @@ void engine_init(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool 
     h.levels = 0;
 }
 
+// Team width of a run: 8 workgroups per tree for small batches (cheaper
+// barriers), growing with the batch to 32 (4 trees x 32 workgroups of 1024
+// threads stay co-resident on the 256 CUs, which the team barrier needs).
+uint32_t team_width(const rg_labels *L, uint32_t n_ops)
+{
+    if (L->team_w) return L->team_w;
+    uint32_t w = 8;
+    while (w < 32 && (uint64_t)w * 2048 < n_ops) w *= 2;
+    return w;
+}
+
+uint32_t team_cap(const rg_labels *L) { return L->block_max == RG_AUTO ? 1024u * L->run_w : L->block_max; }
+
 // A tree's next pass runs in the persistent team launch when it is not a
 // full rebuild and fits the team's work cap; otherwise it is one grid step.
 bool team_ok(const rg_labels *L, const Ctl &h)
 {
-    return h.phase != PH_BUILD_INIT && h.stage == 0 && host_work(h, L->g->V) <= L->block_max;
+    return h.phase != PH_BUILD_INIT && h.stage == 0 && host_work(h, L->g->V) <= team_cap(L);
 }
 
 // Run the engines of labels L[0..n) from `phase0` to completion on the graph
@@ -445,6 +459,7 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
     std::vector<EngineArgs> args(n);
     std::vector<char> active(n, 1), pending_init(n, 0);
     for (int i = 0; i < n; i++) {
+        L[i]->run_w = team_width(L[i], n_ops);
         engine_init(L[i], phase0, seed, n_ops, new_idx);
         // a run that starts in the team hands its initial control block to
         // the kernel through the pinned mirror; otherwise copy it now
@@ -469,20 +484,24 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
             if (team_ok(L[i], h)) {
                 team.push_back(i);
             } else {
+                hipEvent_t ea, eb;
+                prof_begin(g, 2, host_work(h, g->V), st, &ea, &eb);
+                if (ea) HC(hipEventRecord(ea, st));
                 launch_engine_grid_step(args[i], h, st);
+                if (eb) HC(hipEventRecord(eb, st));
                 if (stats) stats->grid_steps++;
             }
         }
         // one launch per group of up to MAXT trees with the same team width
         // and work cap
         while (!team.empty()) {
-            const uint32_t W = L[team[0]]->team_w, cap = L[team[0]]->block_max;
+            const uint32_t W = L[team[0]]->run_w, cap = team_cap(L[team[0]]);
             const EngineArgs *ta[MAXT];
             uint32_t mask = 0;
             int m = 0;
             for (size_t k = 0; k < team.size() && m < MAXT;) {
                 int i = team[k];
-                if (L[i]->team_w != W || L[i]->block_max != cap) { k++; continue; }
+                if (L[i]->run_w != W || team_cap(L[i]) != cap) { k++; continue; }
                 ta[m] = L[i]->dargs;
                 if (pending_init[i]) mask |= 1u << m;
                 pending_init[i] = 0;
@@ -774,8 +793,7 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->tune.recompute_abs = 2000000;
         L->tune.recompute_frac = 0.05f;
         L->tune.walk_limit = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(V / 10, 1u << 16));
-        if (const char *tw = getenv("RG_TEAM_W")) L->team_w = (uint32_t)std::max(1, std::min(64, atoi(tw)));
-        L->block_max = std::max<uint32_t>(4096, 1024 * L->team_w);
+        if (const char *tw = getenv("RG_TEAM_W")) L->team_w = (uint32_t)std::max(1, std::min(32, atoi(tw)));
         if (const char *bm = getenv("RG_BLOCK_MAX")) L->block_max = (uint32_t)atoi(bm);
         HC(hipMemcpyAsync(L->ctl, L->hc, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
         HC(hipStreamSynchronize(g->st));
@@ -1131,13 +1149,13 @@ int rg_profile_enable(rg_graph *g, int on)
     return guard([&] {
         std::lock_guard<std::recursive_mutex> lk(g->mu);
         prof_collect(g);
-        g->prof = on & 3;
+        g->prof = on & 7;
     });
 }
 
 int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint64_t *units)
 {
-    if (!g || kind < 0 || kind > 1) return RG_EINVAL;
+    if (!g || kind < 0 || kind > 2) return RG_EINVAL;
     return guard([&] {
         std::lock_guard<std::recursive_mutex> lk(g->mu);
         prof_collect(g);

@@ -18,13 +18,14 @@
 // p marks v, and the rescue step re-parents v onto a surviving parallel edge.
 //
 // Work is level-synchronous over vertex queues.  A vertex pass gives each
-// queued vertex a 16-lane group; lists longer than 128 entries are split
-// into 512-entry chunk items handled a wave each in the chunk pass, so hub
-// vertices spread over the whole chip.  The same pass functions run either in
-// one persistent 1024-thread workgroup (small frontiers: one launch for the
-// whole update) or as one grid-wide launch per pass (large frontiers); the
-// phase state machine lives in the device control block so the two modes
-// hand over at any pass boundary.
+// queued vertex a 16-lane group; lists longer than 32 entries are split into
+// 128-entry chunk items handled a wave each in the pass's chunk stage, so hub
+// vertices spread over many CUs.  All trees of a batch repair in one
+// persistent launch (the team kernel): each tree gets W workgroups that
+// separate passes with a device barrier, and a pass too large for the team
+// leaves it for grid-wide launches (one per pass); the phase state machine
+// lives in the device control block so the two modes hand over at any pass
+// boundary.
 #include "rg_internal.h"
 
 #include <algorithm>
@@ -75,7 +76,7 @@ __device__ __forceinline__ void ast(int32_t *p, int32_t v)
 // snapshot of the control block taken at a pass boundary
 struct Snap {
     uint32_t phase, stage, level, n_cur, n_tree, n_chunk, n_ops;
-    uint32_t sel_cur, sel_next, sel_tree, epoch, mepoch, seed, rbound;
+    uint32_t sel_cur, sel_next, sel_tree, epoch, mepoch, seed, rbound, flags;
     int32_t idx;
     Ctl *c;   // live control block: LDS copy in block mode, global in grid mode
 };
@@ -97,6 +98,7 @@ __device__ __forceinline__ void take_snap(Ctl *c, Snap &s)
     s.mepoch = ald(&c->mepoch);
     s.seed = ald(&c->seed);
     s.rbound = ald(&c->rbound);
+    s.flags = ald(&c->flags);
     s.idx = ald(&c->idx);
 }
 
@@ -172,16 +174,14 @@ __device__ __forceinline__ void defer_chunks(const EngineArgs &a, const Snap &s,
 // clears the array when the index wraps).
 __device__ __forceinline__ bool marked(int32_t iv, int32_t idx) { return iv == idx || iv == -idx; }
 
-// mark x invalid unless this run already marked (or rescued) it
+// mark x invalid unless this run already marked it: run indices only grow
+// (until the host clears the marks on wrap), so one atomicMax replaces any
+// older mark, 0, or an older rescue mark.  A vertex rescued in this run
+// (-idx, then 0) is never a child of a vertex being expanded: its parent is
+// its new parent outside the subtree.
 __device__ __forceinline__ bool mark_inv(const EngineArgs &a, const Snap &s, uint32_t x)
 {
-    int32_t old = a.inv[x];
-    while (!marked(old, s.idx)) {
-        int32_t prev = atomicCAS(&a.inv[x], old, s.idx);
-        if (prev == old) return true;
-        old = prev;
-    }
-    return false;
+    return atomicMax(&a.inv[x], s.idx) != s.idx;
 }
 
 // has_parents (graph.hpp:1889-1907): true when an ancestor of x (x
@@ -534,7 +534,7 @@ __device__ void pass_chunks(const EngineArgs &a, const Snap &s, uint32_t tid, ui
 // deletions runs them first (roots, levels, pull), and its inserts are seeded
 // in the pull pass, so one push phase settles both; values are the unique
 // fixpoint either way, as in the reference's add-then-delete batch order.
-__device__ void advance(const EngineArgs &a, Ctl *c)
+__device__ void advance_state(const EngineArgs &a, Ctl *c)
 {
     uint32_t ph = ald(&c->phase);
     auto swap_queues = [&]() {
@@ -593,6 +593,23 @@ __device__ void advance(const EngineArgs &a, Ctl *c)
         ast(&c->phase, (uint32_t)PH_DONE);
         break;
     }
+}
+
+// The control block is read once (independent loads, one round trip), the
+// transition runs on the private copy, and only the changed words go back.
+// Nothing else touches the block meanwhile: the caller is the last arriver
+// of a team barrier, or the single thread of the grid-mode advance kernel.
+__device__ void advance(const EngineArgs &a, Ctl *c)
+{
+    constexpr uint32_t NW = sizeof(Ctl) / 4;
+    uint32_t now[NW], was[NW];
+    const uint32_t *cw = (const uint32_t *)c;
+#pragma unroll
+    for (uint32_t i = 0; i < NW; i++) was[i] = now[i] = ald(cw + i);
+    advance_state(a, (Ctl *)now);
+#pragma unroll
+    for (uint32_t i = 0; i < NW; i++)
+        if (now[i] != was[i]) ast((uint32_t *)cw + i, now[i]);
 }
 
 // work of the current pass, in units comparable with the team cap; an op
@@ -702,7 +719,7 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
         __syncthreads();
         const Snap s = snap;
         if (s.phase == PH_DONE || s.phase == PH_REBUILD || s.phase == PH_BUILD_INIT) break;
-        if (ald(&c->flags) & F_ABORT) break;
+        if (s.flags & F_ABORT) break;
         if (phase_work(a, s) > work_max) break;
         // stage clocks of this workgroup: start, vertex stage done, first
         // barrier passed, chunk stage done
@@ -834,7 +851,7 @@ void launch_engine_team(const EngineArgs *const *a, int n, uint32_t W, uint32_t 
     P.ntrees = (uint32_t)n;
     P.W = W;
     P.work_max = work_max;
-    P.chunk_max = 64u * W;
+    P.chunk_max = 256u * W;
     P.init_mask = init_mask;
     hipLaunchKernelGGL(k_team, dim3(n * W), dim3(1024), 0, st, P);
 }

@@ -33,8 +33,8 @@ constexpr uint32_t NBR_MASK = 0x7FFFFFFFu;
 constexpr uint32_t NILP = 0xFFFFFFFFu;
 constexpr uint32_t INDEX_THRESHOLD = 512;   // storage.hpp:304
 constexpr uint64_t HEMPTY = ~0ull;
-constexpr uint32_t LONG_LIST = 128;         // longer lists go to the chunk queue
-constexpr uint32_t CHUNK = 512;             // entries per chunk item
+constexpr uint32_t LONG_LIST = 32;          // longer lists go to the chunk queue
+constexpr uint32_t CHUNK = 128;             // entries per chunk item (two wave steps)
 constexpr int GROUP = 16;                   // lanes per vertex in vertex passes
 constexpr int WAVE = 64;
 
