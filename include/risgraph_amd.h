@@ -83,8 +83,10 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
 int rg_labels_destroy(rg_labels *L);
 /* Strategy knobs of do_update_tree_del (graph.hpp:1252, 1324, 1900) plus the
  * largest pass (queued vertices, or edge ops / 16) that runs inside the
- * persistent team launch; larger passes run grid-wide.  Defaults: rescue
- * while level <= 5 or frontier < 16384; full recompute when level <= 10 and
+ * persistent team launch; larger passes run grid-wide.  A level is rescued
+ * when level <= rescue_level (0xFFFFFFFF: never) or its frontier is below
+ * rescue_frontier.  Defaults: rescue on the first level only (0, 0); the
+ * reference's policy is (5, 16384).  Full recompute when level <= 10 and
  * subtree+frontier > max(0.05*V, 2000000); ancestor walks capped at
  * walk_limit steps (0 = min(V/10, 65536)); block_max = RG_AUTO (1024 per
  * team workgroup; the team width grows with the batch, 8 to 32 workgroups

@@ -319,7 +319,7 @@ class Labels:
             load_library().rg_labels_destroy(self.h)
             self.h = None
 
-    def set_tuning(self, rescue_level=5, rescue_frontier=16384, recompute_level=10,
+    def set_tuning(self, rescue_level=0, rescue_frontier=0, recompute_level=10,
                    recompute_abs=2000000, recompute_frac=0.05, walk_limit=0, block_max=None):
         """Deletion strategy knobs (rg_labels_set_tuning); block_max None = automatic."""
         bm = 0xFFFFFFFF if block_max is None else int(block_max)

@@ -787,8 +787,12 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
             EngineArgs a = engine_args(L);
             launch_engine_fill_init(a, g->st);
         }
-        L->tune.rescue_level = 5;
-        L->tune.rescue_frontier = 16384;
+        // rescue on the first deletion level only: deeper rescue scans cost
+        // more passes than the subtree they save on this device (the
+        // reference's policy, level <= 5 or frontier < 16384, is one
+        // rg_labels_set_tuning call away; values do not depend on it)
+        L->tune.rescue_level = 0;
+        L->tune.rescue_frontier = 0;
         L->tune.recompute_level = 10;
         L->tune.recompute_abs = 2000000;
         L->tune.recompute_frac = 0.05f;

@@ -73,12 +73,28 @@ __device__ __forceinline__ void ast(int32_t *p, int32_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Per-workgroup staging of queue appends in LDS.  Frontier, subtree and
+// chunk-queue appends of a pass go to the workgroup's buffers with LDS
+// atomics; at the end of the stage the workgroup reserves its range of each
+// global queue with one counter atomic and copies the buffer out.  A full
+// buffer spills straight to the global queue.  (With thousands of waves
+// appending, single-counter device atomics were the pass's critical path.)
+constexpr uint32_t SQ_NEXT = 6144, SQ_TREE = 1536, SQ_CHUNK = 1024;
+struct Stage {
+    uint32_t n_next, n_tree, n_chunk, pad;
+    uint32_t base[4];
+    uint32_t next[SQ_NEXT];
+    uint32_t tree[SQ_TREE];
+    uint64_t chunk[SQ_CHUNK];
+};
+
 // snapshot of the control block taken at a pass boundary
 struct Snap {
     uint32_t phase, stage, level, n_cur, n_tree, n_chunk, n_ops;
     uint32_t sel_cur, sel_next, sel_tree, epoch, mepoch, seed, rbound, flags;
     int32_t idx;
-    Ctl *c;   // live control block: LDS copy in block mode, global in grid mode
+    Ctl *c;       // live control block: LDS copy for a one-workgroup team, else HBM
+    Stage *stg;   // this workgroup's append buffers (LDS)
 };
 
 __device__ __forceinline__ void take_snap(Ctl *c, Snap &s)
@@ -139,9 +155,69 @@ __device__ __forceinline__ void q_append(const EngineArgs &a, Ctl *c, uint32_t *
     else atomicOr(&c->flags, F_QUEUE_OVERFLOW);
 }
 
+// append to the next frontier / the subtree list through the LDS stage
+__device__ __forceinline__ void stage_next(const EngineArgs &a, const Snap &s, uint32_t v)
+{
+    const uint32_t pos = wave_reserve(&s.stg->n_next);
+    if (pos < SQ_NEXT) s.stg->next[pos] = v;
+    else q_append(a, s.c, a.q[s.sel_next], &s.c->n_next, v);
+}
+
+__device__ __forceinline__ void stage_tree(const EngineArgs &a, const Snap &s, uint32_t v)
+{
+    const uint32_t pos = wave_reserve(&s.stg->n_tree);
+    if (pos < SQ_TREE) s.stg->tree[pos] = v;
+    else q_append(a, s.c, a.q[s.sel_tree], &s.c->n_tree, v);
+}
+
+__device__ void stage_init(Stage *st)
+{
+    if (threadIdx.x == 0) st->n_next = st->n_tree = st->n_chunk = 0;
+    __syncthreads();
+}
+
+// end of a stage: every thread of the workgroup calls this
+__device__ void stage_flush(const EngineArgs &a, const Snap &s)
+{
+    Stage *st = s.stg;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t nn = min(st->n_next, SQ_NEXT), nt = min(st->n_tree, SQ_TREE);
+        const uint32_t nc = min(st->n_chunk, SQ_CHUNK);
+        st->base[0] = nn ? atomicAdd(&s.c->n_next, nn) : 0;
+        st->base[1] = nt ? atomicAdd(&s.c->n_tree, nt) : 0;
+        st->base[2] = nc ? atomicAdd(&s.c->n_chunk, nc) : 0;
+        st->n_next = nn;
+        st->n_tree = nt;
+        st->n_chunk = nc;
+    }
+    __syncthreads();
+    bool over = false;
+    uint32_t *qn = a.q[s.sel_next], *qt = a.q[s.sel_tree];
+    for (uint32_t i = threadIdx.x; i < st->n_next; i += blockDim.x) {
+        const uint32_t p = st->base[0] + i;
+        if (p < a.V) qn[p] = st->next[i];
+        else over = true;
+    }
+    for (uint32_t i = threadIdx.x; i < st->n_tree; i += blockDim.x) {
+        const uint32_t p = st->base[1] + i;
+        if (p < a.V) qt[p] = st->tree[i];
+        else over = true;
+    }
+    for (uint32_t i = threadIdx.x; i < st->n_chunk; i += blockDim.x) {
+        const uint32_t p = st->base[2] + i;
+        if (p < a.cq_cap) a.cq[p] = st->chunk[i];
+        else over = true;
+    }
+    if (over) atomicOr(&s.c->flags, F_QUEUE_OVERFLOW);
+    __syncthreads();
+    if (threadIdx.x == 0) st->n_next = st->n_tree = st->n_chunk = 0;
+    __syncthreads();
+}
+
 __device__ __forceinline__ void push_next(const EngineArgs &a, const Snap &s, uint32_t v)
 {
-    if (atomicExch(&a.stamp[v], s.epoch) != s.epoch) q_append(a, s.c, a.q[s.sel_next], &s.c->n_next, v);
+    if (atomicExch(&a.stamp[v], s.epoch) != s.epoch) stage_next(a, s, v);
 }
 
 __device__ __forceinline__ void mark_mod(const EngineArgs &a, const Snap &s, uint32_t v)
@@ -158,14 +234,20 @@ enum ChunkOp : uint32_t { CO_PUSH = 0, CO_RESCUE = 1, CO_EXPAND = 2, CO_PULL = 3
 __device__ __forceinline__ void defer_chunks(const EngineArgs &a, const Snap &s, uint32_t v, int L, uint32_t len,
                                              ChunkOp op)
 {
-    uint32_t nch = (len + CHUNK - 1) / CHUNK;
-    uint32_t pos = atomicAdd(&s.c->n_chunk, nch);
-    if (pos + nch > a.cq_cap) {
+    const uint32_t nch = (len + CHUNK - 1) / CHUNK;
+    const uint64_t item = (uint64_t)v | ((uint64_t)L << 32) | ((uint64_t)op << 33);
+    // the items that fit go to the stage, the rest straight to the queue
+    const uint32_t sp = atomicAdd(&s.stg->n_chunk, nch);
+    const uint32_t fit = sp < SQ_CHUNK ? min(nch, SQ_CHUNK - sp) : 0;
+    for (uint32_t k = 0; k < fit; k++) s.stg->chunk[sp + k] = item | ((uint64_t)k << 35);
+    if (fit == nch) return;
+    const uint32_t rest = nch - fit;
+    const uint32_t pos = atomicAdd(&s.c->n_chunk, rest);
+    if (pos + rest > a.cq_cap) {
         atomicOr(&s.c->flags, F_QUEUE_OVERFLOW);
         return;
     }
-    for (uint32_t k = 0; k < nch; k++)
-        a.cq[pos + k] = (uint64_t)v | ((uint64_t)L << 32) | ((uint64_t)op << 33) | ((uint64_t)k << 35);
+    for (uint32_t k = fit; k < nch; k++) a.cq[pos + k - fit] = item | ((uint64_t)k << 35);
 }
 
 // Invalidation marks (graph.hpp:1729) carry the deletion run's index: idx =
@@ -273,12 +355,11 @@ __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uin
 __device__ void expand_range(const EngineArgs &a, const Snap &s, uint32_t x, uint64_t off, uint32_t b,
                              uint32_t e, int lane, int stride)
 {
-    uint32_t *qn = a.q[s.sel_next];
     for (uint32_t i = b + lane; i < e; i += stride) {
         uint32_t c = a.slots.nbr[off + i];
         if (c & DEAD) continue;
         if (vpar(a.val[c]) != x) continue;
-        if (mark_inv(a, s, c)) q_append(a, s.c, qn, &s.c->n_next, c);
+        if (mark_inv(a, s, c)) stage_next(a, s, c);
     }
 }
 
@@ -352,7 +433,7 @@ __device__ __forceinline__ void mark_root(const EngineArgs &a, const Snap &s, ui
     if (mark_inv(a, s, dst)) {
         if constexpr (A == SSWP) atomicMax(&s.c->rbound, vval(w));
         else if constexpr (A != WCC) atomicMin(&s.c->rbound, vval(w));
-        q_append(a, s.c, a.q[s.sel_next], &s.c->n_next, dst);
+        stage_next(a, s, dst);
     }
 }
 
@@ -393,8 +474,105 @@ __device__ void pass_ops(const EngineArgs &a, const Snap &s, uint32_t tid, uint3
 // (its in-list scan was deferred to chunk items), only expand
 constexpr uint32_t Q_EXPAND_ONLY = 0x80000000u;
 
+// Local continuation of the deletion levels: the children a group marks are
+// processed by that group in the same pass, depth first, up to CONT_DEPTH
+// levels below the queued vertex with at most CONT_STACK pending per group;
+// the rest join the next level's queue.  Marking order does not matter for
+// the rescue test (has_parents), since every invalidated vertex already has
+// a marked deletion root above it once the roots pass is done; levels only
+// bound the work per pass.
+constexpr uint32_t CONT_DEPTH = 4;
+constexpr uint32_t CONT_STACK = 32;
+constexpr uint32_t MAX_GROUPS = 1024 / GROUP;   // per workgroup
+constexpr uint32_t STACK_WORDS = MAX_GROUPS * 2 * CONT_STACK;
+// Grid-wide passes use few large blocks (two per CU) with grid-stride loops:
+// each block reserves its queue ranges with one counter atomic per stage,
+// and same-address device atomics are the scarce resource here.
+constexpr uint32_t GRID_BLOCK = 1024;
+constexpr uint32_t GRID_MAX_BLOCKS = 1024;
+constexpr uint32_t GRID_STACK_WORDS = (GRID_BLOCK / GROUP) * 2 * CONT_STACK;
+
+// a group's LDS stack: vertices in v[0, CONT_STACK), depths in d[...]; `top`
+// is the same in all 16 lanes
+struct GStack {
+    uint32_t *v, *d;
+    uint32_t top;
+};
+
+__device__ __forceinline__ uint32_t group_ballot(bool p)
+{
+    const uint64_t b = __ballot(p);
+    return (uint32_t)((b >> (__lane_id() & ~(uint32_t)(GROUP - 1))) & ((1ull << GROUP) - 1));
+}
+
+// expand x's short list with the 16 lanes stepping through it together
+__device__ void expand_cont(const EngineArgs &a, const Snap &s, uint32_t x, uint64_t off, uint32_t len,
+                            int glane, GStack &st, uint32_t depth)
+{
+    for (uint32_t b = 0; b < len; b += GROUP) {
+        const uint32_t i = b + glane;
+        bool mk = false;
+        uint32_t c = 0;
+        if (i < len) {
+            c = a.slots.nbr[off + i];
+            if (!(c & DEAD) && vpar(a.val[c]) == x) mk = mark_inv(a, s, c);
+        }
+        const bool keep = mk && depth < CONT_DEPTH;
+        const uint32_t kb = group_ballot(keep);
+        const uint32_t rank = (uint32_t)__popc(kb & ((1u << glane) - 1));
+        const bool fits = keep && st.top + rank < CONT_STACK;
+        if (fits) {
+            st.v[st.top + rank] = c;
+            st.d[st.top + rank] = depth + 1;
+        }
+        if (mk && !fits) stage_next(a, s, c);
+        st.top = min(CONT_STACK, st.top + (uint32_t)__popc(kb));
+    }
+}
+
+// one vertex of a deletion level: try to rescue it, else keep it invalid,
+// record it for the pull, and mark its children
 template <int A>
-__device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, uint32_t nthr)
+__device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uint32_t depth, int glane,
+                           GStack &st)
+{
+    const uint32_t v = raw & ~Q_EXPAND_ONLY;
+    if (a.inv[v] != s.idx) return;   // rescued
+    if (s.phase == PH_DEL_RESCUE && !(raw & Q_EXPAND_ONLY)) {
+        const uint32_t dv = vval(a.val[v]);
+        bool won = false, deferred = false;
+        for (int li = 0; li < a.npull; li++) {
+            int L = a.pull_l[li];
+            uint32_t len = a.lists[L].len[v];
+            if (len > LONG_LIST) {
+                if (glane == 0) defer_chunks(a, s, v, L, len, CO_RESCUE);
+                deferred = true;
+                continue;
+            }
+            won |= rescue_range<A>(a, s, v, dv, a.lists[L].off[v], 0, len, glane, GROUP);
+        }
+        if (group_any(won)) return;
+        if (deferred) {
+            // the long in-list is scanned in this pass's chunk stage; v is
+            // expanded at the next level unless that rescues it
+            if (glane == 0) stage_next(a, s, v | Q_EXPAND_ONLY);
+            return;
+        }
+    }
+    if (glane == 0) stage_tree(a, s, v);
+    for (int li = 0; li < a.npush; li++) {
+        int L = a.push_l[li];
+        uint32_t len = a.lists[L].len[v];
+        if (len > LONG_LIST) {
+            if (glane == 0) defer_chunks(a, s, v, L, len, CO_EXPAND);
+            continue;
+        }
+        expand_cont(a, s, v, a.lists[L].off[v], len, glane, st, depth);
+    }
+}
+
+template <int A>
+__device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, uint32_t nthr, uint32_t *stk)
 {
     const uint32_t ph = s.phase;
     if (ph == PH_BUILD_INIT) {
@@ -425,39 +603,17 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
                 push_range<A>(a, s, v, uv, a.lists[L].off[v], 0, len, glane, GROUP);
             }
         } else if (ph == PH_DEL_RESCUE || ph == PH_DEL_EXPAND) {
-            // one deletion level: try to rescue v, else keep it invalid and
-            // mark its children (rescue then expand per vertex, one pass)
-            if (a.inv[v] != s.idx) continue;   // rescued
-            if (ph == PH_DEL_RESCUE && !(raw & Q_EXPAND_ONLY)) {
-                const uint32_t dv = vval(a.val[v]);
-                bool won = false, deferred = false;
-                for (int li = 0; li < a.npull; li++) {
-                    int L = a.pull_l[li];
-                    uint32_t len = a.lists[L].len[v];
-                    if (len > LONG_LIST) {
-                        if (glane == 0) defer_chunks(a, s, v, L, len, CO_RESCUE);
-                        deferred = true;
-                        continue;
-                    }
-                    won |= rescue_range<A>(a, s, v, dv, a.lists[L].off[v], 0, len, glane, GROUP);
-                }
-                if (group_any(won)) continue;
-                if (deferred) {
-                    // the long in-list is scanned in this pass's chunk stage;
-                    // v is expanded at the next level unless that rescues it
-                    if (glane == 0) q_append(a, s.c, a.q[s.sel_next], &s.c->n_next, v | Q_EXPAND_ONLY);
-                    continue;
-                }
-            }
-            if (glane == 0) q_append(a, s.c, a.q[s.sel_tree], &s.c->n_tree, v);
-            for (int li = 0; li < a.npush; li++) {
-                int L = a.push_l[li];
-                uint32_t len = a.lists[L].len[v];
-                if (len > LONG_LIST) {
-                    if (glane == 0) defer_chunks(a, s, v, L, len, CO_EXPAND);
-                    continue;
-                }
-                expand_range(a, s, v, a.lists[L].off[v], 0, len, glane, GROUP);
+            // one deletion level (rescue then expand per vertex), continued
+            // depth first through the children this group marks
+            GStack st;
+            st.v = stk + (threadIdx.x / GROUP) * 2 * CONT_STACK;
+            st.d = st.v + CONT_STACK;
+            st.top = 0;
+            del_vertex<A>(a, s, raw, 0, glane, st);
+            while (st.top) {
+                st.top--;
+                const uint32_t c = st.v[st.top], d = st.d[st.top];
+                del_vertex<A>(a, s, c, d, glane, st);
             }
         } else if (ph == PH_DEL_PULL) {
             // reset v and pull its best candidate from in-neighbours outside
@@ -489,13 +645,14 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
 // The vertex stage of a pass: op passes, vertex passes, and the mixed
 // pull pass (tree vertices plus the seeds of the batch's inserts).
 template <int A>
-__device__ __forceinline__ void pass_main(const EngineArgs &a, const Snap &s, uint32_t tid, uint32_t nthr)
+__device__ __forceinline__ void pass_main(const EngineArgs &a, const Snap &s, uint32_t tid, uint32_t nthr,
+                                          uint32_t *stk)
 {
     if (s.phase == PH_SEED || s.phase == PH_DEL_ROOTS) {
         pass_ops<A>(a, s, tid, nthr);
         return;
     }
-    pass_vertices<A>(a, s, tid, nthr);
+    pass_vertices<A>(a, s, tid, nthr, stk);
     if (s.phase == PH_DEL_PULL && s.seed) pass_ops<A>(a, s, tid, nthr);
 }
 
@@ -534,82 +691,101 @@ __device__ void pass_chunks(const EngineArgs &a, const Snap &s, uint32_t tid, ui
 // deletions runs them first (roots, levels, pull), and its inserts are seeded
 // in the pull pass, so one push phase settles both; values are the unique
 // fixpoint either way, as in the reference's add-then-delete batch order.
-__device__ void advance_state(const EngineArgs &a, Ctl *c)
+__device__ __forceinline__ void advance_state(const EngineArgs &a, Ctl &c)
 {
-    uint32_t ph = ald(&c->phase);
+    const uint32_t ph = c.phase;
     auto swap_queues = [&]() {
-        uint32_t cur = ald(&c->sel_cur), nxt = ald(&c->sel_next);
-        ast(&c->sel_cur, nxt);
-        ast(&c->sel_next, cur);
-        ast(&c->n_cur, ald(&c->n_next));
-        ast(&c->n_next, 0u);
-        ast(&c->epoch, ald(&c->epoch) + 1);
-        c->levels += 1;
+        const uint32_t cur = c.sel_cur;
+        c.sel_cur = c.sel_next;
+        c.sel_next = cur;
+        c.n_cur = c.n_next;
+        c.n_next = 0;
+        c.epoch += 1;
+        c.levels += 1;
     };
     auto rescue_on = [&](uint32_t level, uint32_t n) {
-        return level <= a.tune.rescue_level || n < a.tune.rescue_frontier;
+        return (a.tune.rescue_level != RESCUE_NEVER && level <= a.tune.rescue_level) || n < a.tune.rescue_frontier;
     };
     auto recompute = [&](uint32_t level, uint32_t n) {
-        uint64_t tot = (uint64_t)ald(&c->n_tree) + n;
+        uint64_t tot = (uint64_t)c.n_tree + n;
         return level <= a.tune.recompute_level && (double)tot > (double)a.tune.recompute_frac * a.V &&
                tot > a.tune.recompute_abs;
     };
     auto level_next = [&](uint32_t level) {
-        uint32_t n = ald(&c->n_cur);
-        if (ald(&c->flags) & F_NEED_REBUILD) ast(&c->phase, (uint32_t)PH_REBUILD);
-        else if (n == 0) ast(&c->phase, (uint32_t)PH_DEL_PULL);
-        else if (recompute(level, n)) ast(&c->phase, (uint32_t)PH_REBUILD);
-        else ast(&c->phase, rescue_on(level, n) ? (uint32_t)PH_DEL_RESCUE : (uint32_t)PH_DEL_EXPAND);
+        const uint32_t n = c.n_cur;
+        if (c.flags & F_NEED_REBUILD) c.phase = PH_REBUILD;
+        else if (n == 0) c.phase = PH_DEL_PULL;
+        else if (recompute(level, n)) c.phase = PH_REBUILD;
+        else c.phase = rescue_on(level, n) ? PH_DEL_RESCUE : PH_DEL_EXPAND;
     };
-    ast(&c->n_chunk, 0u);
-    ast(&c->stage, 0u);
+    c.n_chunk = 0;
+    c.stage = 0;
     switch (ph) {
     case PH_SEED:
     case PH_BUILD_INIT:
     case PH_PUSH:
     case PH_DEL_PULL:
         swap_queues();
-        ast(&c->seed, 0u);
-        ast(&c->phase, ald(&c->n_cur) ? (uint32_t)PH_PUSH : (uint32_t)PH_DONE);
+        c.seed = 0;
+        c.phase = c.n_cur ? PH_PUSH : PH_DONE;
         break;
     case PH_DEL_ROOTS:
         swap_queues();
-        ast(&c->level, 0u);
-        ast(&c->n_tree, 0u);
-        if (ald(&c->n_cur) == 0) ast(&c->phase, ald(&c->seed) ? (uint32_t)PH_SEED : (uint32_t)PH_DONE);
+        c.level = 0;
+        c.n_tree = 0;
+        if (c.n_cur == 0) c.phase = c.seed ? PH_SEED : PH_DONE;
         else level_next(0);
         break;
     case PH_DEL_RESCUE:
-    case PH_DEL_EXPAND: {
+    case PH_DEL_EXPAND:
         swap_queues();
-        uint32_t level = ald(&c->level) + 1;
-        ast(&c->level, level);
-        level_next(level);
+        c.level += 1;
+        level_next(c.level);
         break;
-    }
     default:
         // unknown phase: stop instead of spinning
-        atomicOr(&c->flags, F_QUEUE_OVERFLOW);
-        ast(&c->phase, (uint32_t)PH_DONE);
+        c.flags |= F_QUEUE_OVERFLOW;
+        c.phase = PH_DONE;
         break;
     }
 }
 
 // The control block is read once (independent loads, one round trip), the
-// transition runs on the private copy, and only the changed words go back.
-// Nothing else touches the block meanwhile: the caller is the last arriver
-// of a team barrier, or the single thread of the grid-mode advance kernel.
+// transition runs on a private copy in registers, and the words that changed
+// go back.  Nothing else touches the block meanwhile: the caller is the last
+// arriver of a team barrier, or the single thread of the grid-mode advance
+// kernel.
 __device__ void advance(const EngineArgs &a, Ctl *c)
 {
-    constexpr uint32_t NW = sizeof(Ctl) / 4;
-    uint32_t now[NW], was[NW];
-    const uint32_t *cw = (const uint32_t *)c;
-#pragma unroll
-    for (uint32_t i = 0; i < NW; i++) was[i] = now[i] = ald(cw + i);
-    advance_state(a, (Ctl *)now);
-#pragma unroll
-    for (uint32_t i = 0; i < NW; i++)
-        if (now[i] != was[i]) ast((uint32_t *)cw + i, now[i]);
+    Ctl l;
+    l.phase = ald(&c->phase);
+    l.stage = ald(&c->stage);
+    l.level = ald(&c->level);
+    l.seed = ald(&c->seed);
+    l.n_cur = ald(&c->n_cur);
+    l.n_next = ald(&c->n_next);
+    l.n_tree = ald(&c->n_tree);
+    l.n_chunk = ald(&c->n_chunk);
+    l.sel_cur = ald(&c->sel_cur);
+    l.sel_next = ald(&c->sel_next);
+    l.epoch = ald(&c->epoch);
+    l.flags = ald(&c->flags);
+    l.levels = c->levels;
+    const Ctl o = l;
+    advance_state(a, l);
+    if (l.phase != o.phase) ast(&c->phase, l.phase);
+    if (l.stage != o.stage) ast(&c->stage, l.stage);
+    if (l.level != o.level) ast(&c->level, l.level);
+    if (l.seed != o.seed) ast(&c->seed, l.seed);
+    if (l.n_cur != o.n_cur) ast(&c->n_cur, l.n_cur);
+    if (l.n_next != o.n_next) ast(&c->n_next, l.n_next);
+    if (l.n_tree != o.n_tree) ast(&c->n_tree, l.n_tree);
+    if (l.n_chunk != o.n_chunk) ast(&c->n_chunk, l.n_chunk);
+    if (l.sel_cur != o.sel_cur) ast(&c->sel_cur, l.sel_cur);
+    if (l.sel_next != o.sel_next) ast(&c->sel_next, l.sel_next);
+    if (l.epoch != o.epoch) ast(&c->epoch, l.epoch);
+    if (l.flags != o.flags) ast(&c->flags, l.flags);
+    c->levels = l.levels;
 }
 
 // work of the current pass, in units comparable with the team cap; an op
@@ -694,8 +870,10 @@ struct TeamParams {
 // stays in HBM and passes are separated by the team barrier.
 template <int A>
 __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32_t work_max, uint32_t chunk_max,
-                                         uint32_t wg, bool init, Ctl *lds_ctl, Snap &snap, uint32_t *sh)
+                                         uint32_t wg, bool init, Ctl *lds_ctl, Snap &snap, uint32_t *sh,
+                                         uint32_t *stk, Stage *stg)
 {
+    stage_init(stg);
     const uint32_t tid = wg * blockDim.x + threadIdx.x, nthr = W * blockDim.x;
     Ctl *c = (W == 1) ? lds_ctl : a.ctl;
     if (threadIdx.x == 0 && (W == 1 || (init && wg == 0))) {
@@ -715,7 +893,10 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
     // only bounds the launch if the state machine were ever corrupted
     for (uint32_t iter = 0; iter < (1u << 24); iter++) {
         __syncthreads();
-        if (threadIdx.x == 0) take_snap(c, snap);
+        if (threadIdx.x == 0) {
+            take_snap(c, snap);
+            snap.stg = stg;
+        }
         __syncthreads();
         const Snap s = snap;
         if (s.phase == PH_DONE || s.phase == PH_REBUILD || s.phase == PH_BUILD_INIT) break;
@@ -727,8 +908,8 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
         tk[0] = wall_clock64();
         tk[1] = tk[2] = tk[0];
         if (s.stage == 0) {
-            pass_main<A>(a, s, tid, nthr);
-            __syncthreads();
+            pass_main<A>(a, s, tid, nthr, stk);
+            stage_flush(a, s);
             tk[1] = wall_clock64();
             // the chunk queue is complete once every workgroup finished
             team_sync(c, W, [&] {
@@ -746,7 +927,7 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
             __syncthreads();
         }
         pass_chunks<A>(a, s, tid, nthr, sh[0]);
-        __syncthreads();
+        stage_flush(a, s);
         tk[3] = wall_clock64();
         team_sync(c, W, [&] {
             if (a.ticks && s.phase < NPHASE) {
@@ -797,6 +978,8 @@ __global__ __launch_bounds__(1024) void k_team(TeamParams P)
     __shared__ Ctl lds_ctl;
     __shared__ Snap snap;
     __shared__ uint32_t sh[2];
+    __shared__ uint32_t stk[STACK_WORDS];
+    __shared__ Stage stg;
     const uint32_t t = blockIdx.x % P.ntrees, wg = blockIdx.x / P.ntrees;
     static_assert(MAXT == 4, "tree selection below");
     static_assert(sizeof(EngineArgs) % 4 == 0, "word copy below");
@@ -806,34 +989,47 @@ __global__ __launch_bounds__(1024) void k_team(TeamParams P)
     __syncthreads();
     const bool init = (P.init_mask >> t) & 1u;
     switch (sa.algo) {
-    case BFS: team_run<BFS>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh); break;
-    case SSSP: team_run<SSSP>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh); break;
-    case SSWP: team_run<SSWP>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh); break;
-    default: team_run<WCC>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh); break;
+    case BFS: team_run<BFS>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
+    case SSSP: team_run<SSSP>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
+    case SSWP: team_run<SSWP>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
+    default: team_run<WCC>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
     }
 }
 
 template <int A>
-__global__ void k_grid_main(EngineArgs a)
+__global__ __launch_bounds__(GRID_BLOCK) void k_grid_main(EngineArgs a)
 {
     __shared__ Snap snap;
-    if (threadIdx.x == 0) take_snap(a.ctl, snap);
+    __shared__ uint32_t stk[GRID_STACK_WORDS];
+    __shared__ Stage stg;
+    stage_init(&stg);
+    if (threadIdx.x == 0) {
+        take_snap(a.ctl, snap);
+        snap.stg = &stg;
+    }
     __syncthreads();
     const Snap s = snap;
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthr = gridDim.x * blockDim.x;
     if (s.stage != 0) return;
-    pass_main<A>(a, s, tid, nthr);
+    pass_main<A>(a, s, tid, nthr, stk);
+    stage_flush(a, s);
 }
 
 template <int A>
-__global__ void k_grid_chunks(EngineArgs a)
+__global__ __launch_bounds__(GRID_BLOCK) void k_grid_chunks(EngineArgs a)
 {
     __shared__ Snap snap;
-    if (threadIdx.x == 0) take_snap(a.ctl, snap);
+    __shared__ Stage stg;
+    stage_init(&stg);
+    if (threadIdx.x == 0) {
+        take_snap(a.ctl, snap);
+        snap.stg = &stg;
+    }
     __syncthreads();
     const Snap s = snap;
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthr = gridDim.x * blockDim.x;
     pass_chunks<A>(a, s, tid, nthr, s.n_chunk);
+    stage_flush(a, s);
 }
 
 __global__ void k_advance(EngineArgs a)
@@ -867,11 +1063,11 @@ static void grid_launch(const EngineArgs &a, const Ctl &h, hipStream_t st)
     case PH_DEL_PULL: threads = std::max<uint64_t>((uint64_t)h.n_tree * GROUP, h.seed ? h.n_ops : 0); break;
     default: threads = (uint64_t)h.n_cur * GROUP; break;
     }
-    uint64_t blocks = (threads + 255) / 256;
+    uint64_t blocks = (threads + GRID_BLOCK - 1) / GRID_BLOCK;
     if (blocks < 1) blocks = 1;
-    if (blocks > 32768) blocks = 32768;
-    if (h.stage == 0) hipLaunchKernelGGL(k_grid_main<A>, dim3((uint32_t)blocks), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_grid_chunks<A>, dim3(2048), dim3(256), 0, st, a);
+    if (blocks > GRID_MAX_BLOCKS) blocks = GRID_MAX_BLOCKS;
+    if (h.stage == 0) hipLaunchKernelGGL(k_grid_main<A>, dim3((uint32_t)blocks), dim3(GRID_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_grid_chunks<A>, dim3(GRID_MAX_BLOCKS / 2), dim3(GRID_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, a);
 }
 

@@ -152,8 +152,10 @@ struct Ctl {
 // synchronise through a barrier in the tree's control block.
 constexpr int MAXT = 4;
 
+constexpr uint32_t RESCUE_NEVER = 0xFFFFFFFFu;   // rescue_level: no level qualifies
+
 struct Tuning {
-    uint32_t rescue_level;        // rescue when level <= this ...
+    uint32_t rescue_level;        // rescue when level <= this (RESCUE_NEVER: none) ...
     uint32_t rescue_frontier;     // ... or frontier < this (graph.hpp:1324)
     uint32_t recompute_level;     // fallback checked up to this level (graph.hpp:1252)
     uint32_t recompute_abs;       // ... when subtree+frontier > this ...

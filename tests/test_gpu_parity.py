@@ -262,9 +262,11 @@ def test_large_batches(name, mode):
 
 @pytest.mark.parametrize("tuning", [
     dict(recompute_abs=0, recompute_frac=0.0),        # every deletion falls back to a full rebuild
-    dict(rescue_level=0, rescue_frontier=0),          # never rescue: reset + pull + push only
+    dict(rescue_level=0xFFFFFFFF, rescue_frontier=0),  # never rescue: reset + pull + push only
+    dict(rescue_level=5, rescue_frontier=16384),      # the reference's rescue policy
+    dict(rescue_frontier=0xFFFFFFFF),                 # rescue at every level
     dict(walk_limit=1),                               # ancestor walks give up: rebuild path
-], ids=["always_recompute", "no_rescue", "tiny_walk"])
+], ids=["always_recompute", "no_rescue", "reference_rescue", "rescue_every_level", "tiny_walk"])
 def test_deletion_strategies_keep_values(tuning):
     for name in ("bfs", "sssp"):
         algo = rg.ALGOS[name]
