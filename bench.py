@@ -5,26 +5,28 @@ Workload (the serving configuration the metric is quoted on): R-MAT scale 24
 (16,777,216 vertices, 268,435,456 edges, a/b/c/d = 0.57/0.19/0.19/0.05,
 u32 weights uniform in [1, 1000]), first 90% loaded, BFS / SSSP / SSWP / WCC
 dependency trees all maintained on one MI355X.  The update stream is the
-all_inc alternation (add edge 0.9E+k, delete edge k); requests arrive by a
-Poisson process from 256 simulated clients.
+all_inc alternation (add edge 0.9E+k, delete edge k) over the circular
+stream (positions mod E, so the sliding window can run past the end);
+requests arrive by a Poisson process from 256 simulated clients.
 
 Legs:
   plain   one update per device batch (rg_serve with batch cap 1), at the
           highest offered rate whose p99.9 stays within the target;
-  serve   continuous batching with the adaptive batch cap, offered rate
-          ramped x1.5 per calibration round; the W warmup and K timed steps
-          then run at the best passing rate (a step = one serving round of
-          ~step_seconds of arrivals);
+  serve   continuous batching with the adaptive batch cap: offered rate
+          ramped x1.5 per short calibration round (two misses in a row end
+          the ramp), then two bisection rounds; the W warmup and K timed
+          steps run at the best passing rate (a step = one serving round of
+          ~step_seconds of arrivals) and must themselves meet the target,
+          else the rate backs off x0.85 and they run again;
   check   every incremental tree is compared with a fresh full rebuild on the
           final graph (values bit-exact);
   cpu     the oracle's sequential port of the reference engine on a bounded
-          sample of the same stream (rank 0, N=1 only).
+          sample of the same stream, after the GPU legs (rank 0, N=1 only).
 """
 import argparse
 import json
 import os
 import sys
-import threading
 import time
 
 import numpy as np
@@ -111,16 +113,13 @@ def main():
     log(f"setup {time.time() - t_setup:.1f}s V={V} E={E} loaded={n0} root={root} build={build_s}")
     info = g.info()
 
-    # CPU baseline setup in the background (C code releases the GIL)
+    # CPU baseline: snapshot the built fixpoint and the stream before any
+    # update runs; the (multi-threaded) host setup runs after the GPU
+    # measurements, so it cannot steal host cores from the serving loop
     cpu = {}
-    cpu_thread = None
+    snap = None
     if not args.no_cpu:
-        # snapshot the built fixpoint and the stream before any update runs
         snap = [L.read() for L in labels]
-        src, dst, w = stream.read(0, E)
-        cpu_thread = threading.Thread(target=cpu_baseline_setup,
-                                      args=(cpu, src, dst, w, snap, labels, V, E, n0, root))
-        cpu_thread.start()
 
     cur = Cursor(n0, E)
 
@@ -193,7 +192,8 @@ def main():
             acc["gpu_ms"] += res["gpu_ms"]
             acc["overloaded"] += res["overloaded"]
             log(f"step {k}: {res['updates_per_s']:.0f} upd/s p99.9={res['p999_ms']:.2f}ms "
-                f"mean_batch={res['mean_batch']:.1f} gpu_busy={res['gpu_ms'] / 1e3 / max(res['wall_s'], 1e-9):.2f}")
+                f"mean_batch={res['mean_batch']:.1f} max_batch={res['max_batch']} rebuilds={res['rebuilds']} "
+                f"gpu_busy={res['gpu_ms'] / 1e3 / max(res['wall_s'], 1e-9):.2f}")
         lat_all = np.concatenate(lats)      # unserved updates of an overloaded step count as +inf
         p = {q: float(np.percentile(lat_all, q)) for q in (50, 99, 99.9)}
         value = acc["total"] / acc["wall"]
@@ -229,8 +229,9 @@ def main():
             fresh.close()
         log(f"values vs full rebuild: {check}")
 
-    if cpu_thread is not None:
-        cpu_thread.join()
+    if snap is not None:
+        src, dst, w = stream.read(0, E)
+        cpu_baseline_setup(cpu, src, dst, w, snap, labels, V, E, n0, root)
         cpu_baseline_run(cpu, args.cpu_seconds)
 
     out = {

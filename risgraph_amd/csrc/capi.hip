@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -398,6 +399,26 @@ uint64_t host_work(const Ctl &h, uint64_t V)
     }
 }
 
+// Team width of a run: 8 workgroups per tree for small batches (cheaper
+// barriers), growing with the batch to 32 (4 trees x 32 workgroups of 1024
+// threads stay co-resident on the 256 CUs, which the team barrier needs).
+uint32_t team_width(const rg_labels *L, uint32_t n_ops)
+{
+    if (L->team_w) return L->team_w;
+    uint32_t w = 8;
+    while (w < 32 && (uint64_t)w * 2048 < n_ops) w *= 2;
+    return w;
+}
+
+uint32_t team_cap(const rg_labels *L) { return L->block_max == RG_AUTO ? 1024u * L->run_w : L->block_max; }
+
+// A tree's next pass runs in the persistent team launch when it is not a
+// full rebuild and fits the team's work cap; otherwise it is one grid step.
+bool team_ok(const rg_labels *L, const Ctl &h)
+{
+    return h.phase != PH_BUILD_INIT && h.stage == 0 && host_work(h, L->g->V) <= team_cap(L);
+}
+
 // seed: a mixed batch starts with its deletions and seeds its inserts in the
 // pull pass (or right after the roots pass when no tree edge was deleted)
 void engine_init(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool new_idx)
@@ -423,28 +444,11 @@ void engine_init(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool 
     h.flags &= F_ALL_MODIFIED;
     h.bar_count = h.bar_gen = h.bar_exit = 0;
     h.rbound = L->algo == SSWP ? 0u : 0xFFFFFFFFu;
+    h.work_max = team_cap(L);
     h.levels = 0;
 }
 
-// Team width of a run: 8 workgroups per tree for small batches (cheaper
-// barriers), growing with the batch to 32 (4 trees x 32 workgroups of 1024
-// threads stay co-resident on the 256 CUs, which the team barrier needs).
-uint32_t team_width(const rg_labels *L, uint32_t n_ops)
-{
-    if (L->team_w) return L->team_w;
-    uint32_t w = 8;
-    while (w < 32 && (uint64_t)w * 2048 < n_ops) w *= 2;
-    return w;
-}
-
-uint32_t team_cap(const rg_labels *L) { return L->block_max == RG_AUTO ? 1024u * L->run_w : L->block_max; }
-
-// A tree's next pass runs in the persistent team launch when it is not a
-// full rebuild and fits the team's work cap; otherwise it is one grid step.
-bool team_ok(const rg_labels *L, const Ctl &h)
-{
-    return h.phase != PH_BUILD_INIT && h.stage == 0 && host_work(h, L->g->V) <= team_cap(L);
-}
+constexpr int GRID_SPEC = 1;   // grid steps enqueued per host round trip
 
 // Run the engines of labels L[0..n) from `phase0` to completion on the graph
 // stream, after the store batch already queued there.  Trees whose passes fit
@@ -476,6 +480,13 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
     int left = n;
     uint64_t guard_iter = 0;
     std::vector<int> team;
+    // progress guard: every round must move some tree's state machine
+    auto state_of = [](const Ctl &h) {
+        return std::array<uint64_t, 4>{h.phase | (uint64_t)h.stage << 32, h.epoch, h.n_cur, h.levels};
+    };
+    std::vector<std::array<uint64_t, 4>> last(n);
+    for (int i = 0; i < n; i++) last[i] = state_of(*L[i]->hc);
+    int stalled = 0;
     while (left) {
         team.clear();
         for (int i = 0; i < n; i++) {
@@ -484,10 +495,13 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
             if (team_ok(L[i], h)) {
                 team.push_back(i);
             } else {
+                // a few grid steps back to back without host round trips;
+                // each turns into a no-op once the tree's passes fit the
+                // team again (or it is done)
                 hipEvent_t ea, eb;
                 prof_begin(g, 2, host_work(h, g->V), st, &ea, &eb);
                 if (ea) HC(hipEventRecord(ea, st));
-                launch_engine_grid_step(args[i], h, st);
+                launch_engine_grid_steps(args[i], h, GRID_SPEC, st);
                 if (eb) HC(hipEventRecord(eb, st));
                 if (stats) stats->grid_steps++;
             }
@@ -539,6 +553,14 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 if (stats) stats->rebuilds++;
             }
         }
+        bool moved = false;
+        for (int i = 0; i < n; i++) {
+            const auto now = state_of(*L[i]->hc);
+            if (now != last[i]) moved = true;
+            last[i] = now;
+        }
+        stalled = moved ? 0 : stalled + 1;
+        if (stalled >= 3) throw RgError{RG_EDEVICE, "engine made no progress"};
         if (++guard_iter > (1ull << 26)) throw RgError{RG_EDEVICE, "engine did not converge"};
     }
 }

@@ -91,7 +91,7 @@ struct Stage {
 // snapshot of the control block taken at a pass boundary
 struct Snap {
     uint32_t phase, stage, level, n_cur, n_tree, n_chunk, n_ops;
-    uint32_t sel_cur, sel_next, sel_tree, epoch, mepoch, seed, rbound, flags;
+    uint32_t sel_cur, sel_next, sel_tree, epoch, mepoch, seed, rbound, flags, work_max;
     int32_t idx;
     Ctl *c;       // live control block: LDS copy for a one-workgroup team, else HBM
     Stage *stg;   // this workgroup's append buffers (LDS)
@@ -115,6 +115,7 @@ __device__ __forceinline__ void take_snap(Ctl *c, Snap &s)
     s.seed = ald(&c->seed);
     s.rbound = ald(&c->rbound);
     s.flags = ald(&c->flags);
+    s.work_max = ald(&c->work_max);
     s.idx = ald(&c->idx);
 }
 
@@ -267,12 +268,18 @@ __device__ __forceinline__ bool mark_inv(const EngineArgs &a, const Snap &s, uin
 }
 
 // has_parents (graph.hpp:1889-1907): true when an ancestor of x (x
-// included) carries a mark of this run; a walk longer than the cap asks for a
-// full rebuild, like the reference's 0.1*V guard.  Values are monotone along
-// tree paths (BFS/SSSP non-decreasing, SSWP non-increasing away from the
-// root), and every marked vertex descends from a deletion root, so an
-// ancestor whose value is below the smallest root value (above the largest,
-// for SSWP) has no marked ancestor either and the walk stops there.
+// included) carries a mark of this run.  Values are monotone along tree
+// paths (BFS/SSSP non-decreasing, SSWP non-increasing away from the root),
+// and every marked vertex descends from a deletion root, so an ancestor whose
+// value is below the smallest root value (above the largest, for SSWP) has
+// no marked ancestor either and the walk stops there.  A walk is given up
+// after RESCUE_WALK steps and the candidate treated as invalid (the vertex
+// then stays in the subtree and is re-pulled: more work, same values); the
+// tuning's walk_limit, when smaller, keeps the reference's 0.1*V guard and
+// asks for a full rebuild instead.
+constexpr uint32_t RESCUE_WALK = 32;
+constexpr uint32_t RESCUE_CHUNKS = 8;   // chunk items of one vertex's rescue scan
+
 template <int A>
 __device__ bool has_parents(const EngineArgs &a, const Snap &s, uint32_t x, int32_t idx)
 {
@@ -293,6 +300,7 @@ __device__ bool has_parents(const EngineArgs &a, const Snap &s, uint32_t x, int3
             atomicOr(&s.c->flags, F_NEED_REBUILD);
             return true;
         }
+        if (steps > RESCUE_WALK) return true;
     }
 }
 
@@ -545,7 +553,10 @@ __device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uin
             int L = a.pull_l[li];
             uint32_t len = a.lists[L].len[v];
             if (len > LONG_LIST) {
-                if (glane == 0) defer_chunks(a, s, v, L, len, CO_RESCUE);
+                // the scan looks for any one valid parent: a bounded prefix
+                // of a hub's in-list is searched, the vertex otherwise stays
+                // invalid (re-pulled later, same values)
+                if (glane == 0) defer_chunks(a, s, v, L, min(len, RESCUE_CHUNKS * CHUNK), CO_RESCUE);
                 deferred = true;
                 continue;
             }
@@ -801,6 +812,15 @@ __device__ __forceinline__ uint32_t phase_work(const EngineArgs &a, const Snap &
     }
 }
 
+// the tree's current pass runs grid-wide: a full rebuild, a chunk stage
+// handed over by the team, or more work than the team takes
+__device__ __forceinline__ bool grid_needed(const EngineArgs &a, const Snap &s)
+{
+    if (s.phase == PH_DONE || s.phase == PH_REBUILD) return false;
+    if (s.flags & (F_QUEUE_OVERFLOW | F_ABORT)) return false;   // the host reports these
+    return s.phase == PH_BUILD_INIT || s.stage != 0 || phase_work(a, s) > s.work_max;
+}
+
 // copy the control block to the pinned host mirror (one thread), so the
 // host reads the state after a stream sync without a copy launch
 __device__ void publish(const EngineArgs &a, const Ctl *c)
@@ -1010,7 +1030,7 @@ __global__ __launch_bounds__(GRID_BLOCK) void k_grid_main(EngineArgs a)
     __syncthreads();
     const Snap s = snap;
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthr = gridDim.x * blockDim.x;
-    if (s.stage != 0) return;
+    if (!grid_needed(a, s) || s.stage != 0) return;
     pass_main<A>(a, s, tid, nthr, stk);
     stage_flush(a, s);
 }
@@ -1028,13 +1048,18 @@ __global__ __launch_bounds__(GRID_BLOCK) void k_grid_chunks(EngineArgs a)
     __syncthreads();
     const Snap s = snap;
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthr = gridDim.x * blockDim.x;
+    if (!grid_needed(a, s)) return;
     pass_chunks<A>(a, s, tid, nthr, s.n_chunk);
     stage_flush(a, s);
 }
 
+// closes a grid step (the same test as its two pass kernels, on the same
+// unchanged fields) and publishes the control block either way
 __global__ void k_advance(EngineArgs a)
 {
-    advance(a, a.ctl);
+    Snap s;
+    take_snap(a.ctl, s);
+    if (grid_needed(a, s)) advance(a, a.ctl);
     publish(a, a.ctl);
 }
 
@@ -1053,7 +1078,7 @@ void launch_engine_team(const EngineArgs *const *a, int n, uint32_t W, uint32_t 
 }
 
 template <int A>
-static void grid_launch(const EngineArgs &a, const Ctl &h, hipStream_t st)
+static void grid_launch(const EngineArgs &a, const Ctl &h, int steps, hipStream_t st)
 {
     uint64_t threads;
     switch (h.phase) {
@@ -1066,18 +1091,20 @@ static void grid_launch(const EngineArgs &a, const Ctl &h, hipStream_t st)
     uint64_t blocks = (threads + GRID_BLOCK - 1) / GRID_BLOCK;
     if (blocks < 1) blocks = 1;
     if (blocks > GRID_MAX_BLOCKS) blocks = GRID_MAX_BLOCKS;
-    if (h.stage == 0) hipLaunchKernelGGL(k_grid_main<A>, dim3((uint32_t)blocks), dim3(GRID_BLOCK), 0, st, a);
-    hipLaunchKernelGGL(k_grid_chunks<A>, dim3(GRID_MAX_BLOCKS / 2), dim3(GRID_BLOCK), 0, st, a);
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, a);
+    for (int k = 0; k < steps; k++) {
+        hipLaunchKernelGGL(k_grid_main<A>, dim3((uint32_t)blocks), dim3(GRID_BLOCK), 0, st, a);
+        hipLaunchKernelGGL(k_grid_chunks<A>, dim3(GRID_MAX_BLOCKS / 2), dim3(GRID_BLOCK), 0, st, a);
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, a);
+    }
 }
 
-void launch_engine_grid_step(const EngineArgs &a, const Ctl &h, hipStream_t st)
+void launch_engine_grid_steps(const EngineArgs &a, const Ctl &h, int steps, hipStream_t st)
 {
     switch (a.algo) {
-    case BFS: grid_launch<BFS>(a, h, st); break;
-    case SSSP: grid_launch<SSSP>(a, h, st); break;
-    case SSWP: grid_launch<SSWP>(a, h, st); break;
-    default: grid_launch<WCC>(a, h, st); break;
+    case BFS: grid_launch<BFS>(a, h, steps, st); break;
+    case SSSP: grid_launch<SSSP>(a, h, steps, st); break;
+    case SSWP: grid_launch<SSWP>(a, h, steps, st); break;
+    default: grid_launch<WCC>(a, h, steps, st); break;
     }
 }
 

@@ -144,7 +144,7 @@ struct Ctl {
     uint32_t bar_gen;      // team barrier: last released episode
     uint32_t bar_exit;     // team workgroups that left the launch
     uint32_t rbound;       // smallest deletion-root value (largest for SSWP)
-    uint32_t pad;
+    uint32_t work_max;     // largest pass the team runs (larger: grid-wide)
     unsigned long long edges, relax_ok, levels, pad2;
 };
 
@@ -219,7 +219,10 @@ void launch_pool_recycle(const StoreArgs &a, hipStream_t s);
 // block is taken from its pinned mirror (first launch of a run).
 void launch_engine_team(const EngineArgs *const *a, int n, uint32_t W, uint32_t work_max, uint32_t init_mask,
                         hipStream_t s);
-void launch_engine_grid_step(const EngineArgs &a, const Ctl &host_ctl, hipStream_t s);
+// `steps` grid-wide passes enqueued back to back, sized for the pass the host
+// last saw; each is a no-op unless the tree's current pass is grid-sized
+// (checked on the device control block)
+void launch_engine_grid_steps(const EngineArgs &a, const Ctl &host_ctl, int steps, hipStream_t s);
 void launch_engine_fill_init(const EngineArgs &a, hipStream_t s);
 void launch_gen_rmat(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, int scale,
                      uint64_t seed, double a, double b, double c, hipStream_t s);

@@ -206,6 +206,10 @@ __device__ __forceinline__ uint64_t gbcast(uint64_t x, const Grp &g)
     return __shfl((unsigned long long)x, 0, G);
 }
 
+// entries per lane kept in flight by the copy and index-build loops (a
+// group owns whole hub lists, so these loops are latency-bound otherwise)
+constexpr int UNR = 8;
+
 // group-parallel (re)build of a list's hash index at capacity `hcap`
 template <int G>
 __device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, const Grp &g)
@@ -219,20 +223,40 @@ __device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, co
     if (h0 == HEMPTY) return false;
     for (uint32_t b = g.lane; b < hcap; b += G) a.slots.hkey[h0 + b] = HEMPTY;
     __threadfence_block();
-    uint32_t mask = hcap - 1;
-    for (uint32_t i = g.lane; i < st.len; i += G) {
-        uint32_t nb = a.slots.nbr[st.off + i];
-        uint32_t w = a.slots.wt ? a.slots.wt[st.off + i] : 0;
-        uint64_t key = hkey_of(nb, w);
-        uint32_t b = (uint32_t)mix64(key) & mask;
-        uint32_t probes = 0;
-        while (atomicCAS((unsigned long long *)&a.slots.hkey[h0 + b], (unsigned long long)HEMPTY,
-                         (unsigned long long)key) != HEMPTY) {
-            b = (b + 1) & mask;
-            if (++probes >= hcap) break;   // cannot happen below 50% load; never spin
+    const uint32_t mask = hcap - 1;
+    for (uint32_t i0 = g.lane; i0 < st.len; i0 += G * UNR) {
+        uint64_t key[UNR];
+        uint32_t bk[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t i = i0 + u * G;
+            key[u] = HEMPTY;
+            if (i < st.len) {
+                const uint32_t nb = a.slots.nbr[st.off + i];
+                const uint32_t w = a.slots.wt ? a.slots.wt[st.off + i] : 0;
+                key[u] = hkey_of(nb, w);
+            }
+            bk[u] = (uint32_t)mix64(key[u]) & mask;
         }
-        if (probes >= hcap) atomicOr(&a.pool->oom, 2u);
-        else a.slots.hval[h0 + b] = i;
+        unsigned long long prev[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++)
+            prev[u] = key[u] == HEMPTY ? 0ull
+                                       : atomicCAS((unsigned long long *)&a.slots.hkey[h0 + bk[u]],
+                                                   (unsigned long long)HEMPTY, (unsigned long long)key[u]);
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            if (key[u] == HEMPTY) continue;
+            uint32_t b = bk[u], probes = 0;
+            while (prev[u] != HEMPTY) {   // taken: linear probing (cannot fill below 50% load)
+                b = (b + 1) & mask;
+                if (++probes >= hcap) break;
+                prev[u] = atomicCAS((unsigned long long *)&a.slots.hkey[h0 + b], (unsigned long long)HEMPTY,
+                                    (unsigned long long)key[u]);
+            }
+            if (probes >= hcap) atomicOr(&a.pool->oom, 2u);
+            else a.slots.hval[h0 + b] = i0 + u * G;
+        }
     }
     __threadfence_block();
     st.hoff = h0;
@@ -252,10 +276,26 @@ __device__ bool grow_block(const StoreArgs &a, ListState &st, uint32_t need, con
     if (g.lane == 0) o = pool_alloc(a, 0, ncap);
     o = gbcast<G>(o, g);
     if (o == HEMPTY) return false;
-    for (uint32_t i = g.lane; i < st.len; i += G) {
-        a.slots.nbr[o + i] = a.slots.nbr[st.off + i];
-        a.slots.cnt[o + i] = a.slots.cnt[st.off + i];
-        if (a.slots.wt) a.slots.wt[o + i] = a.slots.wt[st.off + i];
+    for (uint32_t i0 = g.lane; i0 < st.len; i0 += G * UNR) {
+        uint32_t nb[UNR], cn[UNR], wt[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t i = i0 + u * G;
+            if (i < st.len) {
+                nb[u] = a.slots.nbr[st.off + i];
+                cn[u] = a.slots.cnt[st.off + i];
+                wt[u] = a.slots.wt ? a.slots.wt[st.off + i] : 0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t i = i0 + u * G;
+            if (i < st.len) {
+                a.slots.nbr[o + i] = nb[u];
+                a.slots.cnt[o + i] = cn[u];
+                if (a.slots.wt) a.slots.wt[o + i] = wt[u];
+            }
+        }
     }
     __threadfence_block();
     if (g.lane == 0 && st.cap) pool_free(a, 0, st.off, st.cap);
