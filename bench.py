@@ -59,6 +59,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--json-out", default="")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for rocprofv3 --pmc: after the setup, apply --probe-batches store batches of "
+                         "--probe-pairs insert+delete pairs, then a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--probe-pairs", type=int, default=65536)
+    ap.add_argument("--probe-batches", type=int, default=16)
     return ap.parse_args()
 
 
@@ -112,6 +117,9 @@ def main():
         build_s[nm] = round(time.time() - t, 3)
     log(f"setup {time.time() - t_setup:.1f}s V={V} E={E} loaded={n0} root={root} build={build_s}")
     info = g.info()
+    if args.traffic_probe:
+        traffic_probe(g, labels, stream, n0, E, args)
+        return
 
     # CPU baseline: snapshot the built fixpoint and the stream before any
     # update runs; the (multi-threaded) host setup runs after the GPU
@@ -284,6 +292,49 @@ B_VERTEX = 32                     # per changed vertex: label word r/w + out/in 
 B_EDGE = {"bfs": 12, "sssp": 16, "sswp": 16, "wcc": 12}   # nbr id (+ weight) + neighbour label word
 
 
+def traffic_probe(g, labels, stream, n0, E, args):
+    """Fixed-size store batches (the serve mix: inserts + deletes, all four
+    trees repaired) and a plain device copy of known size, for the HBM
+    FETCH_SIZE / WRITE_SIZE counter runs."""
+    pos = 0
+    m = args.probe_pairs
+    for _ in range(args.probe_batches):
+        g.apply_stream(labels, stream, (n0 + pos) % E, m, pos % E, m, True)
+        pos += m
+    # the HIP runtime's device-to-device copy (its blit kernel) of 1 GiB
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = 1 << 30
+    src_p, dst_p = ctypes.c_void_p(), ctypes.c_void_p()
+    ok = hip.hipMalloc(ctypes.byref(src_p), ctypes.c_size_t(nbytes)) == 0
+    ok = ok and hip.hipMalloc(ctypes.byref(dst_p), ctypes.c_size_t(nbytes)) == 0
+    if ok:
+        for _ in range(3):
+            hip.hipMemcpy(dst_p, src_p, ctypes.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
+        hip.hipFree(src_p)
+        hip.hipFree(dst_p)
+    print(json.dumps({"probe": "traffic", "store_batch_edge_ops": 2 * m, "batches": args.probe_batches,
+                      "copy_ok": bool(ok), "copy_bytes_read": nbytes, "copy_bytes_written": nbytes}),
+          flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per store-apply launch from the committed counter summary
+    (profiles/k_apply_traffic_rmat24.md), with the batch size it was taken at."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_apply_traffic_rmat24.md")
+    try:
+        vals = {}
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_"):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = float(v.strip())
+        return vals or None
+    except OSError:
+        return None
+
+
 def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch):
     """Store-apply kernel: algorithmic bytes (32 B per edge op) over its
     event-timed duration, during a serving round at the operating rate.
@@ -309,9 +360,16 @@ def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch):
         "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
         "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
         "avg_launch_ms": round(launch_ms, 5), "launches": ap["launches"],
-        "engine_block_launches": nb,
-        "engine_block_avg_ms": round((eb1["ms"] - eb0["ms"]) / max(nb, 1), 5),
+        "engine_team_launches": nb,
+        "engine_team_avg_ms": round((eb1["ms"] - eb0["ms"]) / max(nb, 1), 5),
     }
+    tr = measured_traffic()
+    if tr and "traffic_bytes_per_launch" in tr:
+        # HBM bytes per launch from the FETCH_SIZE + WRITE_SIZE counter runs
+        # (copy-kernel calibrated), at the batch size they were taken at
+        roof["traffic"] = tr["traffic_bytes_per_launch"]
+        roof["traffic_batch_edge_ops"] = int(tr.get("traffic_batch_edge_ops", 0))
+        roof["traffic_algorithmic_bytes"] = tr.get("traffic_algorithmic_bytes")
     # batch level, from the changed-vertex set (values before vs after)
     out_deg, in_deg = g.degrees()
     deg = out_deg.astype(np.int64) + in_deg.astype(np.int64)

@@ -384,6 +384,8 @@ EngineArgs engine_args(rg_labels *L)
     a.hctl = L->hc;
     a.ticks = L->ticks;
     a.ops = g->ops;
+    a.ops.n = 0;   // the engine takes the batch size from the control block;
+                   // keeping it out lets the device copy of the arguments stay valid
     a.tune = L->tune;
     return a;
 }
