@@ -175,7 +175,9 @@ int rg_last_gpu_ms(rg_graph *g, double *ms);
  * persistent team engine launches (kind 1) and grid-wide engine passes
  * (kind 2).  rg_profile_read returns the launch count, summed device
  * milliseconds and summed work units (edge ops for kind 0, trees for kind 1,
- * pass work for kind 2) accumulated since the graph was made. */
+ * pass work for kind 2) accumulated since the graph was made; kind 3 + p
+ * reads the grid-wide passes of engine phase p (the phase numbers of
+ * rg_labels_phase_time) alone. */
 int rg_profile_enable(rg_graph *g, int mask);
 int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint64_t *units);
 

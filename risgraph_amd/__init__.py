@@ -283,7 +283,8 @@ class Graph:
 
     def profile_read(self, kind):
         """kind 0 = store apply kernel, 1 = persistent team engine launches,
-        2 = grid-wide engine passes."""
+        2 = grid-wide engine passes, 3 + p = grid-wide passes of phase p
+        (PHASES[p])."""
         n, units, ms = u64(), u64(), ctypes.c_double()
         _check(load_library().rg_profile_read(self.h, kind, ctypes.byref(n), ctypes.byref(ms),
                                               ctypes.byref(units)))

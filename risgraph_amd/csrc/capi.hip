@@ -121,12 +121,15 @@ struct rg_graph {
         int kind;
         hipEvent_t a, b;
         uint64_t units;
+        int phase;   // grid-wide passes: the engine phase the step ran (else -1)
     };
     int prof = 0;   // bit k: time kernels of kind k
     std::vector<hipEvent_t> ev_free;
     std::vector<ProfRec> prof_pending;
-    uint64_t p_launch[3]{}, p_units[3]{};
-    double p_ms[3]{};
+    // kinds 0-2, then 3 + phase: grid-wide passes of each engine phase
+    static constexpr int PKINDS = 3 + NPHASE;
+    uint64_t p_launch[PKINDS]{}, p_units[PKINDS]{};
+    double p_ms[PKINDS]{};
 };
 
 struct rg_labels {
@@ -166,29 +169,18 @@ namespace {
 
 void alloc_list(List &l, uint64_t V, hipStream_t st)
 {
-    l.off = dalloc<uint64_t>(V);
-    l.len = dalloc<uint32_t>(V);
-    l.cap = dalloc<uint32_t>(V);
-    l.deg = dalloc<uint32_t>(V);
-    l.hoff = dalloc<uint64_t>(V);
-    l.hcap = dalloc<uint32_t>(V);
-    l.hcnt = dalloc<uint32_t>(V);
-    l.nops = dalloc<uint32_t>(V);
-    l.seg = dalloc<uint32_t>(V);
-    HC(hipMemsetAsync(l.off, 0, V * 8, st));
-    HC(hipMemsetAsync(l.len, 0, V * 4, st));
-    HC(hipMemsetAsync(l.cap, 0, V * 4, st));
-    HC(hipMemsetAsync(l.deg, 0, V * 4, st));
-    HC(hipMemsetAsync(l.hoff, 0xFF, V * 8, st));
-    HC(hipMemsetAsync(l.hcap, 0, V * 4, st));
-    HC(hipMemsetAsync(l.hcnt, 0, V * 4, st));
-    HC(hipMemsetAsync(l.nops, 0, V * 4, st));
+    l.h = dalloc<ListHdr>(V);
+    launch_list_init(l, (uint32_t)V, st);
+    HC(hipGetLastError());
 }
 
-void free_list(List &l)
+void free_list(List &l) { dfree(l.h); }
+
+// one u32 field of every list header -> host array (strided copy)
+void copy_hdr_field(const List &l, size_t field_off, uint32_t *out, uint64_t V, hipStream_t st)
 {
-    dfree(l.off); dfree(l.len); dfree(l.cap); dfree(l.deg);
-    dfree(l.hoff); dfree(l.hcap); dfree(l.hcnt); dfree(l.nops); dfree(l.seg);
+    HC(hipMemcpy2DAsync(out, 4, reinterpret_cast<const char *>(l.h) + field_off, sizeof(ListHdr), 4, V,
+                        hipMemcpyDeviceToHost, st));
 }
 
 void free_ops(rg_graph *g)
@@ -197,7 +189,7 @@ void free_ops(rg_graph *g)
     dfree(g->ops.old); dfree(g->ops.live);
     dfree(g->lops.v); dfree(g->lops.nbr); dfree(g->lops.w); dfree(g->lops.edge);
     dfree(g->lops.delta); dfree(g->lops.flags); dfree(g->lops.rank); dfree(g->lops.segbuf);
-    dfree(g->lops.slot);
+    dfree(g->lops.slot); dfree(g->lops.heavy); dfree(g->lops.owner);
     dfree(g->d_flag);
     g->op_cap = 0;
 }
@@ -225,6 +217,8 @@ void ensure_ops(rg_graph *g, uint64_t n)
     g->lops.rank = dalloc<uint32_t>(lc);
     g->lops.segbuf = dalloc<uint32_t>(lc);
     g->lops.slot = dalloc<uint32_t>(lc);
+    g->lops.heavy = dalloc<uint32_t>(lc);
+    g->lops.owner = dalloc<uint32_t>(lc);
     g->d_flag = dalloc<uint8_t>(cap);
     g->op_cap = cap;
 }
@@ -309,13 +303,14 @@ hipEvent_t prof_event(rg_graph *g)
 }
 
 // bracket the next launch on stream s with events when profiling
-void prof_begin(rg_graph *g, int kind, uint64_t units, hipStream_t s, hipEvent_t *a, hipEvent_t *b)
+void prof_begin(rg_graph *g, int kind, uint64_t units, hipStream_t s, hipEvent_t *a, hipEvent_t *b,
+                int phase = -1)
 {
     *a = *b = nullptr;
     if (!((g->prof >> kind) & 1)) return;
     *a = prof_event(g);
     *b = prof_event(g);
-    g->prof_pending.push_back({kind, *a, *b, units});
+    g->prof_pending.push_back({kind, *a, *b, units, phase});
     (void)s;
 }
 
@@ -329,6 +324,11 @@ void prof_collect(rg_graph *g)
         g->p_launch[r.kind]++;
         g->p_ms[r.kind] += ms;
         g->p_units[r.kind] += r.units;
+        if (r.phase >= 0 && r.phase < NPHASE) {
+            g->p_launch[3 + r.phase]++;
+            g->p_ms[3 + r.phase] += ms;
+            g->p_units[3 + r.phase] += r.units;
+        }
         g->ev_free.push_back(r.a);
         g->ev_free.push_back(r.b);
     }
@@ -501,7 +501,7 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 // each turns into a no-op once the tree's passes fit the
                 // team again (or it is done)
                 hipEvent_t ea, eb;
-                prof_begin(g, 2, host_work(h, g->V), st, &ea, &eb);
+                prof_begin(g, 2, host_work(h, g->V), st, &ea, &eb, (int)h.phase);
                 if (ea) HC(hipEventRecord(ea, st));
                 launch_engine_grid_steps(args[i], h, GRID_SPEC, st);
                 if (eb) HC(hipEventRecord(eb, st));
@@ -717,7 +717,7 @@ int rg_degree(rg_graph *g, uint64_t v, int incoming, uint64_t *out)
         if (v >= g->V) throw RgError{RG_EINVAL_VERTEX, "VertexId error."};
         const List &l = (incoming && !g->symmetric) ? g->lists[1] : g->lists[0];
         uint32_t d = 0;
-        HC(hipMemcpyAsync(&d, l.deg + v, 4, hipMemcpyDeviceToHost, g->st));
+        HC(hipMemcpyAsync(&d, &l.h[v].deg, 4, hipMemcpyDeviceToHost, g->st));
         HC(hipStreamSynchronize(g->st));
         *out = d;
     });
@@ -729,11 +729,11 @@ int rg_edge_num(rg_graph *g, uint64_t src, uint64_t dst, uint64_t data, uint64_t
     return guard([&] {
         std::lock_guard<std::recursive_mutex> lk(g->mu);
         if (src >= g->V || dst >= g->V) throw RgError{RG_EINVAL_VERTEX, "VertexId error."};
-        uint64_t off = 0;
-        uint32_t len = 0;
-        HC(hipMemcpyAsync(&off, g->lists[0].off + src, 8, hipMemcpyDeviceToHost, g->st));
-        HC(hipMemcpyAsync(&len, g->lists[0].len + src, 4, hipMemcpyDeviceToHost, g->st));
+        ListHdr hd{};
+        HC(hipMemcpyAsync(&hd, &g->lists[0].h[src], sizeof(ListHdr), hipMemcpyDeviceToHost, g->st));
         HC(hipStreamSynchronize(g->st));
+        const uint64_t off = hd.off;
+        const uint32_t len = hd.len;
         std::vector<uint32_t> nb(len), cnt(len), w(len);
         if (len) {
             HC(hipMemcpyAsync(nb.data(), g->slots.nbr + off, len * 4ull, hipMemcpyDeviceToHost, g->st));
@@ -1183,7 +1183,7 @@ int rg_profile_enable(rg_graph *g, int on)
 
 int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint64_t *units)
 {
-    if (!g || kind < 0 || kind > 2) return RG_EINVAL;
+    if (!g || kind < 0 || kind >= rg_graph::PKINDS) return RG_EINVAL;
     return guard([&] {
         std::lock_guard<std::recursive_mutex> lk(g->mu);
         prof_collect(g);
@@ -1242,10 +1242,10 @@ int rg_degrees(rg_graph *g, uint32_t *out_deg, uint32_t *in_deg)
     if (!g) return RG_EINVAL;
     return guard([&] {
         std::lock_guard<std::recursive_mutex> lk(g->mu);
-        if (out_deg) HC(hipMemcpyAsync(out_deg, g->lists[0].deg, g->V * 4, hipMemcpyDeviceToHost, g->st));
+        if (out_deg) copy_hdr_field(g->lists[0], offsetof(ListHdr, deg), out_deg, g->V, g->st);
         if (in_deg) {
             const List &l = g->symmetric ? g->lists[0] : g->lists[1];
-            HC(hipMemcpyAsync(in_deg, l.deg, g->V * 4, hipMemcpyDeviceToHost, g->st));
+            copy_hdr_field(l, offsetof(ListHdr, deg), in_deg, g->V, g->st);
         }
         HC(hipStreamSynchronize(g->st));
     });

@@ -551,7 +551,9 @@ __device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uin
         bool won = false, deferred = false;
         for (int li = 0; li < a.npull; li++) {
             int L = a.pull_l[li];
-            uint32_t len = a.lists[L].len[v];
+            uint64_t off;
+            uint32_t len;
+            list_span(a.lists[L], v, off, len);
             if (len > LONG_LIST) {
                 // the scan looks for any one valid parent: a bounded prefix
                 // of a hub's in-list is searched, the vertex otherwise stays
@@ -560,7 +562,7 @@ __device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uin
                 deferred = true;
                 continue;
             }
-            won |= rescue_range<A>(a, s, v, dv, a.lists[L].off[v], 0, len, glane, GROUP);
+            won |= rescue_range<A>(a, s, v, dv, off, 0, len, glane, GROUP);
         }
         if (group_any(won)) return;
         if (deferred) {
@@ -573,12 +575,14 @@ __device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uin
     if (glane == 0) stage_tree(a, s, v);
     for (int li = 0; li < a.npush; li++) {
         int L = a.push_l[li];
-        uint32_t len = a.lists[L].len[v];
+        uint64_t off;
+        uint32_t len;
+        list_span(a.lists[L], v, off, len);
         if (len > LONG_LIST) {
             if (glane == 0) defer_chunks(a, s, v, L, len, CO_EXPAND);
             continue;
         }
-        expand_cont(a, s, v, a.lists[L].off[v], len, glane, st, depth);
+        expand_cont(a, s, v, off, len, glane, st, depth);
     }
 }
 
@@ -606,12 +610,14 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
             uint32_t uv = vval(a.val[v]);
             for (int li = 0; li < a.npush; li++) {
                 int L = a.push_l[li];
-                uint32_t len = a.lists[L].len[v];
+                uint64_t off;
+                uint32_t len;
+                list_span(a.lists[L], v, off, len);
                 if (len > LONG_LIST) {
                     if (glane == 0) defer_chunks(a, s, v, L, len, CO_PUSH);
                     continue;
                 }
-                push_range<A>(a, s, v, uv, a.lists[L].off[v], 0, len, glane, GROUP);
+                push_range<A>(a, s, v, uv, off, 0, len, glane, GROUP);
             }
         } else if (ph == PH_DEL_RESCUE || ph == PH_DEL_EXPAND) {
             // one deletion level (rescue then expand per vertex), continued
@@ -632,12 +638,14 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
             uint64_t best = (A == SSWP) ? 0ull : ~0ull;
             for (int li = 0; li < a.npull; li++) {
                 int L = a.pull_l[li];
-                uint32_t len = a.lists[L].len[v];
+                uint64_t off;
+                uint32_t len;
+                list_span(a.lists[L], v, off, len);
                 if (len > LONG_LIST) {
                     if (glane == 0) defer_chunks(a, s, v, L, len, CO_PULL);
                     continue;
                 }
-                uint64_t b = pull_range<A>(a, s, a.lists[L].off[v], 0, len, glane, GROUP);
+                uint64_t b = pull_range<A>(a, s, off, 0, len, glane, GROUP);
                 if (A == SSWP ? b > best : b < best) best = b;
             }
             best = reduce_best<A>(best, GROUP);
@@ -680,8 +688,9 @@ __device__ void pass_chunks(const EngineArgs &a, const Snap &s, uint32_t tid, ui
         int L = (int)((item >> 32) & 1);
         uint32_t op = (uint32_t)((item >> 33) & 3);
         uint32_t k = (uint32_t)(item >> 35);
-        uint32_t len = a.lists[L].len[v];
-        uint64_t off = a.lists[L].off[v];
+        uint64_t off;
+        uint32_t len;
+        list_span(a.lists[L], v, off, len);
         uint32_t b = k * CHUNK, e = b + CHUNK < len ? b + CHUNK : len;
         if (op == CO_PUSH) {
             push_range<A>(a, s, v, vval(a.val[v]), off, b, e, lane, WAVE);

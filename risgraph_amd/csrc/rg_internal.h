@@ -9,9 +9,10 @@
 //   hash pool   hkey[H] (u64 key = nbr | w << 32, ~0 = empty), hval[H] (u32
 //               local slot) -- per-list open-addressing index built once a
 //               list reaches 512 entries (storage.hpp:304, 358-367).
-//   headers     per direction (out, in): off, len, cap, deg, hoff, hcap,
-//               hcnt, nops / seg (the current batch's ops on the list and
-//               their segment in the batch's op-index buffer).
+//   headers     per direction (out, in): one 64-byte ListHdr per vertex with
+//               off, len, cap, hoff, hcap, hcnt, deg, and nops / seg (the
+//               current batch's ops on the list and their segment in the
+//               batch's op-index buffer).
 // One labels instance (one algorithm's dependency tree):
 //   val[V]      u64 packed word value << 32 | parent (parent 0xFFFFFFFF =
 //               empty), the device form of VertexTree (graph.hpp:54-59),
@@ -46,17 +47,35 @@ struct Slots {
     uint32_t *hval;
 };
 
-struct List {
-    uint64_t *off;
-    uint32_t *len;
-    uint32_t *cap;
-    uint32_t *deg;
-    uint64_t *hoff;
-    uint32_t *hcap;
-    uint32_t *hcnt;
-    uint32_t *nops;   // list ops of the current batch (0 between batches)
-    uint32_t *seg;    // their segment start in ListOps::segbuf
+// One 64-byte record per (direction, vertex): a list op of the store touches
+// one cache line of header state, and traversal reads the first 16 bytes.
+struct alignas(64) ListHdr {
+    uint64_t off;     // block offset in the slot pool
+    uint32_t len;     // entries used (live and tombstones)
+    uint32_t cap;     // block capacity (power of two, 0 = no block)
+    uint64_t hoff;    // index offset in the hash pool (HEMPTY: not indexed)
+    uint32_t hcap;    // index capacity
+    uint32_t hcnt;    // keys in the index
+    uint32_t deg;     // live multiplicity (storage.hpp:396)
+    uint32_t nops;    // list ops of the current batch (0 between batches)
+    uint32_t seg;     // their segment start in ListOps::segbuf
+    uint32_t pad[5];
 };
+static_assert(sizeof(ListHdr) == 64, "list header is one 64-byte line");
+
+struct List {
+    ListHdr *h;
+};
+
+// block offset and length of v's list: one 16-byte load
+__device__ __forceinline__ void list_span(const List &l, uint32_t v, uint64_t &off, uint32_t &len)
+{
+    const uint4 x = *reinterpret_cast<const uint4 *>(&l.h[v]);
+    off = (uint64_t)x.x | ((uint64_t)x.y << 32);
+    len = x.z;
+}
+
+__device__ __forceinline__ uint32_t list_len(const List &l, uint32_t v) { return l.h[v].len; }
 
 // Slab allocator over the slot pool (0) and the hash pool (1): blocks are
 // powers of two; a freed block goes to its class's pending list and becomes
@@ -70,7 +89,9 @@ struct Pool {
     unsigned long long slot_top, slot_cap;
     unsigned long long hash_top, hash_cap;
     unsigned int oom;
-    unsigned int seg_top;   // segment allocation of the current batch
+    unsigned int n_heavy;   // current batch: heavy lists (a workgroup each)
+    // current batch: segment entries reserved (low word) and light lists (high word)
+    unsigned long long seg_pack;
     unsigned int free_n[2][NCLS];
     unsigned int free_take[2][NCLS];
     unsigned int pend_n[2][NCLS];
@@ -105,6 +126,8 @@ struct ListOps {
     uint32_t *rank;     // position among the batch's ops on the same list
     uint32_t *segbuf;   // list-op indices grouped by list (segments)
     uint32_t *slot;     // local index the op resolved to (NILP: none)
+    uint32_t *owner;    // rank-0 list ops of the light lists (dense)
+    uint32_t *heavy;    // rank-0 list ops of the heavy lists (dense)
     uint32_t n;
 };
 
@@ -212,6 +235,8 @@ void launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a = nul
 void launch_store_count(const StoreArgs &a, hipStream_t s);
 void launch_store_reserve(const StoreArgs &a, hipStream_t s);
 void launch_pool_recycle(const StoreArgs &a, hipStream_t s);
+// empty lists (no block, no index) for vertices [0, V)
+void launch_list_init(List l, uint32_t V, hipStream_t s);
 // Persistent team launch over trees *a[0..n) (n <= MAXT; device copies of
 // their arguments), W workgroups each.
 // A tree leaves the launch when its next pass exceeds work_max (grid mode

@@ -7,16 +7,19 @@
 // of dst, or both endpoint lists of a symmetric graph).  Each list op takes
 // a rank on its list with one atomicAdd; the first op of every list reserves
 // the list's segment of an op-index buffer and every op drops its index into
-// place, so each list is owned by exactly one 16-lane group in the apply
-// kernel: no locks, no cross-group races, and the block of a list may be
-// re-allocated (doubling, from the slab pool) before the group appends to it.
-// The owner applies the list's inserts first, then its deletes, so a batch
-// behaves like the reference batch driver (all adds, then all dels:
-// bfs_inc_batch.cpp:142-178).  Short segments are applied one op at a time
-// (lookups scan the list 16 entries per step, or probe its open-addressing
-// index once it holds >= 512 entries); hub lists with many ops in one batch
-// are indexed and applied 16 ops per round, one op per lane, with same-key
-// ops of a round folded into one count update in lane order.
+// place, so each list has exactly one owner: no locks, no cross-owner races,
+// and the block of a list may be re-allocated (doubling, from the slab pool)
+// before the owner appends to it.  The owner applies the list's inserts
+// first, then its deletes, so a batch behaves like the reference batch driver
+// (all adds, then all dels: bfs_inc_batch.cpp:142-178).
+//   k_apply        one 16-lane group per list; ops one at a time (lookups
+//                  scan the list 16 entries per step, or probe its
+//                  open-addressing index once it holds >= 512 entries).
+//   k_apply_heavy  one 256-thread workgroup per list for hub segments (>= 8
+//                  ops on an indexed list), index builds and rebuilds, and
+//                  block copies longer than 256 entries: 256 ops per round,
+//                  same-key ops of a round folded into one count update
+//                  through an LDS key table.
 #include "rg_internal.h"
 #include "rg_gen.h"
 
@@ -74,7 +77,7 @@ __global__ void k_expand_link(StoreArgs a)
         a.lops.delta[j] = delta;
         a.lops.flags[j] = flags;
         int L = (flags & L_IN) ? 1 : 0;
-        a.lops.rank[j] = atomicAdd(&a.lists[L].nops[v], 1u);
+        a.lops.rank[j] = atomicAdd(&a.lists[L].h[v].nops, 1u);
     };
     // graph.hpp:203-253: a directed op touches out[src] (+ in[dst] unless
     // symmetric); an undirected op is the directed op both ways and returns
@@ -103,32 +106,79 @@ __global__ void k_expand_link(StoreArgs a)
 // ---------------------------------------------------------------------
 // segments: group the batch's list ops by list
 // ---------------------------------------------------------------------
-// the first op of every list reserves the list's segment (one counter
-// atomic per wave, from a wave-wide prefix sum of the segment sizes)
-__global__ void k_segment(StoreArgs a)
+// The first op of every list reserves the list's segment and an entry in one
+// of two dense owner lists (rank-0 ops): light lists, which k_apply walks
+// with a 16-lane group each, and heavy lists, which k_apply_heavy gives a
+// workgroup each.  A list is heavy when its segment is a hub segment, or when
+// its inserts (at most its op count) could need an index build or rebuild or
+// a block copy longer than LIGHT_COPY entries.  A block sums its segments and
+// lists with prefix scans and reserves them with one 64-bit atomic (segment
+// entries low, light lists high) plus one for its heavy lists.
+constexpr int SEG_BLOCK = 1024;
+constexpr uint32_t PAR_MIN = 8;        // hub segments: at least this many ops on an indexed list
+constexpr uint32_t LIGHT_COPY = 256;   // longer lists grow in the heavy kernel
+
+__device__ __forceinline__ bool heavy_list(const ListHdr &h, uint32_t n)
 {
+    const uint32_t need = h.len + n;
+    const bool indexed = h.hoff != HEMPTY;
+    return (n >= PAR_MIN && (indexed || need >= INDEX_THRESHOLD)) || (!indexed && need >= INDEX_THRESHOLD) ||
+           (indexed && 2ull * (h.hcnt + n) > h.hcap) || (need > h.cap && h.len > LIGHT_COPY);
+}
+
+__global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
+{
+    __shared__ uint32_t wsum[SEG_BLOCK / WAVE][3];
+    __shared__ unsigned long long bbase;
+    __shared__ uint32_t hbase;
     const uint32_t nlops = 4 * a.ops.n;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t cnt = 0, v = NILP;
+    uint32_t cnt = 0, lt = 0, hv = 0, v = NILP;
     int L = 0;
     if (j < nlops) {
         v = a.lops.v[j];
         if (v != NILP && a.lops.rank[j] == 0) {
             L = (a.lops.flags[j] & L_IN) ? 1 : 0;
-            cnt = a.lists[L].nops[v];
+            const ListHdr &h = a.lists[L].h[v];
+            cnt = h.nops;
+            if (heavy_list(h, cnt)) hv = 1;
+            else lt = 1;
         }
     }
-    const uint32_t lane = __lane_id();
-    uint32_t incl = cnt;
+    const uint32_t lane = __lane_id(), wid = threadIdx.x / WAVE;
+    uint32_t ic = cnt, il = lt, ih = hv;
     for (int d = 1; d < WAVE; d <<= 1) {
-        uint32_t y = __shfl_up(incl, d);
-        if (lane >= (uint32_t)d) incl += y;
+        const uint32_t yc = __shfl_up(ic, d), yl = __shfl_up(il, d), yh = __shfl_up(ih, d);
+        if (lane >= (uint32_t)d) {
+            ic += yc;
+            il += yl;
+            ih += yh;
+        }
     }
-    const uint32_t total = __shfl(incl, WAVE - 1);
-    uint32_t base = 0;
-    if (lane == WAVE - 1 && total) base = atomicAdd(&a.pool->seg_top, total);
-    base = __shfl(base, WAVE - 1);
-    if (cnt) a.lists[L].seg[v] = base + incl - cnt;
+    if (lane == WAVE - 1) {
+        wsum[wid][0] = ic;
+        wsum[wid][1] = il;
+        wsum[wid][2] = ih;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tc = 0, tl = 0, th = 0;
+        for (int w = 0; w < SEG_BLOCK / WAVE; w++) {
+            const uint32_t c = wsum[w][0], l = wsum[w][1], h = wsum[w][2];
+            wsum[w][0] = tc;
+            wsum[w][1] = tl;
+            wsum[w][2] = th;
+            tc += c;
+            tl += l;
+            th += h;
+        }
+        bbase = (tc | tl) ? atomicAdd(&a.pool->seg_pack, ((unsigned long long)tl << 32) | tc) : 0ull;
+        hbase = th ? atomicAdd(&a.pool->n_heavy, th) : 0u;
+    }
+    __syncthreads();
+    if (lt | hv) a.lists[L].h[v].seg = (uint32_t)bbase + wsum[wid][0] + ic - cnt;
+    if (lt) a.lops.owner[(uint32_t)(bbase >> 32) + wsum[wid][1] + il - 1] = j;
+    if (hv) a.lops.heavy[hbase + wsum[wid][2] + ih - 1] = j;
 }
 
 __global__ void k_scatter(StoreArgs a)
@@ -138,7 +188,7 @@ __global__ void k_scatter(StoreArgs a)
     const uint32_t v = a.lops.v[j];
     if (v == NILP) return;
     const int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
-    a.lops.segbuf[a.lists[L].seg[v] + a.lops.rank[j]] = j;
+    a.lops.segbuf[a.lists[L].h[v].seg + a.lops.rank[j]] = j;
 }
 
 // ---------------------------------------------------------------------
@@ -148,6 +198,30 @@ struct ListState {
     uint64_t off, hoff;
     uint32_t len, cap, deg, hcap, hcnt;
 };
+
+__device__ __forceinline__ ListState load_state(const ListHdr &h)
+{
+    ListState st;
+    st.off = h.off;
+    st.len = h.len;
+    st.cap = h.cap;
+    st.hoff = h.hoff;
+    st.hcap = h.hcap;
+    st.hcnt = h.hcnt;
+    st.deg = h.deg;
+    return st;
+}
+
+__device__ __forceinline__ void store_state(ListHdr &h, const ListState &st)
+{
+    h.off = st.off;
+    h.len = st.len;
+    h.cap = st.cap;
+    h.hoff = st.hoff;
+    h.hcap = st.hcap;
+    h.hcnt = st.hcnt;
+    h.deg = st.deg;
+}
 
 __device__ __forceinline__ int log2_pow2(uint64_t n) { return 63 - __clzll((long long)n); }
 
@@ -267,7 +341,7 @@ __device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, co
 
 // move a list into a block of capacity next_pow2(need) (>= 4); the old block
 // is returned to the slab free lists
-template <int G>
+template <int G, int U = UNR>
 __device__ bool grow_block(const StoreArgs &a, ListState &st, uint32_t need, const Grp &g)
 {
     uint32_t ncap = next_pow2(need);
@@ -276,10 +350,10 @@ __device__ bool grow_block(const StoreArgs &a, ListState &st, uint32_t need, con
     if (g.lane == 0) o = pool_alloc(a, 0, ncap);
     o = gbcast<G>(o, g);
     if (o == HEMPTY) return false;
-    for (uint32_t i0 = g.lane; i0 < st.len; i0 += G * UNR) {
-        uint32_t nb[UNR], cn[UNR], wt[UNR];
+    for (uint32_t i0 = g.lane; i0 < st.len; i0 += G * U) {
+        uint32_t nb[U], cn[U], wt[U];
 #pragma unroll
-        for (int u = 0; u < UNR; u++) {
+        for (int u = 0; u < U; u++) {
             const uint32_t i = i0 + u * G;
             if (i < st.len) {
                 nb[u] = a.slots.nbr[st.off + i];
@@ -288,7 +362,7 @@ __device__ bool grow_block(const StoreArgs &a, ListState &st, uint32_t need, con
             }
         }
 #pragma unroll
-        for (int u = 0; u < UNR; u++) {
+        for (int u = 0; u < U; u++) {
             const uint32_t i = i0 + u * G;
             if (i < st.len) {
                 a.slots.nbr[o + i] = nb[u];
@@ -331,22 +405,34 @@ __device__ uint32_t list_find(const StoreArgs &a, const ListState &st, uint32_t 
         if (ins_bucket) *ins_bucket = NILP;
         return NILP;
     }
-    for (uint32_t b = 0; b < st.len; b += G) {
-        uint32_t i = b + g.lane;
-        bool m = false;
-        if (i < st.len) {
-            uint32_t nb = a.slots.nbr[st.off + i];
-            m = (nb & NBR_MASK) == key_nbr;
-            if (m && a.slots.wt) m = a.slots.wt[st.off + i] == key_w;
+    // Linear scan, 8 entries per lane per step (two 16-byte loads in flight):
+    // blocks hold a power of two >= 4 entries at a multiple-of-4 offset, so
+    // a quad that starts below len lies inside the block.  Keys are unique in
+    // a list, so any match is the match.
+    const uint32_t *nb = a.slots.nbr + st.off;
+    for (uint32_t b = 0; b < st.len; b += 8 * G) {
+        uint32_t found = NILP;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t i0 = b + (uint32_t)h * 4 * G + 4 * (uint32_t)g.lane;
+            uint4 q = make_uint4(NILP, NILP, NILP, NILP);
+            if (i0 < st.len) q = *reinterpret_cast<const uint4 *>(nb + i0);
+            const uint32_t e[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const uint32_t i = i0 + (uint32_t)c;
+                if (found == NILP && i < st.len && (e[c] & NBR_MASK) == key_nbr &&
+                    (!a.slots.wt || a.slots.wt[st.off + i] == key_w))
+                    found = i;
+            }
         }
-        uint32_t hit = gballot<G>(m, g);
-        if (hit) return b + (uint32_t)(__ffs(hit) - 1);
+        const uint32_t hit = gballot<G>(found != NILP, g);
+        if (hit) return (uint32_t)__shfl((int)found, g.base + __ffs(hit) - 1);
     }
     return NILP;
 }
 
-constexpr int AG = 16;        // lanes per list in the apply / reserve kernels
-constexpr uint32_t PAR_MIN = 8;   // lists with this many ops in a batch go 16 ops per round
+constexpr int AG = 16;            // lanes per list in the light apply / reserve kernels
 
 template <int G>
 __device__ __forceinline__ int32_t gsum(int32_t x)
@@ -355,8 +441,10 @@ __device__ __forceinline__ int32_t gsum(int32_t x)
     return x;
 }
 
-// one op at a time, in segment order (short segments)
-__device__ bool apply_serial(const StoreArgs &a, ListState &st, const uint32_t *ops, uint32_t n, const Grp &g)
+// One op at a time, in segment order.  The caller guarantees the list needs
+// no index build or rebuild (a new key of an indexed list finds an empty
+// bucket at load <= 1/2) and has room for every insert.
+__device__ void apply_serial(const StoreArgs &a, ListState &st, const uint32_t *ops, uint32_t n, const Grp &g)
 {
     for (int pass = 0; pass < 2; pass++) {
         for (uint32_t i = 0; i < n; i++) {
@@ -374,25 +462,14 @@ __device__ bool apply_serial(const StoreArgs &a, ListState &st, const uint32_t *
                     a.slots.nbr[st.off + idx] = key_nbr | DEAD;
                     a.slots.cnt[st.off + idx] = 0;
                     if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
+                    if (st.hoff != HEMPTY && bucket != NILP) {
+                        a.slots.hkey[st.hoff + bucket] = hkey_of(key_nbr, key_w);
+                        a.slots.hval[st.hoff + bucket] = idx;
+                    }
                 }
                 st.len++;
+                if (st.hoff != HEMPTY) st.hcnt++;
                 __threadfence_block();
-                if (st.hoff != HEMPTY) {
-                    if (2 * (st.hcnt + 1) > st.hcap) {
-                        if (!build_index<AG>(a, st, st.hcap * 2, g)) return false;
-                    } else {
-                        if (g.lane == 0) {
-                            a.slots.hkey[st.hoff + bucket] = hkey_of(key_nbr, key_w);
-                            a.slots.hval[st.hoff + bucket] = idx;
-                        }
-                        st.hcnt++;
-                        __threadfence_block();
-                    }
-                } else if (st.len >= INDEX_THRESHOLD) {
-                    uint32_t hc = next_pow2(2 * st.len);
-                    if (hc < 1024) hc = 1024;
-                    if (!build_index<AG>(a, st, hc, g)) return false;
-                }
             }
             uint32_t c = idx == NILP ? 0 : a.slots.cnt[st.off + idx];
             uint32_t nc, ret;
@@ -420,36 +497,211 @@ __device__ bool apply_serial(const StoreArgs &a, ListState &st, const uint32_t *
             __threadfence_block();
         }
     }
+}
+
+// liveness of the edge ops whose (src -> dst) entry is on this list, from the
+// count after the whole batch
+__device__ __forceinline__ void write_liveness(const StoreArgs &a, const ListState &st, const uint32_t *ops,
+                                               uint32_t n, uint32_t t0, uint32_t nt)
+{
+    for (uint32_t i = t0; i < n; i += nt) {
+        const uint32_t k = ops[i];
+        if (!(a.lops.flags[k] & L_SLOT)) continue;
+        const uint32_t idx = a.lops.slot[k];
+        a.ops.live[a.lops.edge[k]] = (idx != NILP && a.slots.cnt[st.off + idx] > 0) ? 1u : 0u;
+    }
+}
+
+// Light lists: one 16-lane group per list (its rank-0 op, from the light
+// owner list); k_segment already sent every list that could need an index
+// build, a rebuild, a long copy or hub rounds to the heavy list.
+__global__ void __launch_bounds__(256) k_apply(StoreArgs a)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    Grp g;
+    g.lane = threadIdx.x % AG;
+    g.base = (threadIdx.x % WAVE) - g.lane;
+    const uint32_t gid = t / AG;
+    const uint32_t ng = (gridDim.x * blockDim.x) / AG;
+    const uint32_t nown = (uint32_t)(a.pool->seg_pack >> 32);
+    for (uint32_t oi = gid; oi < nown; oi += ng) {
+        const uint32_t j = a.lops.owner[oi];
+        const uint32_t v = a.lops.v[j];
+        const int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
+        ListHdr &hd = a.lists[L].h[v];
+        const uint32_t n = hd.nops;
+        // a list with one op: that op is j itself
+        const uint32_t *ops = n == 1 ? &a.lops.owner[oi] : a.lops.segbuf + hd.seg;
+        ListState st = load_state(hd);
+
+        int32_t n_ins = 0;
+        for (uint32_t i = g.lane; i < n; i += AG) n_ins += a.lops.delta[ops[i]] > 0 ? 1 : 0;
+        const uint32_t need = st.len + (uint32_t)gsum<AG>(n_ins);
+        bool ok = true;
+        if (need > st.cap) ok = grow_block<AG, 1>(a, st, need, g);
+        if (ok) {
+            apply_serial(a, st, ops, n, g);
+            __threadfence_block();
+            write_liveness(a, st, ops, n, g.lane, AG);
+        }
+        if (g.lane == 0) {
+            store_state(hd, st);
+            hd.nops = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// heavy lists: one 256-thread workgroup per list
+// ---------------------------------------------------------------------
+constexpr int HB = 256;           // threads per heavy list
+constexpr uint32_t HT = 2 * HB;   // LDS key table of one round (load <= 1/2)
+constexpr int HUNR = 8;           // entries per thread in flight in copies
+
+struct HeavyShared {
+    unsigned long long key[HT];
+    uint32_t cnt[HT];
+    uint32_t idx[HT];
+    uint32_t c0[HT];
+    unsigned long long bcast;
+    uint32_t n_app;
+    int32_t sum;
+};
+
+__device__ bool wg_grow(const StoreArgs &a, ListState &st, uint32_t need, HeavyShared &sh)
+{
+    uint32_t ncap = next_pow2(need);
+    if (ncap < 4) ncap = 4;
+    if (threadIdx.x == 0) sh.bcast = pool_alloc(a, 0, ncap);
+    __syncthreads();
+    const uint64_t o = sh.bcast;
+    __syncthreads();
+    if (o == HEMPTY) return false;
+    for (uint32_t i0 = threadIdx.x; i0 < st.len; i0 += HB * HUNR) {
+        uint32_t nb[HUNR], cn[HUNR], wt[HUNR];
+#pragma unroll
+        for (int u = 0; u < HUNR; u++) {
+            const uint32_t i = i0 + u * HB;
+            if (i < st.len) {
+                nb[u] = a.slots.nbr[st.off + i];
+                cn[u] = a.slots.cnt[st.off + i];
+                wt[u] = a.slots.wt ? a.slots.wt[st.off + i] : 0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < HUNR; u++) {
+            const uint32_t i = i0 + u * HB;
+            if (i < st.len) {
+                a.slots.nbr[o + i] = nb[u];
+                a.slots.cnt[o + i] = cn[u];
+                if (a.slots.wt) a.slots.wt[o + i] = wt[u];
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && st.cap) pool_free(a, 0, st.off, st.cap);
+    st.off = o;
+    st.cap = ncap;
     return true;
 }
 
-// 16 ops per round, one per lane (hub lists).  The list is indexed first with
-// room for every insert, so lanes probe independently; ops of a round with
-// the same key are folded: the first such lane (the leader) appends a new
-// entry if needed and applies the summed change to the count, and lane r
-// reports the count it would have seen had the ops run one by one in lane
-// order (clamped at 0 for deletes, as storage.hpp:378/393 do).
-__device__ bool apply_parallel(const StoreArgs &a, ListState &st, const uint32_t *ops, uint32_t n, uint32_t n_ins,
-                               const Grp &g)
+__device__ bool wg_build_index(const StoreArgs &a, ListState &st, uint32_t hcap, HeavyShared &sh)
 {
-    uint32_t hc = next_pow2(2 * (st.len + n_ins));
-    if (hc < 1024) hc = 1024;
-    if (st.hoff == HEMPTY || st.hcap < hc) {
-        if (!build_index<AG>(a, st, hc, g)) return false;
+    if (threadIdx.x == 0) {
+        const uint64_t h0 = pool_alloc(a, 1, hcap);
+        if (h0 != HEMPTY && st.hoff != HEMPTY) pool_free(a, 1, st.hoff, st.hcap);
+        sh.bcast = h0;
     }
+    __syncthreads();
+    const uint64_t h0 = sh.bcast;
+    __syncthreads();
+    if (h0 == HEMPTY) return false;
+    for (uint32_t b = threadIdx.x; b < hcap; b += HB) a.slots.hkey[h0 + b] = HEMPTY;
+    __syncthreads();
+    const uint32_t mask = hcap - 1;
+    for (uint32_t i0 = threadIdx.x; i0 < st.len; i0 += HB * HUNR) {
+        uint64_t key[HUNR];
+        uint32_t bk[HUNR];
+        unsigned long long prev[HUNR];
+#pragma unroll
+        for (int u = 0; u < HUNR; u++) {
+            const uint32_t i = i0 + u * HB;
+            key[u] = HEMPTY;
+            if (i < st.len) key[u] = hkey_of(a.slots.nbr[st.off + i], a.slots.wt ? a.slots.wt[st.off + i] : 0);
+            bk[u] = (uint32_t)mix64(key[u]) & mask;
+        }
+#pragma unroll
+        for (int u = 0; u < HUNR; u++)
+            prev[u] = key[u] == HEMPTY ? 0ull
+                                       : atomicCAS((unsigned long long *)&a.slots.hkey[h0 + bk[u]],
+                                                   (unsigned long long)HEMPTY, (unsigned long long)key[u]);
+#pragma unroll
+        for (int u = 0; u < HUNR; u++) {
+            if (key[u] == HEMPTY) continue;
+            uint32_t b = bk[u], probes = 0;
+            while (prev[u] != HEMPTY) {   // linear probing (cannot fill below 50% load)
+                b = (b + 1) & mask;
+                if (++probes >= hcap) break;
+                prev[u] = atomicCAS((unsigned long long *)&a.slots.hkey[h0 + b], (unsigned long long)HEMPTY,
+                                    (unsigned long long)key[u]);
+            }
+            if (probes >= hcap) atomicOr(&a.pool->oom, 2u);
+            else a.slots.hval[h0 + b] = i0 + u * HB;
+        }
+    }
+    __syncthreads();
+    st.hoff = h0;
+    st.hcap = hcap;
+    st.hcnt = st.len;
+    return true;
+}
+
+// Apply an indexed list's segment HB ops per round, one op per thread.  Ops
+// of a round with the same key meet in an LDS table: each adds its |delta|
+// to the key's total and gets the sum of the ops before it (`pre`), in the
+// order the LDS atomics took them.  The first op of each key (pre == 0)
+// looks the key up, appends a new entry if needed, and applies the total to
+// the count; every op then reports the count it would have seen had the
+// key's ops run one by one in that order (clamped at 0 for deletes, as
+// storage.hpp:378/393 do), so the returns of a key are the reference's
+// multiset.
+__device__ void wg_apply_rounds(const StoreArgs &a, ListState &st, const uint32_t *ops, uint32_t n, HeavyShared &sh)
+{
+    const uint32_t tid = threadIdx.x;
     const uint32_t mask = st.hcap - 1;
     for (int pass = 0; pass < 2; pass++) {
-        for (uint32_t r0 = 0; r0 < n; r0 += AG) {
-            const uint32_t i = r0 + g.lane;
+        for (uint32_t r0 = 0; r0 < n; r0 += HB) {
+            for (uint32_t b = tid; b < HT; b += HB) {
+                sh.key[b] = HEMPTY;
+                sh.cnt[b] = 0;
+            }
+            if (tid == 0) {
+                sh.n_app = 0;
+                sh.sum = 0;
+            }
+            __syncthreads();
+            const uint32_t i = r0 + tid;
             const uint32_t k = i < n ? ops[i] : 0;
             const int8_t delta = i < n ? a.lops.delta[k] : (int8_t)0;
             const bool act = i < n && ((pass == 0) == (delta > 0));
-            const uint32_t key_nbr = act ? a.lops.nbr[k] : 0;
-            const uint32_t key_w = (act && a.slots.wt) ? a.lops.w[k] : 0;
-            const uint64_t key = act ? hkey_of(key_nbr, key_w) : HEMPTY;
-            const uint32_t ad = act ? (uint32_t)(delta < 0 ? -delta : delta) : 0;
-            uint32_t idx = NILP, bucket = NILP;
+            uint32_t key_nbr = 0, key_w = 0, ls = 0, pre = 0;
+            uint64_t key = HEMPTY;
             if (act) {
+                key_nbr = a.lops.nbr[k];
+                key_w = a.slots.wt ? a.lops.w[k] : 0;
+                key = hkey_of(key_nbr, key_w);
+                ls = (uint32_t)mix64(key) & (HT - 1);
+                for (uint32_t p = 0; p < HT; p++) {   // at most HB keys in HT buckets: always ends
+                    const unsigned long long prev =
+                        atomicCAS(&sh.key[ls], (unsigned long long)HEMPTY, (unsigned long long)key);
+                    if (prev == HEMPTY || prev == key) break;
+                    ls = (ls + 1) & (HT - 1);
+                }
+                pre = atomicAdd(&sh.cnt[ls], (uint32_t)(delta < 0 ? -delta : delta));
+            }
+            __syncthreads();
+            if (act && pre == 0) {
+                uint32_t idx = NILP, bucket = NILP;
                 uint32_t b = (uint32_t)mix64(key) & mask;
                 for (uint32_t p = 0; p < st.hcap; p++, b = (b + 1) & mask) {
                     const uint64_t kk = a.slots.hkey[st.hoff + b];
@@ -462,125 +714,106 @@ __device__ bool apply_parallel(const StoreArgs &a, ListState &st, const uint32_t
                         break;
                     }
                 }
-            }
-            // same-key lanes of this round
-            uint32_t leader = g.lane, pre = 0, tot = 0;
-            bool first = true;
-            for (int q = 0; q < AG; q++) {
-                const uint64_t kq = __shfl((unsigned long long)key, g.base + q);
-                const uint32_t aq = __shfl(ad, g.base + q);
-                if (act && aq && kq == key) {
-                    if (first) {
-                        leader = (uint32_t)q;
-                        first = false;
+                uint32_t c = 0;
+                if (idx == NILP && pass == 0 && bucket != NILP) {
+                    // a key new to the list: append an entry with count 0
+                    idx = st.len + atomicAdd(&sh.n_app, 1u);
+                    a.slots.nbr[st.off + idx] = key_nbr | DEAD;
+                    a.slots.cnt[st.off + idx] = 0;
+                    if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
+                    uint32_t bb = bucket;
+                    for (uint32_t p = 0; p < st.hcap; p++, bb = (bb + 1) & mask) {
+                        if (atomicCAS((unsigned long long *)&a.slots.hkey[st.hoff + bb], (unsigned long long)HEMPTY,
+                                      (unsigned long long)key) == HEMPTY) {
+                            a.slots.hval[st.hoff + bb] = idx;
+                            break;
+                        }
                     }
-                    if (q < g.lane) pre += aq;
-                    tot += aq;
+                } else if (idx != NILP) {
+                    c = a.slots.cnt[st.off + idx];
+                } else if (pass == 0) {
+                    atomicOr(&a.pool->oom, 2u);   // index full (cannot happen at load <= 1/2)
                 }
-            }
-            const bool lead = act && leader == (uint32_t)g.lane;
-            // keys new to the list: each leader appends an entry with count 0
-            const bool fresh = pass == 0 && lead && idx == NILP;
-            const uint32_t fb = gballot<AG>(fresh, g);
-            if (fresh) {
-                idx = st.len + (uint32_t)__popc(fb & ((1u << g.lane) - 1));
-                a.slots.nbr[st.off + idx] = key_nbr | DEAD;
-                a.slots.cnt[st.off + idx] = 0;
-                if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
-                uint32_t b = bucket;
-                for (uint32_t p = 0; p < st.hcap; p++, b = (b + 1) & mask) {
-                    if (atomicCAS((unsigned long long *)&a.slots.hkey[st.hoff + b], (unsigned long long)HEMPTY,
-                                  (unsigned long long)key) == HEMPTY) {
-                        a.slots.hval[st.hoff + b] = idx;
-                        break;
-                    }
+                if (idx != NILP) {
+                    const uint32_t tot = sh.cnt[ls];
+                    const uint32_t cf = pass == 0 ? c + tot : (c > tot ? c - tot : 0);
+                    a.slots.cnt[st.off + idx] = cf;
+                    if ((c == 0) != (cf == 0)) a.slots.nbr[st.off + idx] = key_nbr | (cf ? 0u : DEAD);
+                    atomicAdd(&sh.sum, (int32_t)cf - (int32_t)c);
                 }
+                sh.idx[ls] = idx;
+                sh.c0[ls] = c;
             }
-            st.len += (uint32_t)__popc(fb);
-            st.hcnt += (uint32_t)__popc(fb);
-            idx = __shfl(idx, g.base + (int)leader);
-            uint32_t c = 0, cf = 0;
-            if (lead && idx != NILP) {
-                c = a.slots.cnt[st.off + idx];
-                cf = pass == 0 ? c + tot : (c > tot ? c - tot : 0);
-                a.slots.cnt[st.off + idx] = cf;
-                if ((c == 0) != (cf == 0)) a.slots.nbr[st.off + idx] = key_nbr | (cf ? 0u : DEAD);
-            }
-            st.deg = (uint32_t)((int32_t)st.deg + gsum<AG>(lead ? (int32_t)cf - (int32_t)c : 0));
-            c = __shfl(c, g.base + (int)leader);
+            __syncthreads();
             if (act) {
+                const uint32_t idx = sh.idx[ls], c = sh.c0[ls];
                 const uint32_t cb = pass == 0 ? c + pre : (c > pre ? c - pre : 0);
                 const uint32_t ret = delta == -2 ? (cb ? cb - 1 : 0) : cb;
                 if (a.lops.flags[k] & L_RET) a.ops.old[a.lops.edge[k]] = ret;
                 a.lops.slot[k] = idx;
             }
-            __threadfence_block();
+            st.len += sh.n_app;
+            st.hcnt += sh.n_app;
+            st.deg = (uint32_t)((int32_t)st.deg + sh.sum);
+            __syncthreads();
         }
     }
-    return true;
 }
 
-__global__ void k_apply(StoreArgs a)
+__global__ void __launch_bounds__(HB) k_apply_heavy(StoreArgs a)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    Grp g;
-    g.lane = threadIdx.x % AG;
-    g.base = (threadIdx.x % WAVE) - g.lane;
-    const uint32_t gid = t / AG;
-    const uint32_t ng = (gridDim.x * blockDim.x) / AG;
-    const uint32_t nlops = 4 * a.ops.n;
-    for (uint32_t j = gid; j < nlops; j += ng) {
+    __shared__ HeavyShared sh;
+    const uint32_t nh = a.pool->n_heavy;
+    for (uint32_t hi = blockIdx.x; hi < nh; hi += gridDim.x) {
+        const uint32_t j = a.lops.heavy[hi];   // the list's rank-0 op
         const uint32_t v = a.lops.v[j];
-        if (v == NILP) continue;
-        if (a.lops.rank[j] != 0) continue;   // the list's rank-0 op owns it
         const int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
-        const List &li = a.lists[L];
-        const uint32_t n = li.nops[v];
-        const uint32_t *ops = a.lops.segbuf + li.seg[v];
+        ListHdr &hd = a.lists[L].h[v];
+        const uint32_t n = hd.nops;
+        const uint32_t *ops = a.lops.segbuf + hd.seg;
+        ListState st = load_state(hd);
 
-        ListState st;
-        st.off = li.off[v];
-        st.len = li.len[v];
-        st.cap = li.cap[v];
-        st.deg = li.deg[v];
-        st.hoff = li.hoff[v];
-        st.hcap = li.hcap[v];
-        st.hcnt = li.hcnt[v];
-
-        // room for every insert of the segment up front
-        int32_t n_ins = 0;
-        for (uint32_t i = g.lane; i < n; i += AG) n_ins += a.lops.delta[ops[i]] > 0 ? 1 : 0;
-        n_ins = gsum<AG>(n_ins);
+        if (threadIdx.x == 0) sh.sum = 0;
+        __syncthreads();
+        int32_t my = 0;
+        for (uint32_t i = threadIdx.x; i < n; i += HB) my += a.lops.delta[ops[i]] > 0 ? 1 : 0;
+        if (my) atomicAdd(&sh.sum, my);
+        __syncthreads();
+        const uint32_t n_ins = (uint32_t)sh.sum;
+        __syncthreads();
+        const uint32_t need = st.len + n_ins;
         bool ok = true;
-        if (st.len + (uint32_t)n_ins > st.cap) ok = grow_block<AG>(a, st, st.len + (uint32_t)n_ins, g);
+        if (need > st.cap) ok = wg_grow(a, st, need, sh);
         if (ok) {
-            // lists below the index threshold stay unindexed (hash memory is
-            // spent only where the reference indexes too)
-            const bool par = n >= PAR_MIN && (st.hoff != HEMPTY || st.len + (uint32_t)n_ins >= INDEX_THRESHOLD);
-            if (par) ok = apply_parallel(a, st, ops, n, (uint32_t)n_ins, g);
-            else ok = apply_serial(a, st, ops, n, g);
-        }
-        __threadfence_block();
-        // liveness of the edge ops whose (src -> dst) entry is on this list,
-        // from the count after the whole batch
-        if (ok) {
-            for (uint32_t i = g.lane; i < n; i += AG) {
-                const uint32_t k = ops[i];
-                if (!(a.lops.flags[k] & L_SLOT)) continue;
-                const uint32_t idx = a.lops.slot[k];
-                a.ops.live[a.lops.edge[k]] = (idx != NILP && a.slots.cnt[st.off + idx] > 0) ? 1u : 0u;
+            if (st.hoff != HEMPTY || need >= INDEX_THRESHOLD) {
+                // indexed: room for every new key at load <= 1/2
+                uint32_t hc = next_pow2(2 * need);
+                if (hc < 1024) hc = 1024;
+                if (st.hoff == HEMPTY || 2ull * (st.hcnt + n_ins) > st.hcap) ok = wg_build_index(a, st, hc, sh);
+                if (ok) wg_apply_rounds(a, st, ops, n, sh);
+            } else if (threadIdx.x < WAVE) {
+                // a grown short list: the first 16 lanes apply its few ops
+                if (threadIdx.x < AG) {
+                    Grp g;
+                    g.lane = threadIdx.x;
+                    g.base = 0;
+                    apply_serial(a, st, ops, n, g);
+                }
             }
         }
-        if (g.lane == 0) {
-            li.off[v] = st.off;
-            li.len[v] = st.len;
-            li.cap[v] = st.cap;
-            li.deg[v] = st.deg;
-            li.hoff[v] = st.hoff;
-            li.hcap[v] = st.hcap;
-            li.hcnt[v] = st.hcnt;
-            li.nops[v] = 0;
+        // every thread takes the list state of thread 0 (the serial path
+        // advanced only the first group's copy)
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            store_state(hd, st);
+            hd.nops = 0;
         }
+        __syncthreads();
+        if (ok) {
+            st.off = hd.off;
+            write_liveness(a, st, ops, n, threadIdx.x, HB);
+        }
+        __syncthreads();
     }
 }
 
@@ -590,13 +823,16 @@ void launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipE
     const uint32_t n = a.ops.n;
     const uint32_t nl = 4 * n;
     hipLaunchKernelGGL(k_expand_link, dim3((n + 255) / 256), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_segment, dim3((nl + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_segment, dim3((nl + SEG_BLOCK - 1) / SEG_BLOCK), dim3(SEG_BLOCK), 0, s, a);
     hipLaunchKernelGGL(k_scatter, dim3((nl + 255) / 256), dim3(256), 0, s, a);
     uint64_t groups = (uint64_t)nl;
     uint64_t blocks = (groups * AG + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (ev_a) (void)hipEventRecord(ev_a, s);
     hipLaunchKernelGGL(k_apply, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    // heavy lists: at most one per list op; blocks beyond the queue exit at once
+    uint32_t hblocks = nl < 2048 ? nl : 2048;
+    hipLaunchKernelGGL(k_apply_heavy, dim3(hblocks), dim3(HB), 0, s, a);
     if (ev_b) (void)hipEventRecord(ev_b, s);
 }
 
@@ -634,15 +870,8 @@ __global__ void k_reserve(StoreArgs a)
         for (int L = 0; L < nlists; L++) {
             uint32_t add = a.cnt_tmp[L][v];
             if (!add) continue;
-            const List &li = a.lists[L];
-            ListState st;
-            st.off = li.off[v];
-            st.len = li.len[v];
-            st.cap = li.cap[v];
-            st.deg = li.deg[v];
-            st.hoff = li.hoff[v];
-            st.hcap = li.hcap[v];
-            st.hcnt = li.hcnt[v];
+            ListHdr &hd = a.lists[L].h[v];
+            ListState st = load_state(hd);
             uint32_t need = st.len + add;
             bool ok = true;
             if (need > st.cap) ok = grow_block<AG>(a, st, need, g);
@@ -652,11 +881,7 @@ __global__ void k_reserve(StoreArgs a)
                 ok = build_index<AG>(a, st, hc, g);
             }
             if (lane == 0) {
-                li.off[v] = st.off;
-                li.cap[v] = st.cap;
-                li.hoff[v] = st.hoff;
-                li.hcap[v] = st.hcap;
-                li.hcnt[v] = st.hcnt;
+                store_state(hd, st);
                 a.cnt_tmp[L][v] = 0;
             }
         }
@@ -674,6 +899,20 @@ void launch_store_reserve(const StoreArgs &a, hipStream_t s)
     uint64_t blocks = ((uint64_t)a.V * AG + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_reserve, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+}
+
+__global__ void k_list_init(List l, uint32_t V)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    ListHdr h{};
+    h.hoff = HEMPTY;
+    l.h[v] = h;
+}
+
+void launch_list_init(List l, uint32_t V, hipStream_t s)
+{
+    if (V) hipLaunchKernelGGL(k_list_init, dim3((V + 255) / 256), dim3(256), 0, s, l, V);
 }
 
 // pending frees -> free lists, once per batch (one block per pool/class)
@@ -697,7 +936,10 @@ __global__ void k_pool_recycle(StoreArgs a)
         p->free_take[h][c] = 0;
         p->pend_n[h][c] = 0;
         if (mv) atomicAdd(&p->recycled[h], (unsigned long long)mv);
-        if (blockIdx.x == 0) p->seg_top = 0;   // segments of the next batch
+        if (blockIdx.x == 0) {
+            p->seg_pack = 0;   // segments and owners of the next batch
+            p->n_heavy = 0;
+        }
     }
     // publish the batch status (bad ids, pool exhaustion) to pinned host memory
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.hstatus) {
