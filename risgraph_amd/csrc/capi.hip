@@ -821,7 +821,7 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->tune.recompute_abs = 2000000;
         L->tune.recompute_frac = 0.05f;
         L->tune.walk_limit = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(V / 10, 1u << 16));
-        if (const char *tw = getenv("RG_TEAM_W")) L->team_w = (uint32_t)std::max(1, std::min(32, atoi(tw)));
+        if (const char *tw = getenv("RG_TEAM_W")) L->team_w = (uint32_t)std::max(1, std::min(64, atoi(tw)));
         if (const char *bm = getenv("RG_BLOCK_MAX")) L->block_max = (uint32_t)atoi(bm);
         HC(hipMemcpyAsync(L->ctl, L->hc, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
         HC(hipStreamSynchronize(g->st));

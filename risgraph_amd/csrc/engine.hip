@@ -1072,10 +1072,31 @@ __global__ void k_advance(EngineArgs a)
     publish(a, a.ctl);
 }
 
+// Team workgroups that can be resident at once on the current device: the
+// team barrier needs every workgroup of a launch resident, so wider requests
+// are narrowed to fit.
+static uint32_t team_resident_max()
+{
+    static uint32_t cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1;
+    if (!cached[dev]) {
+        hipDeviceProp_t p{};
+        int occ = 0;
+        if (hipGetDeviceProperties(&p, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_team, 1024, 0) != hipSuccess)
+            return 1;
+        cached[dev] = (uint32_t)std::max(1, occ) * (uint32_t)std::max(1, p.multiProcessorCount);
+    }
+    return cached[dev];
+}
+
 void launch_engine_team(const EngineArgs *const *a, int n, uint32_t W, uint32_t work_max, uint32_t init_mask,
                         hipStream_t st)
 {
     if (n <= 0 || n > MAXT || W == 0) return;
+    const uint32_t wmax = std::max<uint32_t>(1, team_resident_max() / (uint32_t)n);
+    if (W > wmax) W = wmax;
     TeamParams P{};
     for (int i = 0; i < MAXT; i++) P.a[i] = a[i < n ? i : 0];
     P.ntrees = (uint32_t)n;

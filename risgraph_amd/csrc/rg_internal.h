@@ -59,7 +59,9 @@ struct alignas(64) ListHdr {
     uint32_t deg;     // live multiplicity (storage.hpp:396)
     uint32_t nops;    // list ops of the current batch (0 between batches)
     uint32_t seg;     // their segment start in ListOps::segbuf
-    uint32_t pad[5];
+    uint32_t ncap;    // block reserved for this batch's growth (0: none) ...
+    uint64_t noff;    // ... and its offset in the slot pool
+    uint32_t pad[2];
 };
 static_assert(sizeof(ListHdr) == 64, "list header is one 64-byte line");
 

@@ -104,6 +104,44 @@ __global__ void k_expand_link(StoreArgs a)
 }
 
 // ---------------------------------------------------------------------
+// slab pool
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int log2_pow2(uint64_t n) { return 63 - __clzll((long long)n); }
+
+// allocate a power-of-two block: reuse a block freed by an earlier batch,
+// else bump the pool pointer
+__device__ uint64_t pool_alloc(const StoreArgs &a, int h, uint64_t n)
+{
+    Pool *p = a.pool;
+    int c = log2_pow2(n);
+    if (c < NCLS) {
+        unsigned int avail = p->free_n[h][c];
+        if (avail) {
+            unsigned int k = atomicAdd(&p->free_take[h][c], 1u);
+            if (k < avail) return a.free_off[((uint64_t)h * NCLS + c) * FCAP + (avail - 1 - k)];
+        }
+    }
+    unsigned long long *top = h ? &p->hash_top : &p->slot_top;
+    unsigned long long cap = h ? p->hash_cap : p->slot_cap;
+    unsigned long long o = atomicAdd(top, (unsigned long long)n);
+    if (o + n > cap) {
+        atomicOr(&p->oom, 1u);
+        return HEMPTY;
+    }
+    return o;
+}
+
+// return a block; it becomes allocatable after this batch
+__device__ void pool_free(const StoreArgs &a, int h, uint64_t off, uint64_t n)
+{
+    if (off == HEMPTY || n == 0) return;
+    int c = log2_pow2(n);
+    if (c >= NCLS) return;
+    unsigned int k = atomicAdd(&a.pool->pend_n[h][c], 1u);
+    if (k < FCAP) a.pend_off[((uint64_t)h * NCLS + c) * FCAP + k] = off;
+}
+
+// ---------------------------------------------------------------------
 // segments: group the batch's list ops by list
 // ---------------------------------------------------------------------
 // The first op of every list reserves the list's segment and an entry in one
@@ -126,6 +164,68 @@ __device__ __forceinline__ bool heavy_list(const ListHdr &h, uint32_t n)
            (indexed && 2ull * (h.hcnt + n) > h.hcap) || (need > h.cap && h.len > LIGHT_COPY);
 }
 
+// Growth blocks of the light lists are reserved here, per block: the lists
+// that may outgrow their block (op count as the bound on inserts) count per
+// size class in LDS, one thread per class pops the class's share of the free
+// list with one atomic, and the rest comes from one bump of the pool pointer
+// for the whole block (a per-list bump is a same-address atomic each).
+__device__ void reserve_growth(const StoreArgs &a, uint32_t cls, bool want, uint64_t &out)
+{
+    __shared__ uint32_t dem[NCLS], take[NCLS], nfree[NCLS], avail[NCLS];
+    __shared__ unsigned long long bump_off[NCLS];
+    __shared__ uint32_t any;
+    if (threadIdx.x < NCLS) dem[threadIdx.x] = 0;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    uint32_t r = 0;
+    if (want) {
+        r = atomicAdd(&dem[cls], 1u);
+        any = 1;
+    }
+    __syncthreads();
+    if (!any) return;
+    Pool *p = a.pool;
+    if (threadIdx.x < NCLS) {
+        const uint32_t c = threadIdx.x, d = dem[c];
+        uint32_t t = 0, f = 0, av = 0;
+        if (d) {
+            av = p->free_n[0][c];
+            if (av) {
+                t = atomicAdd(&p->free_take[0][c], d);
+                f = t < av ? min(d, av - t) : 0;
+            }
+        }
+        take[c] = t;
+        nfree[c] = f;
+        avail[c] = av;
+        bump_off[c] = (unsigned long long)(d - f) << c;   // slots this class needs from the bump pointer
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int c = 0; c < NCLS; c++) {
+            const unsigned long long n = bump_off[c];
+            bump_off[c] = tot;
+            tot += n;
+        }
+        unsigned long long base = tot ? atomicAdd(&p->slot_top, tot) : 0ull;
+        if (tot && base + tot > p->slot_cap) {
+            atomicOr(&p->oom, 1u);
+            base = HEMPTY;
+        }
+        for (int c = 0; c < NCLS; c++) bump_off[c] = base == HEMPTY ? HEMPTY : bump_off[c] + base;
+    }
+    __syncthreads();
+    if (want) {
+        if (r < nfree[cls])
+            out = a.free_off[(uint64_t)cls * FCAP + (avail[cls] - 1 - (take[cls] + r))];
+        else if (bump_off[cls] == HEMPTY)
+            out = HEMPTY;
+        else
+            out = bump_off[cls] + ((uint64_t)(r - nfree[cls]) << cls);
+    }
+}
+
 __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
 {
     __shared__ uint32_t wsum[SEG_BLOCK / WAVE][3];
@@ -133,7 +233,8 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
     __shared__ uint32_t hbase;
     const uint32_t nlops = 4 * a.ops.n;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t cnt = 0, lt = 0, hv = 0, v = NILP;
+    uint32_t cnt = 0, lt = 0, hv = 0, v = NILP, gcls = 0;
+    bool grow = false;
     int L = 0;
     if (j < nlops) {
         v = a.lops.v[j];
@@ -141,9 +242,25 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
             L = (a.lops.flags[j] & L_IN) ? 1 : 0;
             const ListHdr &h = a.lists[L].h[v];
             cnt = h.nops;
-            if (heavy_list(h, cnt)) hv = 1;
-            else lt = 1;
+            if (heavy_list(h, cnt)) {
+                hv = 1;
+            } else {
+                lt = 1;
+                // a single delete never grows its list
+                const bool ins = cnt > 1 || a.lops.delta[j] > 0;
+                if (ins && h.len + cnt > h.cap) {
+                    grow = true;
+                    gcls = (uint32_t)log2_pow2(max(4u, next_pow2(h.len + cnt)));
+                }
+            }
         }
+    }
+    uint64_t noff = HEMPTY;
+    reserve_growth(a, gcls, grow, noff);
+    if (grow) {
+        ListHdr &h = a.lists[L].h[v];
+        h.noff = noff;
+        h.ncap = noff == HEMPTY ? 0u : 1u << gcls;
     }
     const uint32_t lane = __lane_id(), wid = threadIdx.x / WAVE;
     uint32_t ic = cnt, il = lt, ih = hv;
@@ -221,41 +338,6 @@ __device__ __forceinline__ void store_state(ListHdr &h, const ListState &st)
     h.hcap = st.hcap;
     h.hcnt = st.hcnt;
     h.deg = st.deg;
-}
-
-__device__ __forceinline__ int log2_pow2(uint64_t n) { return 63 - __clzll((long long)n); }
-
-// allocate a power-of-two block: reuse a block freed by an earlier batch,
-// else bump the pool pointer
-__device__ uint64_t pool_alloc(const StoreArgs &a, int h, uint64_t n)
-{
-    Pool *p = a.pool;
-    int c = log2_pow2(n);
-    if (c < NCLS) {
-        unsigned int avail = p->free_n[h][c];
-        if (avail) {
-            unsigned int k = atomicAdd(&p->free_take[h][c], 1u);
-            if (k < avail) return a.free_off[((uint64_t)h * NCLS + c) * FCAP + (avail - 1 - k)];
-        }
-    }
-    unsigned long long *top = h ? &p->hash_top : &p->slot_top;
-    unsigned long long cap = h ? p->hash_cap : p->slot_cap;
-    unsigned long long o = atomicAdd(top, (unsigned long long)n);
-    if (o + n > cap) {
-        atomicOr(&p->oom, 1u);
-        return HEMPTY;
-    }
-    return o;
-}
-
-// return a block; it becomes allocatable after this batch
-__device__ void pool_free(const StoreArgs &a, int h, uint64_t off, uint64_t n)
-{
-    if (off == HEMPTY || n == 0) return;
-    int c = log2_pow2(n);
-    if (c >= NCLS) return;
-    unsigned int k = atomicAdd(&a.pool->pend_n[h][c], 1u);
-    if (k < FCAP) a.pend_off[((uint64_t)h * NCLS + c) * FCAP + k] = off;
 }
 
 // A list is owned by a group of G lanes (G = 16 in the apply and reserve
@@ -342,6 +424,9 @@ __device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, co
 // move a list into a block of capacity next_pow2(need) (>= 4); the old block
 // is returned to the slab free lists
 template <int G, int U = UNR>
+__device__ void move_block(const StoreArgs &a, ListState &st, uint64_t o, uint32_t ncap, const Grp &g);
+
+template <int G, int U = UNR>
 __device__ bool grow_block(const StoreArgs &a, ListState &st, uint32_t need, const Grp &g)
 {
     uint32_t ncap = next_pow2(need);
@@ -350,6 +435,14 @@ __device__ bool grow_block(const StoreArgs &a, ListState &st, uint32_t need, con
     if (g.lane == 0) o = pool_alloc(a, 0, ncap);
     o = gbcast<G>(o, g);
     if (o == HEMPTY) return false;
+    move_block<G, U>(a, st, o, ncap, g);
+    return true;
+}
+
+// copy a list into block [o, o + ncap) and free its old block
+template <int G, int U>
+__device__ void move_block(const StoreArgs &a, ListState &st, uint64_t o, uint32_t ncap, const Grp &g)
+{
     for (uint32_t i0 = g.lane; i0 < st.len; i0 += G * U) {
         uint32_t nb[U], cn[U], wt[U];
 #pragma unroll
@@ -375,7 +468,6 @@ __device__ bool grow_block(const StoreArgs &a, ListState &st, uint32_t need, con
     if (g.lane == 0 && st.cap) pool_free(a, 0, st.off, st.cap);
     st.off = o;
     st.cap = ncap;
-    return true;
 }
 
 // find key in the list: local index or NILP.  With `ins_bucket`, returns the
@@ -432,7 +524,7 @@ __device__ uint32_t list_find(const StoreArgs &a, const ListState &st, uint32_t 
     return NILP;
 }
 
-constexpr int AG = 16;            // lanes per list in the light apply / reserve kernels
+constexpr int AG = 4;             // lanes per list in the light apply / reserve kernels
 
 template <int G>
 __device__ __forceinline__ int32_t gsum(int32_t x)
@@ -499,6 +591,75 @@ __device__ void apply_serial(const StoreArgs &a, ListState &st, const uint32_t *
     }
 }
 
+// A light list with exactly one op (the common case): the op's fields come
+// from the caller, the return value and the liveness follow from the one
+// count update.
+// Room for `need` entries in a light list: move into the block k_segment
+// reserved for it (or return that block unused).
+__device__ __forceinline__ bool light_room(const StoreArgs &a, ListState &st, uint32_t need, uint32_t rcap,
+                                           uint64_t roff, const Grp &g)
+{
+    if (need > st.cap) {
+        if (rcap >= need) {
+            move_block<AG, 1>(a, st, roff, rcap, g);
+            return true;
+        }
+        return grow_block<AG, 1>(a, st, need, g);   // no reservation (pool exhausted)
+    }
+    if (rcap && g.lane == 0) pool_free(a, 0, roff, rcap);
+    return true;
+}
+
+__device__ __forceinline__ void apply_single(const StoreArgs &a, ListState &st, uint32_t k, int8_t delta,
+                                             uint8_t flags, uint32_t key_nbr, uint32_t key_w, uint32_t edge,
+                                             uint32_t rcap, uint64_t roff, const Grp &g)
+{
+    if (!light_room(a, st, st.len + (delta > 0 ? 1u : 0u), rcap, roff, g)) return;
+    uint32_t bucket = NILP;
+    uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, &bucket);
+    uint32_t c = 0;
+    if (idx == NILP) {
+        if (delta > 0) {
+            // append a new entry (storage.hpp:371-374, 389-392)
+            idx = st.len;
+            if (g.lane == 0) {
+                a.slots.nbr[st.off + idx] = key_nbr | DEAD;
+                if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
+                if (st.hoff != HEMPTY && bucket != NILP) {
+                    a.slots.hkey[st.hoff + bucket] = hkey_of(key_nbr, key_w);
+                    a.slots.hval[st.hoff + bucket] = idx;
+                }
+            }
+            st.len++;
+            if (st.hoff != HEMPTY) st.hcnt++;
+        }
+    } else {
+        c = a.slots.cnt[st.off + idx];
+    }
+    uint32_t nc, ret;
+    int32_t dchange;
+    switch (delta) {
+    case 1: nc = c + 1; ret = c; dchange = 1; break;
+    case -1: nc = c ? c - 1 : 0; ret = c; dchange = c ? -1 : 0; break;
+    case 2: nc = c + 2; ret = c; dchange = 2; break;
+    default: /* -2: undirected self-loop delete, two -1 steps */
+        ret = c ? c - 1 : 0;
+        nc = c >= 2 ? c - 2 : 0;
+        dchange = -(int32_t)(c >= 2 ? 2 : c);
+        break;
+    }
+    st.deg = (uint32_t)((int32_t)st.deg + dchange);
+    if (g.lane == 0) {
+        if (idx != NILP) {
+            a.slots.cnt[st.off + idx] = nc;
+            if ((c == 0) != (nc == 0)) a.slots.nbr[st.off + idx] = key_nbr | (nc ? 0u : DEAD);
+        }
+        if (flags & L_RET) a.ops.old[edge] = ret;
+        if (flags & L_SLOT) a.ops.live[edge] = (idx != NILP && nc > 0) ? 1u : 0u;
+        a.lops.slot[k] = idx;
+    }
+}
+
 // liveness of the edge ops whose (src -> dst) entry is on this list, from the
 // count after the whole batch
 __device__ __forceinline__ void write_liveness(const StoreArgs &a, const ListState &st, const uint32_t *ops,
@@ -526,20 +687,35 @@ __global__ void __launch_bounds__(256) k_apply(StoreArgs a)
     const uint32_t nown = (uint32_t)(a.pool->seg_pack >> 32);
     for (uint32_t oi = gid; oi < nown; oi += ng) {
         const uint32_t j = a.lops.owner[oi];
+        // the op's own fields are loaded with its vertex, before the header
+        // arrives: most lists have this one op only
         const uint32_t v = a.lops.v[j];
-        const int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
+        const uint8_t fl = a.lops.flags[j];
+        const int8_t dj = a.lops.delta[j];
+        const uint32_t key_nbr = a.lops.nbr[j];
+        const uint32_t key_w = a.slots.wt ? a.lops.w[j] : 0;
+        const uint32_t ej = a.lops.edge[j];
+        const int L = (fl & L_IN) ? 1 : 0;
         ListHdr &hd = a.lists[L].h[v];
         const uint32_t n = hd.nops;
-        // a list with one op: that op is j itself
-        const uint32_t *ops = n == 1 ? &a.lops.owner[oi] : a.lops.segbuf + hd.seg;
+        const uint32_t rcap = hd.ncap;
+        const uint64_t roff = hd.noff;
         ListState st = load_state(hd);
+        if (n == 1) {
+            apply_single(a, st, j, dj, fl, key_nbr, key_w, ej, rcap, roff, g);
+            if (g.lane == 0) {
+                store_state(hd, st);
+                hd.nops = 0;
+                hd.ncap = 0;
+            }
+            continue;
+        }
+        const uint32_t *ops = a.lops.segbuf + hd.seg;
 
         int32_t n_ins = 0;
         for (uint32_t i = g.lane; i < n; i += AG) n_ins += a.lops.delta[ops[i]] > 0 ? 1 : 0;
         const uint32_t need = st.len + (uint32_t)gsum<AG>(n_ins);
-        bool ok = true;
-        if (need > st.cap) ok = grow_block<AG, 1>(a, st, need, g);
-        if (ok) {
+        if (light_room(a, st, need, rcap, roff, g)) {
             apply_serial(a, st, ops, n, g);
             __threadfence_block();
             write_liveness(a, st, ops, n, g.lane, AG);
@@ -547,6 +723,7 @@ __global__ void __launch_bounds__(256) k_apply(StoreArgs a)
         if (g.lane == 0) {
             store_state(hd, st);
             hd.nops = 0;
+            hd.ncap = 0;
         }
     }
 }
