@@ -148,6 +148,7 @@ struct rg_labels {
     Ctl *ctl = nullptr;
     Ctl *hc = nullptr;     // pinned mirror
     unsigned long long *ticks = nullptr;   // per-phase block-mode time (device)
+    uint32_t *bar = nullptr;               // team barrier sub-counters (device)
     hipEvent_t ev_done = nullptr;  // end of the last engine run
     EngineArgs *dargs = nullptr;   // device copy of the engine arguments (team launches)
     EngineArgs *hargs = nullptr;   // pinned staging of the same
@@ -383,6 +384,7 @@ EngineArgs engine_args(rg_labels *L)
     a.ctl = L->ctl;
     a.hctl = L->hc;
     a.ticks = L->ticks;
+    a.bar = L->bar;
     a.ops = g->ops;
     a.ops.n = 0;   // the engine takes the batch size from the control block;
                    // keeping it out lets the device copy of the arguments stay valid
@@ -402,13 +404,14 @@ uint64_t host_work(const Ctl &h, uint64_t V)
 }
 
 // Team width of a run: 8 workgroups per tree for small batches (cheaper
-// barriers), growing with the batch to 32 (4 trees x 32 workgroups of 1024
-// threads stay co-resident on the 256 CUs, which the team barrier needs).
+// barriers), growing with the batch to 64 (4 trees x 64 workgroups of 1024
+// threads, one per CU of a 256-CU device; the launcher narrows a team that
+// would not be co-resident, which the team barrier needs).
 uint32_t team_width(const rg_labels *L, uint32_t n_ops)
 {
     if (L->team_w) return L->team_w;
     uint32_t w = 8;
-    while (w < 32 && (uint64_t)w * 2048 < n_ops) w *= 2;
+    while (w < 64 && (uint64_t)w * 2048 < n_ops) w *= 2;
     return w;
 }
 
@@ -579,6 +582,7 @@ void labels_free(rg_labels *L)
     dfree(L->q[0]); dfree(L->q[1]); dfree(L->q[2]); dfree(L->cq); dfree(L->mlist); dfree(L->ctl);
     dfree(L->init_active);
     dfree(L->ticks);
+    dfree(L->bar);
     dfree(L->dargs);
     hfree(L->hargs);
     hfree(L->hc);
@@ -779,6 +783,8 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->ctl = dalloc<Ctl>(1);
         L->ticks = dalloc<unsigned long long>(NTICK * NPHASE);
         HC(hipMemsetAsync(L->ticks, 0, NTICK * NPHASE * 8, g->st));
+        L->bar = dalloc<uint32_t>(BAR_SUBS * BAR_STRIDE);
+        HC(hipMemsetAsync(L->bar, 0, BAR_SUBS * BAR_STRIDE * 4, g->st));
         L->hc = halloc<Ctl>(1);
         memset(L->hc, 0, sizeof(Ctl));
         L->dargs = dalloc<EngineArgs>(1);

@@ -845,25 +845,39 @@ __device__ void publish(const EngineArgs &a, const Ctl *c)
 // kernels
 // ---------------------------------------------------------------------
 // Team barrier over the W workgroups of one tree (thread 0 of each
-// workgroup arrives).  Arrivals count up monotonically within a run, so
-// episode e is complete when the count reaches e*W; the last arriver runs
-// `last` (the pass's serial step) and then releases the episode.  Waiting is
+// workgroup arrives).  Workgroup wg arrives on sub-counter wg % ns (ns =
+// min(W, BAR_SUBS), each on its own cache line); the last arrival of a
+// sub-group arrives on the control block's counter, so a barrier costs about
+// W/ns + ns serialised same-address atomics instead of W.  Counters grow
+// monotonically within a launch, so episode e of a sub-group of size m is
+// complete when its count reaches e*m; the last arriver overall runs `last`
+// (the pass's serial step) and then releases the episode.  Waiting is
 // bounded in wall-clock time: a timeout sets F_ABORT, which every waiter of
 // the tree sees, so all waves leave the launch and the host reports an error.
 constexpr uint64_t TEAM_SPIN_TICKS = 200000000ull;   // 2 s at the 100 MHz wall clock
 
 template <class F>
-__device__ __forceinline__ void team_sync(Ctl *c, uint32_t W, F &&last)
+__device__ __forceinline__ void team_sync(Ctl *c, uint32_t *bar, uint32_t W, uint32_t wg, F &&last)
 {
     __syncthreads();
     if (threadIdx.x == 0) {
         if (W == 1) {
             last();
         } else {
+            const uint32_t ns = W < BAR_SUBS ? W : BAR_SUBS;
+            const uint32_t sub = wg % ns;
+            const uint32_t m = W / ns + (sub < W % ns ? 1u : 0u);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            const uint32_t old = atomicAdd(&c->bar_count, 1u);
-            const uint32_t ep = old / W + 1;
-            if (old % W == W - 1) {
+            const uint32_t old = atomicAdd(&bar[sub * BAR_STRIDE], 1u);
+            const uint32_t ep = old / m + 1;
+            bool release = false;
+            if (old % m == m - 1) {
+                // the sub-group is complete: pass its writes on to the top count
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+                const uint32_t top = atomicAdd(&c->bar_count, 1u);
+                release = top % ns == ns - 1;
+            }
+            if (release) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 last();
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -917,7 +931,7 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
             else if (i < b0 || i > b2) ast(dst + i, src[i]);
         }
     }
-    if (W > 1 && init) team_sync(c, W, [] {});
+    if (W > 1 && init) team_sync(c, a.bar, W, wg, [] {});
     // every pass either finishes the phase or hands it to grid mode; the cap
     // only bounds the launch if the state machine were ever corrupted
     for (uint32_t iter = 0; iter < (1u << 24); iter++) {
@@ -941,7 +955,7 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
             stage_flush(a, s);
             tk[1] = wall_clock64();
             // the chunk queue is complete once every workgroup finished
-            team_sync(c, W, [&] {
+            team_sync(c, a.bar, W, wg, [&] {
                 if (ald(&c->n_chunk) > chunk_max) ast(&c->stage, 1u);
             });
             tk[2] = wall_clock64();
@@ -958,7 +972,7 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
         pass_chunks<A>(a, s, tid, nthr, sh[0]);
         stage_flush(a, s);
         tk[3] = wall_clock64();
-        team_sync(c, W, [&] {
+        team_sync(c, a.bar, W, wg, [&] {
             if (a.ticks && s.phase < NPHASE) {
                 a.ticks[s.phase] += wall_clock64() - tk[0];
                 a.ticks[NPHASE + s.phase] += 1;
@@ -993,6 +1007,7 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
                 ast(&c->bar_count, 0u);
                 ast(&c->bar_gen, 0u);
                 ast(&c->bar_exit, 0u);
+                for (uint32_t k = 0; k < BAR_SUBS; k++) ast(&a.bar[k * BAR_STRIDE], 0u);
                 publish(a, c);
             }
         }

@@ -176,6 +176,10 @@ struct Ctl {
 // One launch repairs up to MAXT trees; each tree gets W workgroups that
 // synchronise through a barrier in the tree's control block.
 constexpr int MAXT = 4;
+// The team barrier counts arrivals in up to BAR_SUBS sub-counters, one
+// 128-byte line apart, whose last arrivals count on the control block.
+constexpr uint32_t BAR_SUBS = 8;
+constexpr uint32_t BAR_STRIDE = 32;
 
 constexpr uint32_t RESCUE_NEVER = 0xFFFFFFFFu;   // rescue_level: no level qualifies
 
@@ -210,6 +214,7 @@ struct EngineArgs {
     Ctl *ctl;
     Ctl *hctl;                    // pinned host mirror, written at the end of each launch
     unsigned long long *ticks;    // [NTICK*NPHASE] per-phase pass statistics (team mode)
+    uint32_t *bar;                // [BAR_SUBS*BAR_STRIDE] team barrier sub-counters (zero between launches)
     EdgeOps ops;
     Tuning tune;
 };
