@@ -150,6 +150,7 @@ struct rg_labels {
     unsigned long long *ticks = nullptr;   // per-phase block-mode time (device)
     uint32_t *bar = nullptr;               // team barrier sub-counters (device)
     hipEvent_t ev_done = nullptr;  // end of the last engine run
+    hipStream_t st = nullptr;      // this tree's grid-wide passes (concurrent with other trees)
     EngineArgs *dargs = nullptr;   // device copy of the engine arguments (team launches)
     EngineArgs *hargs = nullptr;   // pinned staging of the same
     Tuning tune{};
@@ -177,11 +178,15 @@ void alloc_list(List &l, uint64_t V, hipStream_t st)
 
 void free_list(List &l) { dfree(l.h); }
 
-// one u32 field of every list header -> host array (strided copy)
-void copy_hdr_field(const List &l, size_t field_off, uint32_t *out, uint64_t V, hipStream_t st)
+// live degrees of every list -> host array (gathered on the device, one copy)
+void copy_degrees(const List &l, uint32_t *out, uint64_t V, hipStream_t st)
 {
-    HC(hipMemcpy2DAsync(out, 4, reinterpret_cast<const char *>(l.h) + field_off, sizeof(ListHdr), 4, V,
-                        hipMemcpyDeviceToHost, st));
+    uint32_t *tmp = dalloc<uint32_t>(V);
+    launch_list_degrees(l, (uint32_t)V, tmp, st);
+    HC(hipGetLastError());
+    HC(hipMemcpyAsync(out, tmp, V * 4, hipMemcpyDeviceToHost, st));
+    HC(hipStreamSynchronize(st));
+    dfree(tmp);
 }
 
 void free_ops(rg_graph *g)
@@ -455,10 +460,13 @@ void engine_init(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool 
 
 constexpr int GRID_SPEC = 1;   // grid steps enqueued per host round trip
 
-// Run the engines of labels L[0..n) from `phase0` to completion on the graph
-// stream, after the store batch already queued there.  Trees whose passes fit
-// the team cap share one persistent launch (W workgroups per tree, so the
-// trees repair concurrently on different CUs); larger passes go grid-wide.
+// Run the engines of labels L[0..n) from `phase0` to completion, after the
+// store batch already queued on the graph stream.  Trees whose passes fit the
+// team cap share one persistent launch on the graph stream (W workgroups per
+// tree, so the trees repair concurrently on different CUs); larger passes go
+// grid-wide on the tree's own stream, concurrently with the team launch and
+// with other trees' grid-wide passes.  Every round ends with all streams
+// drained, so the host's view of each control block is current.
 void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t n_ops, bool new_idx,
                       rg_batch_stats *stats)
 {
@@ -482,6 +490,11 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
             HC(hipMemcpyAsync(L[i]->dargs, L[i]->hargs, sizeof(EngineArgs), hipMemcpyHostToDevice, st));
         }
     }
+    // grid-wide passes of different trees run concurrently, each on its
+    // tree's stream, after everything queued on the graph stream so far
+    HC(hipEventRecord(g->ev[2], st));
+    for (int i = 0; i < n; i++) HC(hipStreamWaitEvent(L[i]->st, g->ev[2], 0));
+    std::vector<char> on_own(n, 0);
     int left = n;
     uint64_t guard_iter = 0;
     std::vector<int> team;
@@ -495,19 +508,22 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
     while (left) {
         team.clear();
         for (int i = 0; i < n; i++) {
+            on_own[i] = 0;
             if (!active[i]) continue;
             const Ctl &h = *L[i]->hc;
             if (team_ok(L[i], h)) {
                 team.push_back(i);
             } else {
-                // a few grid steps back to back without host round trips;
-                // each turns into a no-op once the tree's passes fit the
-                // team again (or it is done)
+                // a grid-wide step on the tree's own stream; it turns into a
+                // no-op once the tree's passes fit the team again (or it is
+                // done)
+                hipStream_t ts = L[i]->st;
                 hipEvent_t ea, eb;
-                prof_begin(g, 2, host_work(h, g->V), st, &ea, &eb, (int)h.phase);
-                if (ea) HC(hipEventRecord(ea, st));
-                launch_engine_grid_steps(args[i], h, GRID_SPEC, st);
-                if (eb) HC(hipEventRecord(eb, st));
+                prof_begin(g, 2, host_work(h, g->V), ts, &ea, &eb, (int)h.phase);
+                if (ea) HC(hipEventRecord(ea, ts));
+                launch_engine_grid_steps(args[i], h, GRID_SPEC, ts);
+                if (eb) HC(hipEventRecord(eb, ts));
+                on_own[i] = 1;
                 if (stats) stats->grid_steps++;
             }
         }
@@ -535,6 +551,8 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
         }
         HC(hipGetLastError());
         HC(hipStreamSynchronize(st));
+        for (int i = 0; i < n; i++)
+            if (on_own[i]) HC(hipStreamSynchronize(L[i]->st));
         for (int i = 0; i < n; i++) {
             if (!active[i]) continue;
             Ctl &h = *L[i]->hc;   // published by the last kernel
@@ -554,7 +572,8 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 h.sel_cur = 0; h.sel_next = 1; h.sel_tree = 2;
                 h.epoch += 1;
                 h.flags = (h.flags & ~F_NEED_REBUILD) | F_ALL_MODIFIED;
-                HC(hipMemcpyAsync(L[i]->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, st));
+                // the rebuild runs grid-wide on the tree's stream
+                HC(hipMemcpyAsync(L[i]->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, L[i]->st));
                 if (stats) stats->rebuilds++;
             }
         }
@@ -588,6 +607,8 @@ void labels_free(rg_labels *L)
     hfree(L->hc);
     if (L->ev_done) (void)hipEventDestroy(L->ev_done);
     L->ev_done = nullptr;
+    if (L->st) (void)hipStreamDestroy(L->st);
+    L->st = nullptr;
 }
 
 }  // namespace
@@ -792,6 +813,7 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         memset(L->hargs, 0, sizeof(EngineArgs));
         L->hc->mepoch = 1;
         HC(hipEventCreate(&L->ev_done));
+        HC(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
         HC(hipMemsetAsync(L->inv, 0, V * 4, g->st));
         HC(hipMemsetAsync(L->stamp, 0, V * 4, g->st));
         HC(hipMemsetAsync(L->mstamp, 0, V * 4, g->st));
@@ -1248,11 +1270,8 @@ int rg_degrees(rg_graph *g, uint32_t *out_deg, uint32_t *in_deg)
     if (!g) return RG_EINVAL;
     return guard([&] {
         std::lock_guard<std::recursive_mutex> lk(g->mu);
-        if (out_deg) copy_hdr_field(g->lists[0], offsetof(ListHdr, deg), out_deg, g->V, g->st);
-        if (in_deg) {
-            const List &l = g->symmetric ? g->lists[0] : g->lists[1];
-            copy_hdr_field(l, offsetof(ListHdr, deg), in_deg, g->V, g->st);
-        }
+        if (out_deg) copy_degrees(g->lists[0], out_deg, g->V, g->st);
+        if (in_deg) copy_degrees(g->symmetric ? g->lists[0] : g->lists[1], in_deg, g->V, g->st);
         HC(hipStreamSynchronize(g->st));
     });
 }

@@ -244,6 +244,8 @@ void launch_store_reserve(const StoreArgs &a, hipStream_t s);
 void launch_pool_recycle(const StoreArgs &a, hipStream_t s);
 // empty lists (no block, no index) for vertices [0, V)
 void launch_list_init(List l, uint32_t V, hipStream_t s);
+// out[v] = live degree of v's list (device array of V words)
+void launch_list_degrees(List l, uint32_t V, uint32_t *out, hipStream_t s);
 // Persistent team launch over trees *a[0..n) (n <= MAXT; device copies of
 // their arguments), W workgroups each.
 // A tree leaves the launch when its next pass exceeds work_max (grid mode

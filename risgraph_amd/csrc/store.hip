@@ -1092,6 +1092,17 @@ void launch_list_init(List l, uint32_t V, hipStream_t s)
     if (V) hipLaunchKernelGGL(k_list_init, dim3((V + 255) / 256), dim3(256), 0, s, l, V);
 }
 
+__global__ void k_list_degrees(List l, uint32_t V, uint32_t *out)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) out[v] = l.h[v].deg;
+}
+
+void launch_list_degrees(List l, uint32_t V, uint32_t *out, hipStream_t s)
+{
+    if (V) hipLaunchKernelGGL(k_list_degrees, dim3((V + 255) / 256), dim3(256), 0, s, l, V, out);
+}
+
 // pending frees -> free lists, once per batch (one block per pool/class)
 __global__ void k_pool_recycle(StoreArgs a)
 {
