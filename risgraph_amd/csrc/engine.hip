@@ -407,7 +407,10 @@ template <int A>
 __device__ __forceinline__ void pull_apply(const EngineArgs &a, const Snap &s, uint32_t x, uint64_t best)
 {
     if (A == SSWP ? best == 0ull : best == ~0ull) return;
-    if (relax<A>(a.val, x, vval(best), vpar(best))) mark_mod(a, s, x);
+    if (relax<A>(a.val, x, vval(best), vpar(best))) {
+        push_next(a, s, x);
+        mark_mod(a, s, x);
+    }
 }
 
 // any lane of the calling lane's 16-lane group
@@ -654,7 +657,10 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
                 const bool take = (A == SSWP) ? (best != 0ull && vval(best) > iv)
                                               : (best != ~0ull && vval(best) < iv);
                 a.val[v] = take ? best : pack(iv, NILP);
-                push_next(a, s, v);
+                // a vertex left at its init label improves nobody (except
+                // under WCC, where every init label is a candidate): it is
+                // pushed once a pull chunk or a push gives it a value
+                if (A == WCC || take) push_next(a, s, v);
                 mark_mod(a, s, v);
             }
         }
@@ -1135,10 +1141,15 @@ static void grid_launch(const EngineArgs &a, const Ctl &h, int steps, hipStream_
     }
     uint64_t blocks = (threads + GRID_BLOCK - 1) / GRID_BLOCK;
     if (blocks < 1) blocks = 1;
-    if (blocks > GRID_MAX_BLOCKS) blocks = GRID_MAX_BLOCKS;
+    static const uint32_t max_blocks = [] {
+        const char *e = getenv("RG_GRID_BLOCKS");
+        const long v = e ? atol(e) : 0;
+        return v > 0 && v <= 4096 ? (uint32_t)v : GRID_MAX_BLOCKS;
+    }();
+    if (blocks > max_blocks) blocks = max_blocks;
     for (int k = 0; k < steps; k++) {
         hipLaunchKernelGGL(k_grid_main<A>, dim3((uint32_t)blocks), dim3(GRID_BLOCK), 0, st, a);
-        hipLaunchKernelGGL(k_grid_chunks<A>, dim3(GRID_MAX_BLOCKS / 2), dim3(GRID_BLOCK), 0, st, a);
+        hipLaunchKernelGGL(k_grid_chunks<A>, dim3(std::max<uint32_t>(1, max_blocks / 2)), dim3(GRID_BLOCK), 0, st, a);
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, a);
     }
 }
