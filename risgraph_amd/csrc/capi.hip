@@ -137,10 +137,9 @@ struct rg_labels {
     int algo = 0;
     uint32_t root = 0;
     int trace = 0;
-    uint64_t *val = nullptr;
+    VRec *vr = nullptr;            // label word, mark, frontier stamp per vertex
     uint32_t *init = nullptr;
-    int32_t *inv = nullptr;
-    uint32_t *stamp = nullptr, *mstamp = nullptr;
+    uint32_t *mstamp = nullptr;
     uint32_t *q[3]{};
     uint64_t *cq = nullptr;
     uint32_t cq_cap = 0;
@@ -375,12 +374,11 @@ EngineArgs engine_args(rg_labels *L)
         a.npush = 1; a.push_l[0] = 0;
         a.npull = 1; a.pull_l[0] = 1;
     }
-    a.val = L->val;
+    a.vr = L->vr;
     a.init = L->init;
     a.init_active = L->init_active;
     a.root = L->root;
-    a.inv = L->inv;
-    a.stamp = L->stamp;
+
     a.mstamp = L->mstamp;
     a.q[0] = L->q[0]; a.q[1] = L->q[1]; a.q[2] = L->q[2];
     a.cq = L->cq;
@@ -448,7 +446,7 @@ void engine_init(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool 
         h.idx += 1;
         if (h.idx >= 0x7FFFFFF0) {
             h.idx = 1;
-            HC(hipMemsetAsync(L->inv, 0, L->g->V * 4, L->g->st));
+            launch_clear_marks(L->vr, (uint32_t)L->g->V, L->g->st);
         }
     }
     h.flags &= F_ALL_MODIFIED;
@@ -597,7 +595,7 @@ void engine_run(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool n
 
 void labels_free(rg_labels *L)
 {
-    dfree(L->val); dfree(L->init); dfree(L->inv); dfree(L->stamp); dfree(L->mstamp);
+    dfree(L->vr); dfree(L->init); dfree(L->mstamp);
     dfree(L->q[0]); dfree(L->q[1]); dfree(L->q[2]); dfree(L->cq); dfree(L->mlist); dfree(L->ctl);
     dfree(L->init_active);
     dfree(L->ticks);
@@ -791,10 +789,8 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->algo = algo;
         L->root = (uint32_t)root;
         L->trace = trace_modified;
-        L->val = dalloc<uint64_t>(V);
+        L->vr = dalloc<VRec>(V);
         L->init = dalloc<uint32_t>(V);
-        L->inv = dalloc<int32_t>(V);
-        L->stamp = dalloc<uint32_t>(V);
         L->mstamp = dalloc<uint32_t>(V);
         for (auto &q : L->q) q = dalloc<uint32_t>(V);
         L->cq_cap = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull,
@@ -814,8 +810,7 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->hc->mepoch = 1;
         HC(hipEventCreate(&L->ev_done));
         HC(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
-        HC(hipMemsetAsync(L->inv, 0, V * 4, g->st));
-        HC(hipMemsetAsync(L->stamp, 0, V * 4, g->st));
+        HC(hipMemsetAsync(L->vr, 0, V * sizeof(VRec), g->st));
         HC(hipMemsetAsync(L->mstamp, 0, V * 4, g->st));
         // labels start as (init, empty parent) like alloc_vertex_tree_array
         if (algo == RG_WCC) {
@@ -961,8 +956,12 @@ int rg_labels_read(rg_labels *L, uint64_t first, uint64_t count, uint64_t *value
         if (first + count > g->V || first + count < first) throw RgError{RG_EINVAL_VERTEX, "range out of bounds"};
         if (!count) return;
         std::vector<uint64_t> tmp(count);
-        HC(hipMemcpyAsync(tmp.data(), L->val + first, count * 8, hipMemcpyDeviceToHost, g->st));
+        uint64_t *dv = dalloc<uint64_t>(count);
+        launch_read_labels(L->vr + first, (uint32_t)count, dv, g->st);
+        HC(hipGetLastError());
+        HC(hipMemcpyAsync(tmp.data(), dv, count * 8, hipMemcpyDeviceToHost, g->st));
         HC(hipStreamSynchronize(g->st));
+        dfree(dv);
         for (uint64_t i = 0; i < count; i++) {
             uint32_t p = (uint32_t)tmp[i];
             if (values) values[i] = tmp[i] >> 32;

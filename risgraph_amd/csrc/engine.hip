@@ -121,9 +121,9 @@ __device__ __forceinline__ void take_snap(Ctl *c, Snap &s)
 
 // strict-improvement CAS on the packed label word (update_label, CAS form)
 template <int A>
-__device__ __forceinline__ bool relax(uint64_t *val, uint32_t dst, uint64_t c, uint32_t parent)
+__device__ __forceinline__ bool relax(uint64_t *word, uint64_t c, uint32_t parent)
 {
-    unsigned long long *p = (unsigned long long *)&val[dst];
+    unsigned long long *p = (unsigned long long *)word;
     unsigned long long cur = *p;
     while (better<A>(c, vval(cur))) {
         unsigned long long nw = pack((uint32_t)c, parent);
@@ -218,7 +218,7 @@ __device__ void stage_flush(const EngineArgs &a, const Snap &s)
 
 __device__ __forceinline__ void push_next(const EngineArgs &a, const Snap &s, uint32_t v)
 {
-    if (atomicExch(&a.stamp[v], s.epoch) != s.epoch) stage_next(a, s, v);
+    if (atomicExch(&a.vr[v].stamp, s.epoch) != s.epoch) stage_next(a, s, v);
 }
 
 __device__ __forceinline__ void mark_mod(const EngineArgs &a, const Snap &s, uint32_t v)
@@ -264,7 +264,7 @@ __device__ __forceinline__ bool marked(int32_t iv, int32_t idx) { return iv == i
 // its new parent outside the subtree.
 __device__ __forceinline__ bool mark_inv(const EngineArgs &a, const Snap &s, uint32_t x)
 {
-    return atomicMax(&a.inv[x], s.idx) != s.idx;
+    return atomicMax(&a.vr[x].inv, s.idx) != s.idx;
 }
 
 // has_parents (graph.hpp:1889-1907): true when an ancestor of x (x
@@ -285,8 +285,8 @@ __device__ bool has_parents(const EngineArgs &a, const Snap &s, uint32_t x, int3
 {
     uint32_t steps = 0;
     for (;;) {
-        const uint64_t w = a.val[x];
-        const int32_t iv = a.inv[x];
+        const uint64_t w = a.vr[x].val;
+        const int32_t iv = a.vr[x].inv;
         if constexpr (A == SSWP) {
             if (vval(w) > s.rbound) return false;
         } else if constexpr (A != WCC) {
@@ -324,8 +324,8 @@ __device__ void push_range(const EngineArgs &a, const Snap &s, uint32_t u, uint3
         uint32_t w;
         entry_w<A>(a, off + i, w);
         uint64_t c = cand<A>(uv, w);
-        if (!better<A>(c, vval(a.val[n]))) continue;
-        if (relax<A>(a.val, n, c, u)) {
+        if (!better<A>(c, vval(a.vr[n].val))) continue;
+        if (relax<A>(&a.vr[n].val, c, u)) {
             push_next(a, s, n);
             mark_mod(a, s, n);
         }
@@ -342,16 +342,16 @@ __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uin
                              uint32_t b, uint32_t e, int lane, int stride)
 {
     for (uint32_t i = b + lane; i < e; i += stride) {
-        if (a.inv[d] != s.idx) break;
+        if (a.vr[d].inv != s.idx) break;
         uint32_t n = a.slots.nbr[off + i];
         if (n & DEAD) continue;
         uint32_t w;
         entry_w<A>(a, off + i, w);
-        if (cand<A>(vval(a.val[n]), w) != (uint64_t)dv) continue;
+        if (cand<A>(vval(a.vr[n].val), w) != (uint64_t)dv) continue;
         if (has_parents<A>(a, s, n, s.idx)) continue;
-        if (atomicCAS(&a.inv[d], s.idx, -s.idx) == s.idx) {
-            a.val[d] = pack(dv, n);
-            a.inv[d] = 0;
+        if (atomicCAS(&a.vr[d].inv, s.idx, -s.idx) == s.idx) {
+            a.vr[d].val = pack(dv, n);
+            a.vr[d].inv = 0;
             return true;
         }
     }
@@ -366,7 +366,7 @@ __device__ void expand_range(const EngineArgs &a, const Snap &s, uint32_t x, uin
     for (uint32_t i = b + lane; i < e; i += stride) {
         uint32_t c = a.slots.nbr[off + i];
         if (c & DEAD) continue;
-        if (vpar(a.val[c]) != x) continue;
+        if (vpar(a.vr[c].val) != x) continue;
         if (mark_inv(a, s, c)) stage_next(a, s, c);
     }
 }
@@ -382,10 +382,10 @@ __device__ uint64_t pull_range(const EngineArgs &a, const Snap &s, uint64_t off,
     for (uint32_t i = b + lane; i < e; i += stride) {
         uint32_t n = a.slots.nbr[off + i];
         if (n & DEAD) continue;
-        if (a.inv[n] == s.idx) continue;
+        if (a.vr[n].inv == s.idx) continue;
         uint32_t w;
         entry_w<A>(a, off + i, w);
-        uint64_t c = cand<A>(vval(a.val[n]), w);
+        uint64_t c = cand<A>(vval(a.vr[n].val), w);
         if (A != SSWP && c > 0xFFFFFFFFull) continue;
         uint64_t word = pack((uint32_t)c, n);
         if (A == SSWP ? word > best : word < best) best = word;
@@ -407,7 +407,7 @@ template <int A>
 __device__ __forceinline__ void pull_apply(const EngineArgs &a, const Snap &s, uint32_t x, uint64_t best)
 {
     if (A == SSWP ? best == 0ull : best == ~0ull) return;
-    if (relax<A>(a.val, x, vval(best), vpar(best))) {
+    if (relax<A>(&a.vr[x].val, vval(best), vpar(best))) {
         push_next(a, s, x);
         mark_mod(a, s, x);
     }
@@ -427,9 +427,9 @@ __device__ __forceinline__ bool group_any(bool p)
 template <int A>
 __device__ void seed_edge(const EngineArgs &a, const Snap &s, uint32_t src, uint32_t dst, uint32_t w)
 {
-    uint64_t c = cand<A>(vval(a.val[src]), w);
-    if (!better<A>(c, vval(a.val[dst]))) return;
-    if (relax<A>(a.val, dst, c, src)) {
+    uint64_t c = cand<A>(vval(a.vr[src].val), w);
+    if (!better<A>(c, vval(a.vr[dst].val))) return;
+    if (relax<A>(&a.vr[dst].val, c, src)) {
         push_next(a, s, dst);
         mark_mod(a, s, dst);
     }
@@ -439,7 +439,7 @@ __device__ void seed_edge(const EngineArgs &a, const Snap &s, uint32_t src, uint
 template <int A>
 __device__ __forceinline__ void mark_root(const EngineArgs &a, const Snap &s, uint32_t src, uint32_t dst)
 {
-    const uint64_t w = a.val[dst];
+    const uint64_t w = a.vr[dst].val;
     if (vpar(w) != src) return;
     if (mark_inv(a, s, dst)) {
         if constexpr (A == SSWP) atomicMax(&s.c->rbound, vval(w));
@@ -469,7 +469,7 @@ __device__ void pass_ops(const EngineArgs &a, const Snap &s, uint32_t tid, uint3
             // (bfs_inc.cpp:162-165: add_edge returned 0 and the batch left it)
             const bool seed = (k & K_FORCE) || (a.ops.old[i] == 0 && a.ops.live[i]);
             if (!seed) continue;
-            if (pull && (a.inv[src] == s.idx || a.inv[dst] == s.idx)) continue;
+            if (pull && (a.vr[src].inv == s.idx || a.vr[dst].inv == s.idx)) continue;
             seed_edge<A>(a, s, src, dst, w);
             if (both) seed_edge<A>(a, s, dst, src, w);
         } else {
@@ -526,7 +526,7 @@ __device__ void expand_cont(const EngineArgs &a, const Snap &s, uint32_t x, uint
         uint32_t c = 0;
         if (i < len) {
             c = a.slots.nbr[off + i];
-            if (!(c & DEAD) && vpar(a.val[c]) == x) mk = mark_inv(a, s, c);
+            if (!(c & DEAD) && vpar(a.vr[c].val) == x) mk = mark_inv(a, s, c);
         }
         const bool keep = mk && depth < CONT_DEPTH;
         const uint32_t kb = group_ballot(keep);
@@ -548,9 +548,9 @@ __device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uin
                            GStack &st)
 {
     const uint32_t v = raw & ~Q_EXPAND_ONLY;
-    if (a.inv[v] != s.idx) return;   // rescued
+    if (a.vr[v].inv != s.idx) return;   // rescued
     if (s.phase == PH_DEL_RESCUE && !(raw & Q_EXPAND_ONLY)) {
-        const uint32_t dv = vval(a.val[v]);
+        const uint32_t dv = vval(a.vr[v].val);
         bool won = false, deferred = false;
         for (int li = 0; li < a.npull; li++) {
             int L = a.pull_l[li];
@@ -595,8 +595,8 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
     const uint32_t ph = s.phase;
     if (ph == PH_BUILD_INIT) {
         for (uint32_t v = tid; v < a.V; v += nthr) {
-            a.val[v] = pack(a.init[v], NILP);
-            a.inv[v] = 0;
+            a.vr[v].val = pack(a.init[v], NILP);
+            a.vr[v].inv = 0;
             bool act = a.init_active ? a.init_active[v] != 0 : v == a.root;
             if (act) push_next(a, s, v);
         }
@@ -610,7 +610,7 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
         const uint32_t raw = qv[gi];
         const uint32_t v = raw & ~Q_EXPAND_ONLY;
         if (ph == PH_PUSH) {
-            uint32_t uv = vval(a.val[v]);
+            uint32_t uv = vval(a.vr[v].val);
             for (int li = 0; li < a.npush; li++) {
                 int L = a.push_l[li];
                 uint64_t off;
@@ -656,7 +656,7 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
                 const uint32_t iv = a.init[v];
                 const bool take = (A == SSWP) ? (best != 0ull && vval(best) > iv)
                                               : (best != ~0ull && vval(best) < iv);
-                a.val[v] = take ? best : pack(iv, NILP);
+                a.vr[v].val = take ? best : pack(iv, NILP);
                 // a vertex left at its init label improves nobody (except
                 // under WCC, where every init label is a candidate): it is
                 // pushed once a pull chunk or a push gives it a value
@@ -699,10 +699,10 @@ __device__ void pass_chunks(const EngineArgs &a, const Snap &s, uint32_t tid, ui
         list_span(a.lists[L], v, off, len);
         uint32_t b = k * CHUNK, e = b + CHUNK < len ? b + CHUNK : len;
         if (op == CO_PUSH) {
-            push_range<A>(a, s, v, vval(a.val[v]), off, b, e, lane, WAVE);
+            push_range<A>(a, s, v, vval(a.vr[v].val), off, b, e, lane, WAVE);
         } else if (op == CO_RESCUE) {
-            if (a.inv[v] != s.idx) continue;
-            rescue_range<A>(a, s, v, vval(a.val[v]), off, b, e, lane, WAVE);
+            if (a.vr[v].inv != s.idx) continue;
+            rescue_range<A>(a, s, v, vval(a.vr[v].val), off, b, e, lane, WAVE);
         } else if (op == CO_EXPAND) {
             expand_range(a, s, v, off, b, e, lane, WAVE);
         } else {
@@ -1179,11 +1179,11 @@ __global__ void k_classify(EngineArgs a, uint32_t n, uint8_t *unsafe)
     bool both = a.undirected || (k & K_UNDIR);
     bool u;
     if (k & K_INS) {
-        u = better<A>(cand<A>(vval(a.val[s]), w), vval(a.val[d]));
-        if (both) u = u || better<A>(cand<A>(vval(a.val[d]), w), vval(a.val[s]));
+        u = better<A>(cand<A>(vval(a.vr[s].val), w), vval(a.vr[d].val));
+        if (both) u = u || better<A>(cand<A>(vval(a.vr[d].val), w), vval(a.vr[s].val));
     } else {
-        u = vpar(a.val[d]) == s;
-        if (both) u = u || vpar(a.val[s]) == d;
+        u = vpar(a.vr[d].val) == s;
+        if (both) u = u || vpar(a.vr[s].val) == d;
     }
     unsafe[i] = u;
 }
@@ -1213,6 +1213,26 @@ __global__ void k_fill_init(uint32_t *init, uint32_t V, int algo, uint32_t root)
         else x = v;
         init[v] = x;
     }
+}
+
+__global__ void k_clear_marks(VRec *vr, uint32_t V)
+{
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) vr[v].inv = 0;
+}
+
+void launch_clear_marks(VRec *vr, uint32_t V, hipStream_t st)
+{
+    if (V) hipLaunchKernelGGL(k_clear_marks, dim3(std::min<uint32_t>((V + 255) / 256, 16384)), dim3(256), 0, st, vr, V);
+}
+
+__global__ void k_read_labels(const VRec *vr, uint32_t n, uint64_t *out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = vr[i].val;
+}
+
+void launch_read_labels(const VRec *vr, uint32_t n, uint64_t *out, hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_read_labels, dim3(std::min<uint32_t>((n + 255) / 256, 16384)), dim3(256), 0, st, vr, n, out);
 }
 
 void launch_engine_fill_init(const EngineArgs &a, hipStream_t st)

@@ -14,11 +14,13 @@
 //               current batch's ops on the list and their segment in the
 //               batch's op-index buffer).
 // One labels instance (one algorithm's dependency tree):
-//   val[V]      u64 packed word value << 32 | parent (parent 0xFFFFFFFF =
-//               empty), the device form of VertexTree (graph.hpp:54-59),
-//               updated by 64-bit CAS with a strict-improvement check.
-//   init[V]     init labels; inv[V] invalidation marks (graph.hpp:1729);
-//               stamp[V] / mstamp[V] frontier and modified dedupe epochs;
+//   vr[V]       16-byte records {val, inv, stamp}: val is the u64 packed
+//               word value << 32 | parent (parent 0xFFFFFFFF = empty), the
+//               device form of VertexTree (graph.hpp:54-59), updated by
+//               64-bit CAS with a strict-improvement check; inv the
+//               invalidation mark (graph.hpp:1729); stamp the frontier
+//               dedupe epoch.
+//   init[V]     init labels; mstamp[V] modified-set dedupe epochs;
 //               q[3][V] vertex queues (cur / next / tree); cq chunk queue of
 //               long lists; mlist[V] modified set; ctl control block.
 #pragma once
@@ -183,6 +185,16 @@ constexpr uint32_t BAR_STRIDE = 32;
 
 constexpr uint32_t RESCUE_NEVER = 0xFFFFFFFFu;   // rescue_level: no level qualifies
 
+// One 16-byte record per vertex of a tree: a pass that visits a vertex reads
+// or updates its label word, its invalidation mark and its frontier stamp
+// together, one memory line instead of three.
+struct alignas(16) VRec {
+    uint64_t val;     // value << 32 | parent vertex (parent 0xFFFFFFFF: empty)
+    int32_t inv;      // invalidation mark: run index, -index while rescued, else stale
+    uint32_t stamp;   // frontier dedupe epoch
+};
+static_assert(sizeof(VRec) == 16, "vertex record is 16 bytes");
+
 struct Tuning {
     uint32_t rescue_level;        // rescue when level <= this (RESCUE_NEVER: none) ...
     uint32_t rescue_frontier;     // ... or frontier < this (graph.hpp:1324)
@@ -201,12 +213,11 @@ struct EngineArgs {
     List lists[2];
     int npush, push_l[2];         // lists scanned by push / expand
     int npull, pull_l[2];         // lists scanned by pull / rescue
-    uint64_t *val;
+    VRec *vr;                     // label word, mark and frontier stamp per vertex
     uint32_t *init;
     uint32_t *init_active;        // nullptr: only vertex `root` starts active
     uint32_t root;
-    int32_t *inv;
-    uint32_t *stamp, *mstamp;
+    uint32_t *mstamp;
     uint32_t *q[3];
     uint64_t *cq;
     uint32_t cq_cap;
@@ -258,6 +269,10 @@ void launch_engine_team(const EngineArgs *const *a, int n, uint32_t W, uint32_t 
 // (checked on the device control block)
 void launch_engine_grid_steps(const EngineArgs &a, const Ctl &host_ctl, int steps, hipStream_t s);
 void launch_engine_fill_init(const EngineArgs &a, hipStream_t s);
+// inv = 0 for every vertex record (the mark index wrapped)
+void launch_clear_marks(VRec *vr, uint32_t V, hipStream_t s);
+// out[i] = vr[i].val for i < n
+void launch_read_labels(const VRec *vr, uint32_t n, uint64_t *out, hipStream_t s);
 void launch_gen_rmat(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, int scale,
                      uint64_t seed, double a, double b, double c, hipStream_t s);
 void launch_gen_er(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, uint64_t V,
