@@ -91,7 +91,7 @@ struct Stage {
 // snapshot of the control block taken at a pass boundary
 struct Snap {
     uint32_t phase, stage, level, n_cur, n_tree, n_chunk, n_ops;
-    uint32_t sel_cur, sel_next, sel_tree, epoch, mepoch, seed, rbound, flags, work_max;
+    uint32_t sel_cur, sel_next, sel_tree, epoch, mepoch, seed, rbound, flags, work_max, seq;
     int32_t idx;
     Ctl *c;       // live control block: LDS copy for a one-workgroup team, else HBM
     Stage *stg;   // this workgroup's append buffers (LDS)
@@ -117,6 +117,7 @@ __device__ __forceinline__ void take_snap(Ctl *c, Snap &s)
     s.flags = ald(&c->flags);
     s.work_max = ald(&c->work_max);
     s.idx = ald(&c->idx);
+    s.seq = ald(&c->seq);
 }
 
 // strict-improvement CAS on the packed label word (update_label, CAS form)
@@ -799,6 +800,7 @@ __device__ void advance(const EngineArgs &a, Ctl *c)
     l.levels = c->levels;
     const Ctl o = l;
     advance_state(a, l);
+    ast(&c->seq, ald(&c->seq) + 1u);
     if (l.phase != o.phase) ast(&c->phase, l.phase);
     if (l.stage != o.stage) ast(&c->stage, l.stage);
     if (l.level != o.level) ast(&c->level, l.level);
@@ -956,20 +958,41 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
         uint64_t tk[4];
         tk[0] = wall_clock64();
         tk[1] = tk[2] = tk[0];
+        // the pass's serial step: statistics, then the phase transition
+        auto close_pass = [&] {
+            if (a.ticks && s.phase < NPHASE) {
+                a.ticks[s.phase] += wall_clock64() - tk[0];
+                a.ticks[NPHASE + s.phase] += 1;
+            }
+            advance(a, c);
+        };
         if (s.stage == 0) {
             pass_main<A>(a, s, tid, nthr, stk);
             stage_flush(a, s);
             tk[1] = wall_clock64();
-            // the chunk queue is complete once every workgroup finished
+            // the chunk queue is complete once every workgroup finished; a
+            // pass without chunk items closes right here
             team_sync(c, a.bar, W, wg, [&] {
-                if (ald(&c->n_chunk) > chunk_max) ast(&c->stage, 1u);
+                const uint32_t nch = ald(&c->n_chunk);
+                if (nch > chunk_max) ast(&c->stage, 1u);
+                else if (nch == 0) close_pass();
             });
             tk[2] = wall_clock64();
             if (threadIdx.x == 0) {
                 sh[0] = ald(&c->n_chunk);
                 sh[1] = ald(&c->stage);
+                sh[2] = ald(&c->seq) != s.seq ? 1u : 0u;
             }
             __syncthreads();
+            if (sh[2]) {   // uniform: closed at the first barrier
+                if (a.ticks && wg == 0 && threadIdx.x == 0 && s.phase < NPHASE) {
+                    unsigned long long *d = a.ticks + 2 * NPHASE + s.phase;
+                    d[0] += tk[1] - tk[0];
+                    d[NPHASE] += tk[2] - tk[1];
+                    d[4 * NPHASE] += phase_work(a, s);
+                }
+                continue;
+            }
             if (sh[1] != 0) break;   // uniform: every workgroup read the same block
         } else {
             if (threadIdx.x == 0) sh[0] = ald(&c->n_chunk);
@@ -978,13 +1001,7 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
         pass_chunks<A>(a, s, tid, nthr, sh[0]);
         stage_flush(a, s);
         tk[3] = wall_clock64();
-        team_sync(c, a.bar, W, wg, [&] {
-            if (a.ticks && s.phase < NPHASE) {
-                a.ticks[s.phase] += wall_clock64() - tk[0];
-                a.ticks[NPHASE + s.phase] += 1;
-            }
-            advance(a, c);
-        });
+        team_sync(c, a.bar, W, wg, close_pass);
         if (a.ticks && wg == 0 && threadIdx.x == 0 && s.phase < NPHASE) {
             // workgroup 0's view: vertex stage, first barrier wait, chunk
             // stage, second barrier wait; pass work and chunk items
@@ -1027,7 +1044,7 @@ __global__ __launch_bounds__(1024) void k_team(TeamParams P)
     __shared__ EngineArgs sa;
     __shared__ Ctl lds_ctl;
     __shared__ Snap snap;
-    __shared__ uint32_t sh[2];
+    __shared__ uint32_t sh[3];
     __shared__ uint32_t stk[STACK_WORDS];
     __shared__ Stage stg;
     const uint32_t t = blockIdx.x % P.ntrees, wg = blockIdx.x / P.ntrees;

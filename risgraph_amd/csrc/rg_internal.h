@@ -172,7 +172,9 @@ struct Ctl {
     uint32_t bar_exit;     // team workgroups that left the launch
     uint32_t rbound;       // smallest deletion-root value (largest for SSWP)
     uint32_t work_max;     // largest pass the team runs (larger: grid-wide)
-    unsigned long long edges, relax_ok, levels, pad2;
+    unsigned long long edges, relax_ok, levels;
+    uint32_t seq;          // passes closed (every phase transition)
+    uint32_t pad2;
 };
 
 // One launch repairs up to MAXT trees; each tree gets W workgroups that
