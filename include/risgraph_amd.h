@@ -85,7 +85,8 @@ int rg_labels_destroy(rg_labels *L);
  * largest pass (queued vertices, or edge ops / 16) that runs inside the
  * persistent team launch; larger passes run grid-wide.  A level is rescued
  * when level <= rescue_level (0xFFFFFFFF: never) or its frontier is below
- * rescue_frontier.  Defaults: rescue on the first level only (0, 0); the
+ * rescue_frontier.  Defaults: rescue on the first level only (0, 0) for
+ * BFS / SSSP / SSWP and on every level (0, 0xFFFFFFFF) for WCC; the
  * reference's policy is (5, 16384).  Full recompute when level <= 10 and
  * subtree+frontier > max(0.05*V, 2000000); ancestor walks capped at
  * walk_limit steps (0 = min(V/10, 65536)); block_max = RG_AUTO (1024 per

@@ -320,9 +320,13 @@ class Labels:
             load_library().rg_labels_destroy(self.h)
             self.h = None
 
-    def set_tuning(self, rescue_level=0, rescue_frontier=0, recompute_level=10,
+    def set_tuning(self, rescue_level=0, rescue_frontier=None, recompute_level=10,
                    recompute_abs=2000000, recompute_frac=0.05, walk_limit=0, block_max=None):
-        """Deletion strategy knobs (rg_labels_set_tuning); block_max None = automatic."""
+        """Deletion strategy knobs (rg_labels_set_tuning); block_max None =
+        automatic; rescue_frontier None = the library default for the
+        algorithm (0, or 0xFFFFFFFF for WCC: rescue on every level)."""
+        if rescue_frontier is None:
+            rescue_frontier = 0xFFFFFFFF if self.algo == WCC else 0
         bm = 0xFFFFFFFF if block_max is None else int(block_max)
         _check(load_library().rg_labels_set_tuning(self.h, rescue_level, rescue_frontier, recompute_level,
                                                    recompute_abs, recompute_frac, walk_limit, bm))

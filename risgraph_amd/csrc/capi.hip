@@ -563,6 +563,10 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 HC(hipEventRecord(L[i]->ev_done, st));
             } else if (h.phase == PH_REBUILD) {
                 // deletion fallback (graph.hpp:1252-1321): full recompute
+                static const bool log_rebuild = getenv("RG_LOG_REBUILD") != nullptr;
+                if (log_rebuild)
+                    fprintf(stderr, "[rg] rebuild: algo %d level %u subtree %u frontier %u flags %#x\n", L[i]->algo,
+                            h.level, h.n_tree, h.n_cur, h.flags);
                 h.phase = PH_BUILD_INIT;
                 h.stage = 0;
                 h.seed = 0;
@@ -834,12 +838,17 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
             EngineArgs a = engine_args(L);
             launch_engine_fill_init(a, g->st);
         }
-        // rescue on the first deletion level only: deeper rescue scans cost
-        // more passes than the subtree they save on this device (the
-        // reference's policy, level <= 5 or frontier < 16384, is one
-        // rg_labels_set_tuning call away; values do not depend on it)
+        // BFS / SSSP / SSWP rescue on the first deletion level only: deeper
+        // rescue scans cost more passes than the subtree they save on this
+        // device.  WCC rescues on every level: nearly every vertex of an
+        // invalidated subtree has a neighbour outside it with the same
+        // component label, so rescue stops the expansion at once, where
+        // expanding can invalidate a large part of a component and force a
+        // full recompute.  (The reference's policy, level <= 5 or frontier <
+        // 16384, is one rg_labels_set_tuning call away; values do not depend
+        // on the policy.)
         L->tune.rescue_level = 0;
-        L->tune.rescue_frontier = 0;
+        L->tune.rescue_frontier = algo == RG_WCC ? 0xFFFFFFFFu : 0u;
         L->tune.recompute_level = 10;
         L->tune.recompute_abs = 2000000;
         L->tune.recompute_frac = 0.05f;
