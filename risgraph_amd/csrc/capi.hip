@@ -561,6 +561,15 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 left--;
                 if (stats) stats->levels += h.levels;
                 HC(hipEventRecord(L[i]->ev_done, st));
+                // RG_LOG_SUBTREE=n: report runs whose deletion subtree exceeded n vertices
+                static const long log_subtree = [] {
+                    const char *e = getenv("RG_LOG_SUBTREE");
+                    return e ? atol(e) : 0L;
+                }();
+                if (log_subtree > 0 && phase0 == PH_DEL_ROOTS && h.n_tree > (uint64_t)log_subtree)
+                    fprintf(stderr, "[rg] subtree: algo %d vertices %u levels %llu grid steps so far %llu\n", L[i]->algo,
+                            h.n_tree, (unsigned long long)h.levels,
+                            (unsigned long long)(stats ? stats->grid_steps : 0));
             } else if (h.phase == PH_REBUILD) {
                 // deletion fallback (graph.hpp:1252-1321): full recompute
                 static const bool log_rebuild = getenv("RG_LOG_REBUILD") != nullptr;

@@ -484,3 +484,58 @@ def test_serving_results_same_alone_and_in_the_mix():
     assert res["served"] == 2 * pos and res["batches"] > 1
     for x, z in zip(La, Lc):
         assert np.array_equal(x.read()[0], z.read()[0]), x.algo
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_heavy_lists_multi_round_batches(weighted):
+    """Batches that pile thousands of ops on two hub lists (the out-list of
+    vertex 0 and the in-list of vertex 1), with many repeats of the same keys:
+    the workgroup-per-list apply path with several 256-op rounds, same-key
+    folding in the round's key table, a list grown from empty past the index
+    threshold inside one batch, block copies longer than 256 entries and
+    index rebuilds.  Every op's return value is checked per key against the
+    brute-force model (the multiset of old counts), then degrees, the two hub
+    lists' entry counts and a sample of other keys."""
+    from collections import defaultdict
+    rng = np.random.default_rng(21)
+    V = 4096
+    g = rg.Graph(V, 1 << 16, symmetric=False, dual=True, weighted=weighted)
+    m = MultigraphModel(V, symmetric=False, weighted=weighted)
+    keyspace = 600                       # repeated keys within one batch
+    for rep in range(10):
+        n = 6000
+        kind = rng.random(n)
+        a = rng.integers(0, V, n)
+        b = rng.integers(0, V, n)
+        hub_out = kind < 0.45
+        hub_in = (kind >= 0.45) & (kind < 0.75)
+        a[hub_out] = 0
+        b[hub_out] = rng.integers(0, keyspace + 150 * rep, hub_out.sum())
+        a[hub_in] = rng.integers(0, keyspace + 150 * rep, hub_in.sum())
+        b[hub_in] = 1
+        w = rng.integers(1, 3, n) if weighted else np.zeros(n, np.int64)
+        delta = 1 if rep % 3 != 2 else -1
+        a64, b64, w64 = a.astype(np.uint64), b.astype(np.uint64), w.astype(np.uint64)
+        got = g.update_edges(a64, b64, w64, delta, True)
+        fm = m.add_edge if delta > 0 else m.del_edge
+        want = [fm(int(x), int(y), int(z)) for x, y, z in zip(a, b, w)]
+        gk, wk = defaultdict(list), defaultdict(list)
+        for i in range(n):
+            key = (int(a[i]), int(b[i]), int(w[i]))
+            gk[key].append(int(got[i]))
+            wk[key].append(want[i])
+        for key in wk:
+            assert sorted(gk[key]) == sorted(wk[key]), (rep, key)
+        out, inc = g.degrees()
+        mo, mi = _model_degrees(m, V)
+        assert np.array_equal(out.astype(np.int64), mo), rep
+        assert np.array_equal(inc.astype(np.int64), mi), rep
+    assert g.get_degree(0) > 512 and g.get_degree(1, incoming=True) > 512
+    assert g.info()["hash_used"] > 0
+    for x in range(0, keyspace + 1500, 7):
+        for c in ((1, 2) if weighted else (0,)):
+            assert g.get_edge_num(0, x, c) == m.count(0, x, c), (x, c)
+            assert g.get_edge_num(x, 1, c) == m.count(x, 1, c), (x, c)
+    keys = list(m.out.items())
+    for (a, b, c), cnt in keys[:: max(1, len(keys) // 300)]:
+        assert g.get_edge_num(a, b, c) == cnt, (a, b, c)
