@@ -13,8 +13,9 @@ Legs:
   plain   one update per device batch (rg_serve with batch cap 1), at the
           highest offered rate whose p99.9 stays within the target;
   serve   continuous batching with the adaptive batch cap: offered rate
-          ramped x1.5 per short calibration round (two misses in a row end
-          the ramp), then two bisection rounds; the W warmup and K timed
+          ramped x1.5 per short calibration round while the server keeps up
+          (>= 95% served, p99 within the target; two misses in a row end the
+          ramp), then two bisection rounds; the W warmup and K timed
           steps run at the best passing rate (a step = one serving round of
           ~step_seconds of arrivals) and must themselves meet the target,
           else the rate backs off x0.9 (up to 8 attempts) and they run again;
@@ -148,19 +149,22 @@ def main():
         plain = dict(res, offered=rate)
 
     # ---- batched serve: calibration ramp (x1.5), then two bisection steps ----
+    # a calibration round passes when the server keeps up: not overloaded,
+    # >= 95% of the offered rate served and p99 within the target.  The p99.9
+    # of a short round is set by one or two large deletion subtrees, so it
+    # is left to the timed steps below, which must meet it on their own.
     def calibrate(rate):
         n = int(max(20000, rate * args.calib_seconds))
         res, _ = serve(g, labels, stream, cur, n, rate, 64, 1 << 20, args.target_ms)
-        ok = (not res["overloaded"] and res["p999_ms"] <= args.target_ms
-              and res["updates_per_s"] >= 0.9 * rate)
-        log(f"serve rate={rate:.0f} -> {res['updates_per_s']:.0f} upd/s p99.9={res['p999_ms']:.2f}ms "
-            f"mean_batch={res['mean_batch']:.1f} max_batch={res['max_batch']} rebuilds={res['rebuilds']} "
-            f"{'ok' if ok else ('overloaded' if res['overloaded'] else 'over')}")
+        ok = (not res["overloaded"] and res["p99_ms"] <= args.target_ms
+              and res["updates_per_s"] >= 0.95 * rate)
+        log(f"serve rate={rate:.0f} -> {res['updates_per_s']:.0f} upd/s p99={res['p99_ms']:.2f}ms "
+            f"p99.9={res['p999_ms']:.2f}ms mean_batch={res['mean_batch']:.1f} max_batch={res['max_batch']} "
+            f"rebuilds={res['rebuilds']} {'ok' if ok else ('overloaded' if res['overloaded'] else 'over')}")
         return ok
 
     # the ramp stops at two failures in a row: one slow stretch of the stream
-    # (a deletion that invalidates a large subtree) does not end it; the timed
-    # steps below must meet the target on their own
+    # does not end it
     rate = max(2.0 * plain_capacity, 100000.0)
     best, fail, misses = None, None, 0
     for _ in range(40):
