@@ -79,7 +79,6 @@ __device__ __forceinline__ void list_span(const List &l, uint32_t v, uint64_t &o
     len = x.z;
 }
 
-__device__ __forceinline__ uint32_t list_len(const List &l, uint32_t v) { return l.h[v].len; }
 
 // Slab allocator over the slot pool (0) and the hash pool (1): blocks are
 // powers of two; a freed block goes to its class's pending list and becomes

@@ -12,9 +12,10 @@
 // before the owner appends to it.  The owner applies the list's inserts
 // first, then its deletes, so a batch behaves like the reference batch driver
 // (all adds, then all dels: bfs_inc_batch.cpp:142-178).
-//   k_apply        one 16-lane group per list; ops one at a time (lookups
-//                  scan the list 16 entries per step, or probe its
-//                  open-addressing index once it holds >= 512 entries).
+//   k_apply        one 4-lane group per light list; ops one at a time
+//                  (lookups scan the list 8 entries per lane per step, or
+//                  probe its open-addressing index once it holds >= 512
+//                  entries).
 //   k_apply_heavy  one 256-thread workgroup per list for hub segments (>= 8
 //                  ops on an indexed list), index builds and rebuilds, and
 //                  block copies longer than 256 entries: 256 ops per round,
@@ -673,7 +674,7 @@ __device__ __forceinline__ void write_liveness(const StoreArgs &a, const ListSta
     }
 }
 
-// Light lists: one 16-lane group per list (its rank-0 op, from the light
+// Light lists: one AG-lane group per list (its rank-0 op, from the light
 // owner list); k_segment already sent every list that could need an index
 // build, a rebuild, a long copy or hub rounds to the heavy list.
 __global__ void __launch_bounds__(256) k_apply(StoreArgs a)
