@@ -652,7 +652,7 @@ int rg_graph_create(uint64_t vertices, uint64_t hint_edges, int symmetric, int d
         const char *env = getenv("RG_SLOT_FACTOR");
         double f = env ? atof(env) : 6.0;
         g->slot_cap = std::max<uint64_t>(1ull << 20, (uint64_t)(f * (double)g->hint)) + 4 * vertices;
-        g->hash_cap = std::max<uint64_t>(1ull << 20, 6 * g->hint);
+        g->hash_cap = std::max<uint64_t>(1ull << 20, 8 * g->hint);
         uint64_t nfree = 2ull * NCLS * FCAP;
         g->free_off = dalloc<uint64_t>(nfree);
         g->pend_off = dalloc<uint64_t>(nfree);

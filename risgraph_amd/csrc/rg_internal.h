@@ -8,7 +8,8 @@
 //               IndexedEdgeStorage does (storage.hpp:369-394).
 //   hash pool   hkey[H] (u64 key = nbr | w << 32, ~0 = empty), hval[H] (u32
 //               local slot) -- per-list open-addressing index built once a
-//               list reaches 512 entries (storage.hpp:304, 358-367).
+//               list reaches INDEX_THRESHOLD entries (the reference's is
+//               512, storage.hpp:304, 358-367).
 //   headers     per direction (out, in): one 64-byte ListHdr per vertex with
 //               off, len, cap, hoff, hcap, hcnt, deg, and nops / seg (the
 //               current batch's ops on the list and their segment in the
@@ -34,7 +35,13 @@ enum Algo : int { BFS = 0, SSSP = 1, SSWP = 2, WCC = 3 };
 constexpr uint32_t DEAD = 0x80000000u;
 constexpr uint32_t NBR_MASK = 0x7FFFFFFFu;
 constexpr uint32_t NILP = 0xFFFFFFFFu;
-constexpr uint32_t INDEX_THRESHOLD = 512;   // storage.hpp:304
+// A list gets its open-addressing index at this many entries.  The
+// reference indexes at 512 (storage.hpp:304); on the device a 4-lane scan of
+// a few hundred entries is a chain of dependent loads, so lists are indexed
+// from 128 entries (indexes start at 256 buckets).  Returns and counts do
+// not depend on it.
+constexpr uint32_t INDEX_THRESHOLD = 128;
+constexpr uint32_t INDEX_MIN_CAP = 256;
 constexpr uint64_t HEMPTY = ~0ull;
 constexpr uint32_t LONG_LIST = 32;          // longer lists go to the chunk queue
 constexpr uint32_t CHUNK = 128;             // entries per chunk item (two wave steps)

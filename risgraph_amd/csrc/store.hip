@@ -14,8 +14,8 @@
 // (all adds, then all dels: bfs_inc_batch.cpp:142-178).
 //   k_apply        one 4-lane group per light list; ops one at a time
 //                  (lookups scan the list 8 entries per lane per step, or
-//                  probe its open-addressing index once it holds >= 512
-//                  entries).
+//                  probe its open-addressing index once it holds >=
+//                  INDEX_THRESHOLD entries).
 //   k_apply_heavy  one 256-thread workgroup per list for hub segments (>= 8
 //                  ops on an indexed list), index builds and rebuilds, and
 //                  block copies longer than 256 entries: 256 ops per round,
@@ -966,7 +966,7 @@ __global__ void __launch_bounds__(HB) k_apply_heavy(StoreArgs a)
             if (st.hoff != HEMPTY || need >= INDEX_THRESHOLD) {
                 // indexed: room for every new key at load <= 1/2
                 uint32_t hc = next_pow2(2 * need);
-                if (hc < 1024) hc = 1024;
+                if (hc < INDEX_MIN_CAP) hc = INDEX_MIN_CAP;
                 if (st.hoff == HEMPTY || 2ull * (st.hcnt + n_ins) > st.hcap) ok = wg_build_index(a, st, hc, sh);
                 if (ok) wg_apply_rounds(a, st, ops, n, sh);
             } else if (threadIdx.x < WAVE) {
@@ -1055,7 +1055,7 @@ __global__ void k_reserve(StoreArgs a)
             if (need > st.cap) ok = grow_block<AG>(a, st, need, g);
             if (ok && need >= INDEX_THRESHOLD && (st.hoff == HEMPTY || st.hcap < 2 * need)) {
                 uint32_t hc = next_pow2(2 * need);
-                if (hc < 1024) hc = 1024;
+                if (hc < INDEX_MIN_CAP) hc = INDEX_MIN_CAP;
                 ok = build_index<AG>(a, st, hc, g);
             }
             if (lane == 0) {
