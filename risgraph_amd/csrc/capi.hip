@@ -572,7 +572,10 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                             (unsigned long long)(stats ? stats->grid_steps : 0));
             } else if (h.phase == PH_REBUILD) {
                 // deletion fallback (graph.hpp:1252-1321): full recompute
-                static const bool log_rebuild = getenv("RG_LOG_REBUILD") != nullptr;
+                static const bool log_rebuild = [] {
+                    const char *e = getenv("RG_LOG_REBUILD");
+                    return e && *e && *e != '0';
+                }();
                 if (log_rebuild)
                     fprintf(stderr, "[rg] rebuild: algo %d level %u subtree %u frontier %u flags %#x\n", L[i]->algo,
                             h.level, h.n_tree, h.n_cur, h.flags);
@@ -862,8 +865,10 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->tune.recompute_abs = 2000000;
         L->tune.recompute_frac = 0.05f;
         L->tune.walk_limit = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(V / 10, 1u << 16));
-        if (const char *tw = getenv("RG_TEAM_W")) L->team_w = (uint32_t)std::max(1, std::min(64, atoi(tw)));
-        if (const char *bm = getenv("RG_BLOCK_MAX")) L->block_max = (uint32_t)atoi(bm);
+        // environment overrides; empty values are ignored
+        if (const char *tw = getenv("RG_TEAM_W"); tw && *tw)
+            L->team_w = (uint32_t)std::max(1, std::min(64, atoi(tw)));
+        if (const char *bm = getenv("RG_BLOCK_MAX"); bm && *bm) L->block_max = (uint32_t)strtoul(bm, nullptr, 0);
         HC(hipMemcpyAsync(L->ctl, L->hc, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
         HC(hipStreamSynchronize(g->st));
     });
