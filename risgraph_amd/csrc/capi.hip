@@ -148,7 +148,6 @@ struct rg_labels {
     Ctl *hc = nullptr;     // pinned mirror
     unsigned long long *ticks = nullptr;   // per-phase block-mode time (device)
     uint32_t *bar = nullptr;               // team barrier sub-counters (device)
-    hipEvent_t ev_done = nullptr;  // end of the last engine run
     hipStream_t st = nullptr;      // this tree's grid-wide passes (concurrent with other trees)
     EngineArgs *dargs = nullptr;   // device copy of the engine arguments (team launches)
     EngineArgs *hargs = nullptr;   // pinned staging of the same
@@ -489,10 +488,11 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
         }
     }
     // grid-wide passes of different trees run concurrently, each on its
-    // tree's stream, after everything queued on the graph stream so far
-    HC(hipEventRecord(g->ev[2], st));
-    for (int i = 0; i < n; i++) HC(hipStreamWaitEvent(L[i]->st, g->ev[2], 0));
+    // tree's stream; the first ones wait for what the graph stream still
+    // has queued (the store batch); later rounds start with every stream
+    // drained by the host
     std::vector<char> on_own(n, 0);
+    bool forked = false;
     int left = n;
     uint64_t guard_iter = 0;
     std::vector<int> team;
@@ -516,6 +516,11 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 // no-op once the tree's passes fit the team again (or it is
                 // done)
                 hipStream_t ts = L[i]->st;
+                if (guard_iter == 0) {
+                    if (!forked) HC(hipEventRecord(g->ev[2], st));
+                    forked = true;
+                    HC(hipStreamWaitEvent(ts, g->ev[2], 0));
+                }
                 hipEvent_t ea, eb;
                 prof_begin(g, 2, host_work(h, g->V), ts, &ea, &eb, (int)h.phase);
                 if (ea) HC(hipEventRecord(ea, ts));
@@ -560,7 +565,6 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 active[i] = 0;
                 left--;
                 if (stats) stats->levels += h.levels;
-                HC(hipEventRecord(L[i]->ev_done, st));
                 // RG_LOG_SUBTREE=n: report runs whose deletion subtree exceeded n vertices
                 static const long log_subtree = [] {
                     const char *e = getenv("RG_LOG_SUBTREE");
@@ -601,6 +605,8 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
         if (stalled >= 3) throw RgError{RG_EDEVICE, "engine made no progress"};
         if (++guard_iter > (1ull << 26)) throw RgError{RG_EDEVICE, "engine did not converge"};
     }
+    // end of the run: every stream is drained, the graph stream is the join point
+    HC(hipEventRecord(g->ev[3], st));
 }
 
 void engine_run(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool new_idx,
@@ -619,8 +625,6 @@ void labels_free(rg_labels *L)
     dfree(L->dargs);
     hfree(L->hargs);
     hfree(L->hc);
-    if (L->ev_done) (void)hipEventDestroy(L->ev_done);
-    L->ev_done = nullptr;
     if (L->st) (void)hipStreamDestroy(L->st);
     L->st = nullptr;
 }
@@ -824,7 +828,6 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->hargs = halloc<EngineArgs>(1);
         memset(L->hargs, 0, sizeof(EngineArgs));
         L->hc->mepoch = 1;
-        HC(hipEventCreate(&L->ev_done));
         HC(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
         HC(hipMemsetAsync(L->vr, 0, V * sizeof(VRec), g->st));
         HC(hipMemsetAsync(L->mstamp, 0, V * 4, g->st));
@@ -1154,13 +1157,8 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
     uint8_t ud = directed ? 0 : K_UNDIR;
     const uint32_t *w = g->weighted ? s->w : nullptr;
     if (g->weighted && !w) throw RgError{RG_EINVAL, "weighted graph needs a weighted stream"};
-    auto ops_range = [&](uint64_t first, uint64_t cnt, uint8_t kind, uint32_t base) {
-        uint64_t a = std::min<uint64_t>(cnt, s->n - first);
-        launch_ops_from_stream(s->src, s->dst, w, first, (uint32_t)a, kind, g->ops, base, g->st);
-        if (a < cnt) launch_ops_from_stream(s->src, s->dst, w, 0, (uint32_t)(cnt - a), kind, g->ops, base + (uint32_t)a, g->st);
-    };
-    ops_range(ins_first, n_ins, K_INS | ud, 0);
-    ops_range(del_first, n_del, K_DEL | ud, (uint32_t)n_ins);
+    launch_ops_from_window(s->src, s->dst, w, s->n, ins_first, (uint32_t)n_ins, del_first, (uint32_t)n_del,
+                           (uint8_t)(K_INS | ud), (uint8_t)(K_DEL | ud), g->ops, g->st);
     g->ops.n = (uint32_t)n;
     g->lops.n = (uint32_t)(4 * n);
     HC(hipEventRecord(g->ev[0], g->st));
@@ -1170,14 +1168,13 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
     engine_run_multi(L, nL, ph, n_ins && n_del, (uint32_t)n, n_del > 0, stats);
     HC(hipEventSynchronize(g->ev[1]));
     store_status_check(g);
-    // device time of the batch: store start to the last tree's last kernel
+    // device time of the batch: store start to the end of the engine run
     float ms = 0;
-    HC(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
-    for (int i = 0; i < nL; i++) {
-        float m = 0;
-        HC(hipEventSynchronize(L[i]->ev_done));
-        HC(hipEventElapsedTime(&m, g->ev[0], L[i]->ev_done));
-        ms = std::max(ms, m);
+    if (nL > 0) {
+        HC(hipEventSynchronize(g->ev[3]));
+        HC(hipEventElapsedTime(&ms, g->ev[0], g->ev[3]));
+    } else {
+        HC(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
     }
     g->last_ms = ms;
     if (g->prof) prof_collect(g);

@@ -292,5 +292,8 @@ void launch_max_degree(const uint64_t *src, uint64_t n, uint64_t V, uint32_t *de
 void launch_ops_from_stream(const uint64_t *src, const uint64_t *dst, const uint32_t *w,
                             uint64_t first, uint32_t n, uint8_t kind, EdgeOps ops,
                             uint32_t base, hipStream_t s);
+void launch_ops_from_window(const uint64_t *src, const uint64_t *dst, const uint32_t *w, uint64_t len,
+                            uint64_t ins_first, uint32_t n_ins, uint64_t del_first, uint32_t n_del,
+                            uint8_t kind_ins, uint8_t kind_del, EdgeOps ops, hipStream_t s);
 
 }  // namespace rg

@@ -1243,4 +1243,32 @@ void launch_ops_from_stream(const uint64_t *src, const uint64_t *dst, const uint
                        kind, ops, base);
 }
 
+// one batch of a sliding window over a circular stream of `len` edges: ops
+// [0, n_ins) insert stream edges ins_first.., ops [n_ins, n_ins + n_del)
+// delete stream edges del_first.., both positions taken modulo len
+__global__ void k_ops_from_window(const uint64_t *src, const uint64_t *dst, const uint32_t *w, uint64_t len,
+                                  uint64_t ins_first, uint32_t n_ins, uint64_t del_first, uint32_t n_del,
+                                  uint8_t kind_ins, uint8_t kind_del, EdgeOps ops)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_ins + n_del) return;
+    const bool ins = i < n_ins;
+    uint64_t e = ins ? ins_first + i : del_first + (i - n_ins);
+    e %= len;
+    ops.src[i] = (uint32_t)src[e];
+    ops.dst[i] = (uint32_t)dst[e];
+    ops.w[i] = w ? w[e] : 0;
+    ops.kind[i] = ins ? kind_ins : kind_del;
+}
+
+void launch_ops_from_window(const uint64_t *src, const uint64_t *dst, const uint32_t *w, uint64_t len,
+                            uint64_t ins_first, uint32_t n_ins, uint64_t del_first, uint32_t n_del,
+                            uint8_t kind_ins, uint8_t kind_del, EdgeOps ops, hipStream_t s)
+{
+    const uint32_t n = n_ins + n_del;
+    if (!n) return;
+    hipLaunchKernelGGL(k_ops_from_window, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, w, len, ins_first,
+                       n_ins, del_first, n_del, kind_ins, kind_del, ops);
+}
+
 }  // namespace rg
