@@ -12,7 +12,8 @@ requests arrive by a Poisson process from 256 simulated clients.
 Legs:
   plain   one update per device batch (rg_serve with batch cap 1), at the
           highest offered rate whose p99.9 stays within the target;
-  serve   continuous batching with the adaptive batch cap: offered rate
+  serve   continuous batching (each batch takes every update that has arrived,
+          up to 2^20): offered rate
           ramped x1.5 per short calibration round while the server keeps up
           (>= 95% served, p99 within the target; two misses in a row end the
           ramp), then two bisection rounds; the W warmup and K timed
@@ -86,11 +87,23 @@ class Cursor:
         return ins_first, del_first, n
 
 
+# Batched serving takes every update that has arrived when the device frees
+# up (up to BATCH_MAX per batch): the batch size follows the load by itself,
+# and after a slow batch the backlog drains in the largest, most efficient
+# batches.  (Shrinking batches after a slow one, the library's adaptive cap,
+# slowed that recovery and lengthened the tail.)
+BATCH_MAX = 1 << 20
+
+
 def serve(g, labels, stream, cur, n, rate, cap, cap_max, target_ms):
     ins_first, del_first, n = cur.take(n)
     res, lat = g.serve(labels, stream, ins_first, del_first, n, rate, batch_cap=cap, cap_max=cap_max,
                        target_ms=target_ms, latencies=True)
     return res, lat
+
+
+def serve_batched(g, labels, stream, cur, n, rate, args):
+    return serve(g, labels, stream, cur, n, rate, BATCH_MAX, BATCH_MAX, 0.0)
 
 
 def main():
@@ -155,7 +168,7 @@ def main():
     # is left to the timed steps below, which must meet it on their own.
     def calibrate(rate):
         n = int(max(20000, rate * args.calib_seconds))
-        res, _ = serve(g, labels, stream, cur, n, rate, 64, 1 << 20, args.target_ms)
+        res, _ = serve_batched(g, labels, stream, cur, n, rate, args)
         ok = (not res["overloaded"] and res["p99_ms"] <= args.target_ms
               and res["updates_per_s"] >= 0.95 * rate)
         log(f"serve rate={rate:.0f} -> {res['updates_per_s']:.0f} upd/s p99={res['p99_ms']:.2f}ms "
@@ -192,10 +205,10 @@ def main():
     def timed(rate):
         n_step = int(max(20000, rate * args.step_seconds))
         for _ in range(args.warmup):
-            serve(g, labels, stream, cur, n_step, rate, 64, 1 << 20, args.target_ms)
+            serve_batched(g, labels, stream, cur, n_step, rate, args)
         lats, acc = [], dict(wall=0.0, total=0, batches=0, grid_steps=0, gpu_ms=0.0, overloaded=0)
         for k in range(args.steps):
-            res, lat = serve(g, labels, stream, cur, n_step, rate, 64, 1 << 20, args.target_ms)
+            res, lat = serve_batched(g, labels, stream, cur, n_step, rate, args)
             lats.append(lat)
             acc["wall"] += res["wall_s"]
             acc["total"] += res["served"]
@@ -348,7 +361,7 @@ def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch):
     eb0 = g.profile_read(1)
     g.profile(3)
     n = int(max(20000, rate * min(args.step_seconds, 0.5)))
-    res, _ = serve(g, labels, stream, cur, n, rate, 64, 1 << 20, args.target_ms)
+    res, _ = serve_batched(g, labels, stream, cur, n, rate, args)
     eb1 = g.profile_read(1)
     g.profile(1)
     # every k_apply launch of this process (load batches included), the same
