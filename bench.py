@@ -12,8 +12,8 @@ requests arrive by a Poisson process from 256 simulated clients.
 Legs:
   plain   one update per device batch (rg_serve with batch cap 1), at the
           highest offered rate whose p99.9 stays within the target;
-  serve   continuous batching (each batch takes every update that has arrived,
-          up to 2^20): offered rate
+  serve   continuous batching with the adaptive batch cap (up to 2^20
+          updates per batch): offered rate
           ramped x1.5 per short calibration round while the server keeps up
           (>= 95% served, p99 within the target; two misses in a row end the
           ramp), then two bisection rounds; the W warmup and K timed
@@ -87,11 +87,12 @@ class Cursor:
         return ins_first, del_first, n
 
 
-# Batched serving takes every update that has arrived when the device frees
-# up (up to BATCH_MAX per batch): the batch size follows the load by itself,
-# and after a slow batch the backlog drains in the largest, most efficient
-# batches.  (Shrinking batches after a slow one, the library's adaptive cap,
-# slowed that recovery and lengthened the tail.)
+# Batched serving uses the library's adaptive batch cap (rg_serve): it
+# starts at 64 updates, grows x1.5 while the backlog exceeds it and shrinks
+# x0.7 when a batch's device time exceeds a quarter of the latency target,
+# up to BATCH_MAX.  (Taking every arrived update per batch, with no cap,
+# met the p99.9 target less often: 6 of 8 half-second rounds at 80M
+# updates/s against 7 of 8.)
 BATCH_MAX = 1 << 20
 
 
@@ -103,7 +104,7 @@ def serve(g, labels, stream, cur, n, rate, cap, cap_max, target_ms):
 
 
 def serve_batched(g, labels, stream, cur, n, rate, args):
-    return serve(g, labels, stream, cur, n, rate, BATCH_MAX, BATCH_MAX, 0.0)
+    return serve(g, labels, stream, cur, n, rate, 64, BATCH_MAX, args.target_ms)
 
 
 def main():
