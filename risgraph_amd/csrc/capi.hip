@@ -455,7 +455,18 @@ void engine_init(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool 
     h.levels = 0;
 }
 
-constexpr int GRID_SPEC = 1;   // grid steps enqueued per host round trip
+// grid steps enqueued per host round trip (RG_GRID_SPEC overrides): the
+// steps after the first are speculative and turn into no-ops once the
+// tree's passes fit the team again
+int grid_spec()
+{
+    static const int v = [] {
+        const char *e = getenv("RG_GRID_SPEC");
+        const int x = e && *e ? atoi(e) : 1;
+        return std::max(1, std::min(8, x));
+    }();
+    return v;
+}
 
 // Run the engines of labels L[0..n) from `phase0` to completion, after the
 // store batch already queued on the graph stream.  Trees whose passes fit the
@@ -524,7 +535,7 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 hipEvent_t ea, eb;
                 prof_begin(g, 2, host_work(h, g->V), ts, &ea, &eb, (int)h.phase);
                 if (ea) HC(hipEventRecord(ea, ts));
-                launch_engine_grid_steps(args[i], h, GRID_SPEC, ts);
+                launch_engine_grid_steps(args[i], h, grid_spec(), ts);
                 if (eb) HC(hipEventRecord(eb, ts));
                 on_own[i] = 1;
                 if (stats) stats->grid_steps++;

@@ -1165,6 +1165,8 @@ static void grid_launch(const EngineArgs &a, const Ctl &h, int steps, hipStream_
     }();
     if (blocks > max_blocks) blocks = max_blocks;
     for (int k = 0; k < steps; k++) {
+        // later steps do not know their pass's size: full width
+        if (k == 1) blocks = max_blocks;
         hipLaunchKernelGGL(k_grid_main<A>, dim3((uint32_t)blocks), dim3(GRID_BLOCK), 0, st, a);
         hipLaunchKernelGGL(k_grid_chunks<A>, dim3(std::max<uint32_t>(1, max_blocks / 2)), dim3(GRID_BLOCK), 0, st, a);
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, a);
