@@ -344,8 +344,8 @@ void run_store(rg_graph *g)
     StoreArgs a = store_args(g);
     hipEvent_t ea, eb;
     prof_begin(g, 0, g->ops.n, g->st, &ea, &eb);
-    launch_store_batch(a, g->st, ea, eb);
-    launch_pool_recycle(a, g->st);   // also publishes err/oom to g->h_status
+    if (!launch_store_batch(a, g->st, ea, eb))
+        launch_pool_recycle(a, g->st);   // also publishes err/oom to g->h_status
     HC(hipGetLastError());
 }
 

@@ -255,8 +255,11 @@ struct StoreArgs {
 };
 
 // host-side launchers (store.hip / engine.hip)
-// ev_a / ev_b (may be null) are recorded around the apply kernel
-void launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a = nullptr,
+// ev_a / ev_b (may be null) are recorded around the apply kernels.  Returns
+// true when the batch's pool recycling is already done (small batches take a
+// one-workgroup path that includes it); otherwise launch_pool_recycle must
+// follow.
+bool launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a = nullptr,
                         hipEvent_t ev_b = nullptr);
 void launch_store_count(const StoreArgs &a, hipStream_t s);
 void launch_store_reserve(const StoreArgs &a, hipStream_t s);

@@ -51,10 +51,11 @@ __device__ __forceinline__ uint32_t next_pow2(uint32_t x)
 // ---------------------------------------------------------------------
 // expand edge ops into list ops and link them into per-list chains
 // ---------------------------------------------------------------------
-__global__ void k_expand_link(StoreArgs a)
+// Edge op i -> list ops 4i..4i+3 (unused slots get v = NILP).  With `rank`,
+// each list op takes its rank on its list with one atomicAdd on the header's
+// nops (the segment path); without, the caller groups the ops itself.
+__device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.ops.n) return;
     uint32_t s = a.ops.src[i], d = a.ops.dst[i], w = a.weighted ? a.ops.w[i] : 0;
     uint8_t kind = a.ops.kind[i];
     a.ops.old[i] = 0;
@@ -78,7 +79,7 @@ __global__ void k_expand_link(StoreArgs a)
         a.lops.delta[j] = delta;
         a.lops.flags[j] = flags;
         int L = (flags & L_IN) ? 1 : 0;
-        a.lops.rank[j] = atomicAdd(&a.lists[L].h[v].nops, 1u);
+        if (rank) a.lops.rank[j] = atomicAdd(&a.lists[L].h[v].nops, 1u);
     };
     // graph.hpp:203-253: a directed op touches out[src] (+ in[dst] unless
     // symmetric); an undirected op is the directed op both ways and returns
@@ -102,6 +103,12 @@ __global__ void k_expand_link(StoreArgs a)
         emit(s, s, (int8_t)(2 * sign), L_RET | L_SLOT | L_SELF);
         if (!a.symmetric) emit(s, s, (int8_t)(2 * sign), L_IN | L_SELF);
     }
+}
+
+__global__ void k_expand_link(StoreArgs a)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.ops.n) expand_op(a, i, true);
 }
 
 // ---------------------------------------------------------------------
@@ -938,6 +945,54 @@ __device__ void wg_apply_rounds(const StoreArgs &a, ListState &st, const uint32_
     }
 }
 
+// One heavy list with the whole HB-thread workgroup: room for its inserts,
+// index build or rebuild, then rounds (or, for a short unindexed list that
+// only needed a long copy, the serial path on the first AG lanes); liveness
+// and the header last.
+__device__ void heavy_list_apply(const StoreArgs &a, ListHdr &hd, const uint32_t *ops, uint32_t n, HeavyShared &sh)
+{
+    ListState st = load_state(hd);
+    if (threadIdx.x == 0) sh.sum = 0;
+    __syncthreads();
+    int32_t my = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += HB) my += a.lops.delta[ops[i]] > 0 ? 1 : 0;
+    if (my) atomicAdd(&sh.sum, my);
+    __syncthreads();
+    const uint32_t n_ins = (uint32_t)sh.sum;
+    __syncthreads();
+    const uint32_t need = st.len + n_ins;
+    bool ok = true;
+    if (need > st.cap) ok = wg_grow(a, st, need, sh);
+    if (ok) {
+        if (st.hoff != HEMPTY || need >= INDEX_THRESHOLD) {
+            // indexed: room for every new key at load <= 1/2
+            uint32_t hc = next_pow2(2 * need);
+            if (hc < INDEX_MIN_CAP) hc = INDEX_MIN_CAP;
+            if (st.hoff == HEMPTY || 2ull * (st.hcnt + n_ins) > st.hcap) ok = wg_build_index(a, st, hc, sh);
+            if (ok) wg_apply_rounds(a, st, ops, n, sh);
+        } else if (threadIdx.x < AG) {
+            // a grown short list: the first AG lanes apply its few ops
+            Grp g;
+            g.lane = threadIdx.x;
+            g.base = 0;
+            apply_serial(a, st, ops, n, g);
+        }
+    }
+    // every thread takes the list state of thread 0 (the serial path
+    // advanced only the first group's copy)
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        store_state(hd, st);
+        hd.nops = 0;
+    }
+    __syncthreads();
+    if (ok) {
+        st.off = hd.off;
+        write_liveness(a, st, ops, n, threadIdx.x, HB);
+    }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(HB) k_apply_heavy(StoreArgs a)
 {
     __shared__ HeavyShared sh;
@@ -947,59 +1002,25 @@ __global__ void __launch_bounds__(HB) k_apply_heavy(StoreArgs a)
         const uint32_t v = a.lops.v[j];
         const int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
         ListHdr &hd = a.lists[L].h[v];
-        const uint32_t n = hd.nops;
-        const uint32_t *ops = a.lops.segbuf + hd.seg;
-        ListState st = load_state(hd);
-
-        if (threadIdx.x == 0) sh.sum = 0;
-        __syncthreads();
-        int32_t my = 0;
-        for (uint32_t i = threadIdx.x; i < n; i += HB) my += a.lops.delta[ops[i]] > 0 ? 1 : 0;
-        if (my) atomicAdd(&sh.sum, my);
-        __syncthreads();
-        const uint32_t n_ins = (uint32_t)sh.sum;
-        __syncthreads();
-        const uint32_t need = st.len + n_ins;
-        bool ok = true;
-        if (need > st.cap) ok = wg_grow(a, st, need, sh);
-        if (ok) {
-            if (st.hoff != HEMPTY || need >= INDEX_THRESHOLD) {
-                // indexed: room for every new key at load <= 1/2
-                uint32_t hc = next_pow2(2 * need);
-                if (hc < INDEX_MIN_CAP) hc = INDEX_MIN_CAP;
-                if (st.hoff == HEMPTY || 2ull * (st.hcnt + n_ins) > st.hcap) ok = wg_build_index(a, st, hc, sh);
-                if (ok) wg_apply_rounds(a, st, ops, n, sh);
-            } else if (threadIdx.x < WAVE) {
-                // a grown short list: the first 16 lanes apply its few ops
-                if (threadIdx.x < AG) {
-                    Grp g;
-                    g.lane = threadIdx.x;
-                    g.base = 0;
-                    apply_serial(a, st, ops, n, g);
-                }
-            }
-        }
-        // every thread takes the list state of thread 0 (the serial path
-        // advanced only the first group's copy)
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            store_state(hd, st);
-            hd.nops = 0;
-        }
-        __syncthreads();
-        if (ok) {
-            st.off = hd.off;
-            write_liveness(a, st, ops, n, threadIdx.x, HB);
-        }
-        __syncthreads();
+        heavy_list_apply(a, hd, a.lops.segbuf + hd.seg, hd.nops, sh);
     }
 }
 
-void launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b)
+constexpr uint32_t SMALL_OPS = 64;   // batches up to this many edge ops take the one-workgroup path
+__global__ void __launch_bounds__(HB) k_store_small(StoreArgs a);
+
+bool launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b)
 {
-    if (a.ops.n == 0) return;
+    if (a.ops.n == 0) return false;
     const uint32_t n = a.ops.n;
     const uint32_t nl = 4 * n;
+    if (n <= SMALL_OPS) {
+        // one launch for the whole batch, pool recycling included
+        if (ev_a) (void)hipEventRecord(ev_a, s);
+        hipLaunchKernelGGL(k_store_small, dim3(1), dim3(HB), 0, s, a);
+        if (ev_b) (void)hipEventRecord(ev_b, s);
+        return true;
+    }
     hipLaunchKernelGGL(k_expand_link, dim3((n + 255) / 256), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_segment, dim3((nl + SEG_BLOCK - 1) / SEG_BLOCK), dim3(SEG_BLOCK), 0, s, a);
     hipLaunchKernelGGL(k_scatter, dim3((nl + 255) / 256), dim3(256), 0, s, a);
@@ -1012,6 +1033,7 @@ void launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipE
     uint32_t hblocks = nl < 2048 ? nl : 2048;
     hipLaunchKernelGGL(k_apply_heavy, dim3(hblocks), dim3(HB), 0, s, a);
     if (ev_b) (void)hipEventRecord(ev_b, s);
+    return false;
 }
 
 // ---------------------------------------------------------------------
@@ -1104,10 +1126,10 @@ void launch_list_degrees(List l, uint32_t V, uint32_t *out, hipStream_t s)
     if (V) hipLaunchKernelGGL(k_list_degrees, dim3((V + 255) / 256), dim3(256), 0, s, l, V, out);
 }
 
-// pending frees -> free lists, once per batch (one block per pool/class)
-__global__ void k_pool_recycle(StoreArgs a)
+// pending frees of size class c of pool h -> its free list (threads t0 of
+// nt move the offsets); one caller per class, after every pop of the batch
+__device__ void recycle_class(const StoreArgs &a, int h, int c, uint32_t t0, uint32_t nt)
 {
-    const int h = blockIdx.x / NCLS, c = blockIdx.x % NCLS;
     Pool *p = a.pool;
     unsigned int avail = p->free_n[h][c];
     unsigned int take = p->free_take[h][c];
@@ -1117,25 +1139,162 @@ __global__ void k_pool_recycle(StoreArgs a)
     unsigned int room = FCAP - rem;
     unsigned int mv = pn < room ? pn : room;
     uint64_t base = ((uint64_t)h * NCLS + c) * FCAP;
-    for (unsigned int k = threadIdx.x; k < mv; k += blockDim.x)
-        a.free_off[base + rem + k] = a.pend_off[base + k];
+    for (unsigned int k = t0; k < mv; k += nt) a.free_off[base + rem + k] = a.pend_off[base + k];
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (t0 == 0) {
         p->free_n[h][c] = rem + mv;
         p->free_take[h][c] = 0;
         p->pend_n[h][c] = 0;
         if (mv) atomicAdd(&p->recycled[h], (unsigned long long)mv);
-        if (blockIdx.x == 0) {
-            p->seg_pack = 0;   // segments and owners of the next batch
-            p->n_heavy = 0;
-        }
     }
-    // publish the batch status (bad ids, pool exhaustion) to pinned host memory
-    if (blockIdx.x == 0 && threadIdx.x == 0 && a.hstatus) {
+}
+
+// the batch is over: reset the per-batch counters and publish the batch
+// status (bad ids, pool exhaustion) to pinned host memory
+__device__ void end_batch(const StoreArgs &a)
+{
+    Pool *p = a.pool;
+    p->seg_pack = 0;   // segments and owners of the next batch
+    p->n_heavy = 0;
+    if (a.hstatus) {
         a.hstatus[0] = __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         a.hstatus[1] = __hip_atomic_load(&p->oom, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence_system();
     }
+}
+
+// pending frees -> free lists, once per batch (one block per pool/class)
+__global__ void k_pool_recycle(StoreArgs a)
+{
+    recycle_class(a, blockIdx.x / NCLS, blockIdx.x % NCLS, threadIdx.x, blockDim.x);
+    if (blockIdx.x == 0 && threadIdx.x == 0) end_batch(a);
+}
+
+// A batch of at most SMALL_OPS edge ops in one workgroup (the per-update
+// path): expand, group the list ops by list with a stable rank sort in LDS,
+// apply the light lists with AG-lane groups and the heavy ones with the
+// whole workgroup (the same routines as the batch kernels), then recycle the
+// pool's pending frees and publish the batch status.  One launch instead of
+// six.
+__global__ void __launch_bounds__(HB) k_store_small(StoreArgs a)
+{
+    constexpr uint32_t SL = 4 * SMALL_OPS;
+    static_assert(SL <= HB, "one thread per list op");
+    __shared__ HeavyShared sh;
+    __shared__ uint32_t key[SL], order[SL], sstart[SL + 1], hseg[SL];
+    __shared__ uint32_t wcnt[HB / WAVE];
+    __shared__ uint32_t nvalid, nseg, nheavy;
+    const uint32_t tid = threadIdx.x, lane = __lane_id(), wv = tid / WAVE;
+    const uint32_t n = a.ops.n, nl = 4 * n;
+    if (tid == 0) {
+        nvalid = 0;
+        nheavy = 0;
+    }
+    if (tid < n) expand_op(a, tid, false);
+    __syncthreads();
+    // list key (L << 31 | v) of every list op; NILP for unused slots
+    uint32_t kt = NILP;
+    if (tid < nl) {
+        const uint32_t v = a.lops.v[tid];
+        if (v != NILP) kt = v | ((a.lops.flags[tid] & L_IN) ? 0x80000000u : 0u);
+    }
+    if (tid < SL) key[tid] = kt;
+    if (kt != NILP) atomicAdd(&nvalid, 1u);
+    __syncthreads();
+    // stable rank sort: ops of one list become a contiguous run in op order
+    if (kt != NILP) {
+        uint32_t r = 0;
+        for (uint32_t u = 0; u < nl; u++) {
+            const uint32_t ku = key[u];
+            r += (ku != NILP && (ku < kt || (ku == kt && u < tid))) ? 1u : 0u;
+        }
+        order[r] = tid;
+    }
+    __syncthreads();
+    const uint32_t nv = nvalid;
+    if (tid < nv) a.lops.segbuf[tid] = order[tid];
+    // run starts, compacted in order (wave ballots plus wave offsets)
+    const bool head = tid < nv && (tid == 0 || key[order[tid]] != key[order[tid - 1]]);
+    const uint64_t bal = __ballot(head);
+    if (lane == 0) wcnt[wv] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t q = 0; q < wv; q++) off += wcnt[q];
+    if (head) sstart[off + (uint32_t)__popcll(bal & ((1ull << lane) - 1))] = tid;
+    if (tid == 0) {
+        uint32_t t = 0;
+        for (uint32_t q = 0; q < HB / WAVE; q++) t += wcnt[q];
+        nseg = t;
+        sstart[t] = nv;
+    }
+    __syncthreads();
+    // light lists: one AG-lane group each; heavy ones are queued
+    {
+        Grp g;
+        g.lane = (int)(tid % AG);
+        g.base = (int)(tid % WAVE) - g.lane;
+        for (uint32_t sg = tid / AG; sg < nseg; sg += HB / AG) {
+            const uint32_t b = sstart[sg], cnt = sstart[sg + 1] - b;
+            const uint32_t j0 = order[b];
+            const uint32_t v = a.lops.v[j0];
+            const int L = (a.lops.flags[j0] & L_IN) ? 1 : 0;
+            ListHdr &hd = a.lists[L].h[v];
+            if (heavy_list(hd, cnt)) {
+                if (g.lane == 0) hseg[atomicAdd(&nheavy, 1u)] = sg;
+                continue;
+            }
+            ListState st = load_state(hd);
+            const uint32_t *ops = a.lops.segbuf + b;
+            int32_t n_ins = 0;
+            for (uint32_t i = g.lane; i < cnt; i += AG) n_ins += a.lops.delta[ops[i]] > 0 ? 1 : 0;
+            const uint32_t need = st.len + (uint32_t)gsum<AG>(n_ins);
+            if (light_room(a, st, need, 0u, 0ull, g)) {
+                apply_serial(a, st, ops, cnt, g);
+                __threadfence_block();
+                write_liveness(a, st, ops, cnt, g.lane, AG);
+            }
+            if (g.lane == 0) store_state(hd, st);
+        }
+    }
+    __syncthreads();
+    // heavy lists, one at a time with the whole workgroup
+    const uint32_t nh = nheavy;
+    for (uint32_t q = 0; q < nh; q++) {
+        const uint32_t sg = hseg[q];
+        const uint32_t b = sstart[sg], cnt = sstart[sg + 1] - b;
+        const uint32_t j0 = order[b];
+        ListHdr &hd = a.lists[(a.lops.flags[j0] & L_IN) ? 1 : 0].h[a.lops.v[j0]];
+        heavy_list_apply(a, hd, a.lops.segbuf + b, cnt, sh);
+    }
+    __syncthreads();
+    __threadfence();
+    // every pop of the batch is done: pending frees -> free lists, all
+    // classes at once (HB / (2 NCLS) threads per class; the counters are
+    // read again only by the next batch's kernels)
+    {
+        constexpr uint32_t TPC = HB / (2 * NCLS);
+        static_assert(TPC >= 1, "a thread per class at least");
+        const uint32_t hcls = tid / TPC, t0 = tid % TPC;
+        if (hcls < 2 * NCLS) {
+            const int h = (int)(hcls / NCLS), c = (int)(hcls % NCLS);
+            Pool *p = a.pool;
+            const unsigned int avail = p->free_n[h][c], take = p->free_take[h][c];
+            const unsigned int rem = take >= avail ? 0 : avail - take;
+            unsigned int pn = p->pend_n[h][c];
+            if (pn > FCAP) pn = FCAP;
+            const unsigned int mv = pn < FCAP - rem ? pn : FCAP - rem;
+            const uint64_t base = ((uint64_t)h * NCLS + c) * FCAP;
+            for (unsigned int k = t0; k < mv; k += TPC) a.free_off[base + rem + k] = a.pend_off[base + k];
+            if (t0 == 0) {
+                p->free_n[h][c] = rem + mv;
+                p->free_take[h][c] = 0;
+                p->pend_n[h][c] = 0;
+                if (mv) atomicAdd(&p->recycled[h], (unsigned long long)mv);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) end_batch(a);
 }
 
 void launch_pool_recycle(const StoreArgs &a, hipStream_t s)
