@@ -172,13 +172,15 @@ int rg_stream_plain(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t i
 int rg_last_gpu_ms(rg_graph *g, double *ms);
 
 /* Kernel timing with HIP events on the streams the kernels run on.  Bit k of
- * `mask` times kernels of kind k: the store apply kernel (kind 0), the
+ * `mask` times kernels of kind k: the store apply kernels (kind 0), the
  * persistent team engine launches (kind 1) and grid-wide engine passes
  * (kind 2).  rg_profile_read returns the launch count, summed device
  * milliseconds and summed work units (edge ops for kind 0, trees for kind 1,
  * pass work for kind 2) accumulated since the graph was made; kind 3 + p
  * reads the grid-wide passes of engine phase p (the phase numbers of
- * rg_labels_phase_time) alone. */
+ * rg_labels_phase_time) alone.  Kind 0 counts batches of more than 64 edge
+ * ops (k_apply + k_apply_heavy); batches of at most 64 run the whole store
+ * step in one kernel, timed under mask bit 0 and read as kind 13. */
 int rg_profile_enable(rg_graph *g, int mask);
 int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint64_t *units);
 

@@ -282,9 +282,10 @@ class Graph:
         _check(load_library().rg_profile_enable(self.h, int(mask)))
 
     def profile_read(self, kind):
-        """kind 0 = store apply kernel, 1 = persistent team engine launches,
-        2 = grid-wide engine passes, 3 + p = grid-wide passes of phase p
-        (PHASES[p])."""
+        """kind 0 = store apply kernels (batches over 64 edge ops),
+        1 = persistent team engine launches, 2 = grid-wide engine passes,
+        3 + p = grid-wide passes of phase p (PHASES[p]), 13 = the
+        one-kernel store step of batches of at most 64 edge ops."""
         n, units, ms = u64(), u64(), ctypes.c_double()
         _check(load_library().rg_profile_read(self.h, kind, ctypes.byref(n), ctypes.byref(ms),
                                               ctypes.byref(units)))

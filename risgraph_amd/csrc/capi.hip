@@ -127,7 +127,10 @@ struct rg_graph {
     std::vector<hipEvent_t> ev_free;
     std::vector<ProfRec> prof_pending;
     // kinds 0-2, then 3 + phase: grid-wide passes of each engine phase
-    static constexpr int PKINDS = 3 + NPHASE;
+    // 0 store apply, 1 team, 2 grid, 3 + p grid passes of phase p, then the
+    // one-workgroup store kernel of small batches
+    static constexpr int PKINDS = 4 + NPHASE;
+    static constexpr int KIND_STORE_SMALL = 3 + NPHASE;
     uint64_t p_launch[PKINDS]{}, p_units[PKINDS]{};
     double p_ms[PKINDS]{};
 };
@@ -311,7 +314,8 @@ void prof_begin(rg_graph *g, int kind, uint64_t units, hipStream_t s, hipEvent_t
                 int phase = -1)
 {
     *a = *b = nullptr;
-    if (!((g->prof >> kind) & 1)) return;
+    const int bit = kind == rg_graph::KIND_STORE_SMALL ? 0 : kind;
+    if (!((g->prof >> bit) & 1)) return;
     *a = prof_event(g);
     *b = prof_event(g);
     g->prof_pending.push_back({kind, *a, *b, units, phase});
@@ -343,7 +347,7 @@ void run_store(rg_graph *g)
 {
     StoreArgs a = store_args(g);
     hipEvent_t ea, eb;
-    prof_begin(g, 0, g->ops.n, g->st, &ea, &eb);
+    prof_begin(g, g->ops.n <= STORE_SMALL_OPS ? rg_graph::KIND_STORE_SMALL : 0, g->ops.n, g->st, &ea, &eb);
     if (!launch_store_batch(a, g->st, ea, eb))
         launch_pool_recycle(a, g->st);   // also publishes err/oom to g->h_status
     HC(hipGetLastError());

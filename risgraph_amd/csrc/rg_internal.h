@@ -155,6 +155,8 @@ enum Phase : uint32_t {
 };
 
 constexpr int NPHASE = 10;
+// batches of at most this many edge ops run the one-workgroup store kernel
+constexpr uint32_t STORE_SMALL_OPS = 64;
 // per-phase team statistics, NPHASE words each: wall-clock ticks, passes,
 // then workgroup 0's vertex stage, first barrier, chunk stage and second
 // barrier ticks, summed pass work and summed chunk items

@@ -1006,7 +1006,7 @@ __global__ void __launch_bounds__(HB) k_apply_heavy(StoreArgs a)
     }
 }
 
-constexpr uint32_t SMALL_OPS = 64;   // batches up to this many edge ops take the one-workgroup path
+constexpr uint32_t SMALL_OPS = STORE_SMALL_OPS;
 __global__ void __launch_bounds__(HB) k_store_small(StoreArgs a);
 
 bool launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b)

@@ -486,32 +486,26 @@ def test_serving_results_same_alone_and_in_the_mix():
         assert np.array_equal(x.read()[0], z.read()[0]), x.algo
 
 
-@pytest.mark.parametrize("weighted", [False, True])
-def test_heavy_lists_multi_round_batches(weighted):
-    """Batches that pile thousands of ops on two hub lists (the out-list of
-    vertex 0 and the in-list of vertex 1), with many repeats of the same keys:
-    the workgroup-per-list apply path with several 256-op rounds, same-key
-    folding in the round's key table, a list grown from empty past the index
-    threshold inside one batch, block copies longer than 256 entries and
-    index rebuilds.  Every op's return value is checked per key against the
-    brute-force model (the multiset of old counts), then degrees, the two hub
-    lists' entry counts and a sample of other keys."""
+def _hub_batches(weighted, sizes, seed, keyspace=600, growth=150):
+    """Batches that pile their ops on two hub lists (the out-list of vertex 0
+    and the in-list of vertex 1) with many repeats of the same keys; every
+    op's return value is checked per key against the brute-force model (the
+    multiset of old counts), then degrees, the hub lists' entry counts and a
+    sample of other keys."""
     from collections import defaultdict
-    rng = np.random.default_rng(21)
+    rng = np.random.default_rng(seed)
     V = 4096
     g = rg.Graph(V, 1 << 16, symmetric=False, dual=True, weighted=weighted)
     m = MultigraphModel(V, symmetric=False, weighted=weighted)
-    keyspace = 600                       # repeated keys within one batch
-    for rep in range(10):
-        n = 6000
+    for rep, n in enumerate(sizes):
         kind = rng.random(n)
         a = rng.integers(0, V, n)
         b = rng.integers(0, V, n)
         hub_out = kind < 0.45
         hub_in = (kind >= 0.45) & (kind < 0.75)
         a[hub_out] = 0
-        b[hub_out] = rng.integers(0, keyspace + 150 * rep, hub_out.sum())
-        a[hub_in] = rng.integers(0, keyspace + 150 * rep, hub_in.sum())
+        b[hub_out] = rng.integers(0, keyspace + growth * rep, hub_out.sum())
+        a[hub_in] = rng.integers(0, keyspace + growth * rep, hub_in.sum())
         b[hub_in] = 1
         w = rng.integers(1, 3, n) if weighted else np.zeros(n, np.int64)
         delta = 1 if rep % 3 != 2 else -1
@@ -525,17 +519,53 @@ def test_heavy_lists_multi_round_batches(weighted):
             gk[key].append(int(got[i]))
             wk[key].append(want[i])
         for key in wk:
-            assert sorted(gk[key]) == sorted(wk[key]), (rep, key)
+            assert sorted(gk[key]) == sorted(wk[key]), (rep, n, key)
         out, inc = g.degrees()
         mo, mi = _model_degrees(m, V)
-        assert np.array_equal(out.astype(np.int64), mo), rep
-        assert np.array_equal(inc.astype(np.int64), mi), rep
-    assert g.get_degree(0) > 512 and g.get_degree(1, incoming=True) > 512
-    assert g.info()["hash_used"] > 0
-    for x in range(0, keyspace + 1500, 7):
+        assert np.array_equal(out.astype(np.int64), mo), (rep, n)
+        assert np.array_equal(inc.astype(np.int64), mi), (rep, n)
+    for x in range(0, keyspace + growth * len(sizes), 7):
         for c in ((1, 2) if weighted else (0,)):
             assert g.get_edge_num(0, x, c) == m.count(0, x, c), (x, c)
             assert g.get_edge_num(x, 1, c) == m.count(x, 1, c), (x, c)
     keys = list(m.out.items())
     for (a, b, c), cnt in keys[:: max(1, len(keys) // 300)]:
         assert g.get_edge_num(a, b, c) == cnt, (a, b, c)
+    return g
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_heavy_lists_multi_round_batches(weighted):
+    """Batches of 6000 ops on two hub lists: the workgroup-per-list apply
+    path with several 256-op rounds, same-key folding in the round's key
+    table, a list grown from empty past the index threshold inside one
+    batch, block copies longer than 256 entries and index rebuilds."""
+    g = _hub_batches(weighted, [6000] * 10, 21)
+    assert g.get_degree(0) > 512 and g.get_degree(1, incoming=True) > 512
+    assert g.info()["hash_used"] > 0
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_small_batches_on_hub_lists(weighted):
+    """Batches of 1 to 64 edge ops, the one-kernel store path, piled on two
+    hub lists: light lists, hub lists taken by the whole workgroup (index
+    builds as the lists pass 128 entries, growth copies, same-key ops within
+    one batch), and the size-class recycling between batches.  The store
+    kernel timings of these batches are kept apart from the large-batch
+    apply kernels'."""
+    rng = np.random.default_rng(5)
+    sizes = [int(x) for x in rng.integers(1, 65, 150)] + [64] * 30
+    g = _hub_batches(weighted, sizes, 22, keyspace=40, growth=3)
+    assert g.get_degree(0) > 128 and g.get_degree(1, incoming=True) > 128
+    assert g.info()["hash_used"] > 0
+    g.profile(1)
+    before = [g.profile_read(0), g.profile_read(13)]
+    a = np.arange(1, 41, dtype=np.uint64)
+    g.update_edges(a, a + 1, np.ones(40, np.uint64), 1, True)                        # small
+    big = np.arange(1, 3001, dtype=np.uint64)
+    g.update_edges(big % 4000, (big + 7) % 4000, np.ones(3000, np.uint64), 1, True)  # large
+    after = [g.profile_read(0), g.profile_read(13)]
+    assert after[0]["launches"] - before[0]["launches"] == 1
+    assert after[0]["units"] - before[0]["units"] == 3000
+    assert after[1]["launches"] - before[1]["launches"] == 1
+    assert after[1]["units"] - before[1]["units"] == 40
