@@ -447,7 +447,7 @@ void engine_init(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool 
         // invalidation marks of earlier runs stay in place and never match a
         // fresh index; when the index wraps the marks are cleared
         h.idx += 1;
-        if (h.idx >= 0x7FFFFFF0) {
+        if (h.idx >= 0x3FFFFFF0) {   // the device's valid stamps need idx < 2^30
             h.idx = 1;
             launch_clear_marks(L->vr, (uint32_t)L->g->V, L->g->st);
         }

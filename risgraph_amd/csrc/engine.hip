@@ -278,24 +278,40 @@ __device__ __forceinline__ bool mark_inv(const EngineArgs &a, const Snap &s, uin
 // then stays in the subtree and is re-pulled: more work, same values); the
 // tuning's walk_limit, when smaller, keeps the reference's 0.1*V guard and
 // asks for a full rebuild instead.
-constexpr uint32_t RESCUE_WALK = 32;
+//
+// Walks are remembered.  Every deletion root is marked before the first
+// rescue, and parents change during the deletion levels only for marked
+// vertices (rescue), so a vertex whose ancestor chain ends at the tree root,
+// at a vertex outside the value band or at a vertex already proven valid,
+// with no mark on the way, can never be marked in this run.  Such a walk
+// stamps the vertices it passed with valid_mark(idx), and later walks stop
+// at the first stamp.  A rescued vertex takes the stamp too (its new parent
+// was proven valid).  The stamp is negative: it reads as "not marked" and
+// loses to any later run's index in mark_inv's atomicMax.
+constexpr uint32_t RESCUE_WALK = 64;
 constexpr uint32_t RESCUE_CHUNKS = 8;   // chunk items of one vertex's rescue scan
+constexpr int32_t VALID_BIT = 0x40000000;   // run indices stay below it
+
+__device__ __forceinline__ int32_t valid_mark(int32_t idx) { return -(idx | VALID_BIT); }
 
 template <int A>
 __device__ bool has_parents(const EngineArgs &a, const Snap &s, uint32_t x, int32_t idx)
 {
+    const int32_t vm = valid_mark(idx);
+    const uint32_t x0 = x;
     uint32_t steps = 0;
     for (;;) {
         const uint64_t w = a.vr[x].val;
         const int32_t iv = a.vr[x].inv;
+        if (iv == vm) break;
         if constexpr (A == SSWP) {
-            if (vval(w) > s.rbound) return false;
+            if (vval(w) > s.rbound) break;
         } else if constexpr (A != WCC) {
-            if (vval(w) < s.rbound) return false;
+            if (vval(w) < s.rbound) break;
         }
         if (marked(iv, idx)) return true;
         uint32_t p = vpar(w);
-        if (p == NILP) return false;
+        if (p == NILP) break;
         x = p;
         if (++steps > a.tune.walk_limit) {
             atomicOr(&s.c->flags, F_NEED_REBUILD);
@@ -303,6 +319,15 @@ __device__ bool has_parents(const EngineArgs &a, const Snap &s, uint32_t x, int3
         }
         if (steps > RESCUE_WALK) return true;
     }
+    // x proved the chain valid: stamp the vertices below it (their parents
+    // are fixed for the rest of the run, so this retraces the same path)
+    uint32_t y = x0;
+    for (uint32_t k = 0; k < steps && y != x; k++) {
+        const uint32_t p = vpar(a.vr[y].val);
+        a.vr[y].inv = vm;
+        y = p;
+    }
+    return false;
 }
 
 // ---------------------------------------------------------------------
@@ -335,9 +360,10 @@ __device__ void push_range(const EngineArgs &a, const Snap &s, uint32_t u, uint3
 
 // rescue (graph.hpp:1323-1361): re-parent d onto a live in-edge whose source
 // gives d's current value and lies outside the invalidated subtree.  The
-// winning lane returns true.  The mark goes idx -> -idx -> 0: once d has its
-// new parent it is a valid ancestor again for later rescues (a reader that
-// still sees -idx, or 0 with the old parent, backs off, which is safe).
+// winning lane returns true.  The mark goes idx -> -idx -> valid_mark(idx):
+// once d has its new parent it is a valid ancestor again for later rescues
+// (a reader that still sees -idx, or the stamp with the old parent, backs
+// off or walks the old chain to a mark, which is safe).
 template <int A>
 __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uint32_t dv, uint64_t off,
                              uint32_t b, uint32_t e, int lane, int stride)
@@ -352,7 +378,7 @@ __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uin
         if (has_parents<A>(a, s, n, s.idx)) continue;
         if (atomicCAS(&a.vr[d].inv, s.idx, -s.idx) == s.idx) {
             a.vr[d].val = pack(dv, n);
-            a.vr[d].inv = 0;
+            a.vr[d].inv = valid_mark(s.idx);
             return true;
         }
     }
