@@ -339,8 +339,8 @@ def traffic_probe(g, labels, stream, n0, E, args):
 
 def measured_traffic():
     """HBM bytes per store-apply launch from the committed counter summary
-    (profiles/k_apply_traffic_rmat24.md), with the batch size it was taken at."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_apply_traffic_rmat24.md")
+    (profiles/k_apply_all_traffic_rmat24.md), with the batch size it was taken at."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_apply_all_traffic_rmat24.md")
     try:
         vals = {}
         with open(path) as f:
@@ -365,7 +365,7 @@ def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch):
     res, _ = serve_batched(g, labels, stream, cur, n, rate, args)
     eb1 = g.profile_read(1)
     g.profile(1)
-    # every k_apply launch of this process (load batches included), the same
+    # every k_apply_all launch of this process (load batches included), the same
     # set of dispatches rocprofv3 --stats averages for this command
     ap = g.profile_read(0)
     launch_ms = ap["ms"] / max(ap["launches"], 1)
@@ -373,7 +373,7 @@ def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch):
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
     nb = eb1["launches"] - eb0["launches"]
     roof = {
-        "kernel": "rg::k_apply + rg::k_apply_heavy (store insert/delete batch)",
+        "kernel": "rg::k_apply_all (store insert/delete batch: light and heavy lists in one launch)",
         "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
         "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
@@ -409,7 +409,7 @@ def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch):
                   "changed_vertices_per_batch": round(changed / 3, 1),
                   "batch_gpu_ms": round(tot_ms / 3, 4), "achieved": round(ach, 3), "unit": "GB/s",
                   "frac": round(ach / HBM_PEAK_GBS, 6)}
-    log(f"roofline: k_apply {achieved:.2f} GB/s ({launch_ms * 1e3:.1f} us/launch); "
+    log(f"roofline: k_apply_all {achieved:.2f} GB/s ({launch_ms * 1e3:.1f} us/launch); "
         f"batch {ach:.2f} GB/s over {tot_ms / 3:.3f} ms/batch")
     return roof, batch_roof
 

@@ -179,7 +179,7 @@ int rg_last_gpu_ms(rg_graph *g, double *ms);
  * pass work for kind 2) accumulated since the graph was made; kind 3 + p
  * reads the grid-wide passes of engine phase p (the phase numbers of
  * rg_labels_phase_time) alone.  Kind 0 counts batches of more than 64 edge
- * ops (k_apply + k_apply_heavy); batches of at most 64 run the whole store
+ * ops (the apply kernel k_apply_all); batches of at most 64 run the whole store
  * step in one kernel, timed under mask bit 0 and read as kind 13. */
 int rg_profile_enable(rg_graph *g, int mask);
 int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint64_t *units);

@@ -12,11 +12,12 @@
 // before the owner appends to it.  The owner applies the list's inserts
 // first, then its deletes, so a batch behaves like the reference batch driver
 // (all adds, then all dels: bfs_inc_batch.cpp:142-178).
-//   k_apply        one 4-lane group per light list; ops one at a time
-//                  (lookups scan the list 8 entries per lane per step, or
-//                  probe its open-addressing index once it holds >=
-//                  INDEX_THRESHOLD entries).
-//   k_apply_heavy  one 256-thread workgroup per list for hub segments (>= 8
+// k_apply_all runs both kinds of list side by side in one launch:
+//   light lists    one 4-lane group per list; ops one at a time (lookups
+//                  scan the list 8 entries per lane per step, or probe its
+//                  open-addressing index once it holds >= INDEX_THRESHOLD
+//                  entries).
+//   heavy lists    one 256-thread workgroup per list for hub segments (>= 8
 //                  ops on an indexed list), index builds and rebuilds, and
 //                  block copies longer than 256 entries: 256 ops per round,
 //                  same-key ops of a round folded into one count update
@@ -153,8 +154,8 @@ __device__ void pool_free(const StoreArgs &a, int h, uint64_t off, uint64_t n)
 // segments: group the batch's list ops by list
 // ---------------------------------------------------------------------
 // The first op of every list reserves the list's segment and an entry in one
-// of two dense owner lists (rank-0 ops): light lists, which k_apply walks
-// with a 16-lane group each, and heavy lists, which k_apply_heavy gives a
+// of two dense owner lists (rank-0 ops): light lists, which k_apply_all
+// walks with a 4-lane group each, and heavy lists, which it gives a
 // workgroup each.  A list is heavy when its segment is a hub segment, or when
 // its inserts (at most its op count) could need an index build or rebuild or
 // a block copy longer than LIGHT_COPY entries.  A block sums its segments and
@@ -683,15 +684,16 @@ __device__ __forceinline__ void write_liveness(const StoreArgs &a, const ListSta
 
 // Light lists: one AG-lane group per list (its rank-0 op, from the light
 // owner list); k_segment already sent every list that could need an index
-// build, a rebuild, a long copy or hub rounds to the heavy list.
-__global__ void __launch_bounds__(256) k_apply(StoreArgs a)
+// build, a rebuild, a long copy or hub rounds to the heavy list.  `blk` and
+// `nblk` are this block's index among the light blocks and their count.
+__device__ void light_lists_apply(const StoreArgs &a, uint32_t blk, uint32_t nblk)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blk * blockDim.x + threadIdx.x;
     Grp g;
     g.lane = threadIdx.x % AG;
     g.base = (threadIdx.x % WAVE) - g.lane;
     const uint32_t gid = t / AG;
-    const uint32_t ng = (gridDim.x * blockDim.x) / AG;
+    const uint32_t ng = (nblk * blockDim.x) / AG;
     const uint32_t nown = (uint32_t)(a.pool->seg_pack >> 32);
     for (uint32_t oi = gid; oi < nown; oi += ng) {
         const uint32_t j = a.lops.owner[oi];
@@ -993,11 +995,21 @@ __device__ void heavy_list_apply(const StoreArgs &a, ListHdr &hd, const uint32_t
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(HB) k_apply_heavy(StoreArgs a)
+// The apply step of a batch in one launch: blocks [0, hblocks) take the
+// heavy lists, one list per block at a time; the rest walk the light owner
+// list.  The two sets of lists are disjoint, light lists already hold their
+// growth blocks, and freed blocks only go to the pending lists, so both kinds
+// run side by side (the heavy blocks come first and start first).
+static_assert(HB == 256, "light and heavy blocks share the launch's block size");
+__global__ void __launch_bounds__(HB) k_apply_all(StoreArgs a, uint32_t hblocks)
 {
     __shared__ HeavyShared sh;
+    if (blockIdx.x >= hblocks) {
+        light_lists_apply(a, blockIdx.x - hblocks, gridDim.x - hblocks);
+        return;
+    }
     const uint32_t nh = a.pool->n_heavy;
-    for (uint32_t hi = blockIdx.x; hi < nh; hi += gridDim.x) {
+    for (uint32_t hi = blockIdx.x; hi < nh; hi += hblocks) {
         const uint32_t j = a.lops.heavy[hi];   // the list's rank-0 op
         const uint32_t v = a.lops.v[j];
         const int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
@@ -1027,11 +1039,10 @@ bool launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipE
     uint64_t groups = (uint64_t)nl;
     uint64_t blocks = (groups * AG + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    if (ev_a) (void)hipEventRecord(ev_a, s);
-    hipLaunchKernelGGL(k_apply, dim3((uint32_t)blocks), dim3(256), 0, s, a);
     // heavy lists: at most one per list op; blocks beyond the queue exit at once
-    uint32_t hblocks = nl < 2048 ? nl : 2048;
-    hipLaunchKernelGGL(k_apply_heavy, dim3(hblocks), dim3(HB), 0, s, a);
+    const uint32_t hblocks = nl < 2048 ? nl : 2048;
+    if (ev_a) (void)hipEventRecord(ev_a, s);
+    hipLaunchKernelGGL(k_apply_all, dim3(hblocks + (uint32_t)blocks), dim3(HB), 0, s, a, hblocks);
     if (ev_b) (void)hipEventRecord(ev_b, s);
     return false;
 }
