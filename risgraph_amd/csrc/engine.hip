@@ -81,8 +81,9 @@ __device__ __forceinline__ void ast(int32_t *p, int32_t v)
 // appending, single-counter device atomics were the pass's critical path.)
 constexpr uint32_t SQ_NEXT = 6144, SQ_TREE = 1536, SQ_CHUNK = 1024;
 struct Stage {
-    uint32_t n_next, n_tree, n_chunk, pad;
-    uint32_t base[4];
+    uint32_t n_next, n_tree, n_chunk;
+    uint32_t rb_min, rb_max;   // value band of the deletion roots staged here
+    uint32_t base[3];
     uint32_t next[SQ_NEXT];
     uint32_t tree[SQ_TREE];
     uint64_t chunk[SQ_CHUNK];
@@ -174,7 +175,11 @@ __device__ __forceinline__ void stage_tree(const EngineArgs &a, const Snap &s, u
 
 __device__ void stage_init(Stage *st)
 {
-    if (threadIdx.x == 0) st->n_next = st->n_tree = st->n_chunk = 0;
+    if (threadIdx.x == 0) {
+        st->n_next = st->n_tree = st->n_chunk = 0;
+        st->rb_min = 0xFFFFFFFFu;
+        st->rb_max = 0;
+    }
     __syncthreads();
 }
 
@@ -189,6 +194,11 @@ __device__ void stage_flush(const EngineArgs &a, const Snap &s)
         st->base[0] = nn ? atomicAdd(&s.c->n_next, nn) : 0;
         st->base[1] = nt ? atomicAdd(&s.c->n_tree, nt) : 0;
         st->base[2] = nc ? atomicAdd(&s.c->n_chunk, nc) : 0;
+        // one band update per workgroup instead of one per deletion root
+        if (st->rb_min != 0xFFFFFFFFu) atomicMin(&s.c->rbound, st->rb_min);
+        if (st->rb_max != 0) atomicMax(&s.c->rbound, st->rb_max);
+        st->rb_min = 0xFFFFFFFFu;
+        st->rb_max = 0;
         st->n_next = nn;
         st->n_tree = nt;
         st->n_chunk = nc;
@@ -470,8 +480,8 @@ __device__ __forceinline__ void mark_root(const EngineArgs &a, const Snap &s, ui
     const uint64_t w = a.vr[dst].val;
     if (vpar(w) != src) return;
     if (mark_inv(a, s, dst)) {
-        if constexpr (A == SSWP) atomicMax(&s.c->rbound, vval(w));
-        else if constexpr (A != WCC) atomicMin(&s.c->rbound, vval(w));
+        if constexpr (A == SSWP) atomicMax(&s.stg->rb_max, vval(w));
+        else if constexpr (A != WCC) atomicMin(&s.stg->rb_min, vval(w));
         stage_next(a, s, dst);
     }
 }
