@@ -904,9 +904,13 @@ __device__ void publish(const EngineArgs &a, const Ctl *c)
 // W/ns + ns serialised same-address atomics instead of W.  Counters grow
 // monotonically within a launch, so episode e of a sub-group of size m is
 // complete when its count reaches e*m; the last arriver overall runs `last`
-// (the pass's serial step) and then releases the episode.  Waiting is
-// bounded in wall-clock time: a timeout sets F_ABORT, which every waiter of
-// the tree sees, so all waves leave the launch and the host reports an error.
+// (the pass's serial step) and then releases the episode by writing it into
+// a release word on every sub-counter's line.  Each waiter spins on its own
+// sub-group's line, so the spinning reads stay off the control block, whose
+// line the serial step and the queue counters use.  Waiting is bounded in
+// wall-clock time: a timeout sets F_ABORT, which every waiter of the tree
+// checks every 64 spins, so all waves leave the launch and the host reports
+// an error.
 constexpr uint64_t TEAM_SPIN_TICKS = 200000000ull;   // 2 s at the 100 MHz wall clock
 
 template <class F>
@@ -934,11 +938,15 @@ __device__ __forceinline__ void team_sync(Ctl *c, uint32_t *bar, uint32_t W, uin
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 last();
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                __hip_atomic_store(&c->bar_gen, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (uint32_t k = 0; k < ns; k++)
+                    __hip_atomic_store(&bar[k * BAR_STRIDE + 1], ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
+                const uint32_t *gen = &bar[sub * BAR_STRIDE + 1];
                 const uint64_t t0 = wall_clock64();
-                while ((int32_t)(ald(&c->bar_gen) - ep) < 0) {
+                uint32_t spins = 0;
+                while ((int32_t)(ald(gen) - ep) < 0) {
                     __builtin_amdgcn_s_sleep(1);
+                    if ((++spins & 63u) != 0) continue;
                     if (ald(&c->flags) & F_ABORT) break;
                     if (wall_clock64() - t0 > TEAM_SPIN_TICKS) {
                         atomicOr(&c->flags, F_ABORT);
@@ -1075,7 +1083,10 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
                 ast(&c->bar_count, 0u);
                 ast(&c->bar_gen, 0u);
                 ast(&c->bar_exit, 0u);
-                for (uint32_t k = 0; k < BAR_SUBS; k++) ast(&a.bar[k * BAR_STRIDE], 0u);
+                for (uint32_t k = 0; k < BAR_SUBS; k++) {
+                    ast(&a.bar[k * BAR_STRIDE], 0u);
+                    ast(&a.bar[k * BAR_STRIDE + 1], 0u);
+                }
                 publish(a, c);
             }
         }
