@@ -121,6 +121,35 @@ __device__ __forceinline__ void take_snap(Ctl *c, Snap &s)
     s.seq = ald(&c->seq);
 }
 
+// The same snapshot at the start of a grid-wide kernel, where the block only
+// changes between launches: a launch begins with its caches invalidated, so
+// plain loads (a few wide scalar loads per block) see the previous kernels'
+// writes, and a grid of up to 1024 blocks does not queue 17 single-word
+// reads each on the control block's line.
+__device__ __forceinline__ void take_snap_launch(Ctl *c, Snap &s)
+{
+    const Ctl l = *c;
+    s.c = c;
+    s.phase = l.phase;
+    s.stage = l.stage;
+    s.level = l.level;
+    s.n_cur = l.n_cur;
+    s.n_tree = l.n_tree;
+    s.n_chunk = l.n_chunk;
+    s.n_ops = l.n_ops;
+    s.sel_cur = l.sel_cur;
+    s.sel_next = l.sel_next;
+    s.sel_tree = l.sel_tree;
+    s.epoch = l.epoch;
+    s.mepoch = l.mepoch;
+    s.seed = l.seed;
+    s.rbound = l.rbound;
+    s.flags = l.flags;
+    s.work_max = l.work_max;
+    s.idx = l.idx;
+    s.seq = l.seq;
+}
+
 // strict-improvement CAS on the packed label word (update_label, CAS form)
 template <int A>
 __device__ __forceinline__ bool relax(uint64_t *word, uint64_t c, uint32_t parent)
@@ -1127,7 +1156,7 @@ __global__ __launch_bounds__(GRID_BLOCK) void k_grid_main(EngineArgs a)
     __shared__ Stage stg;
     stage_init(&stg);
     if (threadIdx.x == 0) {
-        take_snap(a.ctl, snap);
+        take_snap_launch(a.ctl, snap);
         snap.stg = &stg;
     }
     __syncthreads();
@@ -1145,7 +1174,7 @@ __global__ __launch_bounds__(GRID_BLOCK) void k_grid_chunks(EngineArgs a)
     __shared__ Stage stg;
     stage_init(&stg);
     if (threadIdx.x == 0) {
-        take_snap(a.ctl, snap);
+        take_snap_launch(a.ctl, snap);
         snap.stg = &stg;
     }
     __syncthreads();
