@@ -194,9 +194,7 @@ void free_ops(rg_graph *g)
 {
     dfree(g->ops.src); dfree(g->ops.dst); dfree(g->ops.w); dfree(g->ops.kind);
     dfree(g->ops.old); dfree(g->ops.live);
-    dfree(g->lops.v); dfree(g->lops.nbr); dfree(g->lops.w); dfree(g->lops.edge);
-    dfree(g->lops.delta); dfree(g->lops.flags); dfree(g->lops.rank); dfree(g->lops.segbuf);
-    dfree(g->lops.slot); dfree(g->lops.heavy); dfree(g->lops.owner);
+    dfree(g->lops.op); dfree(g->lops.segbuf); dfree(g->lops.heavy); dfree(g->lops.owner);
     dfree(g->d_flag);
     g->op_cap = 0;
 }
@@ -215,15 +213,8 @@ void ensure_ops(rg_graph *g, uint64_t n)
     g->ops.old = dalloc<uint32_t>(cap);
     g->ops.live = dalloc<uint32_t>(cap);
     uint64_t lc = 4ull * cap;
-    g->lops.v = dalloc<uint32_t>(lc);
-    g->lops.nbr = dalloc<uint32_t>(lc);
-    g->lops.w = dalloc<uint32_t>(lc);
-    g->lops.edge = dalloc<uint32_t>(lc);
-    g->lops.delta = dalloc<int8_t>(lc);
-    g->lops.flags = dalloc<uint8_t>(lc);
-    g->lops.rank = dalloc<uint32_t>(lc);
+    g->lops.op = dalloc<LOp>(lc);
     g->lops.segbuf = dalloc<uint32_t>(lc);
-    g->lops.slot = dalloc<uint32_t>(lc);
     g->lops.heavy = dalloc<uint32_t>(lc);
     g->lops.owner = dalloc<uint32_t>(lc);
     g->d_flag = dalloc<uint8_t>(cap);

@@ -129,13 +129,25 @@ constexpr uint8_t L_RET = 2;      // its old count is the edge op's return value
 constexpr uint8_t L_SLOT = 4;     // its entry's final count gives the edge op's liveness
 constexpr uint8_t L_SELF = 8;     // undirected self-loop folded into one +-2 op
 
+// One list op as a 32-byte record: the apply reaches a list's ops in
+// segment order, which is not index order, so each op's fields come from one
+// memory line instead of one line per field array.
+struct alignas(16) LOp {
+    uint32_t v;         // list vertex (NILP: unused slot)
+    uint32_t nbr, w;    // the entry's key
+    uint32_t edge;      // edge op it came from
+    uint32_t rank;      // position among the batch's ops on the same list
+    uint32_t slot;      // local index the op resolved to (NILP: none)
+    int8_t delta;
+    uint8_t flags;
+    uint16_t pad0;
+    uint32_t pad1;
+};
+static_assert(sizeof(LOp) == 32, "list op record is 32 bytes");
+
 struct ListOps {
-    uint32_t *v, *nbr, *w, *edge;
-    int8_t *delta;
-    uint8_t *flags;
-    uint32_t *rank;     // position among the batch's ops on the same list
+    LOp *op;
     uint32_t *segbuf;   // list-op indices grouped by list (segments)
-    uint32_t *slot;     // local index the op resolved to (NILP: none)
     uint32_t *owner;    // rank-0 list ops of the light lists (dense)
     uint32_t *heavy;    // rank-0 list ops of the heavy lists (dense)
     uint32_t n;

@@ -62,7 +62,7 @@ __device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank)
     a.ops.old[i] = 0;
     a.ops.live[i] = 0;
     uint32_t base = 4 * i;
-    for (int k = 0; k < 4; k++) a.lops.v[base + k] = NILP;
+    for (int k = 0; k < 4; k++) a.lops.op[base + k].v = NILP;
     if (kind & K_NOSTORE) return;
     if (s >= a.V || d >= a.V) {
         atomicOr(a.err, 1u);
@@ -73,14 +73,17 @@ __device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank)
     uint32_t nl = 0;
     auto emit = [&](uint32_t v, uint32_t nbr, int8_t delta, uint8_t flags) {
         uint32_t j = base + nl++;
-        a.lops.v[j] = v;
-        a.lops.nbr[j] = nbr;
-        a.lops.w[j] = w;
-        a.lops.edge[j] = i;
-        a.lops.delta[j] = delta;
-        a.lops.flags[j] = flags;
+        LOp o{};
+        o.v = v;
+        o.nbr = nbr;
+        o.w = w;
+        o.edge = i;
+        o.delta = delta;
+        o.flags = flags;
+        o.slot = NILP;
         int L = (flags & L_IN) ? 1 : 0;
-        if (rank) a.lops.rank[j] = atomicAdd(&a.lists[L].h[v].nops, 1u);
+        if (rank) o.rank = atomicAdd(&a.lists[L].h[v].nops, 1u);
+        a.lops.op[j] = o;
     };
     // graph.hpp:203-253: a directed op touches out[src] (+ in[dst] unless
     // symmetric); an undirected op is the directed op both ways and returns
@@ -246,9 +249,9 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
     bool grow = false;
     int L = 0;
     if (j < nlops) {
-        v = a.lops.v[j];
-        if (v != NILP && a.lops.rank[j] == 0) {
-            L = (a.lops.flags[j] & L_IN) ? 1 : 0;
+        v = a.lops.op[j].v;
+        if (v != NILP && a.lops.op[j].rank == 0) {
+            L = (a.lops.op[j].flags & L_IN) ? 1 : 0;
             const ListHdr &h = a.lists[L].h[v];
             cnt = h.nops;
             if (heavy_list(h, cnt)) {
@@ -256,7 +259,7 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
             } else {
                 lt = 1;
                 // a single delete never grows its list
-                const bool ins = cnt > 1 || a.lops.delta[j] > 0;
+                const bool ins = cnt > 1 || a.lops.op[j].delta > 0;
                 if (ins && h.len + cnt > h.cap) {
                     grow = true;
                     gcls = (uint32_t)log2_pow2(max(4u, next_pow2(h.len + cnt)));
@@ -311,10 +314,10 @@ __global__ void k_scatter(StoreArgs a)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= 4 * a.ops.n) return;
-    const uint32_t v = a.lops.v[j];
+    const uint32_t v = a.lops.op[j].v;
     if (v == NILP) return;
-    const int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
-    a.lops.segbuf[a.lists[L].h[v].seg + a.lops.rank[j]] = j;
+    const int L = (a.lops.op[j].flags & L_IN) ? 1 : 0;
+    a.lops.segbuf[a.lists[L].h[v].seg + a.lops.op[j].rank] = j;
 }
 
 // ---------------------------------------------------------------------
@@ -550,10 +553,10 @@ __device__ void apply_serial(const StoreArgs &a, ListState &st, const uint32_t *
     for (int pass = 0; pass < 2; pass++) {
         for (uint32_t i = 0; i < n; i++) {
             const uint32_t k = ops[i];
-            int8_t delta = a.lops.delta[k];
+            int8_t delta = a.lops.op[k].delta;
             if ((pass == 0) != (delta > 0)) continue;
-            uint32_t key_nbr = a.lops.nbr[k];
-            uint32_t key_w = a.slots.wt ? a.lops.w[k] : 0;
+            uint32_t key_nbr = a.lops.op[k].nbr;
+            uint32_t key_w = a.slots.wt ? a.lops.op[k].w : 0;
             uint32_t bucket = NILP;
             uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, &bucket);
             if (idx == NILP && delta > 0) {
@@ -592,8 +595,8 @@ __device__ void apply_serial(const StoreArgs &a, ListState &st, const uint32_t *
             }
             st.deg = (uint32_t)((int32_t)st.deg + dchange);
             if (g.lane == 0) {
-                if (a.lops.flags[k] & L_RET) a.ops.old[a.lops.edge[k]] = ret;
-                a.lops.slot[k] = idx;
+                if (a.lops.op[k].flags & L_RET) a.ops.old[a.lops.op[k].edge] = ret;
+                a.lops.op[k].slot = idx;
             }
             __threadfence_block();
         }
@@ -665,7 +668,7 @@ __device__ __forceinline__ void apply_single(const StoreArgs &a, ListState &st, 
         }
         if (flags & L_RET) a.ops.old[edge] = ret;
         if (flags & L_SLOT) a.ops.live[edge] = (idx != NILP && nc > 0) ? 1u : 0u;
-        a.lops.slot[k] = idx;
+        a.lops.op[k].slot = idx;
     }
 }
 
@@ -676,9 +679,9 @@ __device__ __forceinline__ void write_liveness(const StoreArgs &a, const ListSta
 {
     for (uint32_t i = t0; i < n; i += nt) {
         const uint32_t k = ops[i];
-        if (!(a.lops.flags[k] & L_SLOT)) continue;
-        const uint32_t idx = a.lops.slot[k];
-        a.ops.live[a.lops.edge[k]] = (idx != NILP && a.slots.cnt[st.off + idx] > 0) ? 1u : 0u;
+        if (!(a.lops.op[k].flags & L_SLOT)) continue;
+        const uint32_t idx = a.lops.op[k].slot;
+        a.ops.live[a.lops.op[k].edge] = (idx != NILP && a.slots.cnt[st.off + idx] > 0) ? 1u : 0u;
     }
 }
 
@@ -699,12 +702,13 @@ __device__ void light_lists_apply(const StoreArgs &a, uint32_t blk, uint32_t nbl
         const uint32_t j = a.lops.owner[oi];
         // the op's own fields are loaded with its vertex, before the header
         // arrives: most lists have this one op only
-        const uint32_t v = a.lops.v[j];
-        const uint8_t fl = a.lops.flags[j];
-        const int8_t dj = a.lops.delta[j];
-        const uint32_t key_nbr = a.lops.nbr[j];
-        const uint32_t key_w = a.slots.wt ? a.lops.w[j] : 0;
-        const uint32_t ej = a.lops.edge[j];
+        const LOp o = a.lops.op[j];
+        const uint32_t v = o.v;
+        const uint8_t fl = o.flags;
+        const int8_t dj = o.delta;
+        const uint32_t key_nbr = o.nbr;
+        const uint32_t key_w = a.slots.wt ? o.w : 0;
+        const uint32_t ej = o.edge;
         const int L = (fl & L_IN) ? 1 : 0;
         ListHdr &hd = a.lists[L].h[v];
         const uint32_t n = hd.nops;
@@ -723,7 +727,7 @@ __device__ void light_lists_apply(const StoreArgs &a, uint32_t blk, uint32_t nbl
         const uint32_t *ops = a.lops.segbuf + hd.seg;
 
         int32_t n_ins = 0;
-        for (uint32_t i = g.lane; i < n; i += AG) n_ins += a.lops.delta[ops[i]] > 0 ? 1 : 0;
+        for (uint32_t i = g.lane; i < n; i += AG) n_ins += a.lops.op[ops[i]].delta > 0 ? 1 : 0;
         const uint32_t need = st.len + (uint32_t)gsum<AG>(n_ins);
         if (light_room(a, st, need, rcap, roff, g)) {
             apply_serial(a, st, ops, n, g);
@@ -869,13 +873,13 @@ __device__ void wg_apply_rounds(const StoreArgs &a, ListState &st, const uint32_
             __syncthreads();
             const uint32_t i = r0 + tid;
             const uint32_t k = i < n ? ops[i] : 0;
-            const int8_t delta = i < n ? a.lops.delta[k] : (int8_t)0;
+            const int8_t delta = i < n ? a.lops.op[k].delta : (int8_t)0;
             const bool act = i < n && ((pass == 0) == (delta > 0));
             uint32_t key_nbr = 0, key_w = 0, ls = 0, pre = 0;
             uint64_t key = HEMPTY;
             if (act) {
-                key_nbr = a.lops.nbr[k];
-                key_w = a.slots.wt ? a.lops.w[k] : 0;
+                key_nbr = a.lops.op[k].nbr;
+                key_w = a.slots.wt ? a.lops.op[k].w : 0;
                 key = hkey_of(key_nbr, key_w);
                 ls = (uint32_t)mix64(key) & (HT - 1);
                 for (uint32_t p = 0; p < HT; p++) {   // at most HB keys in HT buckets: always ends
@@ -936,8 +940,8 @@ __device__ void wg_apply_rounds(const StoreArgs &a, ListState &st, const uint32_
                 const uint32_t idx = sh.idx[ls], c = sh.c0[ls];
                 const uint32_t cb = pass == 0 ? c + pre : (c > pre ? c - pre : 0);
                 const uint32_t ret = delta == -2 ? (cb ? cb - 1 : 0) : cb;
-                if (a.lops.flags[k] & L_RET) a.ops.old[a.lops.edge[k]] = ret;
-                a.lops.slot[k] = idx;
+                if (a.lops.op[k].flags & L_RET) a.ops.old[a.lops.op[k].edge] = ret;
+                a.lops.op[k].slot = idx;
             }
             st.len += sh.n_app;
             st.hcnt += sh.n_app;
@@ -957,7 +961,7 @@ __device__ void heavy_list_apply(const StoreArgs &a, ListHdr &hd, const uint32_t
     if (threadIdx.x == 0) sh.sum = 0;
     __syncthreads();
     int32_t my = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += HB) my += a.lops.delta[ops[i]] > 0 ? 1 : 0;
+    for (uint32_t i = threadIdx.x; i < n; i += HB) my += a.lops.op[ops[i]].delta > 0 ? 1 : 0;
     if (my) atomicAdd(&sh.sum, my);
     __syncthreads();
     const uint32_t n_ins = (uint32_t)sh.sum;
@@ -1011,8 +1015,8 @@ __global__ void __launch_bounds__(HB) k_apply_all(StoreArgs a, uint32_t hblocks)
     const uint32_t nh = a.pool->n_heavy;
     for (uint32_t hi = blockIdx.x; hi < nh; hi += hblocks) {
         const uint32_t j = a.lops.heavy[hi];   // the list's rank-0 op
-        const uint32_t v = a.lops.v[j];
-        const int L = (a.lops.flags[j] & L_IN) ? 1 : 0;
+        const uint32_t v = a.lops.op[j].v;
+        const int L = (a.lops.op[j].flags & L_IN) ? 1 : 0;
         ListHdr &hd = a.lists[L].h[v];
         heavy_list_apply(a, hd, a.lops.segbuf + hd.seg, hd.nops, sh);
     }
@@ -1206,8 +1210,8 @@ __global__ void __launch_bounds__(HB) k_store_small(StoreArgs a)
     // list key (L << 31 | v) of every list op; NILP for unused slots
     uint32_t kt = NILP;
     if (tid < nl) {
-        const uint32_t v = a.lops.v[tid];
-        if (v != NILP) kt = v | ((a.lops.flags[tid] & L_IN) ? 0x80000000u : 0u);
+        const uint32_t v = a.lops.op[tid].v;
+        if (v != NILP) kt = v | ((a.lops.op[tid].flags & L_IN) ? 0x80000000u : 0u);
     }
     if (tid < SL) key[tid] = kt;
     if (kt != NILP) atomicAdd(&nvalid, 1u);
@@ -1247,8 +1251,8 @@ __global__ void __launch_bounds__(HB) k_store_small(StoreArgs a)
         for (uint32_t sg = tid / AG; sg < nseg; sg += HB / AG) {
             const uint32_t b = sstart[sg], cnt = sstart[sg + 1] - b;
             const uint32_t j0 = order[b];
-            const uint32_t v = a.lops.v[j0];
-            const int L = (a.lops.flags[j0] & L_IN) ? 1 : 0;
+            const uint32_t v = a.lops.op[j0].v;
+            const int L = (a.lops.op[j0].flags & L_IN) ? 1 : 0;
             ListHdr &hd = a.lists[L].h[v];
             if (heavy_list(hd, cnt)) {
                 if (g.lane == 0) hseg[atomicAdd(&nheavy, 1u)] = sg;
@@ -1257,7 +1261,7 @@ __global__ void __launch_bounds__(HB) k_store_small(StoreArgs a)
             ListState st = load_state(hd);
             const uint32_t *ops = a.lops.segbuf + b;
             int32_t n_ins = 0;
-            for (uint32_t i = g.lane; i < cnt; i += AG) n_ins += a.lops.delta[ops[i]] > 0 ? 1 : 0;
+            for (uint32_t i = g.lane; i < cnt; i += AG) n_ins += a.lops.op[ops[i]].delta > 0 ? 1 : 0;
             const uint32_t need = st.len + (uint32_t)gsum<AG>(n_ins);
             if (light_room(a, st, need, 0u, 0ull, g)) {
                 apply_serial(a, st, ops, cnt, g);
@@ -1274,7 +1278,7 @@ __global__ void __launch_bounds__(HB) k_store_small(StoreArgs a)
         const uint32_t sg = hseg[q];
         const uint32_t b = sstart[sg], cnt = sstart[sg + 1] - b;
         const uint32_t j0 = order[b];
-        ListHdr &hd = a.lists[(a.lops.flags[j0] & L_IN) ? 1 : 0].h[a.lops.v[j0]];
+        ListHdr &hd = a.lists[(a.lops.op[j0].flags & L_IN) ? 1 : 0].h[a.lops.op[j0].v];
         heavy_list_apply(a, hd, a.lops.segbuf + b, cnt, sh);
     }
     __syncthreads();
