@@ -109,7 +109,8 @@ struct rg_graph {
     uint32_t *cnt_tmp[2]{};           // bulk reservation counters
     uint32_t *h_status = nullptr;     // pinned: err, oom
     // pinned staging for host-side batches
-    uint32_t *h_src = nullptr, *h_dst = nullptr, *h_w = nullptr, *h_old = nullptr;
+    uint32_t *h_src = nullptr, *h_dst = nullptr, *h_w = nullptr;
+    OpResult *h_res = nullptr;
     uint8_t *h_kind = nullptr, *h_flag = nullptr;
     uint8_t *d_flag = nullptr;
     uint32_t h_cap = 0;
@@ -193,7 +194,7 @@ void copy_degrees(const List &l, uint32_t *out, uint64_t V, hipStream_t st)
 void free_ops(rg_graph *g)
 {
     dfree(g->ops.src); dfree(g->ops.dst); dfree(g->ops.w); dfree(g->ops.kind);
-    dfree(g->ops.old); dfree(g->ops.live);
+    dfree(g->ops.res);
     dfree(g->lops.op); dfree(g->lops.segbuf); dfree(g->lops.heavy); dfree(g->lops.owner);
     dfree(g->d_flag);
     g->op_cap = 0;
@@ -210,8 +211,7 @@ void ensure_ops(rg_graph *g, uint64_t n)
     g->ops.dst = dalloc<uint32_t>(cap);
     g->ops.w = dalloc<uint32_t>(cap);
     g->ops.kind = dalloc<uint8_t>(cap);
-    g->ops.old = dalloc<uint32_t>(cap);
-    g->ops.live = dalloc<uint32_t>(cap);
+    g->ops.res = dalloc<OpResult>(cap);
     uint64_t lc = 4ull * cap;
     g->lops.op = dalloc<LOp>(lc);
     g->lops.segbuf = dalloc<uint32_t>(lc);
@@ -225,11 +225,11 @@ void ensure_host(rg_graph *g, uint64_t n)
 {
     if (n <= g->h_cap) return;
     uint32_t cap = std::max<uint32_t>((uint32_t)n, 1024);
-    hfree(g->h_src); hfree(g->h_dst); hfree(g->h_w); hfree(g->h_old); hfree(g->h_kind); hfree(g->h_flag);
+    hfree(g->h_src); hfree(g->h_dst); hfree(g->h_w); hfree(g->h_res); hfree(g->h_kind); hfree(g->h_flag);
     g->h_src = halloc<uint32_t>(cap);
     g->h_dst = halloc<uint32_t>(cap);
     g->h_w = halloc<uint32_t>(cap);
-    g->h_old = halloc<uint32_t>(cap);
+    g->h_res = halloc<OpResult>(cap);
     g->h_kind = halloc<uint8_t>(cap);
     g->h_flag = halloc<uint8_t>(cap);
     g->h_cap = cap;
@@ -703,7 +703,7 @@ int rg_graph_destroy(rg_graph *g)
     dfree(g->pool); dfree(g->err);
     dfree(g->free_off); dfree(g->pend_off); dfree(g->cnt_tmp[0]); dfree(g->cnt_tmp[1]);
     free_ops(g);
-    hfree(g->h_src); hfree(g->h_dst); hfree(g->h_w); hfree(g->h_old); hfree(g->h_kind); hfree(g->h_flag);
+    hfree(g->h_src); hfree(g->h_dst); hfree(g->h_w); hfree(g->h_res); hfree(g->h_kind); hfree(g->h_flag);
     hfree(g->h_status);
     for (auto &e : g->ev) if (e) (void)hipEventDestroy(e);
     for (auto &r : g->prof_pending) {
@@ -743,11 +743,11 @@ int rg_edge_update_batch(rg_graph *g, const rg_edge *edges, uint64_t n, int delt
         uint8_t kind = (delta > 0 ? K_INS : K_DEL) | (directed ? 0 : K_UNDIR);
         upload_edges(g, edges, n, kind);
         run_store(g);
-        HC(hipMemcpyAsync(g->h_old, g->ops.old, n * 4, hipMemcpyDeviceToHost, g->st));
+        HC(hipMemcpyAsync(g->h_res, g->ops.res, n * sizeof(OpResult), hipMemcpyDeviceToHost, g->st));
         HC(hipStreamSynchronize(g->st));
         store_status_check(g);
         if (old_out)
-            for (uint64_t i = 0; i < n; i++) old_out[i] = g->h_old[i];
+            for (uint64_t i = 0; i < n; i++) old_out[i] = g->h_res[i].old;
     });
 }
 

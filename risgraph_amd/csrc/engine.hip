@@ -534,14 +534,15 @@ __device__ void pass_ops(const EngineArgs &a, const Snap &s, uint32_t tid, uint3
             if (!(k & K_INS)) continue;
             // an insert reaches the engine when it created a live entry
             // (bfs_inc.cpp:162-165: add_edge returned 0 and the batch left it)
-            const bool seed = (k & K_FORCE) || (a.ops.old[i] == 0 && a.ops.live[i]);
+            const OpResult r = a.ops.res[i];
+            const bool seed = (k & K_FORCE) || (r.old == 0 && r.live);
             if (!seed) continue;
             if (pull && (a.vr[src].inv == s.idx || a.vr[dst].inv == s.idx)) continue;
             seed_edge<A>(a, s, src, dst, w);
             if (both) seed_edge<A>(a, s, dst, src, w);
         } else {
             if (!(k & K_DEL)) continue;
-            if (!((k & K_FORCE) || a.ops.old[i] == 1)) continue;
+            if (!((k & K_FORCE) || a.ops.res[i].old == 1)) continue;
             mark_root<A>(a, s, src, dst);
             if (both) mark_root<A>(a, s, dst, src);
         }

@@ -115,11 +115,17 @@ constexpr uint8_t K_FORCE = 4;    // engine-only edge from update_tree_add/del
 constexpr uint8_t K_UNDIR = 8;    // undirected op (both directions)
 constexpr uint8_t K_NOSTORE = 16; // do not touch the store
 
+// What the store reports for one edge op; the list owner that writes the
+// return value usually writes the liveness too, so both share a word pair.
+struct OpResult {
+    uint32_t old;       // reference add/del return value
+    uint32_t live;      // the (src->dst) entry of src's out-list is live after the batch
+};
+
 struct EdgeOps {
     uint32_t *src, *dst, *w;
     uint8_t *kind;
-    uint32_t *old;      // reference add/del return value
-    uint32_t *live;     // the (src->dst) entry of src's out-list is live after the batch
+    OpResult *res;
     uint32_t n;
 };
 

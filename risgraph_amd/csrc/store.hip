@@ -59,8 +59,7 @@ __device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank)
 {
     uint32_t s = a.ops.src[i], d = a.ops.dst[i], w = a.weighted ? a.ops.w[i] : 0;
     uint8_t kind = a.ops.kind[i];
-    a.ops.old[i] = 0;
-    a.ops.live[i] = 0;
+    a.ops.res[i] = OpResult{0u, 0u};
     uint32_t base = 4 * i;
     for (int k = 0; k < 4; k++) a.lops.op[base + k].v = NILP;
     if (kind & K_NOSTORE) return;
@@ -595,7 +594,7 @@ __device__ void apply_serial(const StoreArgs &a, ListState &st, const uint32_t *
             }
             st.deg = (uint32_t)((int32_t)st.deg + dchange);
             if (g.lane == 0) {
-                if (a.lops.op[k].flags & L_RET) a.ops.old[a.lops.op[k].edge] = ret;
+                if (a.lops.op[k].flags & L_RET) a.ops.res[a.lops.op[k].edge].old = ret;
                 a.lops.op[k].slot = idx;
             }
             __threadfence_block();
@@ -666,8 +665,13 @@ __device__ __forceinline__ void apply_single(const StoreArgs &a, ListState &st, 
             a.slots.cnt[st.off + idx] = nc;
             if ((c == 0) != (nc == 0)) a.slots.nbr[st.off + idx] = key_nbr | (nc ? 0u : DEAD);
         }
-        if (flags & L_RET) a.ops.old[edge] = ret;
-        if (flags & L_SLOT) a.ops.live[edge] = (idx != NILP && nc > 0) ? 1u : 0u;
+        const uint32_t live = (idx != NILP && nc > 0) ? 1u : 0u;
+        if ((flags & (L_RET | L_SLOT)) == (L_RET | L_SLOT)) {
+            a.ops.res[edge] = OpResult{ret, live};
+        } else {
+            if (flags & L_RET) a.ops.res[edge].old = ret;
+            if (flags & L_SLOT) a.ops.res[edge].live = live;
+        }
         a.lops.op[k].slot = idx;
     }
 }
@@ -681,7 +685,7 @@ __device__ __forceinline__ void write_liveness(const StoreArgs &a, const ListSta
         const uint32_t k = ops[i];
         if (!(a.lops.op[k].flags & L_SLOT)) continue;
         const uint32_t idx = a.lops.op[k].slot;
-        a.ops.live[a.lops.op[k].edge] = (idx != NILP && a.slots.cnt[st.off + idx] > 0) ? 1u : 0u;
+        a.ops.res[a.lops.op[k].edge].live = (idx != NILP && a.slots.cnt[st.off + idx] > 0) ? 1u : 0u;
     }
 }
 
@@ -940,7 +944,7 @@ __device__ void wg_apply_rounds(const StoreArgs &a, ListState &st, const uint32_
                 const uint32_t idx = sh.idx[ls], c = sh.c0[ls];
                 const uint32_t cb = pass == 0 ? c + pre : (c > pre ? c - pre : 0);
                 const uint32_t ret = delta == -2 ? (cb ? cb - 1 : 0) : cb;
-                if (a.lops.op[k].flags & L_RET) a.ops.old[a.lops.op[k].edge] = ret;
+                if (a.lops.op[k].flags & L_RET) a.ops.res[a.lops.op[k].edge].old = ret;
                 a.lops.op[k].slot = idx;
             }
             st.len += sh.n_app;
