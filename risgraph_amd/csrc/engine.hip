@@ -18,9 +18,9 @@
 // p marks v, and the rescue step re-parents v onto a surviving parallel edge.
 //
 // Work is level-synchronous over vertex queues.  A vertex pass gives each
-// queued vertex a 16-lane group; lists longer than 32 entries are split into
-// 128-entry chunk items handled a wave each in the pass's chunk stage, so hub
-// vertices spread over many CUs.  All trees of a batch repair in one
+// queued vertex a 16-lane group (4 lanes in grid-wide passes); lists longer
+// than 32 entries are split into 128-entry chunk items handled a wave each in
+// the pass's chunk stage, so hub vertices spread over many CUs.  All trees of a batch repair in one
 // persistent launch (the team kernel): each tree gets W workgroups that
 // separate passes with a device barrier, and a pass too large for the team
 // leaves it for grid-wide launches (one per pass); the phase state machine
@@ -370,7 +370,7 @@ __device__ bool has_parents(const EngineArgs &a, const Snap &s, uint32_t x, int3
 }
 
 // ---------------------------------------------------------------------
-// per-entry work shared by vertex groups (16 lanes) and chunk waves (64)
+// per-entry work shared by vertex groups (16 or 4 lanes) and chunk waves (64)
 // ---------------------------------------------------------------------
 template <int A>
 __device__ __forceinline__ void entry_w(const EngineArgs &a, uint64_t slot, uint32_t &w)

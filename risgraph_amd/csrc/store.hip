@@ -320,7 +320,8 @@ __global__ void k_scatter(StoreArgs a)
 }
 
 // ---------------------------------------------------------------------
-// apply: one 16-lane group owns one list (its rank-0 op)
+// apply: one group owns one list (its rank-0 op): AG lanes for a light
+// list, a whole workgroup for a heavy one
 // ---------------------------------------------------------------------
 struct ListState {
     uint64_t off, hoff;
