@@ -145,7 +145,7 @@ struct rg_labels {
     uint32_t *init = nullptr;
     uint32_t *mstamp = nullptr;
     uint32_t *q[3]{};
-    uint64_t *cq = nullptr;
+    ChunkItem *cq = nullptr;
     uint32_t cq_cap = 0;
     uint32_t *mlist = nullptr;
     Ctl *ctl = nullptr;
@@ -821,7 +821,7 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         for (auto &q : L->q) q = dalloc<uint32_t>(V);
         L->cq_cap = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull,
                                                   g->slot_cap / CHUNK + 2 * std::min<uint64_t>(V, g->slot_cap / LONG_LIST) + 1024);
-        L->cq = dalloc<uint64_t>(L->cq_cap);
+        L->cq = dalloc<ChunkItem>(L->cq_cap);
         L->mlist = dalloc<uint32_t>(V);
         L->ctl = dalloc<Ctl>(1);
         L->ticks = dalloc<unsigned long long>(NTICK * NPHASE);

@@ -79,14 +79,14 @@ __device__ __forceinline__ void ast(int32_t *p, int32_t v)
 // global queue with one counter atomic and copies the buffer out.  A full
 // buffer spills straight to the global queue.  (With thousands of waves
 // appending, single-counter device atomics were the pass's critical path.)
-constexpr uint32_t SQ_NEXT = 6144, SQ_TREE = 1536, SQ_CHUNK = 1024;
+constexpr uint32_t SQ_NEXT = 6144, SQ_TREE = 1536, SQ_CHUNK = 512;
 struct Stage {
     uint32_t n_next, n_tree, n_chunk;
     uint32_t rb_min, rb_max;   // value band of the deletion roots staged here
     uint32_t base[3];
     uint32_t next[SQ_NEXT];
     uint32_t tree[SQ_TREE];
-    uint64_t chunk[SQ_CHUNK];
+    ChunkItem chunk[SQ_CHUNK];
 };
 
 // snapshot of the control block taken at a pass boundary
@@ -287,18 +287,24 @@ __device__ __forceinline__ void mark_mod(const EngineArgs &a, const Snap &s, uin
     }
 }
 
-// chunk items: vertex | list << 32 | op << 33 | chunk << 35
 enum ChunkOp : uint32_t { CO_PUSH = 0, CO_RESCUE = 1, CO_EXPAND = 2, CO_PULL = 3 };
 
-__device__ __forceinline__ void defer_chunks(const EngineArgs &a, const Snap &s, uint32_t v, int L, uint32_t len,
-                                             ChunkOp op)
+// the first `len` entries of v's list at slot `off`, as chunk items
+__device__ __forceinline__ void defer_chunks(const EngineArgs &a, const Snap &s, uint32_t v, uint64_t off,
+                                             uint32_t len, ChunkOp op)
 {
     const uint32_t nch = (len + CHUNK - 1) / CHUNK;
-    const uint64_t item = (uint64_t)v | ((uint64_t)L << 32) | ((uint64_t)op << 33);
+    auto item = [&](uint32_t k) {
+        ChunkItem it;
+        it.off = off + (uint64_t)k * CHUNK;
+        it.v = v;
+        it.n_op = min(CHUNK, len - k * CHUNK) | ((uint32_t)op << 16);
+        return it;
+    };
     // the items that fit go to the stage, the rest straight to the queue
     const uint32_t sp = atomicAdd(&s.stg->n_chunk, nch);
     const uint32_t fit = sp < SQ_CHUNK ? min(nch, SQ_CHUNK - sp) : 0;
-    for (uint32_t k = 0; k < fit; k++) s.stg->chunk[sp + k] = item | ((uint64_t)k << 35);
+    for (uint32_t k = 0; k < fit; k++) s.stg->chunk[sp + k] = item(k);
     if (fit == nch) return;
     const uint32_t rest = nch - fit;
     const uint32_t pos = atomicAdd(&s.c->n_chunk, rest);
@@ -306,7 +312,7 @@ __device__ __forceinline__ void defer_chunks(const EngineArgs &a, const Snap &s,
         atomicOr(&s.c->flags, F_QUEUE_OVERFLOW);
         return;
     }
-    for (uint32_t k = fit; k < nch; k++) a.cq[pos + k - fit] = item | ((uint64_t)k << 35);
+    for (uint32_t k = fit; k < nch; k++) a.cq[pos + k - fit] = item(k);
 }
 
 // Invalidation marks (graph.hpp:1729) carry the deletion run's index: idx =
@@ -655,7 +661,7 @@ __device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uin
                 // the scan looks for any one valid parent: a bounded prefix
                 // of a hub's in-list is searched, the vertex otherwise stays
                 // invalid (re-pulled later, same values)
-                if (glane == 0) defer_chunks(a, s, v, L, min(len, RESCUE_CHUNKS * CHUNK), CO_RESCUE);
+                if (glane == 0) defer_chunks(a, s, v, off, min(len, RESCUE_CHUNKS * CHUNK), CO_RESCUE);
                 deferred = true;
                 continue;
             }
@@ -676,7 +682,7 @@ __device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uin
         uint32_t len;
         list_span(a.lists[L], v, off, len);
         if (len > LONG_LIST) {
-            if (glane == 0) defer_chunks(a, s, v, L, len, CO_EXPAND);
+            if (glane == 0) defer_chunks(a, s, v, off, len, CO_EXPAND);
             continue;
         }
         expand_cont<G, CS>(a, s, v, off, len, glane, st, depth);
@@ -711,7 +717,7 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
                 uint32_t len;
                 list_span(a.lists[L], v, off, len);
                 if (len > LONG_LIST) {
-                    if (glane == 0) defer_chunks(a, s, v, L, len, CO_PUSH);
+                    if (glane == 0) defer_chunks(a, s, v, off, len, CO_PUSH);
                     continue;
                 }
                 push_range<A>(a, s, v, uv, off, 0, len, glane, G);
@@ -739,7 +745,7 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
                 uint32_t len;
                 list_span(a.lists[L], v, off, len);
                 if (len > LONG_LIST) {
-                    if (glane == 0) defer_chunks(a, s, v, L, len, CO_PULL);
+                    if (glane == 0) defer_chunks(a, s, v, off, len, CO_PULL);
                     continue;
                 }
                 uint64_t b = pull_range<A>(a, s, off, 0, len, glane, G);
@@ -783,15 +789,10 @@ __device__ void pass_chunks(const EngineArgs &a, const Snap &s, uint32_t tid, ui
     const int lane = tid % WAVE;
     const uint32_t wid = tid / WAVE, nw = nthr / WAVE;
     for (uint32_t ci = wid; ci < n_chunk; ci += nw) {
-        uint64_t item = a.cq[ci];
-        uint32_t v = (uint32_t)item;
-        int L = (int)((item >> 32) & 1);
-        uint32_t op = (uint32_t)((item >> 33) & 3);
-        uint32_t k = (uint32_t)(item >> 35);
-        uint64_t off;
-        uint32_t len;
-        list_span(a.lists[L], v, off, len);
-        uint32_t b = k * CHUNK, e = b + CHUNK < len ? b + CHUNK : len;
+        const ChunkItem it = a.cq[ci];
+        const uint32_t v = it.v, op = it.n_op >> 16;
+        const uint64_t off = it.off;
+        const uint32_t b = 0, e = it.n_op & 0xFFFFu;
         if (op == CO_PUSH) {
             push_range<A>(a, s, v, vval(a.vr[v].val), off, b, e, lane, WAVE);
         } else if (op == CO_RESCUE) {

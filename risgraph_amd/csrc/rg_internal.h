@@ -232,6 +232,15 @@ struct Tuning {
     uint32_t walk_limit;          // has_parents step cap (graph.hpp:1900)
 };
 
+// One chunk of a long list for a pass's chunk stage.  It carries where its
+// entries start, so the wave that takes it goes straight to them instead of
+// first loading the list's header.
+struct alignas(16) ChunkItem {
+    uint64_t off;    // first slot of the chunk
+    uint32_t v;      // the list's vertex
+    uint32_t n_op;   // entries (1..CHUNK) | op << 16
+};
+
 struct EngineArgs {
     uint32_t V;
     int algo;
@@ -247,7 +256,7 @@ struct EngineArgs {
     uint32_t root;
     uint32_t *mstamp;
     uint32_t *q[3];
-    uint64_t *cq;
+    ChunkItem *cq;
     uint32_t cq_cap;
     uint32_t *mlist;
     Ctl *ctl;
