@@ -146,6 +146,7 @@ struct rg_labels {
     uint32_t *mstamp = nullptr;
     uint32_t *q[3]{};
     ChunkItem *cq = nullptr;
+    uint32_t *bnd = nullptr;             // SSWP deletion bounds (device)
     uint32_t cq_cap = 0;
     uint32_t *mlist = nullptr;
     Ctl *ctl = nullptr;
@@ -376,6 +377,7 @@ EngineArgs engine_args(rg_labels *L)
     a.mstamp = L->mstamp;
     a.q[0] = L->q[0]; a.q[1] = L->q[1]; a.q[2] = L->q[2];
     a.cq = L->cq;
+    a.bnd = L->bnd;
     a.cq_cap = L->cq_cap;
     a.mlist = L->mlist;
     a.ctl = L->ctl;
@@ -624,7 +626,7 @@ void engine_run(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool n
 void labels_free(rg_labels *L)
 {
     dfree(L->vr); dfree(L->init); dfree(L->mstamp);
-    dfree(L->q[0]); dfree(L->q[1]); dfree(L->q[2]); dfree(L->cq); dfree(L->mlist); dfree(L->ctl);
+    dfree(L->q[0]); dfree(L->q[1]); dfree(L->q[2]); dfree(L->cq); dfree(L->bnd); dfree(L->mlist); dfree(L->ctl);
     dfree(L->init_active);
     dfree(L->ticks);
     dfree(L->bar);
@@ -822,6 +824,7 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->cq_cap = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull,
                                                   g->slot_cap / CHUNK + 2 * std::min<uint64_t>(V, g->slot_cap / LONG_LIST) + 1024);
         L->cq = dalloc<ChunkItem>(L->cq_cap);
+        if (algo == RG_SSWP) L->bnd = dalloc<uint32_t>(V);
         L->mlist = dalloc<uint32_t>(V);
         L->ctl = dalloc<Ctl>(1);
         L->ticks = dalloc<unsigned long long>(NTICK * NPHASE);

@@ -427,9 +427,14 @@ __device__ void push_range(const EngineArgs &a, const Snap &s, uint32_t u, uint3
 // once d has its new parent it is a valid ancestor again for later rescues
 // (a reader that still sees -idx, or the stamp with the old parent, backs
 // off or walks the old chain to a mark, which is safe).
+//
+// SSWP also collects in `tb` the best value a valid in-neighbour offers below
+// d's value: d's new value is at least that (the neighbour's tree path
+// survives the run), which lets the expand leave alone the children whose
+// value does not exceed it.
 template <int A>
 __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uint32_t dv, uint64_t off,
-                             uint32_t b, uint32_t e, int lane, int stride)
+                             uint32_t b, uint32_t e, int lane, int stride, uint32_t &tb)
 {
     for (uint32_t i = b + lane; i < e; i += stride) {
         if (a.vr[d].inv != s.idx) break;
@@ -437,7 +442,13 @@ __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uin
         if (n & DEAD) continue;
         uint32_t w;
         entry_w<A>(a, off + i, w);
-        if (cand<A>(vval(a.vr[n].val), w) != (uint64_t)dv) continue;
+        const uint64_t c = cand<A>(vval(a.vr[n].val), w);
+        if (c != (uint64_t)dv) {
+            if constexpr (A == SSWP) {
+                if (c > tb && c < dv && !has_parents<A>(a, s, n, s.idx)) tb = (uint32_t)c;
+            }
+            continue;
+        }
         if (has_parents<A>(a, s, n, s.idx)) continue;
         if (atomicCAS(&a.vr[d].inv, s.idx, -s.idx) == s.idx) {
             a.vr[d].val = pack(dv, n);
@@ -449,15 +460,35 @@ __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uin
 }
 
 // children of x through live out-edges whose target still names x as parent
-// join the next level (graph.hpp:1382-1417)
+// join the next level (graph.hpp:1382-1417).
+//
+// SSWP: x's new value is at least bx (its subtree root's bound).  A child c
+// with val[c] <= bx < val[x] has val[c] = w(x, c), so min(new val[x], w(x, c))
+// still equals val[c]: c keeps its value and its parent, and so does its whole
+// subtree (values only shrink along a path).  Such children are not marked;
+// the others inherit bx.
+template <int A>
+__device__ __forceinline__ bool expand_child(const EngineArgs &a, const Snap &s, uint32_t x, uint32_t c,
+                                             uint32_t bx)
+{
+    const uint64_t cv = a.vr[c].val;
+    if (vpar(cv) != x) return false;
+    if constexpr (A == SSWP) {
+        if (vval(cv) <= bx) return false;
+    }
+    if (!mark_inv(a, s, c)) return false;
+    if constexpr (A == SSWP) a.bnd[c] = bx;
+    return true;
+}
+
+template <int A>
 __device__ void expand_range(const EngineArgs &a, const Snap &s, uint32_t x, uint64_t off, uint32_t b,
-                             uint32_t e, int lane, int stride)
+                             uint32_t e, int lane, int stride, uint32_t bx)
 {
     for (uint32_t i = b + lane; i < e; i += stride) {
         uint32_t c = a.slots.nbr[off + i];
         if (c & DEAD) continue;
-        if (vpar(a.vr[c].val) != x) continue;
-        if (mark_inv(a, s, c)) stage_next(a, s, c);
+        if (expand_child<A>(a, s, x, c, bx)) stage_next(a, s, c);
     }
 }
 
@@ -533,6 +564,7 @@ __device__ __forceinline__ void mark_root(const EngineArgs &a, const Snap &s, ui
     const uint64_t w = a.vr[dst].val;
     if (vpar(w) != src) return;
     if (mark_inv(a, s, dst)) {
+        if constexpr (A == SSWP) a.bnd[dst] = 0;   // no bound known yet
         if constexpr (A == SSWP) atomicMax(&s.stg->rb_max, vval(w));
         else if constexpr (A != WCC) atomicMin(&s.stg->rb_min, vval(w));
         stage_next(a, s, dst);
@@ -615,10 +647,13 @@ __device__ __forceinline__ uint32_t group_ballot(bool p)
     return (uint32_t)((b >> (__lane_id() & ~(uint32_t)(G - 1))) & ((1ull << G) - 1));
 }
 
-// expand x's short list with the group's G lanes stepping through it together
-template <int G, uint32_t CS>
+// expand x's short list with the group's G lanes stepping through it
+// together; a stacked child keeps its depth in the low 3 bits of its stack
+// word and its SSWP bound above them
+static_assert(CONT_DEPTH < 8, "depth fits 3 bits");
+template <int A, int G, uint32_t CS>
 __device__ void expand_cont(const EngineArgs &a, const Snap &s, uint32_t x, uint64_t off, uint32_t len,
-                            int glane, GStack &st, uint32_t depth)
+                            int glane, GStack &st, uint32_t depth, uint32_t bx)
 {
     for (uint32_t b = 0; b < len; b += G) {
         const uint32_t i = b + glane;
@@ -626,7 +661,7 @@ __device__ void expand_cont(const EngineArgs &a, const Snap &s, uint32_t x, uint
         uint32_t c = 0;
         if (i < len) {
             c = a.slots.nbr[off + i];
-            if (!(c & DEAD) && vpar(a.vr[c].val) == x) mk = mark_inv(a, s, c);
+            if (!(c & DEAD)) mk = expand_child<A>(a, s, x, c, bx);
         }
         const bool keep = mk && depth < CONT_DEPTH;
         const uint32_t kb = group_ballot<G>(keep);
@@ -634,7 +669,7 @@ __device__ void expand_cont(const EngineArgs &a, const Snap &s, uint32_t x, uint
         const bool fits = keep && st.top + rank < CS;
         if (fits) {
             st.v[st.top + rank] = c;
-            st.d[st.top + rank] = depth + 1;
+            st.d[st.top + rank] = (depth + 1) | (A == SSWP ? bx << 3 : 0u);
         }
         if (mk && !fits) stage_next(a, s, c);
         st.top = min(CS, st.top + (uint32_t)__popc(kb));
@@ -643,15 +678,25 @@ __device__ void expand_cont(const EngineArgs &a, const Snap &s, uint32_t x, uint
 
 // one vertex of a deletion level: try to rescue it, else keep it invalid,
 // record it for the pull, and mark its children
+// `bv` is v's SSWP bound when the caller knows it (a stacked child), else
+// NO_BOUND: read it from the bound array.
+constexpr uint32_t NO_BOUND = 0xFFFFFFFFu;
+
 template <int A, int G, uint32_t CS>
 __device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uint32_t depth, int glane,
-                           GStack &st)
+                           GStack &st, uint32_t bv)
 {
     const uint32_t v = raw & ~Q_EXPAND_ONLY;
     if (a.vr[v].inv != s.idx) return;   // rescued
+    if constexpr (A == SSWP) {
+        if (bv == NO_BOUND) bv = a.bnd[v];
+    } else {
+        bv = 0;
+    }
     if (s.phase == PH_DEL_RESCUE && !(raw & Q_EXPAND_ONLY)) {
         const uint32_t dv = vval(a.vr[v].val);
         bool won = false, deferred = false;
+        uint32_t tb = bv;
         for (int li = 0; li < a.npull; li++) {
             int L = a.pull_l[li];
             uint64_t off;
@@ -665,9 +710,16 @@ __device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uin
                 deferred = true;
                 continue;
             }
-            won |= rescue_range<A>(a, s, v, dv, off, 0, len, glane, G);
+            won |= rescue_range<A>(a, s, v, dv, off, 0, len, glane, G, tb);
         }
         if (group_any<G>(won)) return;
+        if constexpr (A == SSWP) {
+            // the group's best valid offer is a bound on v's new value; the
+            // deferred scan of a long in-list may still raise it
+            for (int m = G / 2; m > 0; m >>= 1) tb = max(tb, (uint32_t)__shfl_xor((int)tb, m, G));
+            bv = tb;
+            if (glane == 0) a.bnd[v] = bv;
+        }
         if (deferred) {
             // the long in-list is scanned in this pass's chunk stage; v is
             // expanded at the next level unless that rescues it
@@ -685,7 +737,7 @@ __device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uin
             if (glane == 0) defer_chunks(a, s, v, off, len, CO_EXPAND);
             continue;
         }
-        expand_cont<G, CS>(a, s, v, off, len, glane, st, depth);
+        expand_cont<A, G, CS>(a, s, v, off, len, glane, st, depth, bv);
     }
 }
 
@@ -729,11 +781,11 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
             st.v = stk + (threadIdx.x / G) * 2 * CS;
             st.d = st.v + CS;
             st.top = 0;
-            del_vertex<A, G, CS>(a, s, raw, 0, glane, st);
+            del_vertex<A, G, CS>(a, s, raw, 0, glane, st, NO_BOUND);
             while (st.top) {
                 st.top--;
                 const uint32_t c = st.v[st.top], d = st.d[st.top];
-                del_vertex<A, G, CS>(a, s, c, d, glane, st);
+                del_vertex<A, G, CS>(a, s, c, d & 7u, glane, st, d >> 3);
             }
         } else if (ph == PH_DEL_PULL) {
             // reset v and pull its best candidate from in-neighbours outside
@@ -797,9 +849,14 @@ __device__ void pass_chunks(const EngineArgs &a, const Snap &s, uint32_t tid, ui
             push_range<A>(a, s, v, vval(a.vr[v].val), off, b, e, lane, WAVE);
         } else if (op == CO_RESCUE) {
             if (a.vr[v].inv != s.idx) continue;
-            rescue_range<A>(a, s, v, vval(a.vr[v].val), off, b, e, lane, WAVE);
+            uint32_t tb = 0;
+            rescue_range<A>(a, s, v, vval(a.vr[v].val), off, b, e, lane, WAVE, tb);
+            if constexpr (A == SSWP) {
+                for (int m = WAVE / 2; m > 0; m >>= 1) tb = max(tb, (uint32_t)__shfl_xor((int)tb, m, WAVE));
+                if (lane == 0 && tb) atomicMax(&a.bnd[v], tb);
+            }
         } else if (op == CO_EXPAND) {
-            expand_range(a, s, v, off, b, e, lane, WAVE);
+            expand_range<A>(a, s, v, off, b, e, lane, WAVE, A == SSWP ? a.bnd[v] : 0u);
         } else {
             uint64_t best = pull_range<A>(a, s, off, b, e, lane, WAVE);
             best = reduce_best<A>(best, WAVE);

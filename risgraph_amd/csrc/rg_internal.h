@@ -256,6 +256,7 @@ struct EngineArgs {
     uint32_t root;
     uint32_t *mstamp;
     uint32_t *q[3];
+    uint32_t *bnd;                // SSWP: lower bound of an invalidated vertex's new value
     ChunkItem *cq;
     uint32_t cq_cap;
     uint32_t *mlist;
