@@ -230,11 +230,12 @@ def test_config1_bfs_plain_stream():
             _check_labels(t["L"], rg.BFS, V, t["ref"], t["root"], ref_vals=ref_vals)
 
 
-@pytest.mark.parametrize("name", ["sssp", "wcc"])
+@pytest.mark.parametrize("name", ["sssp", "sswp", "wcc"])
 @pytest.mark.parametrize("mode", ["auto", "grid"])
 def test_large_batches(name, mode):
     """Batches of 10k and 50k updates on R-MAT scale 16, in the automatic
-    block/grid choice and with every pass forced grid-wide."""
+    block/grid choice and with every pass forced grid-wide (SSWP: large
+    deletion subtrees, where the expand prunes under the roots' bounds)."""
     algo = rg.ALGOS[name]
     V, E = 1 << 16, 1 << 20
     tuning = dict(block_max=0) if mode == "grid" else None
@@ -268,7 +269,7 @@ def test_large_batches(name, mode):
     dict(walk_limit=1),                               # ancestor walks give up: rebuild path
 ], ids=["always_recompute", "no_rescue", "reference_rescue", "rescue_every_level", "tiny_walk"])
 def test_deletion_strategies_keep_values(tuning):
-    for name in ("bfs", "sssp"):
+    for name in ("bfs", "sssp", "sswp"):
         algo = rg.ALGOS[name]
         V = 1024
         t = _setup(algo, "rmat", V, 1 << 14, 0.9, seed=9, tuning=tuning)
