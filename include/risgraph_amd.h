@@ -86,11 +86,11 @@ int rg_labels_destroy(rg_labels *L);
  * persistent team launch; larger passes run grid-wide.  A level is rescued
  * when level <= rescue_level (0xFFFFFFFF: never) or its frontier is below
  * rescue_frontier.  Defaults: rescue on the first level only (0, 0) for
- * BFS / SSSP / SSWP and on every level (0, 0xFFFFFFFF) for WCC; the
+ * BFS / SSSP and on every level (0, 0xFFFFFFFF) for SSWP and WCC; the
  * reference's policy is (5, 16384).  Full recompute when level <= 10 and
  * subtree+frontier > max(0.05*V, 2000000); ancestor walks capped at
  * walk_limit steps (0 = min(V/10, 65536)); block_max = RG_AUTO (1024 per
- * team workgroup; the team width grows with the batch, 8 to 32 workgroups
+ * team workgroup; the team width grows with the batch, 8 to 64 workgroups
  * per tree).  block_max = 0 runs every pass grid-wide.  Values never depend
  * on these; only the work done does. */
 #define RG_AUTO 0xFFFFFFFFu

@@ -327,7 +327,7 @@ class Labels:
         automatic; rescue_frontier None = the library default for the
         algorithm (0, or 0xFFFFFFFF for WCC: rescue on every level)."""
         if rescue_frontier is None:
-            rescue_frontier = 0xFFFFFFFF if self.algo == WCC else 0
+            rescue_frontier = 0xFFFFFFFF if self.algo in (WCC, SSWP) else 0
         bm = 0xFFFFFFFF if block_max is None else int(block_max)
         _check(load_library().rg_labels_set_tuning(self.h, rescue_level, rescue_frontier, recompute_level,
                                                    recompute_abs, recompute_frac, walk_limit, bm))

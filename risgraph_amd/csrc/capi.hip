@@ -862,17 +862,20 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
             EngineArgs a = engine_args(L);
             launch_engine_fill_init(a, g->st);
         }
-        // BFS / SSSP / SSWP rescue on the first deletion level only: deeper
-        // rescue scans cost more passes than the subtree they save on this
-        // device.  WCC rescues on every level: nearly every vertex of an
-        // invalidated subtree has a neighbour outside it with the same
-        // component label, so rescue stops the expansion at once, where
-        // expanding can invalidate a large part of a component and force a
-        // full recompute.  (The reference's policy, level <= 5 or frontier <
-        // 16384, is one rg_labels_set_tuning call away; values do not depend
-        // on the policy.)
+        // BFS / SSSP rescue on the first deletion level only: deeper rescue
+        // scans cost more passes than the subtree they save on this device.
+        // WCC rescues on every level: nearly every vertex of an invalidated
+        // subtree has a neighbour outside it with the same component label,
+        // so rescue stops the expansion at once, where expanding can
+        // invalidate a large part of a component and force a full
+        // recompute.  SSWP rescues on every level too: each scan also raises
+        // the vertex's lower bound, under which the expand leaves children
+        // alone, so the large widest-path subtrees stay small.  (The
+        // reference's policy, level <= 5 or frontier < 16384, is one
+        // rg_labels_set_tuning call away; values do not depend on the
+        // policy.)
         L->tune.rescue_level = 0;
-        L->tune.rescue_frontier = algo == RG_WCC ? 0xFFFFFFFFu : 0u;
+        L->tune.rescue_frontier = (algo == RG_WCC || algo == RG_SSWP) ? 0xFFFFFFFFu : 0u;
         L->tune.recompute_level = 10;
         L->tune.recompute_abs = 2000000;
         L->tune.recompute_frac = 0.05f;

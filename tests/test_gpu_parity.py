@@ -266,8 +266,10 @@ def test_large_batches(name, mode):
     dict(rescue_level=0xFFFFFFFF, rescue_frontier=0),  # never rescue: reset + pull + push only
     dict(rescue_level=5, rescue_frontier=16384),      # the reference's rescue policy
     dict(rescue_frontier=0xFFFFFFFF),                 # rescue at every level
+    dict(rescue_level=0, rescue_frontier=0),          # the first level only (BFS/SSSP default)
     dict(walk_limit=1),                               # ancestor walks give up: rebuild path
-], ids=["always_recompute", "no_rescue", "reference_rescue", "rescue_every_level", "tiny_walk"])
+], ids=["always_recompute", "no_rescue", "reference_rescue", "rescue_every_level", "first_level_only",
+        "tiny_walk"])
 def test_deletion_strategies_keep_values(tuning):
     for name in ("bfs", "sssp", "sswp"):
         algo = rg.ALGOS[name]
