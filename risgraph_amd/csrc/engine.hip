@@ -352,7 +352,6 @@ __device__ __forceinline__ bool mark_inv(const EngineArgs &a, const Snap &s, uin
 // was proven valid).  The stamp is negative: it reads as "not marked" and
 // loses to any later run's index in mark_inv's atomicMax.
 constexpr uint32_t RESCUE_WALK = 64;
-constexpr uint32_t RESCUE_CHUNKS = 8;   // chunk items of one vertex's rescue scan
 constexpr int32_t VALID_BIT = 0x40000000;   // run indices stay below it
 
 __device__ __forceinline__ int32_t valid_mark(int32_t idx) { return -(idx | VALID_BIT); }
@@ -703,10 +702,12 @@ __device__ void del_vertex(const EngineArgs &a, const Snap &s, uint32_t raw, uin
             uint32_t len;
             list_span(a.lists[L], v, off, len);
             if (len > LONG_LIST) {
-                // the scan looks for any one valid parent: a bounded prefix
-                // of a hub's in-list is searched, the vertex otherwise stays
-                // invalid (re-pulled later, same values)
-                if (glane == 0) defer_chunks(a, s, v, off, min(len, RESCUE_CHUNKS * CHUNK), CO_RESCUE);
+                // a long in-list is searched whole, spread over chunk items
+                // (items after a success return at once).  A hub that stays
+                // invalid costs its subtree and its own pull: scanning only a
+                // prefix of hub in-lists (1024 entries) made the mean batch
+                // twice as long on R-MAT scale 24
+                if (glane == 0) defer_chunks(a, s, v, off, len, CO_RESCUE);
                 deferred = true;
                 continue;
             }
