@@ -615,7 +615,7 @@ constexpr uint32_t Q_EXPAND_ONLY = 0x80000000u;
 // the rescue test (has_parents), since every invalidated vertex already has
 // a marked deletion root above it once the roots pass is done; levels only
 // bound the work per pass.
-constexpr uint32_t CONT_DEPTH = 4;
+constexpr uint32_t CONT_DEPTH = 7;   // the most a stack word's 3 depth bits hold
 constexpr uint32_t CONT_STACK = 32;
 constexpr uint32_t MAX_GROUPS = 1024 / GROUP;   // per workgroup
 constexpr uint32_t STACK_WORDS = MAX_GROUPS * 2 * CONT_STACK;
