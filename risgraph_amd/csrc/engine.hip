@@ -20,12 +20,12 @@
 // Work is level-synchronous over vertex queues.  A vertex pass gives each
 // queued vertex a 16-lane group (4 lanes in grid-wide passes); lists longer
 // than 32 entries are split into 128-entry chunk items handled a wave each in
-// the pass's chunk stage, so hub vertices spread over many CUs.  All trees of a batch repair in one
-// persistent launch (the team kernel): each tree gets W workgroups that
-// separate passes with a device barrier, and a pass too large for the team
-// leaves it for grid-wide launches (one per pass); the phase state machine
-// lives in the device control block so the two modes hand over at any pass
-// boundary.
+// the pass's chunk stage, so hub vertices spread over many CUs.  All trees
+// of a batch repair in one persistent launch (the team kernel): each tree
+// gets W workgroups that separate passes with a device barrier, and a pass
+// too large for the team leaves it for grid-wide launches (one per pass);
+// the phase state machine lives in the device control block so the two
+// modes hand over at any pass boundary.
 #include "rg_internal.h"
 
 #include <algorithm>
