@@ -325,7 +325,7 @@ class Labels:
                    recompute_abs=2000000, recompute_frac=0.05, walk_limit=0, block_max=None):
         """Deletion strategy knobs (rg_labels_set_tuning); block_max None =
         automatic; rescue_frontier None = the library default for the
-        algorithm (0, or 0xFFFFFFFF for WCC: rescue on every level)."""
+        algorithm (0, or 0xFFFFFFFF for SSWP and WCC: rescue on every level)."""
         if rescue_frontier is None:
             rescue_frontier = 0xFFFFFFFF if self.algo in (WCC, SSWP) else 0
         bm = 0xFFFFFFFF if block_max is None else int(block_max)
