@@ -88,9 +88,9 @@ class Cursor:
 
 
 # Batched serving uses the library's adaptive batch cap (rg_serve): it
-# starts at 64 updates, grows x1.5 while the backlog exceeds it and shrinks
-# x0.7 when a batch's device time exceeds a quarter of the latency target,
-# up to BATCH_MAX.  (Taking every arrived update per batch, with no cap,
+# starts at 64 updates, grows x1.5 while the backlog exceeds it and
+# otherwise shrinks x0.7 when a batch's device time exceeds a quarter of the
+# latency target, up to BATCH_MAX.  (Taking every arrived update per batch, with no cap,
 # met the p99.9 target less often: 6 of 8 half-second rounds at 80M
 # updates/s against 7 of 8.  A cap of 2^21 instead of 2^20 drains the
 # backlog after a large deletion event in fewer batches: at 170M updates/s

@@ -1370,8 +1370,10 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
             if (p->target_ms > 0) {
                 double service_ms = (tc - tn) * 1e3;
                 uint64_t backlog = (uint64_t)(std::upper_bound(arr.begin() + end, arr.end(), tc) - arr.begin()) - end;
-                if (service_ms > 0.25 * p->target_ms) cap = std::max(1.0, cap * 0.7);
-                else if ((double)backlog >= cap) cap = std::min(cap_max, cap * 1.5 + 16);
+                // a backlog grows the cap first (larger batches drain it in
+                // fewer fixed costs); without one, a slow batch shrinks it
+                if ((double)backlog >= cap) cap = std::min(cap_max, cap * 1.5 + 16);
+                else if (service_ms > 0.25 * p->target_ms) cap = std::max(1.0, cap * 0.7);
             }
             batches++;
             max_batch = std::max<uint64_t>(max_batch, end - next);
