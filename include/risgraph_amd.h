@@ -214,8 +214,8 @@ int rg_degrees(rg_graph *g, uint32_t *out_deg, uint32_t *in_deg);
  * and every labels instance in L, and completes them together.  With
  * target_ms > 0 the cap adapts in the spirit of the reference's
  * num_changes_threshold (bfs_inc_rt.cpp:409-427), but on device service
- * time: x0.7 when a batch took longer than target/4, x1.5+16 while the
- * backlog exceeds the cap, clamped to [1, cap_max].  batch_cap = 1 with no
+ * time: x1.5+16 while the backlog exceeds the cap, otherwise x0.7 when a
+ * batch took longer than target/4, clamped to [1, cap_max].  batch_cap = 1 with no
  * target is the plain one-update-at-a-time path.  When the oldest waiting
  * request is older than max(10 * target, 200 ms) the run is overloaded: the
  * rest is applied untimed in cap_max batches and r->overloaded is set.
