@@ -19,6 +19,14 @@
  *   or_graph_* store  brute-force Python multiset model (old-count return,
  *                     clamping at zero, undirected self-loop rule,
  *                     weighted identity (nbr, w)), plus hub lists >= 512.
+ *   or_need_update_add / or_need_update_del
+ *                     an add is unsafe exactly when the static fixpoint of
+ *                     G + e differs from G's; a delete of a single-copy edge
+ *                     is unsafe exactly when it is some vertex's parent edge
+ *                     (the tree's edges stop being live on G - e), and a
+ *                     safe delete leaves the values and a valid tree; all
+ *                     four algorithms, directed and undirected; planted
+ *                     one-direction and reversed-comparison mistakes fail.
  *   or_build_tree /
  *   or_update_*       equality with or_static_values after every update of
  *                     seeded sliding-window streams (all four algorithms,

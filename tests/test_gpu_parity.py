@@ -114,10 +114,12 @@ def test_invalid_vertex_is_an_error():
     assert g.add_edge(3, 4) == 0
 
 
-def _setup(algo, kind, V, E, load_frac, seed=1, wseed=2, tuning=None):
-    """Device graph + labels and cpu_ref twin, both loaded with the same prefix."""
+def _setup(algo, kind, V, E, load_frac, seed=1, wseed=2, tuning=None, sym=None):
+    """Device graph + labels and cpu_ref twin, both loaded with the same prefix.
+    sym (default: WCC only) makes a symmetric store fed with undirected ops."""
     weighted = _weighted(algo)
-    sym = algo == rg.WCC
+    if sym is None:
+        sym = algo == rg.WCC
     scale = int(np.log2(V))
     s = rg.Stream(kind, scale if kind == "rmat" else V, E, seed=seed, wseed=wseed)
     src, dst, w = s.read()
@@ -215,18 +217,19 @@ def test_reference_driver_loop_through_facade_calls(name):
 
 
 def test_config1_bfs_plain_stream():
-    """R-MAT scale 16, 2^20 edges, 90% loaded, root = max out-degree: every
-    update checked against cpu_ref, every 100th against the closed form."""
+    """Config 1 in full: R-MAT scale 16, 2^20 edges, 90% loaded, root = max
+    out-degree, 10,000 sliding-window updates (bfs_inc.cpp:160-189): every
+    update checked against cpu_ref, every 1000th against the closed form."""
     V, E = 1 << 16, 1 << 20
     t = _setup(rg.BFS, "rmat", V, E, 0.9)
-    for k in range(1000):
+    for k in range(10000):
         i_add, i_del = t["n0"] + k, k
         t["g"].stream_plain([t["L"]], t["s"], i_add, i_del, 1, True, latency=False)
         _ref_step(t, i_add, i_del)
         vals, par = t["L"].read()
         ref_vals = t["RL"].read()[0]
         assert np.array_equal(vals, ref_vals), k
-        if k % 100 == 99:
+        if k % 1000 == 999:
             _check_labels(t["L"], rg.BFS, V, t["ref"], t["root"], ref_vals=ref_vals)
 
 
@@ -344,32 +347,108 @@ def test_wcc_custom_init_labels():
         assert oracle.check_tree(rg.WCC, V, ls, ld, None, 0, vals, par, init=perm) == 0
 
 
-def test_classifier_matches_cpu_ref():
-    algo = rg.SSSP
-    t = _setup(algo, "rmat", 1024, 1 << 14, 0.9)
-    src, dst, w = t["src"], t["dst"], t["w"]
-    idx = np.arange(t["n0"], t["n0"] + 500)
-    got = t["L"].classify(src[idx], dst[idx], w[idx], np.ones(500), True)
-    want = [t["RL"].need_add(int(src[i]), int(dst[i]), int(w[i])) for i in idx]
-    assert got.tolist() == want
-    didx = np.arange(0, 500)
-    got = t["L"].classify(src[didx], dst[didx], w[didx], -np.ones(500), True)
-    vals, par = t["L"].read()
-    want = [int(par[int(dst[i])]) == int(src[i]) for i in didx]
-    assert got.tolist() == want
+@pytest.mark.parametrize("directed", [True, False], ids=["directed", "undirected"])
+@pytest.mark.parametrize("name", ALGO_NAMES)
+def test_classifier_matches_cpu_ref(name, directed):
+    """k_classify against cpu_ref's need_update_tree_add / need_update_tree_del
+    (graph.hpp:987-1000, 1220-1233), whose semantics the CPU suite pins.  Both
+    classify against the same tree: cpu_ref takes the device's values and
+    parents (parent weights resolved from its store).  Adds must agree
+    exactly.  Device parents are vertices only, so for deletes the device's
+    unsafe set contains the reference's, and any extra edge is a copy of the
+    tree edge's endpoints with another weight."""
+    algo = rg.ALGOS[name]
+    if algo == rg.WCC and directed:
+        pytest.skip("WCC runs on symmetric graphs (wcc_inc.cpp:56); the directed-store "
+                    "undirected view is covered by test_classifier_undirected_view")
+    t = _setup(algo, "rmat", 1024, 1 << 14, 0.9, sym=not directed)
+    L, RL = t["L"], t["RL"]
+    vals, par = L.read()
+    RL.assign(vals, par)
+    _, rpar, rpw = RL.read()
+    assert np.array_equal(rpar, par)
+    src, dst, w = t["src"], t["dst"], t["w"] if t["weighted"] else np.zeros_like(t["w"])
+    rng = np.random.default_rng(3)
+    # adds: the next stream edges and random vertex pairs
+    idx = np.arange(t["n0"], t["n0"] + 600)
+    a_s = np.concatenate([src[idx], rng.integers(0, 1024, 600).astype(np.uint64)])
+    a_d = np.concatenate([dst[idx], rng.integers(0, 1024, 600).astype(np.uint64)])
+    a_w = np.concatenate([w[idx], rng.integers(1, 1001, 600).astype(np.uint64) * np.uint64(t["weighted"])])
+    got = L.classify(a_s, a_d, a_w, np.ones(len(a_s)), directed)
+    want = np.array([RL.need_add(int(x), int(y), int(z), directed) for x, y, z in zip(a_s, a_d, a_w)])
+    assert np.array_equal(got, want)
+    assert 0 < want.sum() < len(want)
+    # deletes: loaded edges, plus every tree edge (so the unsafe side is exercised)
+    kids = np.nonzero(par < 1024)[0]
+    d_s = np.concatenate([src[:600], par[kids]])
+    d_d = np.concatenate([dst[:600], kids.astype(np.uint64)])
+    d_w = np.concatenate([w[:600], rpw[kids] * np.uint64(t["weighted"])])
+    got = L.classify(d_s, d_d, d_w, -np.ones(len(d_s)), directed)
+    want = np.array([RL.need_del(int(x), int(y), int(z), directed) for x, y, z in zip(d_s, d_d, d_w)])
+    assert not np.any(want & ~got), "device missed a reference-unsafe delete"
+    for i in np.nonzero(got & ~want)[0]:
+        x, y, z = int(d_s[i]), int(d_d[i]), int(d_w[i])
+        assert t["weighted"], "unweighted parents are exact: no extra unsafe deletes"
+        hit = [(c, p) for c, p in ((y, x), (x, y)) if int(par[c]) == p and (directed is False or c == y)]
+        assert hit and all(int(rpw[c]) != z for c, _ in hit), (x, y, z)
+    assert want.sum() > 0 and (~got).sum() > 0
+
+
+def test_classifier_undirected_view():
+    """WCC over the undirected view of a directed store (the four-tree
+    store): an add is unsafe when it improves either endpoint, a delete when
+    either endpoint's parent is the other (graph.hpp:992-997, 1225-1230)."""
+    V, E = 1024, 1 << 14
+    s = rg.Stream("rmat", 10, E, seed=4, wseed=5)
+    src, dst, w = s.read()
+    n0 = int(0.9 * E)
+    g = rg.Graph(V, E, symmetric=False, dual=True, weighted=True)
+    g.load_stream(s, 0, n0, True)
+    L = g.alloc_vertex_tree_array(rg.WCC, 0)
+    L.build_tree()
+    ref = oracle.RefGraph(V, E, symmetric=True, dual=False, weighted=False)
+    ref.add_edges(src[:n0], dst[:n0], None, False)
+    RL = oracle.RefLabels(ref, rg.WCC, 0)
+    vals, par = L.read()
+    RL.assign(vals, par)
+    idx = np.arange(n0, E)
+    got = L.classify(src[idx], dst[idx], w[idx], np.ones(len(idx)), True)
+    want = np.array([RL.need_add(int(x), int(y), 0, False) for x, y in zip(src[idx], dst[idx])])
+    assert np.array_equal(got, want)
+    kids = np.nonzero(par < V)[0]
+    d_s = np.concatenate([src[:800], par[kids], kids.astype(np.uint64)])
+    d_d = np.concatenate([dst[:800], kids.astype(np.uint64), par[kids]])
+    got = L.classify(d_s, d_d, np.zeros(len(d_s), np.uint64), -np.ones(len(d_s)), True)
+    want = np.array([RL.need_del(int(x), int(y), 0, False) for x, y in zip(d_s, d_d)])
+    assert np.array_equal(got, want)
+    assert want.sum() >= 2 * len(kids)
 
 
 def test_modified_set_tracks_changes():
+    """get_modified (graph.hpp:117-125) holds every vertex whose value
+    changed since clear_modified, across plain and batched updates, and the
+    "everything changed" flag stays off when no full rebuild ran."""
     algo = rg.BFS
     t = _setup(algo, "rmat", 1024, 1 << 14, 0.9)
     L = t["L"]
-    L.clear_modified()
-    before = L.read()[0]
-    t["g"].apply_stream([L], t["s"], t["n0"], 50, 0, 50, True)
-    after = L.read()[0]
-    mods, allf = L.get_modified()
-    changed = set(np.nonzero(before != after)[0].tolist())
-    assert allf or changed <= set(mods.tolist())
+    total = 0
+    pos = 0
+    for b in (1, 1, 5, 50, 200):
+        L.clear_modified()
+        before = L.read()
+        st = t["g"].apply_stream([L], t["s"], t["n0"] + pos, b, pos, b, True)
+        pos += b
+        after = L.read()
+        mods, allf = L.get_modified()
+        assert st["rebuilds"] == 0 and not allf
+        # the set tracks label writes (graph.hpp:1425 and the update_label
+        # call sites); a rescue re-parents without writing the value and is
+        # not traced, as in the reference (graph.hpp:1337-1340)
+        changed = set(np.nonzero(before[0] != after[0])[0].tolist())
+        assert changed <= set(mods.tolist()), b
+        assert len(set(mods.tolist())) == len(mods), "modified set holds no duplicates"
+        total += len(changed)
+    assert total > 0
 
 
 def test_empty_and_degenerate_batches():

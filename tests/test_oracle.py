@@ -372,3 +372,150 @@ def test_cpu_ref_recompute_fallback_keeps_values():
     L.inc_del(0, 1)
     ls, ld, lw = g.live_edges()
     assert np.array_equal(L.read()[0], oracle.static_values(BFS, V, ls, ld, lw, 0))
+
+
+# ---------------------------------------------------------------------------
+# the safe/unsafe classifier (need_update_tree_add / need_update_tree_del,
+# graph.hpp:987-1000, 1220-1233) against what the mathematics fixes:
+#  * an add is unsafe exactly when the fixpoint of G + e differs from that of
+#    G (labels are the unique fixpoint of a monotone relaxation, so an edge
+#    that does not strictly improve its target leaves every value alone);
+#  * a delete of an edge held once is unsafe exactly when the edge is some
+#    vertex's parent edge, i.e. when the tree stops being a set of live
+#    edges on G - e; a safe delete leaves a valid tree with the same values.
+# ---------------------------------------------------------------------------
+def _classifier_case(algo, directed, seed):
+    V = 48
+    m = 40 if algo == WCC else 150   # sparse enough for several components
+    rng = np.random.default_rng(seed)
+    weighted = algo in (SSSP, SSWP)
+    s = rng.integers(0, V, m).astype(np.uint64)
+    d = rng.integers(0, V, m).astype(np.uint64)
+    w = rng.integers(1, 6, m).astype(np.uint64) if weighted else np.zeros(m, np.uint64)
+    sym = not directed
+    g = oracle.RefGraph(V, m, symmetric=sym, dual=not sym, weighted=weighted)
+    g.add_edges(s, d, w if weighted else None, directed)
+    root = int(s[0])
+    L = oracle.RefLabels(g, algo, root)
+    L.build()
+    return V, g, L, root, rng, weighted
+
+
+def _add_changes_fixpoint(algo, V, g, root, a, b, c, directed):
+    ls, ld, lw = g.live_edges()
+    before = oracle.static_values(algo, V, ls, ld, lw, root)
+    ns, nd, nw = [a], [b], [c]
+    if not directed:
+        ns, nd, nw = [a, b], [b, a], [c, c]
+    after = oracle.static_values(algo, V, np.concatenate([ls, np.array(ns, np.uint64)]),
+                                 np.concatenate([ld, np.array(nd, np.uint64)]),
+                                 np.concatenate([lw, np.array(nw, np.uint64)]), root)
+    return not np.array_equal(before, after)
+
+
+def _parent_edges_live_without(g, L, V, a, b, c, directed):
+    """Is every vertex's parent edge (parent vertex, parent weight) still a
+    live edge once the single copy of (a -> b, c) is gone?"""
+    ls, ld, lw = g.live_edges()
+    live = {}
+    for x, y, z in zip(ls.tolist(), ld.tolist(), lw.tolist()):
+        live[(x, y, z)] = live.get((x, y, z), 0) + 1
+    for key in ([(a, b, c)] if directed else [(a, b, c), (b, a, c)]):
+        live[key] -= 1
+    _, par, pw = L.read()
+    for v in range(V):
+        p = int(par[v])
+        if p == V:
+            continue
+        if live.get((p, v, int(pw[v])), 0) <= 0:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("directed", [True, False], ids=["directed", "undirected"])
+@pytest.mark.parametrize("algo", [BFS, SSSP, SSWP, WCC], ids=["bfs", "sssp", "sswp", "wcc"])
+def test_classifier_add_iff_fixpoint_changes(algo, directed):
+    if algo == WCC and directed:
+        pytest.skip("WCC runs on symmetric graphs only (wcc_inc.cpp:56)")
+    for seed in range(3):
+        V, g, L, root, rng, weighted = _classifier_case(algo, directed, 100 + seed)
+        n_unsafe = 0
+        for _ in range(120):
+            a, b = int(rng.integers(0, V)), int(rng.integers(0, V))
+            c = int(rng.integers(1, 6)) if weighted else 0
+            got = L.need_add(a, b, c, directed)
+            assert got == _add_changes_fixpoint(algo, V, g, root, a, b, c, directed), (seed, a, b, c)
+            n_unsafe += got
+        assert 0 < n_unsafe < 120, "the sample must hold both safe and unsafe adds"
+
+
+@pytest.mark.parametrize("directed", [True, False], ids=["directed", "undirected"])
+@pytest.mark.parametrize("algo", [BFS, SSSP, SSWP, WCC], ids=["bfs", "sssp", "sswp", "wcc"])
+def test_classifier_del_iff_parent_edge(algo, directed):
+    if algo == WCC and directed:
+        pytest.skip("WCC runs on symmetric graphs only (wcc_inc.cpp:56)")
+    for seed in range(3):
+        V, g, L, root, rng, weighted = _classifier_case(algo, directed, 200 + seed)
+        ls, ld, lw = g.live_edges()
+        vals = L.read()[0]
+        n_unsafe = 0
+        seen = set()
+        for x, y, z in zip(ls.tolist(), ld.tolist(), lw.tolist()):
+            key = (x, y, z) if directed else (min(x, y), max(x, y), z)
+            if key in seen or g.edge_num(x, y, z) != 1:
+                continue
+            seen.add(key)
+            got = L.need_del(x, y, z, directed)
+            assert got == (not _parent_edges_live_without(g, L, V, x, y, z, directed)), (seed, x, y, z)
+            n_unsafe += got
+            if not got:
+                # a safe delete keeps the same values and a valid tree
+                keep = np.ones(len(ls), bool)
+                for i in range(len(ls)):
+                    if (ls[i], ld[i], lw[i]) == (x, y, z) or (not directed and (ls[i], ld[i], lw[i]) == (y, x, z)):
+                        keep[i] = False
+                exp = oracle.static_values(algo, V, ls[keep], ld[keep], lw[keep], root)
+                assert np.array_equal(vals, exp), (seed, x, y, z)
+                par = L.read()[1]
+                assert oracle.check_tree(algo, V, ls[keep], ld[keep], lw[keep], root, vals, par,
+                                         undirected=not directed or algo == WCC) == 0
+        assert n_unsafe > 0 and len(seen) > n_unsafe
+
+
+def test_classifier_checks_catch_planted_mistakes():
+    """The two properties above reject plausible classifier mistakes: an
+    undirected add or delete that looks at one direction only, and an SSWP
+    add test with the comparison the wrong way round."""
+    # one-direction undirected add: misses an edge that improves src from dst
+    caught = False
+    for seed in range(3):
+        V, g, L, root, rng, _ = _classifier_case(BFS, False, 300 + seed)
+        vals = L.read()[0]
+        for _ in range(200):
+            a, b = int(rng.integers(0, V)), int(rng.integers(0, V))
+            one_way = vals[a] + 1 < vals[b] and vals[a] < 65536
+            if one_way != _add_changes_fixpoint(BFS, V, g, root, a, b, 0, False):
+                caught = True
+    assert caught
+    # SSWP add with the comparison reversed
+    V, g, L, root, rng, _ = _classifier_case(SSWP, True, 310)
+    vals = L.read()[0]
+    caught = False
+    for _ in range(200):
+        a, b, c = int(rng.integers(0, V)), int(rng.integers(0, V)), int(rng.integers(1, 6))
+        wrong = min(int(vals[a]), c) < int(vals[b])
+        if wrong != _add_changes_fixpoint(SSWP, V, g, root, a, b, c, True):
+            caught = True
+    assert caught
+    # one-direction undirected delete: misses a tree edge held as (b -> a)
+    V, g, L, root, rng, _ = _classifier_case(WCC, False, 320)
+    _, par, _ = L.read()
+    ls, ld, lw = g.live_edges()
+    caught = False
+    for x, y, z in zip(ls.tolist(), ld.tolist(), lw.tolist()):
+        if g.edge_num(x, y, z) != 1:
+            continue
+        one_way = int(par[y]) == x
+        if one_way != (not _parent_edges_live_without(g, L, V, x, y, z, False)):
+            caught = True
+    assert caught
