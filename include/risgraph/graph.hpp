@@ -15,22 +15,50 @@
 // driver code like labels[v].data / labels[v].parent.nbr keeps working.
 // parent.data (the parent edge's weight) is not mirrored and reads 0.
 //
+// Copies of a TreeArray are snapshots, as copies of the reference's
+// std::vector<VertexTree> are (`auto labels_pre = labels;`, bfs_inc.cpp:110):
+// the copy holds the labels as they were and is not bound to the device.
+//
 // Thread safety follows the reference: add_edge / del_edge may be called from
 // many threads (the library serialises calls per graph); compute calls are
 // not reentrant (graph.hpp:1728-1730).
+//
+// The standard headers below are the ones the reference's core/graph.hpp
+// brings into a driver (graph.hpp:17-39 and the core headers it includes),
+// so a driver that relies on them (std::atomic_uint64_t at bfs_inc.cpp:117,
+// std::mutex at bfs_inc_rt.cpp:148) compiles with this header in its place.
 #pragma once
+#include <immintrin.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <condition_variable>
 #include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <exception>
 #include <functional>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <type_traits>
+#include <unordered_map>
+#include <unordered_set>
 #include <utility>
 #include <vector>
 
 #include "../risgraph_amd.h"
 #include "atomic.hpp"
 #include "io.hpp"
+#include "mvvec.hpp"
 #include "type.hpp"
 
 namespace risgraph_detail {
@@ -73,17 +101,29 @@ public:
         VertexData data;
     };
 
-    // Labels handle + host mirror.  Copies share the device labels.
+    // Labels handle + host mirror.  A copy is a snapshot of the labels at
+    // the time of the copy (like a copy of the reference's std::vector), not
+    // bound to the device; moves keep the binding.
     template <typename VertexData>
     class TreeArray {
     public:
         TreeArray() = default;
+        TreeArray(TreeArray &&) noexcept = default;
+        TreeArray &operator=(TreeArray &&) noexcept = default;
+        TreeArray(const TreeArray &o) { snapshot_from(o); }
+        TreeArray &operator=(const TreeArray &o)
+        {
+            if (this != &o) snapshot_from(o);
+            return *this;
+        }
         const VertexTree<VertexData> &operator[](uint64_t v) const
         {
             st_->sync();
             return st_->mirror[v];
         }
         uint64_t size() const { return st_->V; }
+        // the labels are bound to the device (false for snapshots)
+        bool bound() const { return st_ && st_->L; }
 
     private:
         friend class Graph;
@@ -111,6 +151,20 @@ public:
             }
         };
         std::shared_ptr<State> st_;
+
+        void snapshot_from(const TreeArray &o)
+        {
+            if (!o.st_) {
+                st_.reset();
+                return;
+            }
+            o.st_->sync();
+            auto st = std::make_shared<State>();
+            st->g = o.st_->g;
+            st->V = o.st_->V;
+            st->mirror = o.st_->mirror;
+            st_ = std::move(st);
+        }
     };
 
     Graph(uint64_t vertices, uint64_t hint_edges, bool symmetric, bool dual, bool trace_modified = false)
@@ -127,6 +181,24 @@ public:
     Graph &operator=(const Graph &) = delete;
 
     uint64_t get_vertices() const { return vertices_; }
+
+    // calling thread's OpenMP thread number (graph.hpp:102-105); 0 outside
+    // a parallel region or without OpenMP
+    uint64_t get_thread_id() const
+    {
+#ifdef _OPENMP
+        return (uint64_t)omp_get_thread_num();
+#else
+        return 0;
+#endif
+    }
+
+    // per-vertex value history for the rt / rpc drivers (graph.hpp:296-299)
+    template <typename VertexData>
+    MVVec<VertexData> alloc_history_array() const
+    {
+        return MVVec<VertexData>(vertices_);
+    }
 
     // add_edge / del_edge (graph.hpp:203-253): old multiplicity
     uint64_t add_edge(edge_type e, bool directed = true) { return update(e, 1, directed); }

@@ -7,7 +7,9 @@ The shipped library (``risgraph_amd``) never imports it.
 ``liboracle.so`` is built from ``oracle/oracle.c`` by ``oracle/Makefile``.
 Which check pins each function is listed at the top of ``oracle.c``.
 The reference engine itself cannot be compiled in this image (core/graph.hpp
-includes tbb/tbb.h; TBB is absent), so there is no ``oracle/_ref`` build.
+includes tbb/tbb.h; TBB is absent).  ``oracle/_ref`` holds the reference's
+plain drivers compiled against this repository's Graph<> facade (the
+drop-in check, ``build_ref_drivers``).
 """
 import ctypes
 import os
@@ -30,6 +32,19 @@ vp = ctypes.c_void_p
 def build():
     """Compile liboracle.so with gcc (via oracle/Makefile)."""
     subprocess.run(["make", "-s", "-B", "-C", _HERE, "liboracle.so"], check=True)
+
+
+REF = "/root/reference"
+
+
+def build_ref_drivers():
+    """Compile the reference's plain drivers against the Graph<> facade into
+    oracle/_ref/ (needs the reference sources and the built library); does
+    nothing where the reference sources are absent."""
+    if not os.path.isdir(os.path.join(REF, "src")):
+        return False
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, f"REF={REF}", "ref_drivers"], check=True)
+    return True
 
 
 def lib():

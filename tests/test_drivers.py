@@ -1,8 +1,16 @@
-"""The Graph<> facade (include/risgraph/graph.hpp) and the sliding-window
-drivers built on it (risgraph_amd/drivers/inc.cpp): the compile check runs on
-the CPU; the GPU tests run each driver on a seeded edge file and compare the
-final labels with the oracle's closed form on the final window."""
+"""The Graph<> facade (include/risgraph/graph.hpp) and the drivers built on it.
+
+* risgraph_amd/drivers/inc.cpp, this repository's sliding-window drivers;
+* the reference's own plain drivers, src/{bfs,sssp,sswp,wcc}_inc.cpp,
+  compiled unchanged against the facade (their "core/graph.hpp" include
+  resolved to include/risgraph/graph.hpp) by `make -C oracle ref_drivers`.
+
+The compile checks run on the CPU (the reference drivers only where the
+reference sources are present); the GPU tests run each driver on a seeded
+edge file and compare what it reports with the oracle's closed form on the
+final window."""
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -13,6 +21,9 @@ import streams
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "risgraph_amd", "drivers", "bin")
+REF = "/root/reference"
+REF_BIN = os.path.join(ROOT, "oracle", "_ref")
+REF_DRIVERS = ["bfs_inc", "sssp_inc", "sswp_inc", "wcc_inc"]
 
 
 def test_facade_compiles_without_gpu(tmp_path):
@@ -57,3 +68,97 @@ def test_driver_final_values_match_oracle(tmp_path, algo, batch):
     w = (fs + fd) % np.uint64(128)
     exp = oracle.static_values(oracle.ALGOS[algo], V, fs, fd, w, root)
     assert np.array_equal(got, exp)
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "src")), reason="reference sources absent")
+@pytest.mark.parametrize("drv", REF_DRIVERS)
+def test_reference_driver_compiles_against_facade(tmp_path, drv):
+    """The reference driver source, unchanged, compiles and links against the
+    facade and librisgraph_amd.so with only `core/` mapped to `risgraph/`."""
+    mapdir = tmp_path / "map"
+    mapdir.mkdir()
+    os.symlink(os.path.join(ROOT, "include", "risgraph"), mapdir / "core")
+    lib = os.path.join(ROOT, "risgraph_amd")
+    cmd = ["g++", "-O0", "-std=c++17", "-fopenmp", "-w", f"-I{mapdir}", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(REF, "src", drv + ".cpp"), "-o", str(tmp_path / drv), f"-L{lib}", "-lrisgraph_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+
+
+def _histogram_line(algo, vals):
+    """The label summary the reference drivers print (bfs_inc.cpp:220-247,
+    sssp_inc.cpp:221-244, sswp_inc.cpp:221-244) for the given labels."""
+    if algo == "bfs":
+        lv = vals[vals != 65536].astype(np.int64)
+        cnt = np.bincount(lv) if len(lv) else np.zeros(0, np.int64)
+        out = []
+        for c in cnt:
+            if c == 0:
+                break
+            out.append(f"{c} ")
+        return "".join(out)
+    keep = vals[vals != 134217728]
+    u, c = np.unique(keep, return_counts=True)
+    return "".join(f"{int(a)}: {int(b)}, " for a, b in zip(u, c))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("drv", REF_DRIVERS)
+def test_reference_driver_runs_on_the_library(tmp_path, drv):
+    """The reference driver, built against the facade, loads 90% of a seeded
+    R-MAT edge file, runs the sliding window over the rest on the MI355X,
+    prints its label summary, rebuilds (build_tree) and prints it again; both
+    summaries must equal the closed form on the final window."""
+    exe = os.path.join(REF_BIN, drv)
+    if not os.path.exists(exe):
+        pytest.skip("reference drivers not built (make -C oracle ref_drivers needs the reference sources)")
+    algo = drv.split("_")[0]
+    n = 6000
+    path = str(tmp_path / "g.bin")
+    s, d = _write_graph(path, 10, n, seed=31)
+    V = int(max(s.max(), d.max())) + 1
+    imported = int(n * 0.9)
+    root = streams.max_out_degree_root(s[:imported], V)
+    args = [exe, path] + ([] if algo == "wcc" else [str(root)]) + ["0.9"]
+    env = dict(os.environ, OMP_NUM_THREADS="4")
+    r = subprocess.run(args, capture_output=True, text=True, env=env, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    lo = n - imported
+    fs, fd = s[lo:], d[lo:]
+    w = (fs + fd) % np.uint64(128)      # the drivers' weights (sssp_inc.cpp:65)
+    exp = oracle.static_values(oracle.ALGOS[algo], V, fs, fd, w, root)
+    if algo == "wcc":
+        counts = [int(x.split(":")[1]) for x in r.stderr.splitlines() if x.startswith("number of communities")]
+        want = int(np.sum(exp == np.arange(V, dtype=np.uint64)))
+        assert len(counts) >= 3 and counts[-1] == counts[-2] == want, (counts, want)
+    else:
+        lines = [x for x in r.stdout.split("\n") if x.strip()]
+        want = _histogram_line(algo, exp)
+        assert len(lines) >= 2
+        assert lines[-2] == want, "incremental labels differ from the closed form"
+        assert lines[-1] == want, "rebuilt labels differ from the closed form"
+
+
+def test_host_mvvec_matches_model(tmp_path):
+    """include/risgraph/mvvec.hpp against a brute-force version map, with
+    concurrent sets and a gc (host only)."""
+    exe = tmp_path / "mvvec_check"
+    cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-Wall", "-Werror", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(ROOT, "tests", "cpp", "mvvec_check.cpp"), "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+def test_facade_label_copies_are_snapshots(tmp_path):
+    exe = tmp_path / "facade_snapshot"
+    lib = os.path.join(ROOT, "risgraph_amd")
+    cmd = ["g++", "-O1", "-std=c++17", "-Wall", "-Werror", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(ROOT, "tests", "cpp", "facade_snapshot.cpp"), "-o", str(exe), f"-L{lib}", "-lrisgraph_amd",
+           f"-Wl,-rpath,{lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
