@@ -249,6 +249,7 @@ StoreArgs store_args(rg_graph *g)
     a.ops = g->ops;
     a.lops = g->lops;
     a.err = g->err;
+    a.hstatus = g->h_status;   // the batch's last kernel publishes err / oom here
     a.free_off = g->free_off;
     a.pend_off = g->pend_off;
     a.cnt_tmp[0] = g->cnt_tmp[0];
