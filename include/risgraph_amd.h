@@ -68,6 +68,22 @@ int rg_edge_update(rg_graph *g, uint64_t src, uint64_t dst, uint64_t data, int d
  * bfs_inc_batch.cpp:142-173).  old_out[i] receives each return value. */
 int rg_edge_update_batch(rg_graph *g, const rg_edge *edges, uint64_t n, int delta, int directed,
                          uint64_t *old_out);
+/* rg_edge_update_batch for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * has no counterpart in the reference.  d_edges (n rg_edge records) and
+ * d_old (n u64 return values, may be NULL) are DEVICE pointers on the
+ * graph's device; the caller's writes to d_edges must be complete before the
+ * call.  The batch's working arrays live in a static device array of
+ * RG_STATIC_MAX_OPS edge ops' worth (about 3 MB per device, held until the
+ * process ends); nothing is allocated and no hipMalloc / hipFree is called.
+ * n > RG_STATIC_MAX_OPS returns RG_EINVAL.  A batch holding an id >= V or a
+ * weight over 32 bits is rejected whole (RG_EINVAL_VERTEX / RG_EINVAL).
+ * Calls are synchronous; every call on a device shares that device's array
+ * and leaves its contents behind, so calls that overlapped on one device
+ * would race: the library serialises them with a process-wide lock. */
+#define RG_STATIC_MAX_OPS 16384
+int rg_edge_update_batch_static(rg_graph *g, const rg_edge *d_edges, uint64_t n, int delta, int directed,
+                                uint64_t *d_old);
 /* get_outgoing_degree / get_incoming_degree (graph.hpp:155-164) and
  * get_edge_num (graph.hpp:198-201). */
 int rg_degree(rg_graph *g, uint64_t v, int incoming, uint64_t *out);

@@ -19,6 +19,7 @@ import numpy as np
 BFS, SSSP, SSWP, WCC = 0, 1, 2, 3
 ALGOS = {"bfs": BFS, "sssp": SSSP, "sswp": SSWP, "wcc": WCC}
 RG_OK, RG_EINVAL_VERTEX, RG_ENOMEM, RG_EALGO, RG_EDEVICE, RG_EINVAL = range(6)
+STATIC_MAX_OPS = 16384   # RG_STATIC_MAX_OPS: largest batch of rg_edge_update_batch_static
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librisgraph_amd.so")
@@ -71,6 +72,7 @@ SIGNATURES = {
     "rg_graph_create": (ci, [u64, u64, ci, ci, ci, ctypes.POINTER(vp)]),
     "rg_graph_destroy": (ci, [vp]),
     "rg_graph_info": (ci, [vp, u64p, u64p, u64p, u64p, u64p]),
+    "rg_edge_update_batch_static": (ci, [vp, vp, u64, ci, ci, vp]),
     "rg_edge_update": (ci, [vp, u64, u64, u64, ci, ci, u64p]),
     "rg_edge_update_batch": (ci, [vp, vp, u64, ci, ci, vp]),
     "rg_degree": (ci, [vp, u64, ci, u64p]),
@@ -223,6 +225,11 @@ class Graph:
         old = np.empty(len(e), np.uint64)
         _check(load_library().rg_edge_update_batch(self.h, _ptr(e), len(e), delta, int(directed), _ptr(old)))
         return old
+
+    def update_edges_static(self, d_edges, n, delta=1, directed=True, d_old=None):
+        """rg_edge_update_batch_static: device pointers (ints) to n rg_edge
+        records and n u64 return values; nothing is allocated."""
+        _check(load_library().rg_edge_update_batch_static(self.h, d_edges, n, delta, int(directed), d_old))
 
     def get_degree(self, v, incoming=False):
         out = u64()

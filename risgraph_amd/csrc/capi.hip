@@ -107,7 +107,8 @@ struct rg_graph {
     uint32_t *err = nullptr;          // device: [0] bad ids
     uint64_t *free_off = nullptr, *pend_off = nullptr;   // slab free lists
     uint32_t *cnt_tmp[2]{};           // bulk reservation counters
-    uint32_t *h_status = nullptr;     // pinned: err, oom
+    uint32_t *h_status = nullptr;     // pinned: err, oom, then the static path's bad-id and bad-weight flags
+    unsigned char *scratch = nullptr; // this device's copy of the static scratch array (no-alloc entry point)
     // pinned staging for host-side batches
     uint32_t *h_src = nullptr, *h_dst = nullptr, *h_w = nullptr;
     OpResult *h_res = nullptr;
@@ -288,6 +289,75 @@ void store_status_check(rg_graph *g)
         throw RgError{RG_EDEVICE, "store op chain corrupt"};
     }
     if (g->h_status[1]) throw RgError{RG_ENOMEM, "adjacency slot or hash pool exhausted"};
+}
+
+// ---------------------------------------------------------------------
+// Scratch of the entry point that allocates nothing
+// (rg_edge_update_batch_static): the store batch's edge-op and list-op
+// arrays for up to STATIC_MAX_OPS edge ops live in one file-scope device
+// array instead of buffers the graph allocates and grows.  Each device holds
+// its own copy; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (taken once per device, with
+// the device current, when a graph is created there).  Calls on a device
+// share its copy, and each call leaves its contents behind; a process-wide
+// lock keeps the calls of this process from overlapping.
+// ---------------------------------------------------------------------
+constexpr uint32_t STATIC_MAX_OPS = RG_STATIC_MAX_OPS;
+constexpr size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t SC_SRC = 0;
+constexpr size_t SC_DST = SC_SRC + al256(4ull * STATIC_MAX_OPS);
+constexpr size_t SC_W = SC_DST + al256(4ull * STATIC_MAX_OPS);
+constexpr size_t SC_KIND = SC_W + al256(4ull * STATIC_MAX_OPS);
+constexpr size_t SC_RES = SC_KIND + al256(1ull * STATIC_MAX_OPS);
+constexpr size_t SC_LOP = SC_RES + al256(sizeof(OpResult) * STATIC_MAX_OPS);
+constexpr size_t SC_SEG = SC_LOP + al256(sizeof(LOp) * 4ull * STATIC_MAX_OPS);
+constexpr size_t SC_OWNER = SC_SEG + al256(16ull * STATIC_MAX_OPS);
+constexpr size_t SC_HEAVY = SC_OWNER + al256(16ull * STATIC_MAX_OPS);
+constexpr size_t SCRATCH_BYTES = SC_HEAVY + al256(16ull * STATIC_MAX_OPS);
+
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+std::mutex g_static_mu;            // one static-path call at a time in this process
+unsigned char *g_scratch_dev[64];  // per-device address of `scratch`, set once
+std::mutex g_scratch_mu;
+
+// this device's address of the scratch array (called with the device current)
+unsigned char *scratch_address()
+{
+    int dev = 0;
+    HC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw RgError{RG_EDEVICE, "device index out of range"};
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw RgError{RG_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+        g_scratch_dev[dev] = (unsigned char *)p;
+    }
+    return g_scratch_dev[dev];
+}
+
+// rg_edge (device) -> edge ops in the scratch; ids and weights validated:
+// bad[0] = 1 for an id >= V, bad[1] = 1 for a weight over 32 bits
+__global__ void k_edges_to_ops(const rg_edge *e, uint32_t n, uint8_t kind, uint64_t V, int weighted, EdgeOps ops,
+                               uint32_t *bad)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const rg_edge x = e[i];
+    if (x.src >= V || x.dst >= V) bad[0] = 1u;
+    if (weighted && x.data > 0xFFFFFFFFull) bad[1] = 1u;
+    ops.src[i] = (uint32_t)x.src;
+    ops.dst[i] = (uint32_t)x.dst;
+    ops.w[i] = weighted ? (uint32_t)x.data : 0u;
+    ops.kind[i] = kind;
+}
+
+__global__ void k_old_out(const OpResult *res, uint32_t n, uint64_t *out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = res[i].old;
 }
 
 hipEvent_t prof_event(rg_graph *g)
@@ -686,6 +756,7 @@ int rg_graph_create(uint64_t vertices, uint64_t hint_edges, int symmetric, int d
         HC(hipMemsetAsync(g->err, 0, 16, g->st));
         g->h_status = halloc<uint32_t>(4);
         memset(g->h_status, 0, 16);
+        g->scratch = scratch_address();   // set-up of the no-alloc entry point on this device
         HC(hipStreamSynchronize(g->st));
     });
     if (rc != RG_OK) {
@@ -759,6 +830,59 @@ int rg_edge_update(rg_graph *g, uint64_t src, uint64_t dst, uint64_t data, int d
 {
     rg_edge e{src, dst, data};
     return rg_edge_update_batch(g, &e, 1, delta, directed, old);
+}
+
+int rg_edge_update_batch_static(rg_graph *g, const rg_edge *d_edges, uint64_t n, int delta, int directed,
+                                uint64_t *d_old)
+{
+    if (!g || (n && !d_edges) || (delta != 1 && delta != -1)) return RG_EINVAL;
+    if (n > STATIC_MAX_OPS) {
+        g_err = "batch exceeds the static scratch (" + std::to_string(STATIC_MAX_OPS) + " edge ops at most)";
+        return RG_EINVAL;
+    }
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
+    std::lock_guard<std::mutex> sl(g_static_mu);
+    try {
+        if (n == 0) return RG_OK;
+        if (!g->scratch) throw RgError{RG_EDEVICE, "static scratch not set up on this device"};
+        unsigned char *sc = g->scratch;
+        EdgeOps ops{};
+        ops.src = (uint32_t *)(sc + SC_SRC);
+        ops.dst = (uint32_t *)(sc + SC_DST);
+        ops.w = (uint32_t *)(sc + SC_W);
+        ops.kind = (uint8_t *)(sc + SC_KIND);
+        ops.res = (OpResult *)(sc + SC_RES);
+        ops.n = (uint32_t)n;
+        ListOps lops{};
+        lops.op = (LOp *)(sc + SC_LOP);
+        lops.segbuf = (uint32_t *)(sc + SC_SEG);
+        lops.owner = (uint32_t *)(sc + SC_OWNER);
+        lops.heavy = (uint32_t *)(sc + SC_HEAVY);
+        lops.n = (uint32_t)(4 * n);
+        // validate and convert first: a batch with a bad id or weight is
+        // rejected whole, like the host-buffer entry point
+        g->h_status[2] = g->h_status[3] = 0;
+        const uint8_t kind = (uint8_t)((delta > 0 ? K_INS : K_DEL) | (directed ? 0 : K_UNDIR));
+        const dim3 grid((uint32_t)((n + 255) / 256)), blk(256);
+        hipLaunchKernelGGL(k_edges_to_ops, grid, blk, 0, g->st, d_edges, (uint32_t)n, kind, g->V, g->weighted, ops,
+                           g->h_status + 2);
+        HC(hipGetLastError());
+        HC(hipStreamSynchronize(g->st));
+        if (g->h_status[2]) throw RgError{RG_EINVAL_VERTEX, "VertexId error."};
+        if (g->h_status[3]) throw RgError{RG_EINVAL, "edge weight exceeds 32 bits"};
+        StoreArgs a = store_args(g);
+        a.ops = ops;
+        a.lops = lops;
+        if (!launch_store_batch(a, g->st)) launch_pool_recycle(a, g->st);
+        if (d_old) hipLaunchKernelGGL(k_old_out, grid, blk, 0, g->st, ops.res, (uint32_t)n, d_old);
+        HC(hipGetLastError());
+        HC(hipStreamSynchronize(g->st));
+        store_status_check(g);
+        return RG_OK;
+    } catch (const RgError &e) {
+        g_err = e.msg;
+        return e.code;
+    }
 }
 
 int rg_degree(rg_graph *g, uint64_t v, int incoming, uint64_t *out)
