@@ -12,8 +12,10 @@ requests arrive by a Poisson process from 256 simulated clients.
 Legs:
   plain   one update per device batch (rg_serve with batch cap 1), at the
           highest offered rate whose p99.9 stays within the target;
-  serve   continuous batching with the adaptive batch cap (up to 2^21
-          updates per batch): offered rate
+  serve   continuous batching with the safe/unsafe split (safe updates
+          complete once their store batch is applied and classified; unsafe
+          ones after the repair) and the adaptive admission cap, which counts
+          unsafe updates (up to 2^21 updates per batch): offered rate
           ramped x1.5 per short calibration round while the server keeps up
           (>= 95% served, p99 within the target; two misses in a row end the
           ramp), then two bisection rounds; the W warmup and K timed
@@ -66,6 +68,10 @@ def parse():
                          "--probe-pairs insert+delete pairs, then a plain 1 GiB device copy (counter calibration)")
     ap.add_argument("--probe-pairs", type=int, default=65536)
     ap.add_argument("--probe-batches", type=int, default=16)
+    ap.add_argument("--no-split", action="store_true",
+                    help="every update of a batch waits for the batch's repair (no safe/unsafe split)")
+    ap.add_argument("--cap", choices=["unsafe", "total"], default="unsafe",
+                    help="what the adaptive admission cap counts: unsafe updates (default) or all updates")
     return ap.parse_args()
 
 
@@ -99,15 +105,18 @@ class Cursor:
 BATCH_MAX = 1 << 21
 
 
-def serve(g, labels, stream, cur, n, rate, cap, cap_max, target_ms):
+def serve(g, labels, stream, cur, n, rate, cap, cap_max, target_ms, split=True, cap_unsafe=False):
     ins_first, del_first, n = cur.take(n)
     res, lat = g.serve(labels, stream, ins_first, del_first, n, rate, batch_cap=cap, cap_max=cap_max,
-                       target_ms=target_ms, latencies=True)
+                       target_ms=target_ms, latencies=True, split=split, cap_unsafe=cap_unsafe)
     return res, lat
 
 
 def serve_batched(g, labels, stream, cur, n, rate, args):
-    return serve(g, labels, stream, cur, n, rate, 64, BATCH_MAX, args.target_ms)
+    # with the unsafe-count cap the initial cap is in unsafe updates
+    cap_unsafe = args.cap == "unsafe" and not args.no_split
+    return serve(g, labels, stream, cur, n, rate, 16 if cap_unsafe else 64, BATCH_MAX, args.target_ms,
+                 split=not args.no_split, cap_unsafe=cap_unsafe)
 
 
 def main():
@@ -210,7 +219,8 @@ def main():
         n_step = int(max(20000, rate * args.step_seconds))
         for _ in range(args.warmup):
             serve_batched(g, labels, stream, cur, n_step, rate, args)
-        lats, acc = [], dict(wall=0.0, total=0, batches=0, grid_steps=0, gpu_ms=0.0, overloaded=0)
+        lats, acc = [], dict(wall=0.0, total=0, batches=0, grid_steps=0, gpu_ms=0.0, overloaded=0, unsafe=0,
+                             safe_p50=[], unsafe_p50=[], unsafe_p999=[])
         for k in range(args.steps):
             res, lat = serve_batched(g, labels, stream, cur, n_step, rate, args)
             lats.append(lat)
@@ -220,6 +230,10 @@ def main():
             acc["grid_steps"] += res["grid_steps"]
             acc["gpu_ms"] += res["gpu_ms"]
             acc["overloaded"] += res["overloaded"]
+            acc["unsafe"] += res["unsafe"]
+            acc["safe_p50"].append(res["p50_safe_ms"])
+            acc["unsafe_p50"].append(res["p50_unsafe_ms"])
+            acc["unsafe_p999"].append(res["p999_unsafe_ms"])
             log(f"step {k}: {res['updates_per_s']:.0f} upd/s p99.9={res['p999_ms']:.2f}ms "
                 f"mean_batch={res['mean_batch']:.1f} max_batch={res['max_batch']} rebuilds={res['rebuilds']} "
                 f"gpu_busy={res['gpu_ms'] / 1e3 / max(res['wall_s'], 1e-9):.2f}")
@@ -282,7 +296,13 @@ def main():
             "offered_rate": round(rate, 1), "target_p999_ms": args.target_ms,
             "step_updates": n_step,
         },
-        "latency_ms": {"p50": round(p[50], 3), "p99": round(p[99], 3), "p99.9": round(p[99.9], 3)},
+        "latency_ms": {"p50": round(p[50], 3), "p99": round(p[99], 3), "p99.9": round(p[99.9], 3),
+                       "p50_safe_median_step": round(float(np.median(acc["safe_p50"])), 3),
+                       "p50_unsafe_median_step": round(float(np.median(acc["unsafe_p50"])), 3),
+                       "p99.9_unsafe_worst_step": round(float(max(acc["unsafe_p999"])), 3)},
+        "split": not args.no_split,
+        "cap_counts": "unsafe updates" if (args.cap == "unsafe" and not args.no_split) else "all updates",
+        "safe_frac": round(1.0 - acc["unsafe"] / max(total, 1), 4),
         "target_met": met,
         "timed_attempts": attempts,
         "plain_gpu": {"updates_per_s": round(plain["updates_per_s"], 1), "offered": round(plain["offered"], 1),

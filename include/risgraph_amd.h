@@ -245,6 +245,16 @@ typedef struct rg_serve_params {
     uint32_t cap_max;
     double target_ms;         /* 0 = fixed cap */
     int directed;
+    /* 1: safe/unsafe split (the safe phase of bfs_inc_rt.cpp:211-326): a
+     * batch of more than 64 updates is classified against the labels before
+     * its repair (need_update_tree_add / _del in every tree, store returns
+     * applied); its safe updates complete as soon as the store batch and the
+     * split are done, only the unsafe ones seed the repair and wait for it */
+    int split;
+    /* 1: the admission cap counts unsafe updates (the rt drivers'
+     * num_changes_threshold, bfs_inc_rt.cpp:409-427): a batch admits
+     * cap / f updates, f = the running unsafe fraction (needs split) */
+    int cap_unsafe;
 } rg_serve_params;
 typedef struct rg_serve_result {
     double wall_s, updates_per_s;
@@ -254,6 +264,9 @@ typedef struct rg_serve_result {
     uint64_t overloaded;      /* 1 when the run fell behind (see above) */
     uint64_t served;          /* updates completed with a measured latency */
     double gpu_ms;            /* summed device time of the batches */
+    uint64_t unsafe;          /* served updates that waited for a repair */
+    double safe_frac;         /* served updates that completed at the split */
+    double p50_safe_ms, p50_unsafe_ms, p999_unsafe_ms;
 } rg_serve_result;
 int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_params *p,
              rg_serve_result *r, double *lat_ms);

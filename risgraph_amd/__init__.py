@@ -45,7 +45,8 @@ class BatchStats(ctypes.Structure):
 class ServeParams(ctypes.Structure):
     _fields_ = [("ins_first", u64), ("del_first", u64), ("n_updates", u64), ("rate", ctypes.c_double),
                 ("seed", u64), ("clients", ctypes.c_uint32), ("batch_cap", ctypes.c_uint32),
-                ("cap_max", ctypes.c_uint32), ("target_ms", ctypes.c_double), ("directed", ci)]
+                ("cap_max", ctypes.c_uint32), ("target_ms", ctypes.c_double), ("directed", ci),
+                ("split", ci), ("cap_unsafe", ci)]
 
 
 class ServeResult(ctypes.Structure):
@@ -53,7 +54,9 @@ class ServeResult(ctypes.Structure):
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("p999_ms", ctypes.c_double),
                 ("max_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("mean_batch", ctypes.c_double),
                 ("batches", u64), ("max_batch", u64), ("grid_steps", u64), ("rebuilds", u64),
-                ("overloaded", u64), ("served", u64), ("gpu_ms", ctypes.c_double)]
+                ("overloaded", u64), ("served", u64), ("gpu_ms", ctypes.c_double), ("unsafe", u64),
+                ("safe_frac", ctypes.c_double), ("p50_safe_ms", ctypes.c_double),
+                ("p50_unsafe_ms", ctypes.c_double), ("p999_unsafe_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: (float(getattr(self, k)) if t is ctypes.c_double else int(getattr(self, k)))
@@ -272,11 +275,11 @@ class Graph:
         return lat, st.as_dict()
 
     def serve(self, labels, stream, ins_first, del_first, n_updates, rate, batch_cap=1, cap_max=0,
-              target_ms=0.0, seed=4, clients=256, directed=True, latencies=False):
+              target_ms=0.0, seed=4, clients=256, directed=True, latencies=False, split=True, cap_unsafe=False):
         """Poisson-arrival serving loop (rg_serve); returns (result dict, latencies_ms or None)."""
         arr = (vp * max(1, len(labels)))(*[L.h for L in labels])
         p = ServeParams(ins_first, del_first, n_updates, rate, seed, clients, batch_cap,
-                        cap_max or batch_cap, target_ms, int(directed))
+                        cap_max or batch_cap, target_ms, int(directed), int(split), int(cap_unsafe))
         r = ServeResult()
         lat = np.empty(n_updates, np.float64) if latencies else None
         _check(load_library().rg_serve(self.h, arr, len(labels), stream.h, ctypes.byref(p),

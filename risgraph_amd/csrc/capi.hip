@@ -103,6 +103,8 @@ struct rg_graph {
     uint64_t slot_cap = 0, hash_cap = 0;
     EdgeOps ops{};
     ListOps lops{};
+    EdgeOps uops{};                   // a stream batch's unsafe ops (the engine's input after the split)
+    uint32_t *ucount = nullptr;       // device: their count
     uint32_t op_cap = 0;
     uint32_t *err = nullptr;          // device: [0] bad ids
     uint64_t *free_off = nullptr, *pend_off = nullptr;   // slab free lists
@@ -117,6 +119,12 @@ struct rg_graph {
     uint32_t h_cap = 0;
     hipEvent_t ev[4]{};
     double last_ms = 0;
+    // the safe/unsafe split's per-op flags and count, copied to the host on
+    // their own stream while the engine runs
+    hipStream_t st_copy = nullptr;
+    hipEvent_t ev_cls = nullptr, ev_safe = nullptr;
+    uint32_t *h_ucount = nullptr;     // pinned
+    uint64_t last_unsafe = 0;         // unsafe ops of the last split batch
     std::recursive_mutex mu;
     // kernel timing (rg_profile_enable)
     struct ProfRec {
@@ -199,6 +207,8 @@ void free_ops(rg_graph *g)
     dfree(g->ops.res);
     dfree(g->lops.op); dfree(g->lops.segbuf); dfree(g->lops.heavy); dfree(g->lops.owner);
     dfree(g->d_flag);
+    dfree(g->uops.src); dfree(g->uops.dst); dfree(g->uops.w); dfree(g->uops.kind); dfree(g->uops.res);
+    dfree(g->ucount);
     g->op_cap = 0;
 }
 
@@ -220,6 +230,12 @@ void ensure_ops(rg_graph *g, uint64_t n)
     g->lops.heavy = dalloc<uint32_t>(lc);
     g->lops.owner = dalloc<uint32_t>(lc);
     g->d_flag = dalloc<uint8_t>(cap);
+    g->uops.src = dalloc<uint32_t>(cap);
+    g->uops.dst = dalloc<uint32_t>(cap);
+    g->uops.w = dalloc<uint32_t>(cap);
+    g->uops.kind = dalloc<uint8_t>(cap);
+    g->uops.res = dalloc<OpResult>(cap);
+    g->ucount = dalloc<uint32_t>(1);
     g->op_cap = cap;
 }
 
@@ -416,7 +432,7 @@ void run_store(rg_graph *g)
     HC(hipGetLastError());
 }
 
-EngineArgs engine_args(rg_labels *L)
+EngineArgs engine_args(rg_labels *L, bool split = false)
 {
     rg_graph *g = L->g;
     EngineArgs a{};
@@ -455,7 +471,7 @@ EngineArgs engine_args(rg_labels *L)
     a.hctl = L->hc;
     a.ticks = L->ticks;
     a.bar = L->bar;
-    a.ops = g->ops;
+    a.ops = split ? g->uops : g->ops;
     a.ops.n = 0;   // the engine takes the batch size from the control block;
                    // keeping it out lets the device copy of the arguments stay valid
     a.tune = L->tune;
@@ -543,8 +559,17 @@ int grid_spec()
 // grid-wide on the tree's own stream, concurrently with the team launch and
 // with other trees' grid-wide passes.  Every round ends with all streams
 // drained, so the host's view of each control block is current.
+// Hooks of a run: `before_launch` is called once every tree's control
+// block is initialised in its pinned mirror and before anything of the run
+// is enqueued (the split kernels go there: they write n_ops into the
+// mirrors); `after_first_enqueue` once the first round's launches are
+// enqueued, before the host waits for them.
+struct RunHooks {
+    std::function<void()> before_launch, after_first_enqueue;
+};
+
 void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t n_ops, bool new_idx,
-                      rg_batch_stats *stats)
+                      rg_batch_stats *stats, bool split = false, const RunHooks *hooks = nullptr)
 {
     if (n <= 0) return;
     rg_graph *g = L[0]->g;
@@ -555,10 +580,13 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
         L[i]->run_w = team_width(L[i], n_ops);
         engine_init(L[i], phase0, seed, n_ops, new_idx);
         // a run that starts in the team hands its initial control block to
-        // the kernel through the pinned mirror; otherwise copy it now
+        // the kernel through the pinned mirror; otherwise it is copied below
         if (team_ok(L[i], *L[i]->hc)) pending_init[i] = 1;
-        else HC(hipMemcpyAsync(L[i]->ctl, L[i]->hc, sizeof(Ctl), hipMemcpyHostToDevice, st));
-        args[i] = engine_args(L[i]);
+    }
+    if (hooks && hooks->before_launch) hooks->before_launch();
+    for (int i = 0; i < n; i++) {
+        if (!pending_init[i]) HC(hipMemcpyAsync(L[i]->ctl, L[i]->hc, sizeof(Ctl), hipMemcpyHostToDevice, st));
+        args[i] = engine_args(L[i], split);
         // the team kernel reads the arguments from device memory; refresh
         // them when they changed (op buffers regrown, tuning)
         if (memcmp(L[i]->hargs, &args[i], sizeof(EngineArgs)) != 0) {
@@ -632,6 +660,7 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
             if (eb) HC(hipEventRecord(eb, st));
         }
         HC(hipGetLastError());
+        if (guard_iter == 0 && hooks && hooks->after_first_enqueue) hooks->after_first_enqueue();
         HC(hipStreamSynchronize(st));
         for (int i = 0; i < n; i++)
             if (on_own[i]) HC(hipStreamSynchronize(L[i]->st));
@@ -731,7 +760,11 @@ int rg_graph_create(uint64_t vertices, uint64_t hint_edges, int symmetric, int d
         g->dual = dual;
         g->weighted = weighted;
         HC(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
+        HC(hipStreamCreateWithFlags(&g->st_copy, hipStreamNonBlocking));
         for (auto &e : g->ev) HC(hipEventCreate(&e));
+        HC(hipEventCreateWithFlags(&g->ev_cls, hipEventDisableTiming));
+        HC(hipEventCreateWithFlags(&g->ev_safe, hipEventDisableTiming));
+        g->h_ucount = halloc<uint32_t>(1);
         alloc_list(g->lists[0], vertices, g->st);
         if (!symmetric) alloc_list(g->lists[1], vertices, g->st);
         else g->lists[1] = g->lists[0];
@@ -771,6 +804,7 @@ int rg_graph_destroy(rg_graph *g)
 {
     if (!g) return RG_OK;
     if (g->st) (void)hipStreamSynchronize(g->st);
+    if (g->st_copy) (void)hipStreamSynchronize(g->st_copy);
     free_list(g->lists[0]);
     if (!g->symmetric) free_list(g->lists[1]);
     dfree(g->slots.nbr); dfree(g->slots.cnt); dfree(g->slots.wt); dfree(g->slots.hkey); dfree(g->slots.hval);
@@ -780,6 +814,10 @@ int rg_graph_destroy(rg_graph *g)
     hfree(g->h_src); hfree(g->h_dst); hfree(g->h_w); hfree(g->h_res); hfree(g->h_kind); hfree(g->h_flag);
     hfree(g->h_status);
     for (auto &e : g->ev) if (e) (void)hipEventDestroy(e);
+    if (g->ev_cls) (void)hipEventDestroy(g->ev_cls);
+    if (g->ev_safe) (void)hipEventDestroy(g->ev_safe);
+    hfree(g->h_ucount);
+    if (g->st_copy) (void)hipStreamDestroy(g->st_copy);
     for (auto &r : g->prof_pending) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -1279,8 +1317,22 @@ int rg_stream_load(rg_graph *g, rg_stream *s, uint64_t first, uint64_t n, int di
     });
 }
 
+// What the safe/unsafe split of one batch reports to the serving loop.
+struct SplitOut {
+    bool flags = false;         // copy the per-op unsafe flags to the host
+    bool done = false;          // the batch was split
+    std::chrono::steady_clock::time_point t_safe{};   // the safe ops completed (host clock)
+    uint64_t unsafe = 0;        // unsafe ops of the batch
+    const uint8_t *flag = nullptr;   // pinned, one per op (ops [0, n_ins) inserts, then deletes)
+};
+
+// batches of more than this many edge ops are split into safe and unsafe
+// ops before the repair (smaller ones run the repair on all their ops)
+constexpr uint64_t SPLIT_MIN_OPS = STORE_SMALL_OPS;
+
 static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first, uint64_t n_ins,
-                        uint64_t del_first, uint64_t n_del, int directed, rg_batch_stats *stats)
+                        uint64_t del_first, uint64_t n_del, int directed, rg_batch_stats *stats,
+                        SplitOut *so = nullptr)
 {
     // ranges wrap around the end of the stream (a sliding window over a
     // circular stream: update k inserts e[(n0 + k) mod n], deletes e[k mod n])
@@ -1289,22 +1341,70 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
     for (int i = 0; i < nL; i++)
         if (!L[i] || L[i]->g != g) throw RgError{RG_EINVAL, "labels belong to another graph"};
     uint64_t n = n_ins + n_del;
+    if (so) so->done = false;
     if (!n) return;
     ensure_ops(g, n);
+    const bool split = nL > 0 && nL <= MAXT && n > SPLIT_MIN_OPS;
+    const bool flags = split && so && so->flags;
+    if (flags) ensure_host(g, n);
     uint8_t ud = directed ? 0 : K_UNDIR;
     const uint32_t *w = g->weighted ? s->w : nullptr;
     if (g->weighted && !w) throw RgError{RG_EINVAL, "weighted graph needs a weighted stream"};
     launch_ops_from_window(s->src, s->dst, w, s->n, ins_first, (uint32_t)n_ins, del_first, (uint32_t)n_del,
-                           (uint8_t)(K_INS | ud), (uint8_t)(K_DEL | ud), g->ops, g->st);
+                           (uint8_t)(K_INS | ud), (uint8_t)(K_DEL | ud), g->ops, split ? g->ucount : nullptr, g->st);
     g->ops.n = (uint32_t)n;
     g->lops.n = (uint32_t)(4 * n);
     HC(hipEventRecord(g->ev[0], g->st));
     run_store(g);
     HC(hipEventRecord(g->ev[1], g->st));
     uint32_t ph = n_del ? PH_DEL_ROOTS : PH_SEED;
-    engine_run_multi(L, nL, ph, n_ins && n_del, (uint32_t)n, n_del > 0, stats);
+    RunHooks hooks;
+    if (split) {
+        SplitParams P{};
+        for (int i = 0; i < nL; i++) {
+            const EngineArgs a = engine_args(L[i]);
+            P.vr[i] = a.vr;
+            P.algo[i] = a.algo;
+            P.undirected[i] = a.undirected;
+            P.hctl[i] = L[i]->hc;
+        }
+        P.ntrees = nL;
+        P.in = g->ops;
+        P.out = g->uops;
+        P.unsafe = flags ? g->d_flag : nullptr;
+        P.count = g->ucount;
+        hooks.before_launch = [g, P, n, flags] {
+            launch_split(P, (uint32_t)n, g->st);
+            HC(hipGetLastError());
+            if (flags) {
+                // the flags travel on their own stream while the repair runs
+                HC(hipEventRecord(g->ev_cls, g->st));
+                HC(hipStreamWaitEvent(g->st_copy, g->ev_cls, 0));
+                HC(hipMemcpyAsync(g->h_flag, g->d_flag, n, hipMemcpyDeviceToHost, g->st_copy));
+                HC(hipMemcpyAsync(g->h_ucount, g->ucount, 4, hipMemcpyDeviceToHost, g->st_copy));
+                HC(hipEventRecord(g->ev_safe, g->st_copy));
+            }
+        };
+        if (flags)
+            hooks.after_first_enqueue = [g, so] {
+                HC(hipEventSynchronize(g->ev_safe));
+                so->t_safe = std::chrono::steady_clock::now();
+            };
+    }
+    engine_run_multi(L, nL, ph, n_ins && n_del, (uint32_t)n, n_del > 0, stats, split, &hooks);
     HC(hipEventSynchronize(g->ev[1]));
     store_status_check(g);
+    if (split) {
+        // each tree's control block kept the compacted op count as n_ops
+        g->last_unsafe = L[0]->hc->n_ops;
+        if (so) {
+            so->done = flags;
+            so->unsafe = g->last_unsafe;
+            so->flag = flags ? g->h_flag : nullptr;
+        }
+    } else {
+        g->last_unsafe = n;
+    }
     // device time of the batch: store start to the end of the engine run
     float ms = 0;
     if (nL > 0) {
@@ -1462,23 +1562,39 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
             arr[u] = t;
         }
         rg_batch_stats st{};
+        // cap: updates per batch, or unsafe updates per batch with cap_unsafe
+        // (a batch then admits cap / frac updates, frac = the running unsafe
+        // fraction of split batches)
+        const bool cap_u = p->cap_unsafe && p->split;
         double cap = p->batch_cap;
+        double frac = 0.25;
         const double cap_max = std::max<double>(p->cap_max ? p->cap_max : p->batch_cap, 1.0);
+        auto limit = [&] { return std::min(cap_max, std::max(1.0, cap_u ? cap / frac : cap)); };
         const double max_wait = std::max(10.0 * p->target_ms, 200.0) * 1e-3;
         uint64_t next = 0, batches = 0, max_batch = 0;
+        std::vector<uint8_t> waited(n, 1);   // 1: the update waited for its batch's repair
         bool overloaded = false;
         double gpu_ms = 0;
         auto t0 = std::chrono::steady_clock::now();
         auto now = [&] {
             return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         };
+        SplitOut so;
+        so.flags = p->split != 0;
         auto run = [&](uint64_t b, uint64_t e) {
             // updates [b, e): adds are the even indices, deletes the odd ones
             uint64_t a0 = (b + 1) / 2, a1 = (e + 1) / 2;
             uint64_t d0 = b / 2, d1 = e / 2;
             apply_batch(g, L, nL, s, (p->ins_first + a0) % s->n, a1 - a0, (p->del_first + d0) % s->n, d1 - d0,
-                        p->directed, &st);
+                        p->directed, &st, p->split ? &so : nullptr);
             gpu_ms += g->last_ms;
+            if (p->split && so.done) {
+                // op index of update u: inserts first, then deletes
+                for (uint64_t u = b; u < e; u++) {
+                    const uint64_t op = (u & 1) ? (a1 - a0) + (u / 2 - d0) : u / 2 - a0;
+                    waited[u] = so.flag[op];
+                }
+            }
         };
         while (next < n) {
             double tn = now();
@@ -1488,16 +1604,19 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
                 break;
             }
             uint64_t arrived = (uint64_t)(std::upper_bound(arr.begin() + next, arr.end(), tn) - arr.begin());
-            uint64_t end = std::min<uint64_t>(arrived, next + (uint64_t)std::max(1.0, std::floor(cap)));
+            uint64_t end = std::min<uint64_t>(arrived, next + (uint64_t)std::floor(limit()));
             run(next, end);
             double tc = now();
-            for (uint64_t u = next; u < end; u++) lat[u] = (tc - arr[u]) * 1e3;
+            const double ts = so.done ? std::chrono::duration<double>(so.t_safe - t0).count() : tc;
+            for (uint64_t u = next; u < end; u++) lat[u] = ((waited[u] ? tc : ts) - arr[u]) * 1e3;
+            if (so.done) frac = std::max(0.01, 0.8 * frac + 0.2 * (double)so.unsafe / (double)(end - next));
             if (p->target_ms > 0) {
                 double service_ms = (tc - tn) * 1e3;
                 uint64_t backlog = (uint64_t)(std::upper_bound(arr.begin() + end, arr.end(), tc) - arr.begin()) - end;
                 // a backlog grows the cap first (larger batches drain it in
                 // fewer fixed costs); without one, a slow batch shrinks it
-                if ((double)backlog >= cap) cap = std::min(cap_max, cap * 1.5 + 16);
+                const double cap_top = cap_u ? cap_max * frac : cap_max;
+                if ((double)backlog >= limit()) cap = std::min(cap_top, cap * 1.5 + 16);
                 else if (service_ms > 0.25 * p->target_ms) cap = std::max(1.0, cap * 0.7);
             }
             batches++;
@@ -1512,6 +1631,15 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
             uint64_t end = std::min<uint64_t>(n, next + (uint64_t)cap_max);
             run(next, end);
             next = end;
+        }
+        {
+            std::vector<double> ls, lu;
+            for (uint64_t u = 0; u < served; u++) (waited[u] ? lu : ls).push_back(lat[u]);
+            r->unsafe = lu.size();
+            r->safe_frac = served ? (double)ls.size() / (double)served : 0;
+            r->p50_safe_ms = percentile(ls, 0.50);
+            r->p50_unsafe_ms = percentile(lu, 0.50);
+            r->p999_unsafe_ms = percentile(lu, 0.999);
         }
         r->overloaded = overloaded ? 1 : 0;
         r->served = served;

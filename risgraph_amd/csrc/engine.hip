@@ -1382,6 +1382,79 @@ void launch_classify(const EngineArgs &a, uint32_t n, uint8_t *unsafe, hipStream
     }
 }
 
+template <int A>
+__device__ __forceinline__ bool op_unsafe(const VRec *vr, bool ins, bool both, uint32_t s, uint32_t d, uint32_t w)
+{
+    if (ins) {
+        bool u = better<A>(cand<A>(vval(vr[s].val), w), vval(vr[d].val));
+        if (both) u = u || better<A>(cand<A>(vval(vr[d].val), w), vval(vr[s].val));
+        return u;
+    }
+    bool u = vpar(vr[d].val) == s;
+    if (both) u = u || vpar(vr[s].val) == d;
+    return u;
+}
+
+// One thread per edge op of a store batch, before any tree's repair starts:
+// the labels are still the fixpoint of the graph before the batch, so an op
+// that is safe here changes no label (an insert that improves nothing, a
+// delete of no tree edge), and the batch's unsafe ops alone seed the repair
+// (a safe insert that a deletion later makes useful is found by the pull or
+// push over the store, where it already is).
+__global__ void k_split(SplitParams P, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool u = false;
+    uint32_t s = 0, d = 0, w = 0;
+    uint8_t k = 0;
+    OpResult r{};
+    if (i < n) {
+        k = P.in.kind[i];
+        r = P.in.res[i];
+        s = P.in.src[i];
+        d = P.in.dst[i];
+        w = P.in.w[i];
+        const bool ins = (k & K_INS) != 0;
+        const bool cand = (k & K_FORCE) || (ins ? (r.old == 0 && r.live != 0) : ((k & K_DEL) && r.old == 1));
+        if (cand) {
+            for (int t = 0; t < P.ntrees && !u; t++) {
+                const bool both = P.undirected[t] || (k & K_UNDIR);
+                switch (P.algo[t]) {
+                case BFS: u = op_unsafe<BFS>(P.vr[t], ins, both, s, d, w); break;
+                case SSSP: u = op_unsafe<SSSP>(P.vr[t], ins, both, s, d, w); break;
+                case SSWP: u = op_unsafe<SSWP>(P.vr[t], ins, both, s, d, w); break;
+                default: u = op_unsafe<WCC>(P.vr[t], ins, both, s, d, w); break;
+                }
+            }
+        }
+        if (P.unsafe) P.unsafe[i] = u ? 1 : 0;
+    }
+    const uint32_t pos = u ? wave_reserve(P.count) : 0u;
+    if (u) {
+        P.out.src[pos] = s;
+        P.out.dst[pos] = d;
+        P.out.w[pos] = w;
+        P.out.kind[pos] = k;
+        P.out.res[pos] = r;
+    }
+}
+
+// the compacted count becomes each tree's n_ops in its pinned control-block
+// mirror, from which the run's first launch takes its control block
+__global__ void k_split_publish(SplitParams P)
+{
+    const uint32_t c = __hip_atomic_load(P.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int t = 0; t < P.ntrees; t++)
+        if (P.hctl[t]) P.hctl[t]->n_ops = c;
+    __threadfence_system();
+}
+
+void launch_split(const SplitParams &p, uint32_t n, hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_split, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    hipLaunchKernelGGL(k_split_publish, dim3(1), dim3(1), 0, st, p);
+}
+
 // init labels of the reference drivers: BFS/SSSP root 0 else MAXL, SSWP
 // root MAXL else 0 (MAXL = 65536 for BFS, 2^27 otherwise); WCC takes the
 // host-provided labels (identity or a permutation) and is not filled here.

@@ -320,13 +320,34 @@ void launch_gen_er(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, uin
                    uint64_t seed, hipStream_t s);
 void launch_gen_weights(uint32_t *w, uint64_t n, uint64_t start, uint64_t seed, hipStream_t s);
 void launch_classify(const EngineArgs &a, uint32_t n, uint8_t *unsafe, hipStream_t s);
+
+// Safe/unsafe split of a store batch for up to MAXT trees (the classify
+// step of the rt drivers' safe phase, bfs_inc_rt.cpp:211-326): an edge op is
+// unsafe when it reached the engine's candidates (an insert that created a
+// live entry, a delete that removed the last copy) and need_update_tree_add
+// / need_update_tree_del holds for it in some tree.  The unsafe ops are
+// compacted into `out` (order not kept), their count goes to *count (zeroed
+// by the caller's earlier kernel) and to each tree's pinned control-block
+// mirror as the engine run's n_ops; unsafe[i] (may be null) flags op i.
+struct SplitParams {
+    const VRec *vr[MAXT];
+    int algo[MAXT];
+    int undirected[MAXT];
+    Ctl *hctl[MAXT];
+    int ntrees;
+    EdgeOps in, out;
+    uint8_t *unsafe;
+    uint32_t *count;
+};
+void launch_split(const SplitParams &p, uint32_t n, hipStream_t s);
 void launch_max_degree(const uint64_t *src, uint64_t n, uint64_t V, uint32_t *deg_tmp,
                        unsigned long long *best, hipStream_t s);
 void launch_ops_from_stream(const uint64_t *src, const uint64_t *dst, const uint32_t *w,
                             uint64_t first, uint32_t n, uint8_t kind, EdgeOps ops,
                             uint32_t base, hipStream_t s);
+// `zero` (may be null) is set to 0 by the launch (the split's counter)
 void launch_ops_from_window(const uint64_t *src, const uint64_t *dst, const uint32_t *w, uint64_t len,
                             uint64_t ins_first, uint32_t n_ins, uint64_t del_first, uint32_t n_del,
-                            uint8_t kind_ins, uint8_t kind_del, EdgeOps ops, hipStream_t s);
+                            uint8_t kind_ins, uint8_t kind_del, EdgeOps ops, uint32_t *zero, hipStream_t s);
 
 }  // namespace rg

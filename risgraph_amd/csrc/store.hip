@@ -1427,9 +1427,10 @@ void launch_ops_from_stream(const uint64_t *src, const uint64_t *dst, const uint
 // delete stream edges del_first.., both positions taken modulo len
 __global__ void k_ops_from_window(const uint64_t *src, const uint64_t *dst, const uint32_t *w, uint64_t len,
                                   uint64_t ins_first, uint32_t n_ins, uint64_t del_first, uint32_t n_del,
-                                  uint8_t kind_ins, uint8_t kind_del, EdgeOps ops)
+                                  uint8_t kind_ins, uint8_t kind_del, EdgeOps ops, uint32_t *zero)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && zero) *zero = 0;
     if (i >= n_ins + n_del) return;
     const bool ins = i < n_ins;
     uint64_t e = ins ? ins_first + i : del_first + (i - n_ins);
@@ -1442,12 +1443,12 @@ __global__ void k_ops_from_window(const uint64_t *src, const uint64_t *dst, cons
 
 void launch_ops_from_window(const uint64_t *src, const uint64_t *dst, const uint32_t *w, uint64_t len,
                             uint64_t ins_first, uint32_t n_ins, uint64_t del_first, uint32_t n_del,
-                            uint8_t kind_ins, uint8_t kind_del, EdgeOps ops, hipStream_t s)
+                            uint8_t kind_ins, uint8_t kind_del, EdgeOps ops, uint32_t *zero, hipStream_t s)
 {
     const uint32_t n = n_ins + n_del;
     if (!n) return;
     hipLaunchKernelGGL(k_ops_from_window, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, w, len, ins_first,
-                       n_ins, del_first, n_del, kind_ins, kind_del, ops);
+                       n_ins, del_first, n_del, kind_ins, kind_del, ops, zero);
 }
 
 }  // namespace rg

@@ -651,3 +651,48 @@ def test_small_batches_on_hub_lists(weighted):
     assert after[0]["units"] - before[0]["units"] == 3000
     assert after[1]["launches"] - before[1]["launches"] == 1
     assert after[1]["units"] - before[1]["units"] == 40
+
+
+@pytest.mark.parametrize("cap_unsafe", [False, True], ids=["cap_total", "cap_unsafe"])
+def test_serving_split_matches_cpu_ref(cap_unsafe):
+    """The serving loop with the safe/unsafe split (bfs_inc_rt.cpp:211-326):
+    safe updates complete at the split, unsafe ones after the repair, and the
+    admission cap counts all updates or only unsafe ones.  The four trees
+    (BFS/SSSP/SSWP on the directed store, WCC on its undirected view) must
+    end bit-exact against cpu_ref running the same all_inc alternation
+    (all_inc.cpp:254-300) and against the closed form on the final window."""
+    V, E = 4096, 1 << 16
+    s = rg.Stream("rmat", 12, E, seed=17, wseed=18)
+    src, dst, w = s.read()
+    w = w.astype(np.uint64)
+    n0 = int(0.9 * E)
+    g = rg.Graph(V, E, symmetric=False, dual=True, weighted=True)
+    g.load_stream(s, 0, n0, True)
+    root = s.max_degree_vertex(n0, V)
+    Ls = [g.alloc_vertex_tree_array(a, root) for a in (rg.BFS, rg.SSSP, rg.SSWP, rg.WCC)]
+    for L in Ls:
+        L.build_tree()
+    n = 12000
+    res, lat = g.serve(Ls, s, n0, 0, n, 4e6, batch_cap=64, cap_max=4096, target_ms=4.0, latencies=True,
+                       split=True, cap_unsafe=cap_unsafe)
+    assert res["served"] == n and not res["overloaded"]
+    assert 0.2 < res["safe_frac"] < 1.0, res
+    assert res["p50_safe_ms"] <= res["p50_unsafe_ms"], res
+    assert np.all(lat >= 0)
+    # cpu_ref on the same alternation: BFS/SSSP/SSWP on a directed graph, WCC on a symmetric one
+    gd = oracle.RefGraph(V, E, symmetric=False, dual=True, weighted=True)
+    gd.bulk_load(src[:n0], dst[:n0], w[:n0], True)
+    gu = oracle.RefGraph(V, E, symmetric=True, dual=False, weighted=False)
+    gu.bulk_load(src[:n0], dst[:n0], None, False)
+    Ld = [oracle.RefLabels(gd, a, root) for a in (rg.BFS, rg.SSSP, rg.SSWP)]
+    Lu = oracle.RefLabels(gu, rg.WCC, root)
+    for R in Ld + [Lu]:
+        R.build()
+    oracle.stream_mixed(gd, Ld, gu, Lu, n, src[n0:], dst[n0:], w[n0:], src, dst, w, latency=False)
+    k = n // 2
+    ls, ld, lw = src[k:n0 + k], dst[k:n0 + k], w[k:n0 + k]
+    for L, R in zip(Ls, Ld + [Lu]):
+        vals, par = L.read()
+        assert np.array_equal(vals, R.read()[0]), L.algo
+        assert np.array_equal(vals, oracle.static_values(L.algo, V, ls, ld, lw, root)), L.algo
+        assert oracle.check_tree(L.algo, V, ls, ld, lw, root, vals, par, undirected=L.algo == rg.WCC) == 0
