@@ -137,6 +137,32 @@ int rg_modified_read(rg_labels *L, uint64_t *vids, uint64_t cap, uint64_t *n, in
 /* clear_modified() (graph.hpp:127-130). */
 int rg_modified_clear(rg_labels *L);
 
+/* Version clock of the device batches: every rg_stream_apply batch (and
+ * every batch of rg_serve) that applies at least one update advances it by
+ * one.  The safe updates of a split batch take effect at the version before
+ * it, its unsafe ones at the version it ends (the rt drivers' label_version,
+ * bfs_inc_rt.cpp:340-390). */
+int rg_graph_version(rg_graph *g, uint64_t *version);
+
+/* Device-resident value history (MVVec, mvvec.hpp:23-162, filled per batch
+ * as bfs_inc_rt.cpp:353-358 fills it from the modified set).  enable takes a
+ * snapshot of the current values as the base at the current version and
+ * keeps a ring of `records` (version, value) records (24 bytes each); after
+ * every batch each vertex whose value changed gets one record of the
+ * batch's version.  The labels must have been created with trace_modified;
+ * the modified set then covers the last batch only.  When the ring is full
+ * the oldest batches are folded into the base (MVVec::gc); a batch changing
+ * more vertices than the ring holds fails with RG_ENOMEM.
+ * read: each vertex's value at `version` (MVVec::get), for versions from
+ * the oldest kept (rg_history_info) to the current one; others RG_EINVAL.
+ * gc: fold every batch of version <= `version` (reads at `version` and
+ * later stay exact).  Only stream / serve batches are recorded; the
+ * host-buffer calls (rg_edge_update*, rg_inc_*, rg_build) are not. */
+int rg_history_enable(rg_labels *L, uint64_t records);
+int rg_history_read(rg_labels *L, uint64_t version, uint64_t first, uint64_t count, uint64_t *values);
+int rg_history_gc(rg_labels *L, uint64_t version);
+int rg_history_info(rg_labels *L, uint64_t *oldest_version, uint64_t *records, uint64_t *capacity);
+
 /* Device-resident synthetic streams (edge arrays that never leave HBM).
  * kind 0 = R-MAT (scale_or_v = scale, a,b,c probabilities), 1 = Erdos-Renyi
  * (scale_or_v = V).  Weights are generated when wseed != 0. */

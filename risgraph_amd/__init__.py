@@ -75,6 +75,11 @@ SIGNATURES = {
     "rg_graph_create": (ci, [u64, u64, ci, ci, ci, ctypes.POINTER(vp)]),
     "rg_graph_destroy": (ci, [vp]),
     "rg_graph_info": (ci, [vp, u64p, u64p, u64p, u64p, u64p]),
+    "rg_graph_version": (ci, [vp, u64p]),
+    "rg_history_enable": (ci, [vp, u64]),
+    "rg_history_read": (ci, [vp, u64, u64, u64, vp]),
+    "rg_history_gc": (ci, [vp, u64]),
+    "rg_history_info": (ci, [vp, u64p, u64p, u64p]),
     "rg_edge_update_batch_static": (ci, [vp, vp, u64, ci, ci, vp]),
     "rg_edge_update": (ci, [vp, u64, u64, u64, ci, ci, u64p]),
     "rg_edge_update_batch": (ci, [vp, vp, u64, ci, ci, vp]),
@@ -307,6 +312,12 @@ class Graph:
         _check(load_library().rg_degrees(self.h, _ptr(out), _ptr(inc)))
         return out, inc
 
+    def version(self):
+        """rg_graph_version: device batches applied so far."""
+        v = u64()
+        _check(load_library().rg_graph_version(self.h, ctypes.byref(v)))
+        return int(v.value)
+
     def last_gpu_ms(self):
         ms = ctypes.c_double()
         _check(load_library().rg_last_gpu_ms(self.h, ctypes.byref(ms)))
@@ -410,6 +421,25 @@ class Labels:
         out = np.empty(int(n.value), np.uint64)
         _check(load_library().rg_modified_read(self.h, _ptr(out), len(out), ctypes.byref(n), ctypes.byref(allf)))
         return out[: int(n.value)], bool(allf.value)
+
+    def enable_history(self, records):
+        """rg_history_enable: keep a device ring of `records` value records."""
+        _check(load_library().rg_history_enable(self.h, int(records)))
+
+    def history_read(self, version, first=0, count=None):
+        """Values at a past version (rg_history_read)."""
+        count = self.g.V - first if count is None else count
+        out = np.empty(count, np.uint64)
+        _check(load_library().rg_history_read(self.h, int(version), first, count, _ptr(out)))
+        return out
+
+    def history_gc(self, version):
+        _check(load_library().rg_history_gc(self.h, int(version)))
+
+    def history_info(self):
+        a, b, c = u64(), u64(), u64()
+        _check(load_library().rg_history_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"oldest_version": int(a.value), "records": int(b.value), "capacity": int(c.value)}
 
     def clear_modified(self):
         _check(load_library().rg_modified_clear(self.h))

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <limits>
 #include <mutex>
@@ -125,6 +126,7 @@ struct rg_graph {
     hipEvent_t ev_cls = nullptr, ev_safe = nullptr;
     uint32_t *h_ucount = nullptr;     // pinned
     uint64_t last_unsafe = 0;         // unsafe ops of the last split batch
+    uint64_t version = 0;             // stream batches applied (the history's version clock)
     std::recursive_mutex mu;
     // kernel timing (rg_profile_enable)
     struct ProfRec {
@@ -170,6 +172,16 @@ struct rg_labels {
     uint32_t block_max = RG_AUTO;  // largest pass (vertices, or ops/16) run by the team
     uint32_t run_w = 8;            // team width of the current run
     uint32_t *init_active = nullptr;
+    // device history (rg_history_*): ring of value records per batch version
+    struct Hist {
+        bool on = false;
+        HistArgs a{};
+        uint64_t top = 0;                 // next logical record index
+        uint64_t base_version = 0;        // base holds the values at this version
+        uint32_t *dcount = nullptr;       // device: records appended by the last append
+        uint32_t *hcount = nullptr;       // pinned copy
+        std::deque<std::pair<uint64_t, uint64_t>> batches;   // (version, first record) in ring order
+    } hist;
 };
 
 struct rg_stream {
@@ -723,8 +735,68 @@ void engine_run(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool n
     engine_run_multi(&L, 1, phase0, seed, n_ops, new_idx, stats);
 }
 
+void hist_free(rg_labels *L)
+{
+    auto &h = L->hist;
+    dfree(h.a.rec); dfree(h.a.head); dfree(h.a.base); dfree(h.dcount);
+    hfree(h.hcount);
+    h = rg_labels::Hist{};
+}
+
+// fold the oldest batches of L's history until `need` more records fit,
+// or every batch with version <= `upto`
+void hist_fold(rg_labels *L, uint64_t need, uint64_t upto)
+{
+    auto &h = L->hist;
+    rg_graph *g = L->g;
+    uint64_t new_tail = h.a.tail;
+    uint64_t base_version = h.base_version;
+    while (!h.batches.empty() &&
+           ((need && h.top + need - new_tail > h.a.cap) || (!need && h.batches.front().first <= upto))) {
+        base_version = h.batches.front().first;
+        h.batches.pop_front();
+        new_tail = h.batches.empty() ? h.top : h.batches.front().second;
+    }
+    if (need && h.top + need - new_tail > h.a.cap)
+        throw RgError{RG_ENOMEM, "history ring smaller than one batch's changed vertices"};
+    if (new_tail == h.a.tail && base_version == h.base_version) return;
+    launch_hist_fold(h.a, new_tail, g->st);
+    HC(hipGetLastError());
+    h.a.tail = new_tail;
+    h.base_version = base_version;
+}
+
+// After a batch's repair: one record per vertex whose value changed
+// (the modified set, or every vertex after a full rebuild), then the
+// modified set starts over for the next batch.
+void hist_commit(rg_labels *L, uint64_t version)
+{
+    auto &h = L->hist;
+    rg_graph *g = L->g;
+    Ctl &c = *L->hc;
+    const bool all = (c.flags & F_ALL_MODIFIED) != 0;
+    const uint64_t n = all ? g->V : std::min<uint64_t>(c.n_mod, g->V);
+    if (n) {
+        hist_fold(L, n, 0);
+        HC(hipMemsetAsync(h.dcount, 0, 4, g->st));
+        launch_hist_append(h.a, L->vr, all ? nullptr : L->mlist, (uint32_t)n, (uint32_t)version, h.top, h.dcount,
+                           g->st);
+        HC(hipGetLastError());
+        HC(hipMemcpyAsync(h.hcount, h.dcount, 4, hipMemcpyDeviceToHost, g->st));
+        HC(hipStreamSynchronize(g->st));
+        if (*h.hcount) {
+            h.batches.emplace_back(version, h.top);
+            h.top += *h.hcount;
+        }
+    }
+    c.n_mod = 0;
+    c.mepoch += 1;
+    c.flags &= ~F_ALL_MODIFIED;
+}
+
 void labels_free(rg_labels *L)
 {
+    hist_free(L);
     dfree(L->vr); dfree(L->init); dfree(L->mstamp);
     dfree(L->q[0]); dfree(L->q[1]); dfree(L->q[2]); dfree(L->cq); dfree(L->bnd); dfree(L->mlist); dfree(L->ctl);
     dfree(L->init_active);
@@ -1148,6 +1220,94 @@ int rg_classify(rg_labels *L, const rg_edge *edges, const int8_t *kind, uint64_t
     });
 }
 
+int rg_graph_version(rg_graph *g, uint64_t *version)
+{
+    if (!g || !version) return RG_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
+    *version = g->version;
+    return RG_OK;
+}
+
+int rg_history_enable(rg_labels *L, uint64_t records)
+{
+    if (!L || records == 0) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (!L->trace) throw RgError{RG_EINVAL, "history needs labels created with trace_modified"};
+        hist_free(L);
+        auto &h = L->hist;
+        h.a.cap = records;
+        h.a.rec = dalloc<HRec>(records);
+        h.a.head = dalloc<uint64_t>(g->V);
+        h.a.base = dalloc<uint32_t>(g->V);
+        h.a.tail = 0;
+        h.dcount = dalloc<uint32_t>(1);
+        h.hcount = halloc<uint32_t>(1);
+        h.top = 0;
+        h.base_version = g->version;
+        launch_hist_init(h.a, L->vr, (uint32_t)g->V, g->st);
+        HC(hipGetLastError());
+        // the snapshot is the base; changes from here on are recorded
+        Ctl &c = *L->hc;
+        c.n_mod = 0;
+        c.mepoch += 1;
+        c.flags &= ~F_ALL_MODIFIED;
+        HC(hipMemcpyAsync(L->ctl, L->hc, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
+        HC(hipStreamSynchronize(g->st));
+        h.on = true;
+    });
+}
+
+int rg_history_read(rg_labels *L, uint64_t version, uint64_t first, uint64_t count, uint64_t *values)
+{
+    if (!L) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        auto &h = L->hist;
+        if (!h.on) throw RgError{RG_EINVAL, "history is not enabled on these labels"};
+        if (first + count > g->V || first + count < first) throw RgError{RG_EINVAL_VERTEX, "range out of bounds"};
+        if (version > g->version) throw RgError{RG_EINVAL, "version is in the future"};
+        if (version < h.base_version) throw RgError{RG_EINVAL, "version was folded (older than the history keeps)"};
+        if (!count) return;
+        uint64_t *dv = dalloc<uint64_t>(count);
+        launch_hist_read(h.a, (uint32_t)first, (uint32_t)count, (uint32_t)version, dv, g->st);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(values, dv, count * 8, hipMemcpyDeviceToHost, g->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(g->st);
+        dfree(dv);
+        hipchk(e, "history read");
+    });
+}
+
+int rg_history_gc(rg_labels *L, uint64_t version)
+{
+    if (!L) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (!L->hist.on) throw RgError{RG_EINVAL, "history is not enabled on these labels"};
+        hist_fold(L, 0, version);
+        HC(hipStreamSynchronize(g->st));
+    });
+}
+
+int rg_history_info(rg_labels *L, uint64_t *oldest_version, uint64_t *records, uint64_t *capacity)
+{
+    if (!L) return RG_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(L->g->mu);
+    const auto &h = L->hist;
+    if (!h.on) {
+        g_err = "history is not enabled on these labels";
+        return RG_EINVAL;
+    }
+    if (oldest_version) *oldest_version = h.base_version;
+    if (records) *records = h.top - h.a.tail;
+    if (capacity) *capacity = h.a.cap;
+    return RG_OK;
+}
+
 int rg_labels_read(rg_labels *L, uint64_t first, uint64_t count, uint64_t *values, uint64_t *parents)
 {
     if (!L) return RG_EINVAL;
@@ -1394,6 +1554,9 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
     engine_run_multi(L, nL, ph, n_ins && n_del, (uint32_t)n, n_del > 0, stats, split, &hooks);
     HC(hipEventSynchronize(g->ev[1]));
     store_status_check(g);
+    g->version += 1;
+    for (int i = 0; i < nL; i++)
+        if (L[i]->hist.on) hist_commit(L[i], g->version);
     if (split) {
         // each tree's control block kept the compacted op count as n_ops
         g->last_unsafe = L[0]->hc->n_ops;

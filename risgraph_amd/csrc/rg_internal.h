@@ -350,4 +350,29 @@ void launch_ops_from_window(const uint64_t *src, const uint64_t *dst, const uint
                             uint64_t ins_first, uint32_t n_ins, uint64_t del_first, uint32_t n_del,
                             uint8_t kind_ins, uint8_t kind_del, EdgeOps ops, uint32_t *zero, hipStream_t s);
 
+// Device history of one labels instance (history.hip): a ring of records
+// addressed by logical index, per-vertex chains through `prev`.
+constexpr uint64_t HNIL = ~0ull;
+struct HRec {
+    uint64_t prev;      // older record of the same vertex (logical index), HNIL = none
+    uint32_t version;
+    uint32_t value;
+    uint32_t vertex;
+    uint32_t pad;
+};
+static_assert(sizeof(HRec) == 24, "history record is 24 bytes");
+struct HistArgs {
+    HRec *rec;
+    uint64_t *head;      // [V] newest record of each vertex
+    uint32_t *base;      // [V] value at the base version
+    uint64_t cap;        // ring slots
+    uint64_t tail;       // records below this logical index are folded
+};
+void launch_hist_init(const HistArgs &h, const VRec *vr, uint32_t V, hipStream_t s);
+// list == nullptr: vertices [0, n)
+void launch_hist_append(const HistArgs &h, const VRec *vr, const uint32_t *list, uint32_t n, uint32_t version,
+                        uint64_t top, uint32_t *count, hipStream_t s);
+void launch_hist_fold(const HistArgs &h, uint64_t new_tail, hipStream_t s);
+void launch_hist_read(const HistArgs &h, uint32_t first, uint32_t n, uint32_t version, uint64_t *out, hipStream_t s);
+
 }  // namespace rg
