@@ -137,9 +137,26 @@ int rg_modified_read(rg_labels *L, uint64_t *vids, uint64_t cap, uint64_t *n, in
 /* clear_modified() (graph.hpp:127-130). */
 int rg_modified_clear(rg_labels *L);
 
-/* Version clock of the device batches: every rg_stream_apply batch (and
- * every batch of rg_serve) that applies at least one update advances it by
- * one.  The safe updates of a split batch take effect at the version before
+/* A batch of transactions from host buffers (the transaction server of
+ * all_inc_rpc_txn.cpp; ClientTxnUpdateRequest, rpc.hpp:40-61): ntxn
+ * transactions of txn_len[t] updates each (1..RG_TXN_MAX), the updates laid
+ * out one transaction after another in edges / kind (+1 add, -1 del).  All
+ * of them go to the store as one device batch (inserts before deletes, the
+ * batch drivers' order, bfs_inc_batch.cpp:142-178), then the updates are
+ * classified against the labels of every tree in L[0..nL) (1 <= nL <= 4):
+ * a transaction is unsafe when any of its updates is (all_inc_rpc_txn.cpp:
+ * is_safe is the AND over its parts), and only unsafe updates seed the
+ * repair.  txn_unsafe[t] (may be NULL) receives that flag and txn_version[t]
+ * (may be NULL) the version at which transaction t takes effect: the
+ * version before the batch for a safe transaction, the one the batch ends
+ * at for an unsafe one; one version per transaction either way. */
+#define RG_TXN_MAX 16
+int rg_txn_apply(rg_graph *g, rg_labels **L, int nL, const rg_edge *edges, const int8_t *kind, uint64_t n,
+                 const uint32_t *txn_len, uint64_t ntxn, int directed, uint8_t *txn_unsafe, uint64_t *txn_version);
+
+/* Version clock of the device batches: every rg_stream_apply and
+ * rg_txn_apply batch (and every batch of rg_serve) that applies at least one
+ * update advances it by one.  The safe updates of a split batch take effect at the version before
  * it, its unsafe ones at the version it ends (the rt drivers' label_version,
  * bfs_inc_rt.cpp:340-390). */
 int rg_graph_version(rg_graph *g, uint64_t *version);

@@ -75,6 +75,7 @@ SIGNATURES = {
     "rg_graph_create": (ci, [u64, u64, ci, ci, ci, ctypes.POINTER(vp)]),
     "rg_graph_destroy": (ci, [vp]),
     "rg_graph_info": (ci, [vp, u64p, u64p, u64p, u64p, u64p]),
+    "rg_txn_apply": (ci, [vp, vp, ci, vp, vp, u64, vp, u64, ci, vp, vp]),
     "rg_graph_version": (ci, [vp, u64p]),
     "rg_history_enable": (ci, [vp, u64]),
     "rg_history_read": (ci, [vp, u64, u64, u64, vp]),
@@ -311,6 +312,18 @@ class Graph:
         inc = np.empty(self.V, np.uint32)
         _check(load_library().rg_degrees(self.h, _ptr(out), _ptr(inc)))
         return out, inc
+
+    def txn_apply(self, labels, src, dst, w, kind, txn_len, directed=True):
+        """rg_txn_apply: a batch of transactions; returns (unsafe flags, versions) per transaction."""
+        arr = (vp * max(1, len(labels)))(*[L.h for L in labels])
+        e = _edges(src, dst, w)
+        k = np.ascontiguousarray(kind, dtype=np.int8)
+        tl = np.ascontiguousarray(txn_len, dtype=np.uint32)
+        unsafe = np.empty(len(tl), np.uint8)
+        ver = np.empty(len(tl), np.uint64)
+        _check(load_library().rg_txn_apply(self.h, arr, len(labels), _ptr(e), _ptr(k), len(e), _ptr(tl), len(tl),
+                                           int(directed), _ptr(unsafe), _ptr(ver)))
+        return unsafe.astype(bool), ver
 
     def version(self):
         """rg_graph_version: device batches applied so far."""

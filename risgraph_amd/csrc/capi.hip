@@ -1490,34 +1490,15 @@ struct SplitOut {
 // ops before the repair (smaller ones run the repair on all their ops)
 constexpr uint64_t SPLIT_MIN_OPS = STORE_SMALL_OPS;
 
-static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first, uint64_t n_ins,
-                        uint64_t del_first, uint64_t n_del, int directed, rg_batch_stats *stats,
-                        SplitOut *so = nullptr)
+// The repair of a store batch already enqueued on the graph stream (the
+// batch's n edge ops in g->ops, bracketed by ev[0] / ev[1]): the split of the
+// ops into safe and unsafe ones (when `split`), the engine run of every tree
+// on the unsafe ops, the store status, the version clock and the histories.
+static void repair_batch(rg_graph *g, rg_labels **L, int nL, uint64_t n, bool has_ins, bool has_del,
+                         rg_batch_stats *stats, bool split, SplitOut *so)
 {
-    // ranges wrap around the end of the stream (a sliding window over a
-    // circular stream: update k inserts e[(n0 + k) mod n], deletes e[k mod n])
-    if (ins_first >= s->n || del_first >= s->n || n_ins > s->n || n_del > s->n)
-        throw RgError{RG_EINVAL, "range out of bounds"};
-    for (int i = 0; i < nL; i++)
-        if (!L[i] || L[i]->g != g) throw RgError{RG_EINVAL, "labels belong to another graph"};
-    uint64_t n = n_ins + n_del;
-    if (so) so->done = false;
-    if (!n) return;
-    ensure_ops(g, n);
-    const bool split = nL > 0 && nL <= MAXT && n > SPLIT_MIN_OPS;
     const bool flags = split && so && so->flags;
-    if (flags) ensure_host(g, n);
-    uint8_t ud = directed ? 0 : K_UNDIR;
-    const uint32_t *w = g->weighted ? s->w : nullptr;
-    if (g->weighted && !w) throw RgError{RG_EINVAL, "weighted graph needs a weighted stream"};
-    launch_ops_from_window(s->src, s->dst, w, s->n, ins_first, (uint32_t)n_ins, del_first, (uint32_t)n_del,
-                           (uint8_t)(K_INS | ud), (uint8_t)(K_DEL | ud), g->ops, split ? g->ucount : nullptr, g->st);
-    g->ops.n = (uint32_t)n;
-    g->lops.n = (uint32_t)(4 * n);
-    HC(hipEventRecord(g->ev[0], g->st));
-    run_store(g);
-    HC(hipEventRecord(g->ev[1], g->st));
-    uint32_t ph = n_del ? PH_DEL_ROOTS : PH_SEED;
+    uint32_t ph = has_del ? PH_DEL_ROOTS : PH_SEED;
     RunHooks hooks;
     if (split) {
         SplitParams P{};
@@ -1551,7 +1532,7 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
                 so->t_safe = std::chrono::steady_clock::now();
             };
     }
-    engine_run_multi(L, nL, ph, n_ins && n_del, (uint32_t)n, n_del > 0, stats, split, &hooks);
+    engine_run_multi(L, nL, ph, has_ins && has_del, (uint32_t)n, has_del, stats, split, &hooks);
     HC(hipEventSynchronize(g->ev[1]));
     store_status_check(g);
     g->version += 1;
@@ -1578,10 +1559,84 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
     }
     g->last_ms = ms;
     if (g->prof) prof_collect(g);
+}
+
+static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first, uint64_t n_ins,
+                        uint64_t del_first, uint64_t n_del, int directed, rg_batch_stats *stats,
+                        SplitOut *so = nullptr)
+{
+    // ranges wrap around the end of the stream (a sliding window over a
+    // circular stream: update k inserts e[(n0 + k) mod n], deletes e[k mod n])
+    if (ins_first >= s->n || del_first >= s->n || n_ins > s->n || n_del > s->n)
+        throw RgError{RG_EINVAL, "range out of bounds"};
+    for (int i = 0; i < nL; i++)
+        if (!L[i] || L[i]->g != g) throw RgError{RG_EINVAL, "labels belong to another graph"};
+    uint64_t n = n_ins + n_del;
+    if (so) so->done = false;
+    if (!n) return;
+    ensure_ops(g, n);
+    const bool split = nL > 0 && nL <= MAXT && n > SPLIT_MIN_OPS;
+    if (split && so && so->flags) ensure_host(g, n);
+    uint8_t ud = directed ? 0 : K_UNDIR;
+    const uint32_t *w = g->weighted ? s->w : nullptr;
+    if (g->weighted && !w) throw RgError{RG_EINVAL, "weighted graph needs a weighted stream"};
+    launch_ops_from_window(s->src, s->dst, w, s->n, ins_first, (uint32_t)n_ins, del_first, (uint32_t)n_del,
+                           (uint8_t)(K_INS | ud), (uint8_t)(K_DEL | ud), g->ops, split ? g->ucount : nullptr, g->st);
+    g->ops.n = (uint32_t)n;
+    g->lops.n = (uint32_t)(4 * n);
+    HC(hipEventRecord(g->ev[0], g->st));
+    run_store(g);
+    HC(hipEventRecord(g->ev[1], g->st));
+    repair_batch(g, L, nL, n, n_ins > 0, n_del > 0, stats, split, so);
     if (stats) {
         stats->seeds += n_ins;
         stats->roots += n_del;
     }
+}
+
+int rg_txn_apply(rg_graph *g, rg_labels **L, int nL, const rg_edge *edges, const int8_t *kind, uint64_t n,
+                 const uint32_t *txn_len, uint64_t ntxn, int directed, uint8_t *txn_unsafe, uint64_t *txn_version)
+{
+    if (!g || (nL && !L) || nL < 1 || nL > MAXT || (n && (!edges || !kind)) || (ntxn && !txn_len)) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        for (int i = 0; i < nL; i++)
+            if (!L[i] || L[i]->g != g) throw RgError{RG_EINVAL, "labels belong to another graph"};
+        uint64_t tot = 0;
+        for (uint64_t t = 0; t < ntxn; t++) {
+            if (txn_len[t] < 1 || txn_len[t] > RG_TXN_MAX)
+                throw RgError{RG_EINVAL, "a transaction holds 1 to 16 updates"};
+            tot += txn_len[t];
+        }
+        if (tot != n) throw RgError{RG_EINVAL, "transaction lengths do not add up to the update count"};
+        for (uint64_t i = 0; i < n; i++)
+            if (kind[i] != 1 && kind[i] != -1) throw RgError{RG_EINVAL, "kind must be +1 (add) or -1 (del)"};
+        const uint64_t v0 = g->version;
+        if (n == 0) return;
+        upload_edges(g, edges, n, 0);
+        const uint8_t ud = directed ? 0 : K_UNDIR;
+        bool has_ins = false, has_del = false;
+        for (uint64_t i = 0; i < n; i++) {
+            g->h_kind[i] = (uint8_t)((kind[i] > 0 ? K_INS : K_DEL) | ud);
+            has_ins |= kind[i] > 0;
+            has_del |= kind[i] < 0;
+        }
+        HC(hipMemcpyAsync(g->ops.kind, g->h_kind, n, hipMemcpyHostToDevice, g->st));
+        HC(hipMemsetAsync(g->ucount, 0, 4, g->st));
+        HC(hipEventRecord(g->ev[0], g->st));
+        run_store(g);
+        HC(hipEventRecord(g->ev[1], g->st));
+        SplitOut so;
+        so.flags = true;
+        repair_batch(g, L, nL, n, has_ins, has_del, nullptr, true, &so);
+        uint64_t k = 0;
+        for (uint64_t t = 0; t < ntxn; t++) {
+            bool u = false;
+            for (uint32_t j = 0; j < txn_len[t]; j++, k++) u = u || so.flag[k];
+            if (txn_unsafe) txn_unsafe[t] = u ? 1 : 0;
+            if (txn_version) txn_version[t] = u ? g->version : v0;
+        }
+    });
 }
 
 int rg_stream_apply(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first, uint64_t n_ins,
