@@ -154,6 +154,40 @@ int rg_modified_clear(rg_labels *L);
 int rg_txn_apply(rg_graph *g, rg_labels **L, int nL, const rg_edge *edges, const int8_t *kind, uint64_t n,
                  const uint32_t *txn_len, uint64_t ntxn, int directed, uint8_t *txn_unsafe, uint64_t *txn_version);
 
+/* Durability (the reference's WAL, all_inc_rpc.cpp:305-336, 572-590, plus
+ * the snapshot and replay it lacks).
+ * rg_wal_open: every later rg_txn_apply batch is appended to the file at
+ * `path` (created when absent; an existing log of the same graph shape is
+ * appended to) as one record of 32-byte entries before it is applied, and
+ * with sync = 1 fdatasync'ed first (group commit: the whole batch's
+ * transactions are durable before any of them is acknowledged).
+ * rg_wal_replay applies, in order, the logged batches whose version is
+ * above the graph's (the ones a snapshot does not hold) and stops at the
+ * first record that is cut short or fails its checksum; *batches (may be
+ * NULL) counts the batches applied.  Only rg_txn_apply batches are logged. */
+int rg_wal_open(rg_graph *g, const char *path, int sync);
+int rg_wal_close(rg_graph *g);
+int rg_wal_replay(rg_graph *g, rg_labels **L, int nL, const char *path, uint64_t *batches);
+/* rg_snapshot_save writes the graph's shape and version, the live edge
+ * multiset of its out-lists and, for each labels instance in L[0..nL), its
+ * algorithm, root, WCC init labels, values and parents (written to
+ * path.tmp, fsync'ed, renamed).  rg_snapshot_load makes a new graph and
+ * labels from such a file (at most `cap` labels into L; *nL = how many),
+ * at the snapshot's version; the graph's edge lists are rebuilt from the
+ * multiset, so their entry order (and dropped tombstones) may differ. */
+int rg_snapshot_save(rg_graph *g, rg_labels **L, int nL, const char *path);
+int rg_snapshot_load(const char *path, rg_graph **g, rg_labels **L, int cap, int *nL);
+/* A device-resident stream made from host edges (an edge file read with
+ * mmap_binary, io.hpp:26-36, or a snapshot's edges); weights from data. */
+int rg_stream_from_edges(const rg_edge *edges, uint64_t n, int weighted, rg_stream **out);
+/* The shape a graph was created with, and a labels instance's algorithm and
+ * root (for handles that rg_snapshot_load made). */
+int rg_graph_shape(rg_graph *g, int *symmetric, int *dual, int *weighted);
+int rg_labels_algo(rg_labels *L, int *algo, uint64_t *root);
+/* Overwrite a labels instance with given values and parent vertices
+ * (parent V = empty); the modified set then reports every vertex. */
+int rg_labels_assign(rg_labels *L, const uint64_t *values, const uint64_t *parents);
+
 /* Version clock of the device batches: every rg_stream_apply and
  * rg_txn_apply batch (and every batch of rg_serve) that applies at least one
  * update advances it by one.  The safe updates of a split batch take effect at the version before

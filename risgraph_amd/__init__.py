@@ -76,6 +76,15 @@ SIGNATURES = {
     "rg_graph_destroy": (ci, [vp]),
     "rg_graph_info": (ci, [vp, u64p, u64p, u64p, u64p, u64p]),
     "rg_txn_apply": (ci, [vp, vp, ci, vp, vp, u64, vp, u64, ci, vp, vp]),
+    "rg_wal_open": (ci, [vp, ctypes.c_char_p, ci]),
+    "rg_wal_close": (ci, [vp]),
+    "rg_wal_replay": (ci, [vp, vp, ci, ctypes.c_char_p, u64p]),
+    "rg_snapshot_save": (ci, [vp, vp, ci, ctypes.c_char_p]),
+    "rg_snapshot_load": (ci, [ctypes.c_char_p, ctypes.POINTER(vp), vp, ci, ctypes.POINTER(ci)]),
+    "rg_stream_from_edges": (ci, [vp, u64, ci, ctypes.POINTER(vp)]),
+    "rg_labels_assign": (ci, [vp, vp, vp]),
+    "rg_graph_shape": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
+    "rg_labels_algo": (ci, [vp, ctypes.POINTER(ci), u64p]),
     "rg_graph_version": (ci, [vp, u64p]),
     "rg_history_enable": (ci, [vp, u64]),
     "rg_history_read": (ci, [vp, u64, u64, u64, vp]),
@@ -173,6 +182,16 @@ class Stream:
         w = np.empty(n, np.uint32) if self.weighted else None
         _check(load_library().rg_stream_read(self.h, first, n, _ptr(s), _ptr(d), _ptr(w)))
         return s, d, w
+
+    @classmethod
+    def from_edges(cls, src, dst, w=None):
+        """rg_stream_from_edges: a device stream of the given host edges."""
+        e = _edges(src, dst, w)
+        h = vp()
+        _check(load_library().rg_stream_from_edges(_ptr(e), len(e), int(w is not None), ctypes.byref(h)))
+        obj = cls.__new__(cls)
+        obj.h, obj.n, obj.weighted = h, len(e), w is not None
+        return obj
 
     def max_degree_vertex(self, n, V):
         out = u64()
@@ -313,6 +332,35 @@ class Graph:
         _check(load_library().rg_degrees(self.h, _ptr(out), _ptr(inc)))
         return out, inc
 
+    @classmethod
+    def _wrap(cls, h):
+        obj = cls.__new__(cls)
+        obj.h = h
+        V = u64()
+        _check(load_library().rg_graph_info(h, ctypes.byref(V), None, None, None, None))
+        sym, dual, wt = ci(), ci(), ci()
+        _check(load_library().rg_graph_shape(h, ctypes.byref(sym), ctypes.byref(dual), ctypes.byref(wt)))
+        obj.V, obj.symmetric, obj.weighted, obj._labels = int(V.value), bool(sym.value), bool(wt.value), []
+        return obj
+
+    def wal_open(self, path, sync=True):
+        """rg_wal_open: log every later transaction batch to `path`."""
+        _check(load_library().rg_wal_open(self.h, os.fsencode(path), int(sync)))
+
+    def wal_close(self):
+        _check(load_library().rg_wal_close(self.h))
+
+    def wal_replay(self, labels, path):
+        """rg_wal_replay: apply the logged batches above the graph's version; returns their count."""
+        arr = (vp * max(1, len(labels)))(*[L.h for L in labels])
+        n = u64()
+        _check(load_library().rg_wal_replay(self.h, arr, len(labels), os.fsencode(path), ctypes.byref(n)))
+        return int(n.value)
+
+    def snapshot_save(self, labels, path):
+        arr = (vp * max(1, len(labels)))(*[L.h for L in labels])
+        _check(load_library().rg_snapshot_save(self.h, arr, len(labels), os.fsencode(path)))
+
     def txn_apply(self, labels, src, dst, w, kind, txn_len, directed=True):
         """rg_txn_apply: a batch of transactions; returns (unsafe flags, versions) per transaction."""
         arr = (vp * max(1, len(labels)))(*[L.h for L in labels])
@@ -435,6 +483,21 @@ class Labels:
         _check(load_library().rg_modified_read(self.h, _ptr(out), len(out), ctypes.byref(n), ctypes.byref(allf)))
         return out[: int(n.value)], bool(allf.value)
 
+    @classmethod
+    def _wrap(cls, g, h):
+        obj = cls.__new__(cls)
+        a, r = ci(), u64()
+        _check(load_library().rg_labels_algo(h, ctypes.byref(a), ctypes.byref(r)))
+        obj.g, obj.h, obj.algo, obj.root = g, h, int(a.value), int(r.value)
+        g._labels.append(obj)
+        return obj
+
+    def assign(self, values, parents):
+        """rg_labels_assign: set values and parent vertices (V = empty)."""
+        v = np.ascontiguousarray(values, dtype=np.uint64)
+        p = np.ascontiguousarray(parents, dtype=np.uint64)
+        _check(load_library().rg_labels_assign(self.h, _ptr(v), _ptr(p)))
+
     def enable_history(self, records):
         """rg_history_enable: keep a device ring of `records` value records."""
         _check(load_library().rg_history_enable(self.h, int(records)))
@@ -456,3 +519,13 @@ class Labels:
 
     def clear_modified(self):
         _check(load_library().rg_modified_clear(self.h))
+
+
+def snapshot_load(path, cap=8):
+    """rg_snapshot_load: (Graph, [Labels]) rebuilt from a snapshot file."""
+    g = vp()
+    arr = (vp * cap)()
+    n = ci()
+    _check(load_library().rg_snapshot_load(os.fsencode(path), ctypes.byref(g), arr, cap, ctypes.byref(n)))
+    G = Graph._wrap(g)
+    return G, [Labels._wrap(G, vp(arr[i])) for i in range(n.value)]

@@ -13,8 +13,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cerrno>
 #include <cstring>
 #include <deque>
+#include <fcntl.h>
+#include <unistd.h>
 #include <functional>
 #include <limits>
 #include <mutex>
@@ -127,6 +130,8 @@ struct rg_graph {
     uint32_t *h_ucount = nullptr;     // pinned
     uint64_t last_unsafe = 0;         // unsafe ops of the last split batch
     uint64_t version = 0;             // stream batches applied (the history's version clock)
+    int wal_fd = -1;                  // write-ahead log of transaction batches (rg_wal_open)
+    int wal_sync = 1;                 // fdatasync after every logged batch (group commit)
     std::recursive_mutex mu;
     // kernel timing (rg_profile_enable)
     struct ProfRec {
@@ -877,6 +882,7 @@ int rg_graph_destroy(rg_graph *g)
     if (!g) return RG_OK;
     if (g->st) (void)hipStreamSynchronize(g->st);
     if (g->st_copy) (void)hipStreamSynchronize(g->st_copy);
+    if (g->wal_fd >= 0) (void)close(g->wal_fd);
     free_list(g->lists[0]);
     if (!g->symmetric) free_list(g->lists[1]);
     dfree(g->slots.nbr); dfree(g->slots.cnt); dfree(g->slots.wt); dfree(g->slots.hkey); dfree(g->slots.hval);
@@ -1594,6 +1600,119 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
     }
 }
 
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------
+// Write-ahead log of transaction batches (the WAL of all_inc_rpc.cpp:305-336,
+// 572-590: 32-byte entries {type, client, src, dst, data}, group-committed
+// before the responses go out).  File: a 32-byte header {magic "RGWAL001",
+// V, weighted, symmetric}, then one record per batch: a 24-byte header
+// {magic, count, version the batch produces, FNV-1a checksum of the
+// entries} and `count` 32-byte entries {type (bit 0 delete, bit 1
+// undirected), transaction, src, dst, data}.  A batch is written and
+// (with sync) fdatasync'ed before it is applied; replay stops at the first
+// record that is cut short or fails its checksum (a batch that was never
+// acknowledged).
+// ---------------------------------------------------------------------
+constexpr char WAL_MAGIC[8] = {'R', 'G', 'W', 'A', 'L', '0', '0', '1'};
+constexpr uint32_t WAL_BATCH_MAGIC = 0x31424752u;   // "RGB1"
+struct WalFileHdr {
+    char magic[8];
+    uint64_t V;
+    uint32_t weighted, symmetric;
+    uint64_t pad;
+};
+struct WalBatchHdr {
+    uint32_t magic, count;
+    uint64_t version;
+    uint64_t checksum;
+};
+struct WalEntry {
+    uint32_t type, txn;
+    uint64_t src, dst, data;
+};
+static_assert(sizeof(WalFileHdr) == 32 && sizeof(WalBatchHdr) == 24 && sizeof(WalEntry) == 32, "WAL layout");
+
+uint64_t fnv1a(const void *p, size_t n)
+{
+    const unsigned char *b = (const unsigned char *)p;
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 0x100000001b3ull;
+    return h;
+}
+
+void write_all(int fd, const void *p, size_t n, const char *what)
+{
+    const char *c = (const char *)p;
+    while (n) {
+        ssize_t k = write(fd, c, n);
+        if (k < 0) {
+            if (errno == EINTR) continue;
+            throw RgError{RG_EDEVICE, std::string(what) + ": " + strerror(errno)};
+        }
+        c += k;
+        n -= (size_t)k;
+    }
+}
+
+bool read_all(int fd, void *p, size_t n)
+{
+    char *c = (char *)p;
+    while (n) {
+        ssize_t k = read(fd, c, n);
+        if (k < 0 && errno == EINTR) continue;
+        if (k <= 0) return false;
+        c += k;
+        n -= (size_t)k;
+    }
+    return true;
+}
+
+void wal_append(rg_graph *g, const rg_edge *e, const int8_t *kind, const uint32_t *txn_of, uint64_t n, int directed,
+                uint64_t version)
+{
+    std::vector<WalEntry> buf(n);
+    for (uint64_t i = 0; i < n; i++)
+        buf[i] = WalEntry{(kind[i] < 0 ? 1u : 0u) | (directed ? 0u : 2u), txn_of ? txn_of[i] : (uint32_t)i, e[i].src,
+                          e[i].dst, e[i].data};
+    WalBatchHdr h{WAL_BATCH_MAGIC, (uint32_t)n, version, fnv1a(buf.data(), n * sizeof(WalEntry))};
+    std::vector<char> rec(sizeof(h) + n * sizeof(WalEntry));
+    memcpy(rec.data(), &h, sizeof(h));
+    memcpy(rec.data() + sizeof(h), buf.data(), n * sizeof(WalEntry));
+    write_all(g->wal_fd, rec.data(), rec.size(), "WAL write");
+    if (g->wal_sync && fdatasync(g->wal_fd) != 0) throw RgError{RG_EDEVICE, std::string("WAL fdatasync: ") + strerror(errno)};
+}
+
+// One device batch of mixed host-buffer updates: logged first when a WAL is
+// open (and `log`), then the store batch and the repair of every tree, split
+// into safe and unsafe updates (so->flag[i] for update i).
+void ops_batch(rg_graph *g, rg_labels **L, int nL, const rg_edge *e, const int8_t *kind, const uint32_t *txn_of,
+               uint64_t n, int directed, SplitOut *so, bool log)
+{
+    if (log && g->wal_fd >= 0) wal_append(g, e, kind, txn_of, n, directed, g->version + 1);
+    upload_edges(g, e, n, 0);
+    const uint8_t ud = directed ? 0 : K_UNDIR;
+    bool has_ins = false, has_del = false;
+    for (uint64_t i = 0; i < n; i++) {
+        g->h_kind[i] = (uint8_t)((kind[i] > 0 ? K_INS : K_DEL) | ud);
+        has_ins |= kind[i] > 0;
+        has_del |= kind[i] < 0;
+    }
+    HC(hipMemcpyAsync(g->ops.kind, g->h_kind, n, hipMemcpyHostToDevice, g->st));
+    HC(hipMemsetAsync(g->ucount, 0, 4, g->st));
+    HC(hipEventRecord(g->ev[0], g->st));
+    run_store(g);
+    HC(hipEventRecord(g->ev[1], g->st));
+    const bool split = nL > 0 && nL <= MAXT;
+    repair_batch(g, L, nL, n, has_ins, has_del, nullptr, split, split ? so : nullptr);
+}
+
+}  // namespace
+
+extern "C" {
+
 int rg_txn_apply(rg_graph *g, rg_labels **L, int nL, const rg_edge *edges, const int8_t *kind, uint64_t n,
                  const uint32_t *txn_len, uint64_t ntxn, int directed, uint8_t *txn_unsafe, uint64_t *txn_version)
 {
@@ -1613,22 +1732,12 @@ int rg_txn_apply(rg_graph *g, rg_labels **L, int nL, const rg_edge *edges, const
             if (kind[i] != 1 && kind[i] != -1) throw RgError{RG_EINVAL, "kind must be +1 (add) or -1 (del)"};
         const uint64_t v0 = g->version;
         if (n == 0) return;
-        upload_edges(g, edges, n, 0);
-        const uint8_t ud = directed ? 0 : K_UNDIR;
-        bool has_ins = false, has_del = false;
-        for (uint64_t i = 0; i < n; i++) {
-            g->h_kind[i] = (uint8_t)((kind[i] > 0 ? K_INS : K_DEL) | ud);
-            has_ins |= kind[i] > 0;
-            has_del |= kind[i] < 0;
-        }
-        HC(hipMemcpyAsync(g->ops.kind, g->h_kind, n, hipMemcpyHostToDevice, g->st));
-        HC(hipMemsetAsync(g->ucount, 0, 4, g->st));
-        HC(hipEventRecord(g->ev[0], g->st));
-        run_store(g);
-        HC(hipEventRecord(g->ev[1], g->st));
+        std::vector<uint32_t> txn_of(n);
+        for (uint64_t t = 0, k = 0; t < ntxn; t++)
+            for (uint32_t j = 0; j < txn_len[t]; j++) txn_of[k++] = (uint32_t)t;
         SplitOut so;
         so.flags = true;
-        repair_batch(g, L, nL, n, has_ins, has_del, nullptr, true, &so);
+        ops_batch(g, L, nL, edges, kind, txn_of.data(), n, directed, &so, true);
         uint64_t k = 0;
         for (uint64_t t = 0; t < ntxn; t++) {
             bool u = false;
@@ -1637,6 +1746,374 @@ int rg_txn_apply(rg_graph *g, rg_labels **L, int nL, const rg_edge *edges, const
             if (txn_version) txn_version[t] = u ? g->version : v0;
         }
     });
+}
+
+int rg_wal_open(rg_graph *g, const char *path, int sync)
+{
+    if (!g || !path) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (g->wal_fd >= 0) throw RgError{RG_EINVAL, "a WAL is already open on this graph"};
+        int fd = open(path, O_RDWR | O_CREAT | O_APPEND, 0644);
+        if (fd < 0) throw RgError{RG_EINVAL, std::string("WAL open: ") + strerror(errno)};
+        off_t size = lseek(fd, 0, SEEK_END);
+        if (size == 0) {
+            WalFileHdr h{};
+            memcpy(h.magic, WAL_MAGIC, 8);
+            h.V = g->V;
+            h.weighted = (uint32_t)g->weighted;
+            h.symmetric = (uint32_t)g->symmetric;
+            try {
+                write_all(fd, &h, sizeof(h), "WAL header");
+            } catch (...) {
+                close(fd);
+                throw;
+            }
+            if (sync) (void)fsync(fd);
+        } else {
+            WalFileHdr h{};
+            if (pread(fd, &h, sizeof(h), 0) != (ssize_t)sizeof(h) || memcmp(h.magic, WAL_MAGIC, 8) != 0 ||
+                h.V != g->V || h.weighted != (uint32_t)g->weighted) {
+                close(fd);
+                throw RgError{RG_EINVAL, "WAL file belongs to another graph"};
+            }
+        }
+        g->wal_fd = fd;
+        g->wal_sync = sync ? 1 : 0;
+    });
+}
+
+int rg_wal_close(rg_graph *g)
+{
+    if (!g) return RG_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
+    if (g->wal_fd >= 0) {
+        (void)fdatasync(g->wal_fd);
+        (void)close(g->wal_fd);
+    }
+    g->wal_fd = -1;
+    return RG_OK;
+}
+
+int rg_wal_replay(rg_graph *g, rg_labels **L, int nL, const char *path, uint64_t *batches)
+{
+    if (!g || !path || (nL && !L) || nL < 0 || nL > MAXT) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        if (batches) *batches = 0;
+        int fd = open(path, O_RDONLY);
+        if (fd < 0) throw RgError{RG_EINVAL, std::string("WAL open: ") + strerror(errno)};
+        struct Closer {
+            int fd;
+            ~Closer() { close(fd); }
+        } closer{fd};
+        WalFileHdr fh{};
+        if (!read_all(fd, &fh, sizeof(fh)) || memcmp(fh.magic, WAL_MAGIC, 8) != 0)
+            throw RgError{RG_EINVAL, "not a WAL file"};
+        if (fh.V != g->V || fh.weighted != (uint32_t)g->weighted)
+            throw RgError{RG_EINVAL, "WAL file belongs to another graph"};
+        std::vector<WalEntry> ent;
+        std::vector<rg_edge> e;
+        std::vector<int8_t> kind;
+        for (;;) {
+            WalBatchHdr h{};
+            if (!read_all(fd, &h, sizeof(h)) || h.magic != WAL_BATCH_MAGIC) break;   // end or torn tail
+            ent.resize(h.count);
+            if (!read_all(fd, ent.data(), h.count * sizeof(WalEntry))) break;
+            if (fnv1a(ent.data(), h.count * sizeof(WalEntry)) != h.checksum) break;
+            if (h.version <= g->version) continue;   // already in the snapshot
+            if (h.version != g->version + 1) throw RgError{RG_EINVAL, "WAL has a gap after the graph's version"};
+            // a logged batch has one direction for all its entries
+            const int directed = h.count && (ent[0].type & 2u) ? 0 : 1;
+            e.resize(h.count);
+            kind.resize(h.count);
+            for (uint32_t i = 0; i < h.count; i++) {
+                e[i] = rg_edge{ent[i].src, ent[i].dst, ent[i].data};
+                kind[i] = (ent[i].type & 1u) ? -1 : 1;
+            }
+            if (h.count) ops_batch(g, L, nL, e.data(), kind.data(), nullptr, h.count, directed, nullptr, false);
+            else g->version += 1;
+            if (batches) (*batches)++;
+        }
+    });
+}
+
+int rg_graph_shape(rg_graph *g, int *symmetric, int *dual, int *weighted)
+{
+    if (!g) return RG_EINVAL;
+    if (symmetric) *symmetric = g->symmetric;
+    if (dual) *dual = g->dual;
+    if (weighted) *weighted = g->weighted;
+    return RG_OK;
+}
+
+int rg_labels_algo(rg_labels *L, int *algo, uint64_t *root)
+{
+    if (!L) return RG_EINVAL;
+    if (algo) *algo = L->algo;
+    if (root) *root = L->root;
+    return RG_OK;
+}
+
+int rg_stream_from_edges(const rg_edge *edges, uint64_t n, int weighted, rg_stream **out)
+{
+    if (!out || (n && !edges)) return RG_EINVAL;
+    *out = nullptr;
+    rg_stream *s = new rg_stream();
+    int rc = guard([&] {
+        HC(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        s->n = n;
+        s->src = dalloc<uint64_t>(n);
+        s->dst = dalloc<uint64_t>(n);
+        if (weighted) s->w = dalloc<uint32_t>(n);
+        const uint64_t B = 1ull << 22;
+        std::vector<uint64_t> a(std::min(B, std::max<uint64_t>(n, 1))), b(a.size());
+        std::vector<uint32_t> c(weighted ? a.size() : 0);
+        for (uint64_t f = 0; f < n; f += B) {
+            const uint64_t m = std::min(B, n - f);
+            for (uint64_t i = 0; i < m; i++) {
+                const rg_edge &e = edges[f + i];
+                if (e.src > MAX_V || e.dst > MAX_V) throw RgError{RG_EINVAL_VERTEX, "VertexId error."};
+                if (weighted && e.data > 0xFFFFFFFFull) throw RgError{RG_EINVAL, "edge weight exceeds 32 bits"};
+                a[i] = e.src;
+                b[i] = e.dst;
+                if (weighted) c[i] = (uint32_t)e.data;
+            }
+            HC(hipMemcpy(s->src + f, a.data(), m * 8, hipMemcpyHostToDevice));
+            HC(hipMemcpy(s->dst + f, b.data(), m * 8, hipMemcpyHostToDevice));
+            if (weighted) HC(hipMemcpy(s->w + f, c.data(), m * 4, hipMemcpyHostToDevice));
+        }
+    });
+    if (rc != RG_OK) {
+        rg_stream_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return RG_OK;
+}
+
+int rg_labels_assign(rg_labels *L, const uint64_t *values, const uint64_t *parents)
+{
+    if (!L || !values || !parents) return RG_EINVAL;
+    return guard([&] {
+        rg_graph *g = L->g;
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        const uint64_t V = g->V;
+        std::vector<uint64_t> w(V);
+        for (uint64_t v = 0; v < V; v++) {
+            if (values[v] > 0xFFFFFFFFull) throw RgError{RG_EINVAL, "label exceeds 32 bits"};
+            if (parents[v] > V) throw RgError{RG_EINVAL_VERTEX, "parent out of range"};
+            w[v] = (values[v] << 32) | (parents[v] == V ? (uint64_t)NILP : parents[v]);
+        }
+        uint64_t *dw = dalloc<uint64_t>(V);
+        hipError_t e = hipMemcpyAsync(dw, w.data(), V * 8, hipMemcpyHostToDevice, g->st);
+        if (e == hipSuccess) {
+            launch_write_labels(L->vr, (uint32_t)V, dw, g->st);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(g->st);
+        dfree(dw);
+        hipchk(e, "labels assign");
+        // every vertex may have changed
+        L->hc->flags |= F_ALL_MODIFIED;
+        HC(hipMemcpyAsync(L->ctl, L->hc, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
+        HC(hipStreamSynchronize(g->st));
+    });
+}
+
+// ---------------------------------------------------------------------
+// Snapshots.  File: header {magic "RGSNAP01", V, hint, symmetric, dual,
+// weighted, labels count, version, edge count}; the live multiset of the
+// out-lists as rg_edge records (one per copy, so reloading them as directed
+// inserts rebuilds every list and count); then per labels instance {algo,
+// trace, root, WCC init labels flag} + [init labels] + values + parents.
+// ---------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+constexpr char SNAP_MAGIC[8] = {'R', 'G', 'S', 'N', 'A', 'P', '0', '1'};
+struct SnapHdr {
+    char magic[8];
+    uint64_t V, hint;
+    uint32_t symmetric, dual, weighted, nL;
+    uint64_t version, n_edges;
+};
+struct SnapLabelsHdr {
+    uint32_t algo, trace;
+    uint64_t root;
+    uint32_t has_init, pad;
+};
+
+struct FileCloser {
+    FILE *f;
+    ~FileCloser()
+    {
+        if (f) fclose(f);
+    }
+};
+
+void fwrite_all(FILE *f, const void *p, size_t n)
+{
+    if (n && fwrite(p, 1, n, f) != n) throw RgError{RG_EDEVICE, std::string("snapshot write: ") + strerror(errno)};
+}
+
+void fread_all(FILE *f, void *p, size_t n)
+{
+    if (n && fread(p, 1, n, f) != n) throw RgError{RG_EINVAL, "snapshot file is cut short"};
+}
+
+}  // namespace
+
+extern "C" {
+
+int rg_snapshot_save(rg_graph *g, rg_labels **L, int nL, const char *path)
+{
+    if (!g || !path || (nL && !L) || nL < 0) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        for (int i = 0; i < nL; i++)
+            if (!L[i] || L[i]->g != g) throw RgError{RG_EINVAL, "labels belong to another graph"};
+        const uint64_t V = g->V;
+        // per-vertex live degrees of the out-lists -> record offsets
+        std::vector<uint32_t> deg(V);
+        copy_degrees(g->lists[0], deg.data(), V, g->st);
+        std::vector<uint64_t> off(V + 1, 0);
+        for (uint64_t v = 0; v < V; v++) off[v + 1] = off[v] + deg[v];
+        const std::string tmp = std::string(path) + ".tmp";
+        FILE *f = fopen(tmp.c_str(), "wb");
+        if (!f) throw RgError{RG_EINVAL, std::string("snapshot open: ") + strerror(errno)};
+        FileCloser fc{f};
+        SnapHdr h{};
+        memcpy(h.magic, SNAP_MAGIC, 8);
+        h.V = V;
+        h.hint = g->hint;
+        h.symmetric = (uint32_t)g->symmetric;
+        h.dual = (uint32_t)g->dual;
+        h.weighted = (uint32_t)g->weighted;
+        h.nL = (uint32_t)nL;
+        h.version = g->version;
+        h.n_edges = off[V];
+        fwrite_all(f, &h, sizeof(h));
+        // the edges in vertex ranges of at most ~2^24 records
+        const uint64_t CH = 1ull << 24;
+        uint64_t *doff = dalloc<uint64_t>(V + 1);
+        rg_edge_rec *drec = dalloc<rg_edge_rec>(std::max<uint64_t>(1, std::min(CH + 1, off[V])));
+        std::vector<rg_edge_rec> hrec;
+        try {
+            HC(hipMemcpyAsync(doff, off.data(), (V + 1) * 8, hipMemcpyHostToDevice, g->st));
+            uint64_t v0 = 0;
+            while (v0 < V) {
+                uint64_t v1 = v0 + 1;
+                while (v1 < V && off[v1 + 1] - off[v0] <= CH) v1++;
+                const uint64_t m = off[v1] - off[v0];
+                if (m > CH + 1) {
+                    // one vertex above the chunk: a buffer of its own size
+                    dfree(drec);
+                    drec = dalloc<rg_edge_rec>(m);
+                }
+                launch_list_dump(g->lists[0], g->slots, (uint32_t)v0, (uint32_t)v1, doff + v0, drec, g->st);
+                HC(hipGetLastError());
+                hrec.resize(m);
+                if (m) HC(hipMemcpyAsync(hrec.data(), drec, m * sizeof(rg_edge_rec), hipMemcpyDeviceToHost, g->st));
+                HC(hipStreamSynchronize(g->st));
+                fwrite_all(f, hrec.data(), m * sizeof(rg_edge_rec));
+                v0 = v1;
+            }
+        } catch (...) {
+            dfree(doff);
+            dfree(drec);
+            throw;
+        }
+        dfree(doff);
+        dfree(drec);
+        std::vector<uint64_t> val(V), par(V), init;
+        for (int i = 0; i < nL; i++) {
+            SnapLabelsHdr lh{};
+            lh.algo = (uint32_t)L[i]->algo;
+            lh.trace = (uint32_t)L[i]->trace;
+            lh.root = L[i]->root;
+            lh.has_init = L[i]->algo == RG_WCC ? 1u : 0u;
+            fwrite_all(f, &lh, sizeof(lh));
+            if (lh.has_init) {
+                init.resize(V);
+                uint64_t *di = dalloc<uint64_t>(V);
+                launch_read_init(L[i]->init, (uint32_t)V, di, g->st);
+                hipError_t e = hipGetLastError();
+                if (e == hipSuccess) e = hipMemcpyAsync(init.data(), di, V * 8, hipMemcpyDeviceToHost, g->st);
+                if (e == hipSuccess) e = hipStreamSynchronize(g->st);
+                dfree(di);
+                hipchk(e, "snapshot init labels");
+                fwrite_all(f, init.data(), V * 8);
+            }
+            if (rg_labels_read(L[i], 0, V, val.data(), par.data()) != RG_OK) throw RgError{RG_EDEVICE, g_err};
+            fwrite_all(f, val.data(), V * 8);
+            fwrite_all(f, par.data(), V * 8);
+        }
+        if (fflush(f) != 0 || fsync(fileno(f)) != 0)
+            throw RgError{RG_EDEVICE, std::string("snapshot sync: ") + strerror(errno)};
+        fclose(f);
+        fc.f = nullptr;
+        if (rename(tmp.c_str(), path) != 0) throw RgError{RG_EDEVICE, std::string("snapshot rename: ") + strerror(errno)};
+    });
+}
+
+int rg_snapshot_load(const char *path, rg_graph **gout, rg_labels **Lout, int cap, int *nL_out)
+{
+    if (!path || !gout || (cap && !Lout) || cap < 0 || !nL_out) return RG_EINVAL;
+    *gout = nullptr;
+    *nL_out = 0;
+    rg_graph *g = nullptr;
+    std::vector<rg_labels *> made;
+    int rc = guard([&] {
+        FILE *f = fopen(path, "rb");
+        if (!f) throw RgError{RG_EINVAL, std::string("snapshot open: ") + strerror(errno)};
+        FileCloser fc{f};
+        SnapHdr h{};
+        fread_all(f, &h, sizeof(h));
+        if (memcmp(h.magic, SNAP_MAGIC, 8) != 0) throw RgError{RG_EINVAL, "not a snapshot file"};
+        if ((int)h.nL > cap) throw RgError{RG_EINVAL, "more labels in the snapshot than the output array holds"};
+        int r = rg_graph_create(h.V, h.hint, (int)h.symmetric, (int)h.dual, (int)h.weighted, &g);
+        if (r != RG_OK) throw RgError{r, g_err};
+        // the edges: a device stream, loaded as directed inserts
+        std::vector<rg_edge> e(h.n_edges);
+        fread_all(f, e.data(), h.n_edges * sizeof(rg_edge));
+        rg_stream *s = nullptr;
+        r = rg_stream_from_edges(e.data(), h.n_edges, (int)h.weighted, &s);
+        if (r != RG_OK) throw RgError{r, g_err};
+        r = rg_stream_load(g, s, 0, h.n_edges, 1);
+        rg_stream_destroy(s);
+        if (r != RG_OK) throw RgError{r, g_err};
+        e.clear();
+        e.shrink_to_fit();
+        std::vector<uint64_t> val(h.V), par(h.V), init;
+        for (uint32_t i = 0; i < h.nL; i++) {
+            SnapLabelsHdr lh{};
+            fread_all(f, &lh, sizeof(lh));
+            if (lh.has_init) {
+                init.resize(h.V);
+                fread_all(f, init.data(), h.V * 8);
+            }
+            fread_all(f, val.data(), h.V * 8);
+            fread_all(f, par.data(), h.V * 8);
+            rg_labels *L = nullptr;
+            r = rg_labels_create(g, (int)lh.algo, lh.root, lh.has_init ? init.data() : nullptr, (int)lh.trace, &L);
+            if (r != RG_OK) throw RgError{r, g_err};
+            made.push_back(L);
+            r = rg_labels_assign(L, val.data(), par.data());
+            if (r != RG_OK) throw RgError{r, g_err};
+        }
+        g->version = h.version;
+    });
+    if (rc != RG_OK) {
+        for (rg_labels *L : made) rg_labels_destroy(L);
+        if (g) rg_graph_destroy(g);
+        return rc;
+    }
+    for (size_t i = 0; i < made.size(); i++) Lout[i] = made[i];
+    *nL_out = (int)made.size();
+    *gout = g;
+    return RG_OK;
 }
 
 int rg_stream_apply(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first, uint64_t n_ins,

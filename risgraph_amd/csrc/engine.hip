@@ -1480,6 +1480,30 @@ void launch_clear_marks(VRec *vr, uint32_t V, hipStream_t st)
     if (V) hipLaunchKernelGGL(k_clear_marks, dim3(std::min<uint32_t>((V + 255) / 256, 16384)), dim3(256), 0, st, vr, V);
 }
 
+// vr[i].val = words[i] with the marks cleared (labels restored from a snapshot)
+__global__ void k_write_labels(VRec *vr, uint32_t n, const uint64_t *words)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        vr[i].val = words[i];
+        vr[i].inv = 0;
+    }
+}
+
+void launch_write_labels(VRec *vr, uint32_t n, const uint64_t *words, hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_write_labels, dim3(std::min<uint32_t>((n + 255) / 256, 16384)), dim3(256), 0, st, vr, n, words);
+}
+
+__global__ void k_read_init(const uint32_t *init, uint32_t n, uint64_t *out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = init[i];
+}
+
+void launch_read_init(const uint32_t *init, uint32_t n, uint64_t *out, hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_read_init, dim3(std::min<uint32_t>((n + 255) / 256, 16384)), dim3(256), 0, st, init, n, out);
+}
+
 __global__ void k_read_labels(const VRec *vr, uint32_t n, uint64_t *out)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = vr[i].val;

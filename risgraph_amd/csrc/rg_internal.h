@@ -298,6 +298,14 @@ void launch_pool_recycle(const StoreArgs &a, hipStream_t s);
 void launch_list_init(List l, uint32_t V, hipStream_t s);
 // out[v] = live degree of v's list (device array of V words)
 void launch_list_degrees(List l, uint32_t V, uint32_t *out, hipStream_t s);
+// an edge record as the C ABI's rg_edge (src, dst, data)
+struct rg_edge_rec {
+    uint64_t src, dst, data;
+};
+// live multiset of lists [v0, v1) as records: off[k] = first record of
+// vertex v0 + k (relative to off[0]), off has v1 - v0 + 1 entries
+void launch_list_dump(List l, Slots sl, uint32_t v0, uint32_t v1, const uint64_t *off, rg_edge_rec *out,
+                      hipStream_t s);
 // Persistent team launch over trees *a[0..n) (n <= MAXT; device copies of
 // their arguments), W workgroups each.
 // A tree leaves the launch when its next pass exceeds work_max (grid mode
@@ -314,6 +322,10 @@ void launch_engine_fill_init(const EngineArgs &a, hipStream_t s);
 void launch_clear_marks(VRec *vr, uint32_t V, hipStream_t s);
 // out[i] = vr[i].val for i < n
 void launch_read_labels(const VRec *vr, uint32_t n, uint64_t *out, hipStream_t s);
+// vr[i].val = words[i], marks cleared, for i < n
+void launch_write_labels(VRec *vr, uint32_t n, const uint64_t *words, hipStream_t s);
+// out[i] = init[i] for i < n
+void launch_read_init(const uint32_t *init, uint32_t n, uint64_t *out, hipStream_t s);
 void launch_gen_rmat(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, int scale,
                      uint64_t seed, double a, double b, double c, hipStream_t s);
 void launch_gen_er(uint64_t *src, uint64_t *dst, uint64_t n, uint64_t start, uint64_t V,

@@ -1146,6 +1146,37 @@ void launch_list_degrees(List l, uint32_t V, uint32_t *out, hipStream_t s)
     if (V) hipLaunchKernelGGL(k_list_degrees, dim3((V + 255) / 256), dim3(256), 0, s, l, V, out);
 }
 
+// The live multiset of vertices [v0, v1)'s lists as edge records, `deg`
+// copies of each (v, nbr, w) entry with its count (snapshots): vertex v's
+// records start at out[off[v - v0] - off[0]].  One thread per vertex; a
+// list writes at most its header's live degree (the sum of its counts).
+__global__ void k_list_dump(List l, Slots sl, uint32_t v0, uint32_t v1, const uint64_t *off, rg_edge_rec *out)
+{
+    const uint32_t v = v0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= v1) return;
+    const ListHdr h = l.h[v];
+    uint64_t o = off[v - v0] - off[0];
+    const uint64_t end = o + h.deg;
+    for (uint32_t i = 0; i < h.len && o < end; i++) {
+        const uint64_t slot = h.off + i;
+        uint32_t c = sl.cnt[slot];
+        const uint32_t nb = sl.nbr[slot] & NBR_MASK;
+        const uint64_t w = sl.wt ? sl.wt[slot] : 0;
+        for (; c > 0 && o < end; c--, o++) {
+            out[o].src = v;
+            out[o].dst = nb;
+            out[o].data = w;
+        }
+    }
+}
+
+void launch_list_dump(List l, Slots sl, uint32_t v0, uint32_t v1, const uint64_t *off, rg_edge_rec *out,
+                      hipStream_t s)
+{
+    if (v1 > v0)
+        hipLaunchKernelGGL(k_list_dump, dim3((v1 - v0 + 255) / 256), dim3(256), 0, s, l, sl, v0, v1, off, out);
+}
+
 // pending frees of size class c of pool h -> its free list (threads t0 of
 // nt move the offsets); one caller per class, after every pop of the batch
 __device__ void recycle_class(const StoreArgs &a, int h, int c, uint32_t t0, uint32_t nt)
