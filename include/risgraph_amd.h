@@ -188,6 +188,42 @@ int rg_labels_algo(rg_labels *L, int *algo, uint64_t *root);
  * (parent V = empty); the modified set then reports every vertex. */
 int rg_labels_assign(rg_labels *L, const uint64_t *values, const uint64_t *parents);
 
+/* Request server: the network front-end of the reference's RPC servers
+ * (all_inc_rpc.cpp:397-542, all_inc_rpc_txn.cpp; UCX there, TCP on
+ * 127.0.0.1 here) feeding a device request loop.  Requests are the
+ * ClientUpdateRequest / ClientTxnUpdateRequest fields of rpc.hpp:19-61 in a
+ * packed little-endian layout:
+ *   update (txn = 0), 40 bytes: u32 client_id, u32 type (0 add, 1 del,
+ *     2 end), u64 src, u64 dst, u64 data, i64 request_time;
+ *   transaction (txn = 1), 464 bytes: u32 client_id, u32 num_updates (1..16),
+ *     u32 types[16], u64 srcs[16], u64 dsts[16], u64 datas[16],
+ *     i64 request_time;
+ * replies: update, 24 bytes: u32 client_id, u32 status (0 or an RG_* code),
+ *   u64 version_id, i64 request_time (echoed); transaction, 144 bytes: u32
+ *   client_id, u32 status, u64 version_ids[16], i64 request_time.
+ * Every request that arrived while the device was busy is admitted as one
+ * batch (up to cap_max updates; 0 = 2^20): logged when the graph has a WAL
+ * open (rg_wal_open), stored, split into safe and unsafe requests against
+ * the trees in L[0..nL) (1 <= nL <= 4); safe requests are answered as soon
+ * as the split is on the host, with the version before the batch, unsafe
+ * ones after the repair, with the version after it.  A type-2 request closes
+ * its connection.  port 0 picks a free port (rg_server_port).  While the
+ * server runs, other calls on the graph wait for the batch in progress. */
+typedef struct rg_server rg_server;
+typedef struct rg_server_params {
+    uint16_t port;
+    int txn;                  /* 1: transaction requests, else single updates */
+    int directed;
+    uint32_t cap_max;
+} rg_server_params;
+int rg_server_start(rg_graph *g, rg_labels **L, int nL, const rg_server_params *p, rg_server **out);
+int rg_server_port(rg_server *s, uint32_t *port);
+/* requests and updates served, device batches, unsafe requests; RG_EDEVICE
+ * when a batch failed (its requests got the error code as status) */
+int rg_server_stats(rg_server *s, uint64_t *requests, uint64_t *updates, uint64_t *batches, uint64_t *unsafe);
+/* stops accepting, answers what is queued, closes every connection */
+int rg_server_stop(rg_server *s);
+
 /* Version clock of the device batches: every rg_stream_apply and
  * rg_txn_apply batch (and every batch of rg_serve) that applies at least one
  * update advances it by one.  The safe updates of a split batch take effect at the version before

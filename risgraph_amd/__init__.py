@@ -85,6 +85,10 @@ SIGNATURES = {
     "rg_labels_assign": (ci, [vp, vp, vp]),
     "rg_graph_shape": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
     "rg_labels_algo": (ci, [vp, ctypes.POINTER(ci), u64p]),
+    "rg_server_start": (ci, [vp, vp, ci, vp, ctypes.POINTER(vp)]),
+    "rg_server_port": (ci, [vp, ctypes.POINTER(ctypes.c_uint32)]),
+    "rg_server_stats": (ci, [vp, u64p, u64p, u64p, u64p]),
+    "rg_server_stop": (ci, [vp]),
     "rg_graph_version": (ci, [vp, u64p]),
     "rg_history_enable": (ci, [vp, u64]),
     "rg_history_read": (ci, [vp, u64, u64, u64, vp]),
@@ -529,3 +533,35 @@ def snapshot_load(path, cap=8):
     _check(load_library().rg_snapshot_load(os.fsencode(path), ctypes.byref(g), arr, cap, ctypes.byref(n)))
     G = Graph._wrap(g)
     return G, [Labels._wrap(G, vp(arr[i])) for i in range(n.value)]
+
+
+class ServerParams(ctypes.Structure):
+    _fields_ = [("port", ctypes.c_uint16), ("txn", ci), ("directed", ci), ("cap_max", ctypes.c_uint32)]
+
+
+class Server:
+    """rg_server_*: TCP request front-end (127.0.0.1) feeding the device request loop."""
+
+    def __init__(self, graph, labels, port=0, txn=False, directed=True, cap_max=0):
+        self.graph, self.labels = graph, list(labels)
+        arr = (vp * len(self.labels))(*[L.h for L in self.labels])
+        p = ServerParams(port, int(txn), int(directed), cap_max)
+        h = vp()
+        _check(load_library().rg_server_start(graph.h, arr, len(self.labels), ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        pt = ctypes.c_uint32()
+        _check(load_library().rg_server_port(h, ctypes.byref(pt)))
+        self.port = int(pt.value)
+
+    def stats(self):
+        v = [u64() for _ in range(4)]
+        rc = load_library().rg_server_stats(self.h, *[ctypes.byref(x) for x in v])
+        d = dict(zip(("requests", "updates", "batches", "unsafe"), (int(x.value) for x in v)))
+        d["ok"] = rc == RG_OK
+        return d
+
+    def stop(self):
+        if getattr(self, "h", None):
+            rc = load_library().rg_server_stop(self.h)
+            self.h = None
+            _check(rc)

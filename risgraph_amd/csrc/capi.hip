@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/risgraph_amd.h"
+#include "host_batch.h"
 #include "rg_internal.h"
 
 using namespace rg;
@@ -1490,6 +1491,8 @@ struct SplitOut {
     std::chrono::steady_clock::time_point t_safe{};   // the safe ops completed (host clock)
     uint64_t unsafe = 0;        // unsafe ops of the batch
     const uint8_t *flag = nullptr;   // pinned, one per op (ops [0, n_ins) inserts, then deletes)
+    // called once the flags are on the host, while the repair still runs
+    std::function<void(const uint8_t *flag)> on_safe;
 };
 
 // batches of more than this many edge ops are split into safe and unsafe
@@ -1536,6 +1539,7 @@ static void repair_batch(rg_graph *g, rg_labels **L, int nL, uint64_t n, bool ha
             hooks.after_first_enqueue = [g, so] {
                 HC(hipEventSynchronize(g->ev_safe));
                 so->t_safe = std::chrono::steady_clock::now();
+                if (so->on_safe) so->on_safe(g->h_flag);
             };
     }
     engine_run_multi(L, nL, ph, has_ins && has_del, (uint32_t)n, has_del, stats, split, &hooks);
@@ -1747,6 +1751,38 @@ int rg_txn_apply(rg_graph *g, rg_labels **L, int nL, const rg_edge *edges, const
         }
     });
 }
+
+}  // extern "C"
+
+// One batch of host-buffer updates for the request server (host_batch.h).
+int rg::host_batch(rg_graph *g, rg_labels **L, int nL, const rg_edge *e, const int8_t *kind, const uint32_t *txn_of,
+                   uint64_t n, int directed, uint8_t *flags, const std::function<void(const uint8_t *)> &on_safe,
+                   uint64_t *v_before, uint64_t *v_after, std::string *err)
+{
+    try {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        *v_before = *v_after = g->version;
+        if (!n) return RG_OK;
+        SplitOut so;
+        so.flags = true;
+        so.on_safe = on_safe;
+        ops_batch(g, L, nL, e, kind, txn_of, n, directed, &so, true);
+        if (flags) {
+            if (so.flag) memcpy(flags, so.flag, n);
+            else memset(flags, 1, n);   // no trees to split for: everything waited for the batch
+        }
+        *v_after = g->version;
+        return RG_OK;
+    } catch (const RgError &x) {
+        if (err) *err = x.msg;
+        return x.code;
+    } catch (const std::exception &x) {
+        if (err) *err = x.what();
+        return RG_EDEVICE;
+    }
+}
+
+extern "C" {
 
 int rg_wal_open(rg_graph *g, const char *path, int sync)
 {
