@@ -24,8 +24,10 @@ Legs:
           else the rate backs off x0.9 (up to 8 attempts) and they run again;
   check   every incremental tree is compared with a fresh full rebuild on the
           final graph (values bit-exact);
-  cpu     the oracle's sequential port of the reference engine on a bounded
-          sample of the same stream, after the GPU legs (rank 0, N=1 only).
+  cpu     the reference's rt serving loop restated on all host threads
+          (OpenMP) on the same stream, after the GPU legs (rank 0, N=1
+          only): the highest offered rate whose ~1 s round meets p99.9 <=
+          20 ms, and the one-thread back-to-back rate beside it.
 """
 import argparse
 import json
@@ -59,7 +61,7 @@ def parse():
     ap.add_argument("--step-seconds", type=float, default=0.5)
     ap.add_argument("--calib-seconds", type=float, default=0.3)
     ap.add_argument("--plain-updates", type=int, default=4000)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--json-out", default="")
@@ -464,34 +466,82 @@ def cpu_baseline_setup(cpu, src, dst, w, snap, labels, V, E, n0, root):
         log(f"cpu baseline setup failed: {e!r}")
 
 
+def host_info():
+    """Host facts the CPU baseline depends on: core count, model, sockets, RTM
+    (without RTM the reference takes its lock path for weighted labels,
+    graph.hpp:1781)."""
+    info = {"nproc": os.cpu_count(), "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        with open("/proc/cpuinfo") as f:
+            txt = f.read()
+        models = [l.split(":", 1)[1].strip() for l in txt.splitlines() if l.startswith("model name")]
+        phys = {l.split(":", 1)[1].strip() for l in txt.splitlines() if l.startswith("physical id")}
+        flags = next((l.split(":", 1)[1].split() for l in txt.splitlines() if l.startswith("flags")), [])
+        info.update(model=models[0] if models else None, sockets=len(phys) or 1, rtm="rtm" in flags)
+    except OSError:
+        pass
+    return info
+
+
 def cpu_baseline_run(cpu, seconds):
+    """The reference's rt serving loop on all host threads (OpenMP restatement,
+    oracle.serve_rt: parallel safe phase, serial unsafe phase, the
+    num_changes_threshold cap; bfs_inc_rt.cpp:154-431) with Poisson arrivals:
+    offered rate ramped x1.5 from 50k/s while a ~1 s round meets p99.9 <= 20 ms
+    and keeps up; the answer is the best passing round's served rate.  Then the
+    one-thread back-to-back rate of the same engine as a second figure."""
     if "error" in cpu:
         cpu["result"] = {"error": cpu["error"]}
         return
     import oracle
     n0, src, dst, w = cpu["n0"], cpu["src"], cpu["dst"], cpu["w"]
-    # calibrate on 2,000 updates, then time a sample of about `seconds`
-    done = 0
-    chunk = 2000
+    rng = np.random.default_rng(4)
+    k = 0            # stream updates consumed (even: whole add/del pairs)
+    rate, best, trials = 50000.0, None, []
+    t_start = time.time()
+    while time.time() - t_start < seconds:
+        n = int(max(20000, rate * 1.0))
+        n -= n % 2
+        if n0 + (k + n) // 2 >= len(src):
+            break
+        arrival = np.cumsum(rng.exponential(1.0 / rate, n))
+        h = k // 2
+        st, lat = oracle.serve_rt(cpu["gd"], cpu["Ld"], cpu["gu"], cpu["Lu"], n, src[n0 + h:], dst[n0 + h:],
+                                  w[n0 + h:], src[h:], dst[h:], w[h:], arrival, timeout_ms=20.0)
+        served = st["served"]
+        wall = st["wall_ns"] / 1e9
+        p999 = float(np.percentile(lat[:served], 99.9)) if served else float("inf")
+        ok = served == n and not st["overloaded"] and p999 <= 20.0
+        trials.append({"offered": round(rate, 1), "served_per_s": round(served / max(wall, 1e-9), 1),
+                       "p99.9_ms": round(p999, 3), "unsafe_frac": round(st["unsafe"] / max(served, 1), 4),
+                       "threshold": st["threshold"], "ok": ok})
+        log(f"cpu rt rate={rate:.0f} -> {served / max(wall, 1e-9):.0f} upd/s p99.9={p999:.2f}ms "
+            f"threads={st['threads']} {'ok' if ok else 'miss'}")
+        threads = st["threads"]
+        if not ok:
+            break
+        best = trials[-1]
+        k += n
+        rate *= 1.5
+    # one thread, back to back (the previous baseline), a bounded sample
+    h = k // 2
     t = time.time()
-    oracle.stream_mixed(cpu["gd"], cpu["Ld"], cpu["gu"], cpu["Lu"], chunk, src[n0:], dst[n0:], w[n0:],
-                        src, dst, w, latency=False)
-    done += chunk
-    per = (time.time() - t) / chunk
-    n = int(min(2 * (len(src) - n0) - done, max(10000, seconds / max(per, 1e-9))))
-    n -= n % 2
-    k = done // 2
-    t = time.time()
-    _, lat = oracle.stream_mixed(cpu["gd"], cpu["Ld"], cpu["gu"], cpu["Lu"], n, src[n0 + k:], dst[n0 + k:],
-                                 w[n0 + k:], src[k:], dst[k:], w[k:], latency=True)
-    el = time.time() - t
+    n1 = 20000
+    _, lat1 = oracle.stream_mixed(cpu["gd"], cpu["Ld"], cpu["gu"], cpu["Lu"], n1, src[n0 + h:], dst[n0 + h:],
+                                  w[n0 + h:], src[h:], dst[h:], w[h:], latency=True)
+    el1 = time.time() - t
     cpu["result"] = {
-        "value": round(n / el, 1), "unit": "updates/s", "cores": 1, "kind": "port",
-        "sample": f"{n} consecutive updates of the same all_inc stream (after 2,000 warm-up updates), "
-                  f"4 trees on the full scale graph, started from the device's built fixpoint; "
-                  f"sequential C restatement of the reference engine",
-        "p50_ms": round(float(np.percentile(lat, 50)) / 1e6, 4),
-        "p99.9_ms": round(float(np.percentile(lat, 99.9)) / 1e6, 4),
+        "value": best["served_per_s"] if best else 0.0, "unit": "updates/s at p99.9 <= 20 ms",
+        "cores": threads if trials else 0, "kind": "port",
+        "sample": "the reference's rt serving loop (parallel safe phase, serial unsafe phase, unsafe-change cap) "
+                  "restated in C with OpenMP, on the same all_inc stream and 4 trees at full scale, started "
+                  "from the device's built fixpoint; Poisson arrivals, ~1 s rounds at rates x1.5 from 50k/s "
+                  "until one misses p99.9 <= 20 ms",
+        "trials": trials,
+        "host": host_info(),
+        "single_thread_back_to_back": {"value": round(n1 / el1, 1), "unit": "updates/s", "cores": 1,
+                                       "p50_ms": round(float(np.percentile(lat1, 50)) / 1e6, 4),
+                                       "p99.9_ms": round(float(np.percentile(lat1, 99.9)) / 1e6, 4)},
         "setup_s": round(cpu["setup_s"], 1),
     }
 

@@ -81,6 +81,8 @@ def lib():
             "or_recompute_count": (u64, [vp]),
             "or_stream_plain": (u64, [vp, vp, u64, vp, vp, vp, vp, vp, vp, ctypes.c_int, vp]),
             "or_stream_mixed": (u64, [vp, vp, ctypes.c_int, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp]),
+            "or_serve_rt": (u64, [vp, vp, ctypes.c_int, vp, vp, u64, vp, vp, vp, vp, vp, vp, vp,
+                                  ctypes.c_double, ctypes.c_double, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -256,3 +258,17 @@ def stream_mixed(gd, Ld, gu, Lu, n_updates, add_s, add_d, add_w, del_s, del_d, d
     unsafe = lib().or_stream_mixed(gd.h if gd else None, arr, len(Ld), gu.h if gu else None,
                                    Lu.h if Lu else None, n_updates, *[_p(x) for x in a], _p(lat))
     return int(unsafe), lat
+
+
+def serve_rt(gd, Ld, gu, Lu, n_updates, add_s, add_d, add_w, del_s, del_d, del_w, arrival,
+             timeout_ms=20.0, target_timeout_rate=0.001):
+    """The rt driver's serving loop on host threads (or_serve_rt); returns (stats, latencies_ms)."""
+    arr = (vp * max(1, len(Ld)))(*[L.h for L in Ld])
+    a = [_u64(x) for x in (add_s, add_d, add_w, del_s, del_d, del_w)]
+    t = np.ascontiguousarray(arrival, dtype=np.float64)
+    lat = np.full(n_updates, np.inf, np.float64)
+    st = np.zeros(7, np.uint64)
+    lib().or_serve_rt(gd.h if gd else None, arr, len(Ld), gu.h if gu else None, Lu.h if Lu else None,
+                      n_updates, *[_p(x) for x in a], _p(t), timeout_ms, target_timeout_rate, _p(lat), _p(st))
+    keys = ("served", "rounds", "unsafe", "threshold", "overloaded", "threads", "wall_ns")
+    return dict(zip(keys, (int(x) for x in st))), lat

@@ -27,6 +27,9 @@
  *                     safe delete leaves the values and a valid tree; all
  *                     four algorithms, directed and undirected; planted
  *                     one-direction and reversed-comparison mistakes fail.
+ *   or_serve_rt       the rt loop's final values equal the closed form on the
+ *                     final window for all four trees (valid trees too),
+ *                     under 1, 2, 4 and 8 threads; every update served.
  *   or_build_tree /
  *   or_update_*       equality with or_static_values after every update of
  *                     seeded sliding-window streams (all four algorithms,
@@ -45,9 +48,13 @@
  */
 #define _POSIX_C_SOURCE 199309L
 #include "oracle.h"
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ------------------------------------------------------------------ */
 /* algorithm lambdas (bfs_inc.cpp:77-92, sssp_inc.cpp:82-97,            */
@@ -1071,4 +1078,194 @@ uint64_t or_stream_mixed(or_graph *gd, or_labels **Ld, int nLd, or_graph *gu, or
         }
     }
     return unsafe;
+}
+
+
+/* ------------------------------------------------------------------ */
+/* The rt driver's serving loop on host threads (bfs_inc_rt.cpp:154-431), */
+/* over the four trees of the all_inc stream: a parallel safe phase and a */
+/* serial unsafe phase per round, with the num_changes_threshold cap.   */
+/* ------------------------------------------------------------------ */
+
+static double now_s(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+static int u64_cmp(const void *a, const void *b)
+{
+    const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+/* per-list spin locks: the safe phase's concurrent add_edge / del_edge
+ * (IndexedEdgeStorage locks each list, storage.hpp:348) */
+static void lock_list(uint8_t *l)
+{
+    while (__atomic_test_and_set(l, __ATOMIC_ACQUIRE))
+        while (__atomic_load_n(l, __ATOMIC_RELAXED)) { }
+}
+static void unlock_list(uint8_t *l) { __atomic_clear(l, __ATOMIC_RELEASE); }
+
+static uint64_t locked_directed(or_graph *g, uint8_t *lk, uint64_t s, uint64_t d, uint64_t w, int upd)
+{
+    uint64_t data = g->weighted ? w : 0;
+    lock_list(&lk[s]);
+    uint64_t cur = store_update(&g->out, s, d, data, upd);
+    unlock_list(&lk[s]);
+    if (!g->symmetric) {
+        lock_list(&lk[g->V + d]);
+        store_update(&g->in, d, s, data, upd);
+        unlock_list(&lk[g->V + d]);
+    }
+    return cur;
+}
+
+static void locked_update(or_graph *g, uint8_t *lk, uint64_t s, uint64_t d, uint64_t w, int directed, int upd)
+{
+    locked_directed(g, lk, s, d, w, upd);
+    if (!directed) locked_directed(g, lk, d, s, w, upd);
+}
+
+/* update u of the all_inc alternation: add edge u/2 (even u) or delete it (odd u) */
+static int rt_unsafe(or_graph *gd, or_labels **Ld, int nLd, or_graph *gu, or_labels *Lu, int add, uint64_t s,
+                     uint64_t d, uint64_t w)
+{
+    for (int i = 0; i < nLd; i++)
+        if (add ? or_need_update_add(gd, Ld[i], s, d, w, 1) : or_need_update_del(gd, Ld[i], s, d, w, 1)) return 1;
+    if (gu && Lu && (add ? or_need_update_add(gu, Lu, s, d, 0, 0) : or_need_update_del(gu, Lu, s, d, 0, 0)))
+        return 1;
+    return 0;
+}
+
+uint64_t or_serve_rt(or_graph *gd, or_labels **Ld, int nLd, or_graph *gu, or_labels *Lu, uint64_t n_updates,
+                     const uint64_t *as, const uint64_t *ad, const uint64_t *aw, const uint64_t *ds,
+                     const uint64_t *dd, const uint64_t *dw, const double *arrival, double timeout_ms,
+                     double target_timeout_rate, double *lat_ms, uint64_t *stats)
+{
+    const uint64_t V = gd ? gd->V : gu->V;
+    uint8_t *lkd = gd ? calloc(2 * V, 1) : NULL;
+    uint8_t *lku = gu ? calloc(2 * V, 1) : NULL;
+    uint64_t *unsafe = malloc((n_updates ? n_updates : 1) * sizeof(uint64_t));
+    int threads = 1;
+#ifdef _OPENMP
+    threads = omp_get_max_threads();
+#endif
+    uint64_t threshold = (uint64_t)threads, rounds = 0, n_unsafe_total = 0, timeouts = 0;
+    uint64_t last_timeouts = 0, counted = 0, last_counted = 0, overloaded = 0;
+    uint8_t *state = calloc(n_updates ? n_updates : 1, 1);   /* 1: done (or queued this round) */
+    uint64_t lo = 0;             /* every update below lo is done */
+    const double t0 = now_s();
+    const double to = timeout_ms * 1e-3;
+    while (lo < n_updates) {
+        double tn = now_s() - t0;
+        while (arrival[lo] > tn) tn = now_s() - t0;   /* idle until a request arrives */
+        if (tn - arrival[lo] > 10.0 * to + 0.2) { overloaded = 1; break; }
+        /* the updates that have arrived */
+        uint64_t l = lo, h = n_updates;
+        while (l < h) {
+            const uint64_t m = l + (h - l) / 2;
+            if (arrival[m] <= tn) l = m + 1; else h = m;
+        }
+        const uint64_t hi = l;
+        /* safe phase (bfs_inc_rt.cpp:211-326): threads classify arrived
+         * updates against the labels; safe ones are applied to the store at
+         * once and complete, unsafe ones wait for the serial phase.  The
+         * phase stops taking updates once the unsafe ones exceed the
+         * threshold or the oldest of them has waited 0.8 x timeout
+         * (need_break_func, bfs_inc_rt.cpp:156-164); the rest stay pending */
+        uint64_t n_uns = 0;
+        uint64_t first_uns = UINT64_MAX;   /* smallest unsafe update (arrivals grow with u) */
+        int stop = 0;
+#pragma omp parallel for schedule(dynamic, 64)
+        for (uint64_t u = lo; u < hi; u++) {
+            int st;
+#pragma omp atomic read
+            st = stop;
+            if (st || state[u]) continue;
+            uint64_t nu;
+#pragma omp atomic read
+            nu = n_uns;
+            const uint64_t fu = __atomic_load_n(&first_uns, __ATOMIC_RELAXED);
+            const double t = now_s() - t0;
+            if (nu > threshold || (fu != UINT64_MAX && t - arrival[fu] > 0.8 * to)) {
+#pragma omp atomic write
+                stop = 1;
+                continue;
+            }
+            const uint64_t k = u / 2;
+            const int add = (u & 1) == 0;
+            const uint64_t s = add ? as[k] : ds[k], d = add ? ad[k] : dd[k];
+            const uint64_t w = add ? (aw ? aw[k] : 0) : (dw ? dw[k] : 0);
+            if (rt_unsafe(gd, Ld, nLd, gu, Lu, add, s, d, w)) {
+                uint64_t pos;
+#pragma omp atomic capture
+                pos = n_uns++;
+                unsafe[pos] = u;
+                uint64_t cur = __atomic_load_n(&first_uns, __ATOMIC_RELAXED);
+                while (u < cur && !__atomic_compare_exchange_n(&first_uns, &cur, u, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+                }
+            } else {
+                if (gd) locked_update(gd, lkd, s, d, w, 1, add ? 1 : -1);
+                if (gu) locked_update(gu, lku, s, d, 0, 0, add ? 1 : -1);
+                lat_ms[u] = (now_s() - t0 - arrival[u]) * 1e3;
+            }
+            state[u] = 1;
+        }
+        /* serial phase, in arrival order (bfs_inc_rt.cpp:329-405) */
+        qsort(unsafe, n_uns, sizeof(uint64_t), u64_cmp);
+        for (uint64_t i = 0; i < n_uns; i++) {
+            const uint64_t u = unsafe[i], k = u / 2;
+            const int add = (u & 1) == 0;
+            const uint64_t s = add ? as[k] : ds[k], d = add ? ad[k] : dd[k], w = add ? (aw ? aw[k] : 0) : (dw ? dw[k] : 0);
+            if (add) {
+                if (gd && or_add_edge(gd, s, d, w, 1) == 0)
+                    for (int t = 0; t < nLd; t++) or_update_add(gd, Ld[t], s, d, w, 1);
+                if (gu && or_add_edge(gu, s, d, 0, 0) == 0 && Lu) or_update_add(gu, Lu, s, d, 0, 0);
+            } else {
+                if (gd && or_del_edge(gd, s, d, w, 1) == 1)
+                    for (int t = 0; t < nLd; t++) or_update_del(gd, Ld[t], s, d, w, 1);
+                if (gu && or_del_edge(gu, s, d, 0, 0) == 1 && Lu) or_update_del(gu, Lu, s, d, 0, 0);
+            }
+            lat_ms[u] = (now_s() - t0 - arrival[u]) * 1e3;
+        }
+        n_unsafe_total += n_uns;
+        rounds++;
+        while (lo < n_updates && state[lo]) lo++;
+        /* AIMD on the unsafe-change threshold every 3 rounds (bfs_inc_rt.cpp:409-427) */
+        for (; counted < lo; counted++)
+            if (lat_ms[counted] > timeout_ms) timeouts++;
+        if (rounds % 3 == 0 && counted > last_counted) {
+            const double local = (double)(timeouts - last_timeouts) / (double)(counted - last_counted);
+            const double global = (double)timeouts / (double)counted;
+            if (local > 0.8 * target_timeout_rate) {
+                threshold = (uint64_t)(threshold * 0.9);
+                if (threshold < (uint64_t)threads) threshold = (uint64_t)threads;
+            }
+            if (local < 0.8 * target_timeout_rate && global < target_timeout_rate) {
+                if (threshold < 1000) threshold += 10;
+                else threshold = (uint64_t)(threshold * 1.01);
+                if (threshold > 1000000) threshold = 1000000;
+            }
+            last_timeouts = timeouts;
+            last_counted = counted;
+        }
+    }
+    const double wall = now_s() - t0;
+    if (stats) {
+        stats[0] = lo;                         /* served (every update below lo) */
+        stats[1] = rounds;
+        stats[2] = n_unsafe_total;
+        stats[3] = threshold;
+        stats[4] = overloaded;
+        stats[5] = (uint64_t)threads;
+        stats[6] = (uint64_t)(wall * 1e9);
+    }
+    free(lkd);
+    free(lku);
+    free(unsafe);
+    free(state);
+    return lo;
 }

@@ -113,6 +113,25 @@ uint64_t or_stream_mixed(or_graph *gd, or_labels **Ld, int nLd, or_graph *gu, or
                          uint64_t n_updates, const uint64_t *as, const uint64_t *ad, const uint64_t *aw,
                          const uint64_t *ds, const uint64_t *dd, const uint64_t *dw, double *lat_ns);
 
+/* The rt driver's serving loop (bfs_inc_rt.cpp:154-431) on host threads
+ * (OpenMP, omp_get_max_threads()) over the all_inc stream's four trees
+ * (BFS/SSSP/SSWP labels Ld on gd, WCC Lu on gu; update u adds edge u/2 for
+ * even u, deletes it for odd u).  Update u arrives at arrival[u] seconds
+ * after the call starts.  Rounds: a parallel safe phase classifies arrived
+ * updates (need_update_tree_add / _del in every tree), applies the safe
+ * ones to the store under per-list locks and completes them, queues the
+ * unsafe ones until their count passes the threshold or the oldest has
+ * waited 0.8 x timeout; then a serial phase applies the queued ones in
+ * arrival order with the incremental engine.  The threshold starts at the
+ * thread count and moves by the reference's rule every 3 rounds.
+ * lat_ms[u] = completion - arrival.  stats[7]: served, rounds, unsafe,
+ * final threshold, overloaded (a request waited 10 x timeout), threads,
+ * wall ns.  Returns the number served. */
+uint64_t or_serve_rt(or_graph *gd, or_labels **Ld, int nLd, or_graph *gu, or_labels *Lu, uint64_t n_updates,
+                     const uint64_t *as, const uint64_t *ad, const uint64_t *aw, const uint64_t *ds,
+                     const uint64_t *dd, const uint64_t *dw, const double *arrival, double timeout_ms,
+                     double target_timeout_rate, double *lat_ms, uint64_t *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -519,3 +519,33 @@ def test_classifier_checks_catch_planted_mistakes():
         if one_way != (not _parent_edges_live_without(g, L, V, x, y, z, False)):
             caught = True
     assert caught
+
+
+def test_rt_serving_loop_matches_static():
+    """or_serve_rt (the rt driver's safe / unsafe rounds on host threads)
+    ends at the same values as the closed form on the final window, for all
+    four trees, with every update served and its latency recorded."""
+    V, E = 1024, 12000
+    s, d = streams.rmat(10, E, 3)
+    w = streams.weights(E, 4)
+    n0 = 9000
+    gd = oracle.RefGraph(V, E, False, True, True)
+    gd.add_edges(s[:n0], d[:n0], w[:n0], True)
+    gu = oracle.RefGraph(V, E, True, False, False)
+    gu.add_edges(s[:n0], d[:n0], None, False)
+    root = streams.max_out_degree_root(s[:n0], V)
+    Ld = [oracle.RefLabels(gd, a, root) for a in (BFS, SSSP, SSWP)]
+    Lu = oracle.RefLabels(gu, WCC, root)
+    for L in Ld + [Lu]:
+        L.build()
+    n = 4000
+    arrival = np.arange(n) / 2e5          # 200k updates/s offered
+    st, lat = oracle.serve_rt(gd, Ld, gu, Lu, n, s[n0:], d[n0:], w[n0:], s, d, w, arrival)
+    assert st["served"] == n and not st["overloaded"] and st["threads"] >= 1
+    assert 0 < st["unsafe"] < n and np.all(np.isfinite(lat)) and np.all(lat >= 0)
+    k = n // 2
+    ls, ld, lw = s[k:n0 + k], d[k:n0 + k], w[k:n0 + k]
+    for L in Ld + [Lu]:
+        vals, par, _ = L.read()
+        assert np.array_equal(vals, oracle.static_values(L.algo, V, ls, ld, lw, root)), L.algo
+        assert oracle.check_tree(L.algo, V, ls, ld, lw, root, vals, par, undirected=L.algo == WCC) == 0
