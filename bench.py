@@ -68,8 +68,11 @@ def parse():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc: after the setup, apply --probe-batches store batches of "
                          "--probe-pairs insert+delete pairs, then a plain 1 GiB device copy (counter calibration)")
-    ap.add_argument("--probe-pairs", type=int, default=65536)
+    ap.add_argument("--probe-pairs", type=int, default=150000)
     ap.add_argument("--probe-batches", type=int, default=16)
+    ap.add_argument("--serve-only", type=float, default=0.0, metavar="RATE",
+                    help="for profiling: skip the plain leg and the ramp, serve the warmup and timed steps at RATE "
+                         "updates/s (the line's plain and ramp fields are then empty)")
     ap.add_argument("--no-split", action="store_true",
                     help="every update of a batch waits for the batch's repair (no safe/unsafe split)")
     ap.add_argument("--cap", choices=["unsafe", "total"], default="unsafe",
@@ -134,7 +137,7 @@ def main():
     t_setup = time.time()
     stream = rg.Stream("rmat", scale, E, seed=1, wseed=2)
     g = rg.Graph(V, E, symmetric=False, dual=True, weighted=True)
-    g.profile(1)          # HIP events around every store apply kernel of this process
+    g.profile(3)          # HIP events around every store apply kernel and engine team launch of this process
     g.load_stream(stream, 0, n0, True)
     root = stream.max_degree_vertex(n0, V)
     names = ["bfs", "sssp", "sswp", "wcc"]
@@ -159,6 +162,9 @@ def main():
         snap = [L.read() for L in labels]
 
     cur = Cursor(n0, E)
+    if args.serve_only:
+        serve_only(g, labels, stream, cur, args, snap, names, root, V, E, n0, info, build_s)
+        return
 
     # ---- plain path: one update per device batch ----
     res, _ = serve(g, labels, stream, cur, args.plain_updates, 1e9, 1, 1, 0.0)
@@ -223,6 +229,7 @@ def main():
             serve_batched(g, labels, stream, cur, n_step, rate, args)
         lats, acc = [], dict(wall=0.0, total=0, batches=0, grid_steps=0, gpu_ms=0.0, overloaded=0, unsafe=0,
                              safe_p50=[], unsafe_p50=[], unsafe_p999=[])
+        tp0 = g.profile_read(1)   # the timed steps' engine team launches (HIP events)
         for k in range(args.steps):
             res, lat = serve_batched(g, labels, stream, cur, n_step, rate, args)
             lats.append(lat)
@@ -239,6 +246,9 @@ def main():
             log(f"step {k}: {res['updates_per_s']:.0f} upd/s p99.9={res['p999_ms']:.2f}ms "
                 f"mean_batch={res['mean_batch']:.1f} max_batch={res['max_batch']} rebuilds={res['rebuilds']} "
                 f"gpu_busy={res['gpu_ms'] / 1e3 / max(res['wall_s'], 1e-9):.2f}")
+        tp1 = g.profile_read(1)
+        acc["team_launches"] = tp1["launches"] - tp0["launches"]
+        acc["team_ms"] = tp1["ms"] - tp0["ms"]
         lat_all = np.concatenate(lats)      # unserved updates of an overloaded step count as +inf
         p = {q: float(np.percentile(lat_all, q)) for q in (50, 99, 99.9)}
         value = acc["total"] / acc["wall"]
@@ -259,7 +269,9 @@ def main():
     grid_steps, gpu_ms, overloaded = acc["grid_steps"], acc["gpu_ms"], acc["overloaded"]
 
     # ---- roofline: dominant kernel (store apply) timed live with HIP events ----
-    roof, batch_roof = roofline_sample(g, labels, stream, cur, rate, n_step, args, int(total / max(batches, 1)))
+    roof, roof_store, batch_roof = roofline_sample(g, labels, stream, cur, rate, n_step, args,
+                                                   int(total / max(batches, 1)),
+                                                   (acc["team_launches"], acc["team_ms"]))
 
     # ---- correctness: incremental trees vs a fresh rebuild ----
     check = None
@@ -317,6 +329,7 @@ def main():
         "build_s": build_s,
         "store": info,
         "roofline": roof,
+        "roofline_store": roof_store,
         "batch_roofline": batch_roof,
         "cpu_baseline": cpu.get("result"),
     }
@@ -325,6 +338,39 @@ def main():
     if args.json_out:
         with open(args.json_out, "w") as f:
             f.write(line + "\n")
+
+
+def serve_only(g, labels, stream, cur, args, snap, names, root, V, E, n0, info, build_s):
+    """Profiling form of the run: no plain leg and no ramp; W warmup and K timed
+    steps of serving at a fixed offered rate, then the roofline sample, so a
+    kernel trace of the command holds the serving kernels alone."""
+    rate = args.serve_only
+    n_step = int(max(20000, rate * args.step_seconds))
+    lats, wall, total, batches = [], 0.0, 0, 0
+    tp0 = None
+    for k in range(args.warmup + args.steps):
+        if k == args.warmup:
+            tp0 = g.profile_read(1)
+        res, lat = serve_batched(g, labels, stream, cur, n_step, rate, args)
+        log(f"serve-only step {k}: {res['updates_per_s']:.0f} upd/s p99.9={res['p999_ms']:.2f}ms "
+            f"mean_batch={res['mean_batch']:.1f}")
+        if k >= args.warmup:
+            lats.append(lat)
+            wall += res["wall_s"]
+            total += res["served"]
+            batches += res["batches"]
+    lat_all = np.concatenate(lats)
+    tp1 = g.profile_read(1)
+    roof, roof_store, batch_roof = roofline_sample(g, labels, stream, cur, rate, n_step, args,
+                                                   int(total / max(batches, 1)),
+                                                   (tp1["launches"] - tp0["launches"], tp1["ms"] - tp0["ms"]))
+    out = {"mode": "serve-only profiling run", "offered_rate": rate, "value": round(total / max(wall, 1e-9), 1),
+           "p99.9_ms": round(float(np.percentile(lat_all, 99.9)), 3), "mean_batch": round(total / max(batches, 1), 1),
+           "roofline": roof, "roofline_store": roof_store, "batch_roofline": batch_roof}
+    print(json.dumps(out), flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(json.dumps(out) + "\n")
 
 
 # ---------------------------------------------------------------------------
@@ -341,8 +387,20 @@ def traffic_probe(g, labels, stream, n0, E, args):
     FETCH_SIZE / WRITE_SIZE counter runs."""
     pos = 0
     m = args.probe_pairs
+    names = ["bfs", "sssp", "sswp", "wcc"]
+    out_deg, in_deg = g.degrees()
+    deg = out_deg.astype(np.int64) + in_deg.astype(np.int64)
+    eng_bytes, team_l, grid_l, changed = 0, 0, 0, 0
     for _ in range(args.probe_batches):
+        before = [L.read()[0] for L in labels]
+        t0, r0 = g.profile_read(1), g.profile_read(2)
         g.apply_stream(labels, stream, (n0 + pos) % E, m, pos % E, m, True)
+        t1, r1 = g.profile_read(1), g.profile_read(2)
+        b, c = delta_bytes(names, labels, before, deg)
+        eng_bytes += b
+        changed += c
+        team_l += t1["launches"] - t0["launches"]
+        grid_l += r1["launches"] - r0["launches"]
         pos += m
     # the HIP runtime's device-to-device copy (its blit kernel) of 1 GiB
     import ctypes
@@ -358,7 +416,10 @@ def traffic_probe(g, labels, stream, n0, E, args):
         hip.hipFree(src_p)
         hip.hipFree(dst_p)
     print(json.dumps({"probe": "traffic", "store_batch_edge_ops": 2 * m, "batches": args.probe_batches,
-                      "copy_ok": bool(ok), "copy_bytes_read": nbytes, "copy_bytes_written": nbytes}),
+                      "copy_ok": bool(ok), "copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                      "engine_algorithmic_bytes": eng_bytes, "changed_vertices": changed,
+                      "team_launches": team_l, "grid_steps": grid_l,
+                      "engine_bytes_per_team_launch": round(eng_bytes / max(team_l, 1), 1)}),
           flush=True)
 
 
@@ -378,65 +439,118 @@ def measured_traffic():
         return None
 
 
-def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch):
-    """Store-apply kernel: algorithmic bytes (32 B per edge op) over its
-    event-timed duration, during a serving round at the operating rate.
-    Batch level: bytes from the changed-vertex set of whole batches over the
-    batch's device time (store + all four trees)."""
+def engine_traffic():
+    """HBM bytes per engine team launch (rg::k_team) from the committed counter
+    summary (profiles/k_team_traffic_rmat24.md), with the algorithmic bytes per
+    launch of the same probe batches."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_team_traffic_rmat24.md")
+    try:
+        vals = {}
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_"):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = float(v.strip())
+        return vals or None
+    except OSError:
+        return None
+
+
+def delta_bytes(names, labels, before, deg):
+    """Algorithmic bytes of one batch's repair from its changed-vertex set:
+    per changed vertex B_VERTEX + (out+in live degree) x B_EDGE[tree]."""
+    b, changed = 0, 0
+    for nm, L, v0 in zip(names, labels, before):
+        d = np.nonzero(L.read()[0] != v0)[0]
+        changed += len(d)
+        b += len(d) * B_VERTEX + int(deg[d].sum()) * B_EDGE[nm]
+    return b, changed
+
+
+def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch, team_timed):
+    """The dominant kernel is the persistent engine launch (rg::k_team): its
+    algorithmic bytes are the changed-vertex bytes of whole batches (the
+    store's 32 B per edge op left out), divided over the batches' team
+    launches; its average duration is the HIP-event average over every team
+    launch of the process (the dispatches rocprofv3 averages for this
+    command).  Grid-wide passes of the same batches, which share that work,
+    are added into a second, engine-level figure.  The store apply kernel
+    (rg::k_apply_all) keeps its own line.  Batch level: all bytes over the
+    batch's device time."""
     names = ["bfs", "sssp", "sswp", "wcc"]
-    eb0 = g.profile_read(1)
-    g.profile(3)
-    n = int(max(20000, rate * min(args.step_seconds, 0.5)))
-    res, _ = serve_batched(g, labels, stream, cur, n, rate, args)
-    eb1 = g.profile_read(1)
-    g.profile(1)
-    # every k_apply_all launch of this process (load batches included), the same
-    # set of dispatches rocprofv3 --stats averages for this command
-    ap = g.profile_read(0)
-    launch_ms = ap["ms"] / max(ap["launches"], 1)
-    bytes_per_launch = B_STORE * ap["units"] / max(ap["launches"], 1)
-    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
-    nb = eb1["launches"] - eb0["launches"]
-    roof = {
-        "kernel": "rg::k_apply_all (store insert/delete batch: light and heavy lists in one launch)",
-        "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-        "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
-        "avg_launch_ms": round(launch_ms, 5), "launches": ap["launches"],
-        "engine_team_launches": nb,
-        "engine_team_avg_ms": round((eb1["ms"] - eb0["ms"]) / max(nb, 1), 5),
-    }
-    tr = measured_traffic()
-    if tr and "traffic_bytes_per_launch" in tr:
-        # HBM bytes per launch from the FETCH_SIZE + WRITE_SIZE counter runs
-        # (copy-kernel calibrated), at the batch size they were taken at
-        roof["traffic"] = tr["traffic_bytes_per_launch"]
-        roof["traffic_batch_edge_ops"] = int(tr.get("traffic_batch_edge_ops", 0))
-        roof["traffic_algorithmic_bytes"] = tr.get("traffic_algorithmic_bytes")
-    # batch level, from the changed-vertex set (values before vs after)
     out_deg, in_deg = g.degrees()
     deg = out_deg.astype(np.int64) + in_deg.astype(np.int64)
-    tot_bytes, tot_ms, changed = 0, 0.0, 0
+    g.profile(7)
+    tot_bytes, eng_bytes, tot_ms, changed = 0, 0, 0.0, 0
+    team_l, team_ms, grid_ms = 0, 0.0, 0.0
     batch = max(2, mean_batch - mean_batch % 2)
-    for _ in range(3):
+    nb = 3
+    for _ in range(nb):
         before = [L.read()[0] for L in labels]
+        t0, r0 = g.profile_read(1), g.profile_read(2)
         ins_first, del_first, m = cur.take(batch)
         g.apply_stream(labels, stream, ins_first, m // 2, del_first, m // 2, True)
         tot_ms += g.last_gpu_ms()
-        b = B_STORE * m
-        for nm, L, v0 in zip(names, labels, before):
-            d = np.nonzero(L.read()[0] != v0)[0]
-            changed += len(d)
-            b += len(d) * B_VERTEX + int(deg[d].sum()) * B_EDGE[nm]
-        tot_bytes += b
+        t1, r1 = g.profile_read(1), g.profile_read(2)
+        team_l += t1["launches"] - t0["launches"]
+        team_ms += t1["ms"] - t0["ms"]
+        grid_ms += r1["ms"] - r0["ms"]
+        b, c = delta_bytes(names, labels, before, deg)
+        changed += c
+        eng_bytes += b
+        tot_bytes += b + B_STORE * m
+    g.profile(3)
+    # the team launches of the timed serving steps: the dispatches that
+    # rocprofv3 averages for the bench's serve-only form at the same rate
+    team_l_timed, team_ms_timed = team_timed
+    team_avg_ms = team_ms_timed / max(team_l_timed, 1)
+    per_launch = eng_bytes / max(team_l, 1)
+    achieved = per_launch / (team_avg_ms * 1e-3) / 1e9 if team_avg_ms > 0 else 0.0
+    eng_ach = eng_bytes / ((team_ms + grid_ms) * 1e-3) / 1e9 if team_ms + grid_ms > 0 else 0.0
+    roof = {
+        "kernel": "rg::k_team (persistent engine launch: all four trees' repair passes of a batch)",
+        "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+        "algorithmic_bytes_per_launch": round(per_launch, 1),
+        "bytes_basis": "changed-vertex set of 3 sample batches at the operating batch size "
+                       f"({batch} updates): {B_VERTEX} B per changed vertex + live degree x "
+                       f"{B_EDGE} B per tree, over their {team_l} team launches",
+        "avg_launch_ms": round(team_avg_ms, 5), "launches": team_l_timed,
+        "avg_basis": "HIP events around every team launch of the timed serving steps",
+        "engine_level": {"bytes": eng_bytes, "team_ms": round(team_ms, 4), "grid_ms": round(grid_ms, 4),
+                         "achieved": round(eng_ach, 3), "frac": round(eng_ach / HBM_PEAK_GBS, 6)},
+    }
+    tr = engine_traffic()
+    if tr and "traffic_bytes_per_launch" in tr:
+        roof["traffic"] = tr["traffic_bytes_per_launch"]
+        roof["traffic_algorithmic_bytes_per_launch"] = tr.get("traffic_algorithmic_bytes_per_launch")
+        roof["traffic_batch_pairs"] = int(tr.get("traffic_batch_pairs", 0))
+    # the store apply kernel (one launch per batch of more than 64 edge ops)
+    ap = g.profile_read(0)
+    launch_ms = ap["ms"] / max(ap["launches"], 1)
+    bytes_per_launch = B_STORE * ap["units"] / max(ap["launches"], 1)
+    st_ach = bytes_per_launch / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else 0.0
+    roof_store = {
+        "kernel": "rg::k_apply_all (store insert/delete batch: light and heavy lists in one launch)",
+        "bound": "hbm", "achieved": round(st_ach, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(st_ach / HBM_PEAK_GBS, 6), "traffic": None,
+        "algorithmic_bytes_per_launch": round(bytes_per_launch, 1),
+        "avg_launch_ms": round(launch_ms, 5), "launches": ap["launches"],
+    }
+    tr = measured_traffic()
+    if tr and "traffic_bytes_per_launch" in tr:
+        roof_store["traffic"] = tr["traffic_bytes_per_launch"]
+        roof_store["traffic_batch_edge_ops"] = int(tr.get("traffic_batch_edge_ops", 0))
+        roof_store["traffic_algorithmic_bytes"] = tr.get("traffic_algorithmic_bytes")
     ach = tot_bytes / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
-    batch_roof = {"batch_updates": batch, "algorithmic_bytes_per_batch": round(tot_bytes / 3, 1),
-                  "changed_vertices_per_batch": round(changed / 3, 1),
-                  "batch_gpu_ms": round(tot_ms / 3, 4), "achieved": round(ach, 3), "unit": "GB/s",
+    batch_roof = {"batch_updates": batch, "algorithmic_bytes_per_batch": round(tot_bytes / nb, 1),
+                  "changed_vertices_per_batch": round(changed / nb, 1),
+                  "batch_gpu_ms": round(tot_ms / nb, 4), "achieved": round(ach, 3), "unit": "GB/s",
                   "frac": round(ach / HBM_PEAK_GBS, 6)}
-    log(f"roofline: k_apply_all {achieved:.2f} GB/s ({launch_ms * 1e3:.1f} us/launch); "
-        f"batch {ach:.2f} GB/s over {tot_ms / 3:.3f} ms/batch")
-    return roof, batch_roof
+    log(f"roofline: k_team {achieved:.2f} GB/s ({per_launch / 1e6:.2f} MB per launch, "
+        f"{team_avg_ms * 1e3:.1f} us avg); engine {eng_ach:.2f} GB/s; k_apply_all {st_ach:.2f} GB/s; "
+        f"batch {ach:.2f} GB/s over {tot_ms / nb:.3f} ms/batch")
+    return roof, roof_store, batch_roof
 
 
 # ---------------------------------------------------------------------------

@@ -124,11 +124,9 @@ struct rg_graph {
     uint32_t h_cap = 0;
     hipEvent_t ev[4]{};
     double last_ms = 0;
-    // the safe/unsafe split's per-op flags and count, copied to the host on
-    // their own stream while the engine runs
-    hipStream_t st_copy = nullptr;
-    hipEvent_t ev_cls = nullptr, ev_safe = nullptr;
-    uint32_t *h_ucount = nullptr;     // pinned
+    // the safe/unsafe split's per-op flags reach the host (pinned) when
+    // ev_safe completes, while the repair runs
+    hipEvent_t ev_safe = nullptr;
     uint64_t last_unsafe = 0;         // unsafe ops of the last split batch
     uint64_t version = 0;             // stream batches applied (the history's version clock)
     int wal_fd = -1;                  // write-ahead log of transaction batches (rg_wal_open)
@@ -838,11 +836,8 @@ int rg_graph_create(uint64_t vertices, uint64_t hint_edges, int symmetric, int d
         g->dual = dual;
         g->weighted = weighted;
         HC(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
-        HC(hipStreamCreateWithFlags(&g->st_copy, hipStreamNonBlocking));
         for (auto &e : g->ev) HC(hipEventCreate(&e));
-        HC(hipEventCreateWithFlags(&g->ev_cls, hipEventDisableTiming));
         HC(hipEventCreateWithFlags(&g->ev_safe, hipEventDisableTiming));
-        g->h_ucount = halloc<uint32_t>(1);
         alloc_list(g->lists[0], vertices, g->st);
         if (!symmetric) alloc_list(g->lists[1], vertices, g->st);
         else g->lists[1] = g->lists[0];
@@ -882,7 +877,6 @@ int rg_graph_destroy(rg_graph *g)
 {
     if (!g) return RG_OK;
     if (g->st) (void)hipStreamSynchronize(g->st);
-    if (g->st_copy) (void)hipStreamSynchronize(g->st_copy);
     if (g->wal_fd >= 0) (void)close(g->wal_fd);
     free_list(g->lists[0]);
     if (!g->symmetric) free_list(g->lists[1]);
@@ -893,10 +887,7 @@ int rg_graph_destroy(rg_graph *g)
     hfree(g->h_src); hfree(g->h_dst); hfree(g->h_w); hfree(g->h_res); hfree(g->h_kind); hfree(g->h_flag);
     hfree(g->h_status);
     for (auto &e : g->ev) if (e) (void)hipEventDestroy(e);
-    if (g->ev_cls) (void)hipEventDestroy(g->ev_cls);
     if (g->ev_safe) (void)hipEventDestroy(g->ev_safe);
-    hfree(g->h_ucount);
-    if (g->st_copy) (void)hipStreamDestroy(g->st_copy);
     for (auto &r : g->prof_pending) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -1521,19 +1512,15 @@ static void repair_batch(rg_graph *g, rg_labels **L, int nL, uint64_t n, bool ha
         P.ntrees = nL;
         P.in = g->ops;
         P.out = g->uops;
-        P.unsafe = flags ? g->d_flag : nullptr;
+        // the split writes its flags straight into pinned host memory: a copy
+        // on another stream would wait for CUs behind the repair's
+        // persistent launch, and the safe updates with it
+        P.unsafe = flags ? g->h_flag : nullptr;
         P.count = g->ucount;
         hooks.before_launch = [g, P, n, flags] {
             launch_split(P, (uint32_t)n, g->st);
             HC(hipGetLastError());
-            if (flags) {
-                // the flags travel on their own stream while the repair runs
-                HC(hipEventRecord(g->ev_cls, g->st));
-                HC(hipStreamWaitEvent(g->st_copy, g->ev_cls, 0));
-                HC(hipMemcpyAsync(g->h_flag, g->d_flag, n, hipMemcpyDeviceToHost, g->st_copy));
-                HC(hipMemcpyAsync(g->h_ucount, g->ucount, 4, hipMemcpyDeviceToHost, g->st_copy));
-                HC(hipEventRecord(g->ev_safe, g->st_copy));
-            }
+            if (flags) HC(hipEventRecord(g->ev_safe, g->st));
         };
         if (flags)
             hooks.after_first_enqueue = [g, so] {
