@@ -77,6 +77,9 @@ def parse():
                     help="every update of a batch waits for the batch's repair (no safe/unsafe split)")
     ap.add_argument("--cap", choices=["unsafe", "total"], default="unsafe",
                     help="what the adaptive admission cap counts: unsafe updates (default) or all updates")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each batch's window, store and split kernels one by one instead of replaying "
+                         "the captured hipGraph of the batch's size bucket")
     return ap.parse_args()
 
 
@@ -137,7 +140,10 @@ def main():
     t_setup = time.time()
     stream = rg.Stream("rmat", scale, E, seed=1, wseed=2)
     g = rg.Graph(V, E, symmetric=False, dual=True, weighted=True)
-    g.profile(3)          # HIP events around every store apply kernel and engine team launch of this process
+    # HIP events around every engine team launch of this process; the store
+    # kernels run in captured graphs here and are timed in the sample batches
+    g.profile(2)
+    g.capture(not args.no_graph)
     g.load_stream(stream, 0, n0, True)
     root = stream.max_degree_vertex(n0, V)
     names = ["bfs", "sssp", "sswp", "wcc"]
@@ -315,6 +321,7 @@ def main():
                        "p50_unsafe_median_step": round(float(np.median(acc["unsafe_p50"])), 3),
                        "p99.9_unsafe_worst_step": round(float(max(acc["unsafe_p999"])), 3)},
         "split": not args.no_split,
+        "graph_replay": g.capture_info(),
         "cap_counts": "unsafe updates" if (args.cap == "unsafe" and not args.no_split) else "all updates",
         "safe_frac": round(1.0 - acc["unsafe"] / max(total, 1), 4),
         "target_met": met,
@@ -499,7 +506,7 @@ def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch, team
         changed += c
         eng_bytes += b
         tot_bytes += b + B_STORE * m
-    g.profile(3)
+    g.profile(2)
     # the team launches of the timed serving steps: the dispatches that
     # rocprofv3 averages for the bench's serve-only form at the same rate
     team_l_timed, team_ms_timed = team_timed
@@ -525,7 +532,9 @@ def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch, team
         roof["traffic"] = tr["traffic_bytes_per_launch"]
         roof["traffic_algorithmic_bytes_per_launch"] = tr.get("traffic_algorithmic_bytes_per_launch")
         roof["traffic_batch_pairs"] = int(tr.get("traffic_batch_pairs", 0))
-    # the store apply kernel (one launch per batch of more than 64 edge ops)
+    # the store apply kernel (one launch per batch of more than 64 edge ops),
+    # timed in the sample batches above: timing bit 0 launches the store
+    # kernels of a batch one by one, outside a captured graph
     ap = g.profile_read(0)
     launch_ms = ap["ms"] / max(ap["launches"], 1)
     bytes_per_launch = B_STORE * ap["units"] / max(ap["launches"], 1)

@@ -180,6 +180,16 @@ int rg_snapshot_load(const char *path, rg_graph **g, rg_labels **L, int cap, int
 /* A device-resident stream made from host edges (an edge file read with
  * mmap_binary, io.hpp:26-36, or a snapshot's edges); weights from data. */
 int rg_stream_from_edges(const rg_edge *edges, uint64_t n, int weighted, rg_stream **out);
+/* Stream batches of more than 64 edge ops with trees to repair run their
+ * window, store and split launches as one captured hipGraph per batch-size
+ * bucket (powers of two from 128), replayed with the batch's size in a
+ * device descriptor; on = 0 launches them one by one.  The default is on
+ * unless the environment sets RG_GRAPH=0.  No counterpart in the reference
+ * (its batches are host loops); results are the same either way. */
+int rg_graph_capture(rg_graph *g, int on);
+/* Whether capture is on (it turns off by itself where the runtime cannot
+ * capture), the pipelines captured so far and the batches replayed. */
+int rg_graph_capture_info(rg_graph *g, int *on, uint32_t *pipelines, uint64_t *replays);
 /* The shape a graph was created with, and a labels instance's algorithm and
  * root (for handles that rg_snapshot_load made). */
 int rg_graph_shape(rg_graph *g, int *symmetric, int *dual, int *weighted);

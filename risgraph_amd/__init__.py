@@ -84,6 +84,8 @@ SIGNATURES = {
     "rg_stream_from_edges": (ci, [vp, u64, ci, ctypes.POINTER(vp)]),
     "rg_labels_assign": (ci, [vp, vp, vp]),
     "rg_graph_shape": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
+    "rg_graph_capture": (ci, [vp, ci]),
+    "rg_graph_capture_info": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ctypes.c_uint32), u64p]),
     "rg_labels_algo": (ci, [vp, ctypes.POINTER(ci), u64p]),
     "rg_server_start": (ci, [vp, vp, ci, vp, ctypes.POINTER(vp)]),
     "rg_server_port": (ci, [vp, ctypes.POINTER(ctypes.c_uint32)]),
@@ -346,6 +348,17 @@ class Graph:
         _check(load_library().rg_graph_shape(h, ctypes.byref(sym), ctypes.byref(dual), ctypes.byref(wt)))
         obj.V, obj.symmetric, obj.weighted, obj._labels = int(V.value), bool(sym.value), bool(wt.value), []
         return obj
+
+    def capture(self, on=True):
+        """rg_graph_capture: replay stream batches' window/store/split
+        launches from captured hipGraphs (on) or launch them one by one."""
+        _check(load_library().rg_graph_capture(self.h, int(on)))
+
+    def capture_info(self):
+        """rg_graph_capture_info: {"on", "pipelines", "replays"}."""
+        on, n, r = ci(), ctypes.c_uint32(), u64()
+        _check(load_library().rg_graph_capture_info(self.h, ctypes.byref(on), ctypes.byref(n), ctypes.byref(r)))
+        return {"on": bool(on.value), "pipelines": int(n.value), "replays": int(r.value)}
 
     def wal_open(self, path, sync=True):
         """rg_wal_open: log every later transaction batch to `path`."""

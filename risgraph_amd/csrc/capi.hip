@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -129,6 +130,23 @@ struct rg_graph {
     hipEvent_t ev_safe = nullptr;
     uint64_t last_unsafe = 0;         // unsafe ops of the last split batch
     uint64_t version = 0;             // stream batches applied (the history's version clock)
+    // captured pipelines of stream batches (window -> store -> split), one
+    // per batch-size bucket, tree set and window source (rg_graph_capture);
+    // keyed by the argument values the captured launches hold
+    struct Captured {
+        uint32_t bucket = 0;
+        SplitParams P{};
+        const uint64_t *src = nullptr, *dst = nullptr;
+        const uint32_t *w = nullptr;
+        uint64_t len = 0;
+        uint8_t kind_ins = 0, kind_del = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Captured> captured;
+    int capture = -1;                 // -1: RG_GRAPH decides (default on)
+    uint64_t replays = 0;             // batches run through a captured pipeline
+    BatchDesc *h_desc = nullptr;      // pinned
+    BatchDesc *d_desc = nullptr;
     int wal_fd = -1;                  // write-ahead log of transaction batches (rg_wal_open)
     int wal_sync = 1;                 // fdatasync after every logged batch (group commit)
     std::recursive_mutex mu;
@@ -228,10 +246,19 @@ void free_ops(rg_graph *g)
     g->op_cap = 0;
 }
 
+// captured pipelines hold the op buffers' and pinned flags' addresses
+void captured_clear(rg_graph *g)
+{
+    for (auto &c : g->captured)
+        if (c.exec) (void)hipGraphExecDestroy(c.exec);
+    g->captured.clear();
+}
+
 void ensure_ops(rg_graph *g, uint64_t n)
 {
     if (n > 0xFFFFFFFFull / 4) throw RgError{RG_EINVAL, "batch too large"};
     if (n <= g->op_cap) return;
+    captured_clear(g);
     uint32_t cap = std::max<uint32_t>((uint32_t)n, 1024);
     HC(hipStreamSynchronize(g->st));
     free_ops(g);
@@ -258,6 +285,7 @@ void ensure_ops(rg_graph *g, uint64_t n)
 void ensure_host(rg_graph *g, uint64_t n)
 {
     if (n <= g->h_cap) return;
+    captured_clear(g);
     uint32_t cap = std::max<uint32_t>((uint32_t)n, 1024);
     hfree(g->h_src); hfree(g->h_dst); hfree(g->h_w); hfree(g->h_res); hfree(g->h_kind); hfree(g->h_flag);
     g->h_src = halloc<uint32_t>(cap);
@@ -838,6 +866,8 @@ int rg_graph_create(uint64_t vertices, uint64_t hint_edges, int symmetric, int d
         HC(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
         for (auto &e : g->ev) HC(hipEventCreate(&e));
         HC(hipEventCreateWithFlags(&g->ev_safe, hipEventDisableTiming));
+        g->h_desc = halloc<BatchDesc>(1);
+        g->d_desc = dalloc<BatchDesc>(1);
         alloc_list(g->lists[0], vertices, g->st);
         if (!symmetric) alloc_list(g->lists[1], vertices, g->st);
         else g->lists[1] = g->lists[0];
@@ -888,6 +918,9 @@ int rg_graph_destroy(rg_graph *g)
     hfree(g->h_status);
     for (auto &e : g->ev) if (e) (void)hipEventDestroy(e);
     if (g->ev_safe) (void)hipEventDestroy(g->ev_safe);
+    captured_clear(g);
+    hfree(g->h_desc);
+    dfree(g->d_desc);
     for (auto &r : g->prof_pending) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -1448,6 +1481,7 @@ int rg_stream_load(rg_graph *g, rg_stream *s, uint64_t first, uint64_t n, int di
         // size every list (and hub index) once for the whole load
         for (int L = 0; L < (g->symmetric ? 1 : 2); L++) {
             if (!g->cnt_tmp[L]) {
+                captured_clear(g);   // captured store launches hold the buffer set
                 g->cnt_tmp[L] = dalloc<uint32_t>(g->V);
                 HC(hipMemsetAsync(g->cnt_tmp[L], 0, g->V * 4, g->st));
             }
@@ -1494,8 +1528,98 @@ constexpr uint64_t SPLIT_MIN_OPS = STORE_SMALL_OPS;
 // batch's n edge ops in g->ops, bracketed by ev[0] / ev[1]): the split of the
 // ops into safe and unsafe ones (when `split`), the engine run of every tree
 // on the unsafe ops, the store status, the version clock and the histories.
+static bool capture_on(const rg_graph *g)
+{
+    if (g->capture >= 0) return g->capture != 0;
+    static const bool env = [] {
+        const char *e = getenv("RG_GRAPH");
+        return !(e && *e == '0');
+    }();
+    return env;
+}
+
+// The captured window -> store -> split pipeline for a batch-size bucket:
+// the descriptor copy (pinned -> device), the window's edge ops, the store
+// kernels and the pool recycle, the split and its count publication, every
+// kernel sized for `bucket` edge ops and taking the batch's size from the
+// device descriptor.  Captured once per (bucket, trees, stream, kinds) and
+// replayed; null when capture is unavailable (the caller runs the launches).
+static hipGraphExec_t captured_pipeline(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint32_t bucket,
+                                        uint8_t kind_ins, uint8_t kind_del, bool flags)
+{
+    const uint32_t *w = g->weighted ? s->w : nullptr;
+    StoreArgs a = store_args(g);
+    a.dyn_n = &g->d_desc->n;
+    SplitParams P{};
+    for (int i = 0; i < nL; i++) {
+        const EngineArgs e = engine_args(L[i]);
+        P.vr[i] = e.vr;
+        P.algo[i] = e.algo;
+        P.undirected[i] = e.undirected;
+        P.hctl[i] = L[i]->hc;
+    }
+    P.ntrees = nL;
+    P.in = g->ops;
+    P.out = g->uops;
+    P.unsafe = flags ? g->h_flag : nullptr;
+    P.count = g->ucount;
+    P.dyn_n = &g->d_desc->n;
+    P.in.n = P.out.n = 0;   // the count comes from the descriptor
+    auto same_split = [nL](const SplitParams &x, const SplitParams &y) {
+        bool eq = x.ntrees == y.ntrees && x.unsafe == y.unsafe && x.count == y.count && x.dyn_n == y.dyn_n;
+        for (int i = 0; eq && i < nL; i++)
+            eq = x.vr[i] == y.vr[i] && x.algo[i] == y.algo[i] && x.undirected[i] == y.undirected[i] &&
+                 x.hctl[i] == y.hctl[i];
+        for (const EdgeOps *e : {&x.in, &x.out}) {
+            const EdgeOps &f = e == &x.in ? y.in : y.out;
+            eq = eq && e->src == f.src && e->dst == f.dst && e->w == f.w && e->kind == f.kind && e->res == f.res;
+        }
+        return eq;
+    };
+    for (const auto &c : g->captured)
+        if (c.bucket == bucket && same_split(c.P, P) && c.src == s->src && c.dst == s->dst && c.w == w &&
+            c.len == s->n && c.kind_ins == kind_ins && c.kind_del == kind_del)
+            return c.exec;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (hipStreamBeginCapture(g->st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        g->capture = 0;
+        return nullptr;
+    }
+    (void)hipMemcpyAsync(g->d_desc, g->h_desc, sizeof(BatchDesc), hipMemcpyHostToDevice, g->st);
+    launch_ops_from_window_dyn(s->src, s->dst, w, s->n, g->d_desc, bucket, kind_ins, kind_del, g->ops, g->ucount,
+                               g->st);
+    launch_store_batch_dyn(a, bucket, g->st);
+    launch_pool_recycle(a, g->st);
+    launch_split(P, bucket, g->st);
+    hipError_t e = hipStreamEndCapture(g->st, &graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    if (e != hipSuccess || !exec) {
+        // capture is not available here: every later batch launches eagerly
+        g->capture = 0;
+        return nullptr;
+    }
+    rg_graph::Captured c;
+    c.bucket = bucket;
+    c.P = P;
+    c.src = s->src;
+    c.dst = s->dst;
+    c.w = w;
+    c.len = s->n;
+    c.kind_ins = kind_ins;
+    c.kind_del = kind_del;
+    c.exec = exec;
+    g->captured.push_back(c);
+    return exec;
+}
+
+// `pipeline` (may be null): a captured window -> store -> split pipeline
+// that launches in place of the split (the store is then not yet enqueued).
 static void repair_batch(rg_graph *g, rg_labels **L, int nL, uint64_t n, bool has_ins, bool has_del,
-                         rg_batch_stats *stats, bool split, SplitOut *so)
+                         rg_batch_stats *stats, bool split, SplitOut *so,
+                         const std::function<void()> *pipeline = nullptr)
 {
     const bool flags = split && so && so->flags;
     uint32_t ph = has_del ? PH_DEL_ROOTS : PH_SEED;
@@ -1517,8 +1641,9 @@ static void repair_batch(rg_graph *g, rg_labels **L, int nL, uint64_t n, bool ha
         // persistent launch, and the safe updates with it
         P.unsafe = flags ? g->h_flag : nullptr;
         P.count = g->ucount;
-        hooks.before_launch = [g, P, n, flags] {
-            launch_split(P, (uint32_t)n, g->st);
+        hooks.before_launch = [g, P, n, flags, pipeline] {
+            if (pipeline) (*pipeline)();
+            else launch_split(P, (uint32_t)n, g->st);
             HC(hipGetLastError());
             if (flags) HC(hipEventRecord(g->ev_safe, g->st));
         };
@@ -1571,12 +1696,40 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
     uint64_t n = n_ins + n_del;
     if (so) so->done = false;
     if (!n) return;
-    ensure_ops(g, n);
     const bool split = nL > 0 && nL <= MAXT && n > SPLIT_MIN_OPS;
-    if (split && so && so->flags) ensure_host(g, n);
+    const bool flags = split && so && so->flags;
     uint8_t ud = directed ? 0 : K_UNDIR;
     const uint32_t *w = g->weighted ? s->w : nullptr;
     if (g->weighted && !w) throw RgError{RG_EINVAL, "weighted graph needs a weighted stream"};
+    // a split batch whose store kernels are not timed one by one runs the
+    // window -> store -> split part as one captured graph of its size bucket
+    if (split && capture_on(g) && !(g->prof & 1)) {
+        uint32_t bucket = 128;
+        while (bucket < n) bucket <<= 1;
+        ensure_ops(g, bucket);
+        if (flags) ensure_host(g, bucket);
+        g->ops.n = (uint32_t)n;
+        g->lops.n = (uint32_t)(4 * n);
+        hipGraphExec_t exec = captured_pipeline(g, L, nL, s, bucket, (uint8_t)(K_INS | ud), (uint8_t)(K_DEL | ud),
+                                                flags);
+        if (exec) {
+            const std::function<void()> pipe = [&] {
+                *g->h_desc = BatchDesc{ins_first, del_first, (uint32_t)n_ins, (uint32_t)n_del, (uint32_t)n, 0};
+                HC(hipEventRecord(g->ev[0], g->st));
+                HC(hipGraphLaunch(exec, g->st));
+                g->replays++;
+                HC(hipEventRecord(g->ev[1], g->st));
+            };
+            repair_batch(g, L, nL, n, n_ins > 0, n_del > 0, stats, split, so, &pipe);
+            if (stats) {
+                stats->seeds += n_ins;
+                stats->roots += n_del;
+            }
+            return;
+        }
+    }
+    ensure_ops(g, n);
+    if (flags) ensure_host(g, n);
     launch_ops_from_window(s->src, s->dst, w, s->n, ins_first, (uint32_t)n_ins, del_first, (uint32_t)n_del,
                            (uint8_t)(K_INS | ud), (uint8_t)(K_DEL | ud), g->ops, split ? g->ucount : nullptr, g->st);
     g->ops.n = (uint32_t)n;
@@ -1859,6 +2012,24 @@ int rg_wal_replay(rg_graph *g, rg_labels **L, int nL, const char *path, uint64_t
             if (batches) (*batches)++;
         }
     });
+}
+
+int rg_graph_capture(rg_graph *g, int on)
+{
+    if (!g) return RG_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
+    g->capture = on ? 1 : 0;
+    return RG_OK;
+}
+
+int rg_graph_capture_info(rg_graph *g, int *on, uint32_t *pipelines, uint64_t *replays)
+{
+    if (!g) return RG_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(g->mu);
+    if (on) *on = capture_on(g) ? 1 : 0;
+    if (pipelines) *pipelines = (uint32_t)g->captured.size();
+    if (replays) *replays = g->replays;
+    return RG_OK;
 }
 
 int rg_graph_shape(rg_graph *g, int *symmetric, int *dual, int *weighted)

@@ -1403,6 +1403,7 @@ __device__ __forceinline__ bool op_unsafe(const VRec *vr, bool ins, bool both, u
 // push over the store, where it already is).
 __global__ void k_split(SplitParams P, uint32_t n)
 {
+    if (P.dyn_n) n = *P.dyn_n;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool u = false;
     uint32_t s = 0, d = 0, w = 0;

@@ -282,6 +282,17 @@ struct StoreArgs {
     uint32_t *err;   // bad vertex ids etc.
     uint32_t *hstatus;  // pinned host words: err, oom (written after each batch)
     uint32_t *cnt_tmp[2];  // per-vertex insert counts for bulk reservation
+    // captured batch pipelines: the batch's edge-op count comes from this
+    // device word instead of ops.n (the launch grids are sized for the
+    // bucket's largest batch); null = ops.n
+    const uint32_t *dyn_n;
+};
+
+// The window of one captured stream batch: written by the host into pinned
+// memory and copied to the device by the graph's first node.
+struct BatchDesc {
+    uint64_t ins_first, del_first;
+    uint32_t n_ins, n_del, n, pad;
 };
 
 // host-side launchers (store.hip / engine.hip)
@@ -350,6 +361,7 @@ struct SplitParams {
     EdgeOps in, out;
     uint8_t *unsafe;
     uint32_t *count;
+    const uint32_t *dyn_n;   // captured pipelines: the op count (else the launch's n)
 };
 void launch_split(const SplitParams &p, uint32_t n, hipStream_t s);
 void launch_max_degree(const uint64_t *src, uint64_t n, uint64_t V, uint32_t *deg_tmp,
@@ -357,6 +369,14 @@ void launch_max_degree(const uint64_t *src, uint64_t n, uint64_t V, uint32_t *de
 void launch_ops_from_stream(const uint64_t *src, const uint64_t *dst, const uint32_t *w,
                             uint64_t first, uint32_t n, uint8_t kind, EdgeOps ops,
                             uint32_t base, hipStream_t s);
+// the same window read from a device BatchDesc, with a grid for up to
+// n_max edge ops (captured pipelines)
+void launch_ops_from_window_dyn(const uint64_t *src, const uint64_t *dst, const uint32_t *w, uint64_t len,
+                                const BatchDesc *desc, uint32_t n_max, uint8_t kind_ins, uint8_t kind_del,
+                                EdgeOps ops, uint32_t *zero, hipStream_t s);
+// store batch launches for a captured pipeline: every kernel takes its size
+// from a.dyn_n, grids sized for n_max edge ops (n_max > STORE_SMALL_OPS)
+void launch_store_batch_dyn(const StoreArgs &a, uint32_t n_max, hipStream_t s);
 // `zero` (may be null) is set to 0 by the launch (the split's counter)
 void launch_ops_from_window(const uint64_t *src, const uint64_t *dst, const uint32_t *w, uint64_t len,
                             uint64_t ins_first, uint32_t n_ins, uint64_t del_first, uint32_t n_del,

@@ -108,8 +108,18 @@ __device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank)
     }
 }
 
+// captured pipelines take the batch size from device memory
+__device__ __forceinline__ void dyn_size(StoreArgs &a)
+{
+    if (a.dyn_n) {
+        a.ops.n = *a.dyn_n;
+        a.lops.n = 4 * a.ops.n;
+    }
+}
+
 __global__ void k_expand_link(StoreArgs a)
 {
+    dyn_size(a);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < a.ops.n) expand_op(a, i, true);
 }
@@ -239,6 +249,7 @@ __device__ void reserve_growth(const StoreArgs &a, uint32_t cls, bool want, uint
 
 __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
 {
+    dyn_size(a);
     __shared__ uint32_t wsum[SEG_BLOCK / WAVE][3];
     __shared__ unsigned long long bbase;
     __shared__ uint32_t hbase;
@@ -311,6 +322,7 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
 
 __global__ void k_scatter(StoreArgs a)
 {
+    dyn_size(a);
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= 4 * a.ops.n) return;
     const uint32_t v = a.lops.op[j].v;
@@ -1013,6 +1025,7 @@ static_assert(HB == 256, "light and heavy blocks share the launch's block size")
 __global__ void __launch_bounds__(HB) k_apply_all(StoreArgs a, uint32_t hblocks)
 {
     __shared__ HeavyShared sh;
+    dyn_size(a);
     if (blockIdx.x >= hblocks) {
         light_lists_apply(a, blockIdx.x - hblocks, gridDim.x - hblocks);
         return;
@@ -1054,6 +1067,18 @@ bool launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipE
     hipLaunchKernelGGL(k_apply_all, dim3(hblocks + (uint32_t)blocks), dim3(HB), 0, s, a, hblocks);
     if (ev_b) (void)hipEventRecord(ev_b, s);
     return false;
+}
+
+void launch_store_batch_dyn(const StoreArgs &a, uint32_t n_max, hipStream_t s)
+{
+    const uint32_t nl = 4 * n_max;
+    hipLaunchKernelGGL(k_expand_link, dim3((n_max + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_segment, dim3((nl + SEG_BLOCK - 1) / SEG_BLOCK), dim3(SEG_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(k_scatter, dim3((nl + 255) / 256), dim3(256), 0, s, a);
+    uint64_t blocks = ((uint64_t)nl * AG + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    const uint32_t hblocks = nl < 2048 ? nl : 2048;
+    hipLaunchKernelGGL(k_apply_all, dim3(hblocks + (uint32_t)blocks), dim3(HB), 0, s, a, hblocks);
 }
 
 // ---------------------------------------------------------------------
@@ -1470,6 +1495,31 @@ __global__ void k_ops_from_window(const uint64_t *src, const uint64_t *dst, cons
     ops.dst[i] = (uint32_t)dst[e];
     ops.w[i] = w ? w[e] : 0;
     ops.kind[i] = ins ? kind_ins : kind_del;
+}
+
+__global__ void k_ops_from_window_dyn(const uint64_t *src, const uint64_t *dst, const uint32_t *w, uint64_t len,
+                                      const BatchDesc *desc, uint8_t kind_ins, uint8_t kind_del, EdgeOps ops,
+                                      uint32_t *zero)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && zero) *zero = 0;
+    const BatchDesc d = *desc;
+    if (i >= d.n_ins + d.n_del) return;
+    const bool ins = i < d.n_ins;
+    uint64_t e = ins ? d.ins_first + i : d.del_first + (i - d.n_ins);
+    e %= len;
+    ops.src[i] = (uint32_t)src[e];
+    ops.dst[i] = (uint32_t)dst[e];
+    ops.w[i] = w ? w[e] : 0;
+    ops.kind[i] = ins ? kind_ins : kind_del;
+}
+
+void launch_ops_from_window_dyn(const uint64_t *src, const uint64_t *dst, const uint32_t *w, uint64_t len,
+                                const BatchDesc *desc, uint32_t n_max, uint8_t kind_ins, uint8_t kind_del,
+                                EdgeOps ops, uint32_t *zero, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ops_from_window_dyn, dim3((n_max + 255) / 256), dim3(256), 0, s, src, dst, w, len, desc,
+                       kind_ins, kind_del, ops, zero);
 }
 
 void launch_ops_from_window(const uint64_t *src, const uint64_t *dst, const uint32_t *w, uint64_t len,
