@@ -55,7 +55,7 @@ __device__ __forceinline__ uint32_t next_pow2(uint32_t x)
 // Edge op i -> list ops 4i..4i+3 (unused slots get v = NILP).  With `rank`,
 // each list op takes its rank on its list with one atomicAdd on the header's
 // nops (the segment path); without, the caller groups the ops itself.
-__device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank)
+__device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank, uint32_t n_ops)
 {
     uint32_t s = a.ops.src[i], d = a.ops.dst[i], w = a.weighted ? a.ops.w[i] : 0;
     uint8_t kind = a.ops.kind[i];
@@ -97,7 +97,7 @@ __device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank)
         emit(s, d, sign, L_RET | L_SLOT);
         if (!a.symmetric) emit(d, s, sign, L_IN);
     } else if (s != d) {
-        bool ret_sd = a.ops.n > 1 && s < d;
+        bool ret_sd = n_ops > 1 && s < d;
         emit(s, d, sign, (uint8_t)(L_SLOT | (ret_sd ? L_RET : 0)));
         if (!a.symmetric) emit(d, s, sign, L_IN);
         emit(d, s, sign, (uint8_t)(ret_sd ? 0 : L_RET));
@@ -108,20 +108,19 @@ __device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank)
     }
 }
 
-// captured pipelines take the batch size from device memory
-__device__ __forceinline__ void dyn_size(StoreArgs &a)
+// The batch's edge-op count: captured pipelines take it from device memory.
+// (Kernels read it into a local and never write their StoreArgs argument: a
+// written argument becomes a per-lane private copy of the whole struct.)
+__device__ __forceinline__ uint32_t op_count(const StoreArgs &a)
 {
-    if (a.dyn_n) {
-        a.ops.n = *a.dyn_n;
-        a.lops.n = 4 * a.ops.n;
-    }
+    return a.dyn_n ? *a.dyn_n : a.ops.n;
 }
 
 __global__ void k_expand_link(StoreArgs a)
 {
-    dyn_size(a);
+    const uint32_t n = op_count(a);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < a.ops.n) expand_op(a, i, true);
+    if (i < n) expand_op(a, i, true, n);
 }
 
 // ---------------------------------------------------------------------
@@ -249,11 +248,10 @@ __device__ void reserve_growth(const StoreArgs &a, uint32_t cls, bool want, uint
 
 __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
 {
-    dyn_size(a);
     __shared__ uint32_t wsum[SEG_BLOCK / WAVE][3];
     __shared__ unsigned long long bbase;
     __shared__ uint32_t hbase;
-    const uint32_t nlops = 4 * a.ops.n;
+    const uint32_t nlops = 4 * op_count(a);
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t cnt = 0, lt = 0, hv = 0, v = NILP, gcls = 0;
     bool grow = false;
@@ -322,9 +320,8 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
 
 __global__ void k_scatter(StoreArgs a)
 {
-    dyn_size(a);
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= 4 * a.ops.n) return;
+    if (j >= 4 * op_count(a)) return;
     const uint32_t v = a.lops.op[j].v;
     if (v == NILP) return;
     const int L = (a.lops.op[j].flags & L_IN) ? 1 : 0;
@@ -1025,7 +1022,6 @@ static_assert(HB == 256, "light and heavy blocks share the launch's block size")
 __global__ void __launch_bounds__(HB) k_apply_all(StoreArgs a, uint32_t hblocks)
 {
     __shared__ HeavyShared sh;
-    dyn_size(a);
     if (blockIdx.x >= hblocks) {
         light_lists_apply(a, blockIdx.x - hblocks, gridDim.x - hblocks);
         return;
@@ -1266,7 +1262,7 @@ __global__ void __launch_bounds__(HB) k_store_small(StoreArgs a)
         nvalid = 0;
         nheavy = 0;
     }
-    if (tid < n) expand_op(a, tid, false);
+    if (tid < n) expand_op(a, tid, false, n);
     __syncthreads();
     // list key (L << 31 | v) of every list op; NILP for unused slots
     uint32_t kt = NILP;
