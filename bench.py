@@ -335,6 +335,7 @@ def main():
         "values_match_rebuild": check,
         "build_s": build_s,
         "store": info,
+        "pool": g.pool_info(),
         "roofline": roof,
         "roofline_store": roof_store,
         "batch_roofline": batch_roof,

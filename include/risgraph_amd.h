@@ -57,6 +57,12 @@ int rg_graph_destroy(rg_graph *g);
 /* vertices, slot-pool use/capacity (entries), hash-pool use/capacity */
 int rg_graph_info(rg_graph *g, uint64_t *vertices, uint64_t *slots_used, uint64_t *slots_cap,
                   uint64_t *hash_used, uint64_t *hash_cap);
+/* Slab-pool recycling, six counters: blocks recycled (slot pool, hash pool),
+ * blocks dropped (freed past the 262,144 pending frees per size class and
+ * batch, or into a full free list: never reused), and the entries those
+ * dropped blocks held.  No counterpart in the reference (its per-vertex
+ * vectors reallocate through the host allocator). */
+int rg_graph_pool_info(rg_graph *g, uint64_t out[6]);
 
 /* add_edge / del_edge -- graph.hpp:203-253 over IndexedEdgeStorage::update_edge
  * (storage.hpp:345-398).  delta = +1 add, -1 del.  *old = the reference return

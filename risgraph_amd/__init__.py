@@ -85,6 +85,7 @@ SIGNATURES = {
     "rg_labels_assign": (ci, [vp, vp, vp]),
     "rg_graph_shape": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
     "rg_graph_capture": (ci, [vp, ci]),
+    "rg_graph_pool_info": (ci, [vp, vp]),
     "rg_graph_capture_info": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ctypes.c_uint32), u64p]),
     "rg_labels_algo": (ci, [vp, ctypes.POINTER(ci), u64p]),
     "rg_server_start": (ci, [vp, vp, ci, vp, ctypes.POINTER(vp)]),
@@ -280,6 +281,14 @@ class Graph:
         _check(load_library().rg_graph_info(self.h, *[ctypes.byref(v) for v in vals]))
         keys = ("vertices", "slots_used", "slots_cap", "hash_used", "hash_cap")
         return {k: int(v.value) for k, v in zip(keys, vals)}
+
+    def pool_info(self):
+        """rg_graph_pool_info: recycled / dropped blocks and dropped entries
+        of the slot pool and the hash pool."""
+        out = np.zeros(6, np.uint64)
+        _check(load_library().rg_graph_pool_info(self.h, _ptr(out)))
+        o = [int(x) for x in out]
+        return {"recycled": o[0:2], "dropped": o[2:4], "dropped_entries": o[4:6]}
 
     def load_stream(self, stream, first, n, directed=True):
         _check(load_library().rg_stream_load(self.h, stream.h, first, n, int(directed)))

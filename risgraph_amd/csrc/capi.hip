@@ -107,6 +107,7 @@ struct rg_graph {
     List lists[2]{};
     Pool *pool = nullptr;
     uint64_t slot_cap = 0, hash_cap = 0;
+    uint32_t fcap = 0;                // entries per size-class free list of the slab pools
     EdgeOps ops{};
     ListOps lops{};
     EdgeOps uops{};                   // a stream batch's unsafe ops (the engine's input after the split)
@@ -315,6 +316,7 @@ StoreArgs store_args(rg_graph *g)
     a.pend_off = g->pend_off;
     a.cnt_tmp[0] = g->cnt_tmp[0];
     a.cnt_tmp[1] = g->cnt_tmp[1];
+    a.fcap = g->fcap;
     return a;
 }
 
@@ -875,7 +877,11 @@ int rg_graph_create(uint64_t vertices, uint64_t hint_edges, int symmetric, int d
         double f = env ? atof(env) : 6.0;
         g->slot_cap = std::max<uint64_t>(1ull << 20, (uint64_t)(f * (double)g->hint)) + 4 * vertices;
         g->hash_cap = std::max<uint64_t>(1ull << 20, 8 * g->hint);
-        uint64_t nfree = 2ull * NCLS * FCAP;
+        // a batch frees at most a few blocks per list op; the class lists hold
+        // one per 1,024 pool entries (4,096 to 524,288)
+        g->fcap = 4096;
+        while (g->fcap < (1u << 19) && (uint64_t)g->fcap * 1024 < g->slot_cap) g->fcap <<= 1;
+        uint64_t nfree = 2ull * NCLS * g->fcap;
         g->free_off = dalloc<uint64_t>(nfree);
         g->pend_off = dalloc<uint64_t>(nfree);
         g->slots.nbr = dalloc<uint32_t>(g->slot_cap);
@@ -945,6 +951,22 @@ int rg_graph_info(rg_graph *g, uint64_t *vertices, uint64_t *slots_used, uint64_
         if (slots_cap) *slots_cap = p.slot_cap;
         if (hash_used) *hash_used = std::min<uint64_t>(p.hash_top, p.hash_cap);
         if (hash_cap) *hash_cap = p.hash_cap;
+    });
+}
+
+int rg_graph_pool_info(rg_graph *g, uint64_t *out)
+{
+    if (!g || !out) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        Pool p{};
+        HC(hipMemcpyAsync(&p, g->pool, sizeof(Pool), hipMemcpyDeviceToHost, g->st));
+        HC(hipStreamSynchronize(g->st));
+        for (int h = 0; h < 2; h++) {
+            out[h] = p.recycled[h];
+            out[2 + h] = p.dropped[h];
+            out[4 + h] = p.dropped_entries[h];
+        }
     });
 }
 

@@ -93,7 +93,8 @@ __device__ __forceinline__ void list_span(const List &l, uint32_t v, uint64_t &o
 // push never touch the same list inside one kernel.  Pops take from the end
 // of the free list; the bump pointer serves what the free lists cannot.
 constexpr int NCLS = 32;
-constexpr uint32_t FCAP = 1u << 16;   // entries per class list (excess frees leak)
+// Entries per class list (StoreArgs::fcap, sized from the slot pool at
+// graph creation); frees past it are counted (Pool::dropped) and leak.
 
 struct Pool {
     unsigned long long slot_top, slot_cap;
@@ -106,6 +107,9 @@ struct Pool {
     unsigned int free_take[2][NCLS];
     unsigned int pend_n[2][NCLS];
     unsigned long long recycled[2];
+    // frees that found no room (pending list past fcap, or free list full):
+    // blocks and entries that no later batch can reuse
+    unsigned long long dropped[2], dropped_entries[2];
 };
 
 // edge-op kinds (bitmask)
@@ -275,8 +279,8 @@ struct StoreArgs {
     Slots slots;
     List lists[2];
     Pool *pool;
-    uint64_t *free_off;   // [2][NCLS][FCAP] reusable block offsets
-    uint64_t *pend_off;   // [2][NCLS][FCAP] blocks freed in the current batch
+    uint64_t *free_off;   // [2][NCLS][fcap] reusable block offsets
+    uint64_t *pend_off;   // [2][NCLS][fcap] blocks freed in the current batch
     EdgeOps ops;
     ListOps lops;
     uint32_t *err;   // bad vertex ids etc.
@@ -286,6 +290,7 @@ struct StoreArgs {
     // device word instead of ops.n (the launch grids are sized for the
     // bucket's largest batch); null = ops.n
     const uint32_t *dyn_n;
+    uint32_t fcap;   // entries per size-class free list
 };
 
 // The window of one captured stream batch: written by the host into pinned

@@ -138,7 +138,7 @@ __device__ uint64_t pool_alloc(const StoreArgs &a, int h, uint64_t n)
         unsigned int avail = p->free_n[h][c];
         if (avail) {
             unsigned int k = atomicAdd(&p->free_take[h][c], 1u);
-            if (k < avail) return a.free_off[((uint64_t)h * NCLS + c) * FCAP + (avail - 1 - k)];
+            if (k < avail) return a.free_off[((uint64_t)h * NCLS + c) * a.fcap + (avail - 1 - k)];
         }
     }
     unsigned long long *top = h ? &p->hash_top : &p->slot_top;
@@ -158,7 +158,7 @@ __device__ void pool_free(const StoreArgs &a, int h, uint64_t off, uint64_t n)
     int c = log2_pow2(n);
     if (c >= NCLS) return;
     unsigned int k = atomicAdd(&a.pool->pend_n[h][c], 1u);
-    if (k < FCAP) a.pend_off[((uint64_t)h * NCLS + c) * FCAP + k] = off;
+    if (k < a.fcap) a.pend_off[((uint64_t)h * NCLS + c) * a.fcap + k] = off;
 }
 
 // ---------------------------------------------------------------------
@@ -238,7 +238,7 @@ __device__ void reserve_growth(const StoreArgs &a, uint32_t cls, bool want, uint
     __syncthreads();
     if (want) {
         if (r < nfree[cls])
-            out = a.free_off[(uint64_t)cls * FCAP + (avail[cls] - 1 - (take[cls] + r))];
+            out = a.free_off[(uint64_t)cls * a.fcap + (avail[cls] - 1 - (take[cls] + r))];
         else if (bump_off[cls] == HEMPTY)
             out = HEMPTY;
         else
@@ -1206,11 +1206,11 @@ __device__ void recycle_class(const StoreArgs &a, int h, int c, uint32_t t0, uin
     unsigned int avail = p->free_n[h][c];
     unsigned int take = p->free_take[h][c];
     unsigned int rem = take >= avail ? 0 : avail - take;   // pops took from the end
-    unsigned int pn = p->pend_n[h][c];
-    if (pn > FCAP) pn = FCAP;
-    unsigned int room = FCAP - rem;
+    const unsigned int pn_all = p->pend_n[h][c];
+    const unsigned int pn = pn_all > a.fcap ? a.fcap : pn_all;
+    unsigned int room = a.fcap - rem;
     unsigned int mv = pn < room ? pn : room;
-    uint64_t base = ((uint64_t)h * NCLS + c) * FCAP;
+    uint64_t base = ((uint64_t)h * NCLS + c) * a.fcap;
     for (unsigned int k = t0; k < mv; k += nt) a.free_off[base + rem + k] = a.pend_off[base + k];
     __syncthreads();
     if (t0 == 0) {
@@ -1218,6 +1218,10 @@ __device__ void recycle_class(const StoreArgs &a, int h, int c, uint32_t t0, uin
         p->free_take[h][c] = 0;
         p->pend_n[h][c] = 0;
         if (mv) atomicAdd(&p->recycled[h], (unsigned long long)mv);
+        if (pn_all > mv) {
+            atomicAdd(&p->dropped[h], (unsigned long long)(pn_all - mv));
+            atomicAdd(&p->dropped_entries[h], (unsigned long long)(pn_all - mv) << c);
+        }
     }
 }
 
@@ -1352,16 +1356,20 @@ __global__ void __launch_bounds__(HB) k_store_small(StoreArgs a)
             Pool *p = a.pool;
             const unsigned int avail = p->free_n[h][c], take = p->free_take[h][c];
             const unsigned int rem = take >= avail ? 0 : avail - take;
-            unsigned int pn = p->pend_n[h][c];
-            if (pn > FCAP) pn = FCAP;
-            const unsigned int mv = pn < FCAP - rem ? pn : FCAP - rem;
-            const uint64_t base = ((uint64_t)h * NCLS + c) * FCAP;
+            const unsigned int pn_all = p->pend_n[h][c];
+            const unsigned int pn = pn_all > a.fcap ? a.fcap : pn_all;
+            const unsigned int mv = pn < a.fcap - rem ? pn : a.fcap - rem;
+            const uint64_t base = ((uint64_t)h * NCLS + c) * a.fcap;
             for (unsigned int k = t0; k < mv; k += TPC) a.free_off[base + rem + k] = a.pend_off[base + k];
             if (t0 == 0) {
                 p->free_n[h][c] = rem + mv;
                 p->free_take[h][c] = 0;
                 p->pend_n[h][c] = 0;
                 if (mv) atomicAdd(&p->recycled[h], (unsigned long long)mv);
+                if (pn_all > mv) {
+                    atomicAdd(&p->dropped[h], (unsigned long long)(pn_all - mv));
+                    atomicAdd(&p->dropped_entries[h], (unsigned long long)(pn_all - mv) << c);
+                }
             }
         }
     }

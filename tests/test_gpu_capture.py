@@ -76,6 +76,10 @@ def test_captured_batches_match_eager_launches():
     # buckets 128, 256, 512, 2048, 4096, 8192, 16384
     assert ic["pipelines"] == 7, ic
     assert not ie["on"] and ie["replays"] == 0 and ie["pipelines"] == 0
+    # every block freed by these batches went back to a free list
+    for g in (gc, ge):
+        pi = g.pool_info()
+        assert pi["dropped"] == [0, 0] and pi["dropped_entries"] == [0, 0] and pi["recycled"][0] > 0, pi
 
 
 def test_captured_batches_after_new_trees_and_stream():
