@@ -119,6 +119,15 @@ int rg_labels_destroy(rg_labels *L);
 int rg_labels_set_tuning(rg_labels *L, uint32_t rescue_level, uint32_t rescue_frontier,
                          uint32_t recompute_level, uint32_t recompute_abs, double recompute_frac,
                          uint32_t walk_limit, uint32_t block_max);
+/* Direction of a build's passes (stream_edges, graph.hpp:323-394): with
+ * mode 1 a push pass whose frontier's live out-edges reach both V and a
+ * twentieth of the live edges runs dense instead, every vertex pulling its
+ * best candidate over its in-edges (graph.hpp:380-394); mode 0 (the
+ * default, also RG_DENSE=0; RG_DENSE=1 makes 1 the default) keeps builds
+ * push-only, which measured faster on MI355X; -1 leaves the mode.  Returns in dense_passes the
+ * dense passes this instance's builds and rebuilds ran so far.  Values never
+ * depend on the mode. */
+int rg_labels_dense(rg_labels *L, int mode, uint64_t *dense_passes);
 
 /* build_tree -- graph.hpp:873-985: full fixpoint from the init labels. */
 int rg_build(rg_labels *L, uint64_t *result);

@@ -128,6 +128,7 @@ SIGNATURES = {
     "rg_profile_read": (ci, [vp, ci, u64p, ctypes.POINTER(ctypes.c_double), u64p]),
     "rg_degrees": (ci, [vp, vp, vp]),
     "rg_labels_phase_time": (ci, [vp, vp, vp]),
+    "rg_labels_dense": (ci, [vp, ci, u64p]),
     "rg_labels_phase_detail": (ci, [vp, vp, vp, vp]),
 }
 
@@ -439,6 +440,14 @@ class Labels:
         bm = 0xFFFFFFFF if block_max is None else int(block_max)
         _check(load_library().rg_labels_set_tuning(self.h, rescue_level, rescue_frontier, recompute_level,
                                                    recompute_abs, recompute_frac, walk_limit, bm))
+
+    def dense(self, mode=-1):
+        """rg_labels_dense: set the build direction rule (1 reference
+        push/pull switch, 0 push only, -1 unchanged); returns the dense
+        passes run so far."""
+        n = u64()
+        _check(load_library().rg_labels_dense(self.h, int(mode), ctypes.byref(n)))
+        return int(n.value)
 
     def build_tree(self):
         r = u64()

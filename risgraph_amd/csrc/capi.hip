@@ -186,6 +186,7 @@ struct rg_labels {
     Ctl *ctl = nullptr;
     Ctl *hc = nullptr;     // pinned mirror
     unsigned long long *ticks = nullptr;   // per-phase block-mode time (device)
+    uint64_t dense_passes = 0;             // pull passes of its builds so far
     uint32_t *bar = nullptr;               // team barrier sub-counters (device)
     hipStream_t st = nullptr;      // this tree's grid-wide passes (concurrent with other trees)
     EngineArgs *dargs = nullptr;   // device copy of the engine arguments (team launches)
@@ -531,6 +532,7 @@ uint64_t host_work(const Ctl &h, uint64_t V)
     case PH_DEL_ROOTS: return h.n_ops / 16;   // same units as the device's phase_work
     case PH_DEL_PULL: return h.n_tree + (h.seed ? h.n_ops / 16 : 0);
     case PH_BUILD_INIT: return V;
+    case PH_PUSH: return (h.flags & F_DENSE) ? V : h.n_cur;
     default: return h.n_cur;
     }
 }
@@ -553,7 +555,8 @@ uint32_t team_cap(const rg_labels *L) { return L->block_max == RG_AUTO ? 1024u *
 // full rebuild and fits the team's work cap; otherwise it is one grid step.
 bool team_ok(const rg_labels *L, const Ctl &h)
 {
-    return h.phase != PH_BUILD_INIT && h.stage == 0 && host_work(h, L->g->V) <= team_cap(L);
+    return h.phase != PH_BUILD_INIT && !(h.phase == PH_PUSH && (h.flags & F_DENSE)) && h.stage == 0 &&
+           host_work(h, L->g->V) <= team_cap(L);
 }
 
 // seed: a mixed batch starts with its deletions and seeds its inserts in the
@@ -579,6 +582,9 @@ void engine_init(rg_labels *L, uint32_t phase0, bool seed, uint32_t n_ops, bool 
         }
     }
     h.flags &= F_ALL_MODIFIED;
+    if (phase0 == PH_BUILD_INIT && L->tune.dense) h.flags |= F_BUILD;
+    h.e_next = h.e_live = 0;
+    h.dense = 0;
     h.bar_count = h.bar_gen = h.bar_exit = 0;
     h.rbound = L->algo == SSWP ? 0u : 0xFFFFFFFFu;
     h.work_max = team_cap(L);
@@ -719,6 +725,7 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 active[i] = 0;
                 left--;
                 if (stats) stats->levels += h.levels;
+                L[i]->dense_passes += h.dense;
                 // RG_LOG_SUBTREE=n: report runs whose deletion subtree exceeded n vertices
                 static const long log_subtree = [] {
                     const char *e = getenv("RG_LOG_SUBTREE");
@@ -743,7 +750,8 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 h.n_cur = h.n_next = h.n_tree = h.n_chunk = 0;
                 h.sel_cur = 0; h.sel_next = 1; h.sel_tree = 2;
                 h.epoch += 1;
-                h.flags = (h.flags & ~F_NEED_REBUILD) | F_ALL_MODIFIED;
+                h.flags = (h.flags & ~(F_NEED_REBUILD | F_DENSE)) | F_ALL_MODIFIED | (L[i]->tune.dense ? F_BUILD : 0u);
+                h.e_next = h.e_live = 0;
                 // the rebuild runs grid-wide on the tree's stream
                 HC(hipMemcpyAsync(L[i]->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, L[i]->st));
                 if (stats) stats->rebuilds++;
@@ -1168,6 +1176,12 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         L->tune.recompute_abs = 2000000;
         L->tune.recompute_frac = 0.05f;
         L->tune.walk_limit = (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(V / 10, 1u << 16));
+        // builds stay push-only unless asked (rg_labels_dense, RG_DENSE=1):
+        // on this GPU the reference's pull passes made scale-24 builds
+        // 25-80% slower (a pull pass reads every in-edge; pushes from the
+        // frontier are cheap atomics here)
+        L->tune.dense = 0;
+        if (const char *de = getenv("RG_DENSE"); de && *de == '1') L->tune.dense = 1;
         // environment overrides; empty values are ignored
         if (const char *tw = getenv("RG_TEAM_W"); tw && *tw)
             L->team_w = (uint32_t)std::max(1, std::min(64, atoi(tw)));
@@ -2386,6 +2400,15 @@ int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint6
         if (ms) *ms = g->p_ms[kind];
         if (units) *units = g->p_units[kind];
     });
+}
+
+int rg_labels_dense(rg_labels *L, int mode, uint64_t *dense_passes)
+{
+    if (!L || mode < -1 || mode > 1) return RG_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(L->g->mu);
+    if (mode >= 0) L->tune.dense = (uint32_t)mode;
+    if (dense_passes) *dense_passes = L->dense_passes;
+    return RG_OK;
 }
 
 int rg_labels_phase_time(rg_labels *L, double *us, uint64_t *passes)

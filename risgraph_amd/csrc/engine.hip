@@ -81,6 +81,7 @@ __device__ __forceinline__ void ast(int32_t *p, int32_t v)
 // appending, single-counter device atomics were the pass's critical path.)
 constexpr uint32_t SQ_NEXT = 6144, SQ_TREE = 1536, SQ_CHUNK = 512;
 struct Stage {
+    unsigned long long e_next, e_live;   // builds: degree sums (frontier, whole graph)
     uint32_t n_next, n_tree, n_chunk;
     uint32_t rb_min, rb_max;   // value band of the deletion roots staged here
     uint32_t base[3];
@@ -224,6 +225,7 @@ __device__ void stage_init(Stage *st)
 {
     if (threadIdx.x == 0) {
         st->n_next = st->n_tree = st->n_chunk = 0;
+        st->e_next = st->e_live = 0;
         st->rb_min = 0xFFFFFFFFu;
         st->rb_max = 0;
     }
@@ -244,6 +246,9 @@ __device__ void stage_flush(const EngineArgs &a, const Snap &s)
         // one band update per workgroup instead of one per deletion root
         if (st->rb_min != 0xFFFFFFFFu) atomicMin(&s.c->rbound, st->rb_min);
         if (st->rb_max != 0) atomicMax(&s.c->rbound, st->rb_max);
+        if (st->e_next) atomicAdd(&s.c->e_next, st->e_next);
+        if (st->e_live) atomicAdd(&s.c->e_live, st->e_live);
+        st->e_next = st->e_live = 0;
         st->rb_min = 0xFFFFFFFFu;
         st->rb_max = 0;
         st->n_next = nn;
@@ -274,9 +279,22 @@ __device__ void stage_flush(const EngineArgs &a, const Snap &s)
     __syncthreads();
 }
 
+// live edges a push from v goes over (graph.hpp:327-331 sums the out-degrees
+// of the active vertices)
+__device__ __forceinline__ uint32_t push_degree(const EngineArgs &a, uint32_t v)
+{
+    uint32_t d = 0;
+    for (int li = 0; li < a.npush; li++) d += a.lists[a.push_l[li]].h[v].deg;
+    return d;
+}
+
 __device__ __forceinline__ void push_next(const EngineArgs &a, const Snap &s, uint32_t v)
 {
-    if (atomicExch(&a.vr[v].stamp, s.epoch) != s.epoch) stage_next(a, s, v);
+    if (atomicExch(&a.vr[v].stamp, s.epoch) != s.epoch) {
+        stage_next(a, s, v);
+        // a build sizes its next pass by the frontier's edges
+        if (s.flags & F_BUILD) atomicAdd(&s.stg->e_next, (unsigned long long)push_degree(a, v));
+    }
 }
 
 __device__ __forceinline__ void mark_mod(const EngineArgs &a, const Snap &s, uint32_t v)
@@ -493,7 +511,8 @@ __device__ void expand_range(const EngineArgs &a, const Snap &s, uint32_t x, uin
 
 // best candidate word over a range (min or max of value<<32|parent); in-
 // neighbours inside the invalidated subtree are skipped, their values are
-// being reset in the same pass
+// being reset in the same pass (a build's dense passes have no subtree: its
+// init pass cleared every mark)
 template <int A>
 __device__ uint64_t pull_range(const EngineArgs &a, const Snap &s, uint64_t off, uint32_t b, uint32_t e,
                                int lane, int stride)
@@ -502,7 +521,7 @@ __device__ uint64_t pull_range(const EngineArgs &a, const Snap &s, uint64_t off,
     for (uint32_t i = b + lane; i < e; i += stride) {
         uint32_t n = a.slots.nbr[off + i];
         if (n & DEAD) continue;
-        if (a.vr[n].inv == s.idx) continue;
+        if (!(s.flags & F_BUILD) && a.vr[n].inv == s.idx) continue;
         uint32_t w;
         entry_w<A>(a, off + i, w);
         uint64_t c = cand<A>(vval(a.vr[n].val), w);
@@ -747,16 +766,48 @@ __device__ void pass_vertices(const EngineArgs &a, const Snap &s, uint32_t tid, 
 {
     const uint32_t ph = s.phase;
     if (ph == PH_BUILD_INIT) {
+        // the init pass also counts the live edges of the push lists: the
+        // dense-pass rule compares the frontier's edges with a twentieth of them
+        unsigned long long e = 0;
         for (uint32_t v = tid; v < a.V; v += nthr) {
             a.vr[v].val = pack(a.init[v], NILP);
             a.vr[v].inv = 0;
             bool act = a.init_active ? a.init_active[v] != 0 : v == a.root;
             if (act) push_next(a, s, v);
+            if constexpr (G == GRID_GROUP) {
+                if (a.tune.dense) e += push_degree(a, v);
+            }
         }
+        for (int m = WAVE / 2; m > 0; m >>= 1) e += __shfl_xor(e, m, WAVE);
+        if (__lane_id() == 0 && e) atomicAdd(&s.stg->e_live, e);
         return;
     }
     const int glane = tid % G;
     const uint32_t gid = tid / G, ngr = nthr / G;
+    // (build init and dense passes always run grid-wide)
+    if constexpr (G == GRID_GROUP) if (ph == PH_PUSH && (s.flags & F_DENSE)) {
+        // dense pass of a build (graph.hpp:380-394): every vertex takes the
+        // best candidate over its live in-edges; long in-lists finish in
+        // pull chunks
+        for (uint32_t v = gid; v < a.V; v += ngr) {
+            uint64_t best = (A == SSWP) ? 0ull : ~0ull;
+            for (int li = 0; li < a.npull; li++) {
+                int L = a.pull_l[li];
+                uint64_t off;
+                uint32_t len;
+                list_span(a.lists[L], v, off, len);
+                if (len > LONG_LIST) {
+                    if (glane == 0) defer_chunks(a, s, v, off, len, CO_PULL);
+                    continue;
+                }
+                uint64_t b = pull_range<A>(a, s, off, 0, len, glane, G);
+                if (A == SSWP ? b > best : b < best) best = b;
+            }
+            best = reduce_best<A>(best, G);
+            if (glane == 0) pull_apply<A>(a, s, v, best);
+        }
+        return;
+    }
     const uint32_t n = (ph == PH_DEL_PULL) ? s.n_tree : s.n_cur;
     const uint32_t *qv = (ph == PH_DEL_PULL) ? a.q[s.sel_tree] : a.q[s.sel_cur];
     for (uint32_t gi = gid; gi < n; gi += ngr) {
@@ -907,6 +958,15 @@ __device__ __forceinline__ void advance_state(const EngineArgs &a, Ctl &c)
         swap_queues();
         c.seed = 0;
         c.phase = c.n_cur ? PH_PUSH : PH_DONE;
+        if (c.flags & F_BUILD) {
+            // graph.hpp:334: a pass goes dense when its frontier's edges
+            // reach both the vertex count and a twentieth of the live edges
+            const unsigned long long e = c.e_next;
+            c.e_next = 0;
+            const bool dense = a.tune.dense && c.phase == PH_PUSH && a.npull > 0 && e >= a.V && e >= c.e_live / 20;
+            c.flags = dense ? (c.flags | F_DENSE) : (c.flags & ~F_DENSE);
+            if (dense) c.dense += 1;
+        }
         break;
     case PH_DEL_ROOTS:
         swap_queues();
@@ -950,6 +1010,9 @@ __device__ void advance(const EngineArgs &a, Ctl *c)
     l.epoch = ald(&c->epoch);
     l.flags = ald(&c->flags);
     l.levels = c->levels;
+    l.e_next = __hip_atomic_load(&c->e_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    l.e_live = __hip_atomic_load(&c->e_live, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    l.dense = c->dense;
     const Ctl o = l;
     advance_state(a, l);
     ast(&c->seq, ald(&c->seq) + 1u);
@@ -966,6 +1029,8 @@ __device__ void advance(const EngineArgs &a, Ctl *c)
     if (l.epoch != o.epoch) ast(&c->epoch, l.epoch);
     if (l.flags != o.flags) ast(&c->flags, l.flags);
     c->levels = l.levels;
+    if (l.e_next != o.e_next) __hip_atomic_store(&c->e_next, l.e_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    c->dense = l.dense;
 }
 
 // work of the current pass, in units comparable with the team cap; an op
@@ -977,6 +1042,7 @@ __device__ __forceinline__ uint32_t phase_work(const EngineArgs &a, const Snap &
     case PH_DEL_ROOTS: return s.n_ops / 16;
     case PH_DEL_PULL: return s.n_tree + (s.seed ? s.n_ops / 16 : 0);
     case PH_BUILD_INIT: return a.V;
+    case PH_PUSH: return (s.flags & F_DENSE) ? a.V : s.n_cur;
     default: return s.n_cur;
     }
 }
@@ -987,7 +1053,8 @@ __device__ __forceinline__ bool grid_needed(const EngineArgs &a, const Snap &s)
 {
     if (s.phase == PH_DONE || s.phase == PH_REBUILD) return false;
     if (s.flags & (F_QUEUE_OVERFLOW | F_ABORT)) return false;   // the host reports these
-    return s.phase == PH_BUILD_INIT || s.stage != 0 || phase_work(a, s) > s.work_max;
+    return s.phase == PH_BUILD_INIT || (s.phase == PH_PUSH && (s.flags & F_DENSE)) || s.stage != 0 ||
+           phase_work(a, s) > s.work_max;
 }
 
 // copy the control block to the pinned host mirror (one thread), so the
@@ -1113,6 +1180,7 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
         if (s.phase == PH_DONE || s.phase == PH_REBUILD || s.phase == PH_BUILD_INIT) break;
         if (s.flags & F_ABORT) break;
         if (phase_work(a, s) > work_max) break;
+        if (s.phase == PH_PUSH && (s.flags & F_DENSE)) break;   // dense passes run grid-wide
         // stage clocks of this workgroup: start, vertex stage done, first
         // barrier passed, chunk stage done
         uint64_t tk[4];
@@ -1319,6 +1387,7 @@ static void grid_launch(const EngineArgs &a, const Ctl &h, int steps, hipStream_
     case PH_DEL_PULL: threads = std::max<uint64_t>((uint64_t)h.n_tree * GRID_GROUP, h.seed ? h.n_ops : 0); break;
     default: threads = (uint64_t)h.n_cur * GRID_GROUP; break;
     }
+    if (h.phase == PH_PUSH && (h.flags & F_DENSE)) threads = (uint64_t)a.V * GRID_GROUP;
     uint64_t blocks = (threads + GRID_BLOCK - 1) / GRID_BLOCK;
     if (blocks < 1) blocks = 1;
     static const uint32_t max_blocks = [] {

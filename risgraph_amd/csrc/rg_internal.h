@@ -188,6 +188,8 @@ constexpr uint32_t F_NEED_REBUILD = 1;
 constexpr uint32_t F_QUEUE_OVERFLOW = 2;
 constexpr uint32_t F_ALL_MODIFIED = 4;
 constexpr uint32_t F_ABORT = 8;        // a team barrier timed out
+constexpr uint32_t F_BUILD = 16;       // a full build: its push passes may run dense (pull)
+constexpr uint32_t F_DENSE = 32;       // the current push pass pulls over every vertex
 
 struct Ctl {
     uint32_t phase, stage, level, seed;   // seed: inserts still to seed (mixed batch)
@@ -202,9 +204,11 @@ struct Ctl {
     uint32_t bar_exit;     // team workgroups that left the launch
     uint32_t rbound;       // smallest deletion-root value (largest for SSWP)
     uint32_t work_max;     // largest pass the team runs (larger: grid-wide)
-    unsigned long long edges, relax_ok, levels;
+    // builds: live edges the next frontier pushes over (sum of its push-list
+    // degrees), and every live edge of the push lists (from the init pass)
+    unsigned long long e_next, e_live, levels;
     uint32_t seq;          // passes closed (every phase transition)
-    uint32_t pad2;
+    uint32_t dense;        // dense passes of the run
 };
 
 // One launch repairs up to MAXT trees; each tree gets W workgroups that
@@ -234,6 +238,7 @@ struct Tuning {
     uint32_t recompute_abs;       // ... when subtree+frontier > this ...
     float recompute_frac;         // ... and > frac * V
     uint32_t walk_limit;          // has_parents step cap (graph.hpp:1900)
+    uint32_t dense;               // builds switch to pull passes by the reference's rule (0: push only)
 };
 
 // One chunk of a long list for a pass's chunk stage.  It carries where its

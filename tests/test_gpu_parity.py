@@ -278,11 +278,16 @@ def test_deletion_strategies_keep_values(tuning):
         algo = rg.ALGOS[name]
         V = 1024
         t = _setup(algo, "rmat", V, 1 << 14, 0.9, seed=9, tuning=tuning)
+        if tuning.get("recompute_abs") == 0:
+            t["L"].dense(1)   # rebuilds with the reference's push/pull switch
         for k in range(60):
             i_add, i_del = t["n0"] + k, k
             t["g"].stream_plain([t["L"]], t["s"], i_add, i_del, 1, t["directed"], latency=False)
             _ref_step(t, i_add, i_del)
             _check_labels(t["L"], algo, V, t["ref"], t["root"])
+        if tuning.get("recompute_abs") == 0:
+            # the rebuilds switch to dense passes (graph.hpp:334)
+            assert t["L"].dense() > 0, name
 
 
 def test_four_trees_on_one_store():
