@@ -77,6 +77,10 @@ def parse():
                     help="every update of a batch waits for the batch's repair (no safe/unsafe split)")
     ap.add_argument("--cap", choices=["unsafe", "total"], default="unsafe",
                     help="what the adaptive admission cap counts: unsafe updates (default) or all updates")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the engine autotuner (rg_autotune) before serving")
+    ap.add_argument("--batch-max", type=int, default=BATCH_MAX,
+                    help="largest serving batch (updates); 0 = the autotuner's batch cap for the latency target")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each batch's window, store and split kernels one by one instead of replaying "
                          "the captured hipGraph of the batch's size bucket")
@@ -123,7 +127,7 @@ def serve(g, labels, stream, cur, n, rate, cap, cap_max, target_ms, split=True, 
 def serve_batched(g, labels, stream, cur, n, rate, args):
     # with the unsafe-count cap the initial cap is in unsafe updates
     cap_unsafe = args.cap == "unsafe" and not args.no_split
-    return serve(g, labels, stream, cur, n, rate, 16 if cap_unsafe else 64, BATCH_MAX, args.target_ms,
+    return serve(g, labels, stream, cur, n, rate, 16 if cap_unsafe else 64, args.batch_max, args.target_ms,
                  split=not args.no_split, cap_unsafe=cap_unsafe)
 
 
@@ -168,6 +172,19 @@ def main():
         snap = [L.read() for L in labels]
 
     cur = Cursor(n0, E)
+    # engine settings per batch size, timed on forward/inverse batches of the
+    # current window (the graph and the values return to where they were)
+    tune = None
+    if not args.no_autotune:
+        t = time.time()
+        tune = g.autotune(labels, stream, n0, 0, target_ms=args.target_ms)
+        tune["seconds"] = round(time.time() - t, 2)
+        log(f"autotune {tune['seconds']}s: " + "; ".join(
+            f"{b['ops']} ops cap x{b['cap_x4'] / 4:g} w {b['weight']} {b['default_ms']:.3f}->{b['best_ms']:.3f} ms"
+            for b in tune["buckets"]) + f"; batch cap {tune['batch_cap']}")
+    if args.batch_max == 0:
+        args.batch_max = tune["batch_cap"] if tune else BATCH_MAX
+    args.autotune = tune
     if args.serve_only:
         serve_only(g, labels, stream, cur, args, snap, names, root, V, E, n0, info, build_s)
         return
@@ -322,6 +339,8 @@ def main():
                        "p99.9_unsafe_worst_step": round(float(max(acc["unsafe_p999"])), 3)},
         "split": not args.no_split,
         "graph_replay": g.capture_info(),
+        "autotune": args.autotune,
+        "batch_max": args.batch_max,
         "cap_counts": "unsafe updates" if (args.cap == "unsafe" and not args.no_split) else "all updates",
         "safe_frac": round(1.0 - acc["unsafe"] / max(total, 1), 4),
         "target_met": met,
@@ -374,7 +393,8 @@ def serve_only(g, labels, stream, cur, args, snap, names, root, V, E, n0, info, 
                                                    (tp1["launches"] - tp0["launches"], tp1["ms"] - tp0["ms"]))
     out = {"mode": "serve-only profiling run", "offered_rate": rate, "value": round(total / max(wall, 1e-9), 1),
            "p99.9_ms": round(float(np.percentile(lat_all, 99.9)), 3), "mean_batch": round(total / max(batches, 1), 1),
-           "roofline": roof, "roofline_store": roof_store, "batch_roofline": batch_roof}
+           "roofline": roof, "roofline_store": roof_store, "batch_roofline": batch_roof,
+           "autotune": args.autotune, "batch_max": args.batch_max}
     print(json.dumps(out), flush=True)
     if args.json_out:
         with open(args.json_out, "w") as f:

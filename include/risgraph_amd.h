@@ -128,6 +128,12 @@ int rg_labels_set_tuning(rg_labels *L, uint32_t rescue_level, uint32_t rescue_fr
  * dense passes this instance's builds and rebuilds ran so far.  Values never
  * depend on the mode. */
 int rg_labels_dense(rg_labels *L, int mode, uint64_t *dense_passes);
+/* The tree's share of a batch's persistent repair launch: the launch's
+ * eight XCD slots (each a 1/8 of its workgroups) are split over its trees by
+ * weight, one slot each at least.  weight 0 (default) = by algorithm (BFS 1,
+ * SSSP 3, SSWP 2, WCC 2), else 1-64.  No counterpart in the reference (its
+ * trees share the host's OpenMP threads); values never depend on it. */
+int rg_labels_team_weight(rg_labels *L, uint32_t weight);
 
 /* build_tree -- graph.hpp:873-985: full fixpoint from the init labels. */
 int rg_build(rg_labels *L, uint64_t *result);
@@ -205,6 +211,42 @@ int rg_graph_capture(rg_graph *g, int on);
 /* Whether capture is on (it turns off by itself where the runtime cannot
  * capture), the pipelines captured so far and the batches replayed. */
 int rg_graph_capture_info(rg_graph *g, int *on, uint32_t *pipelines, uint64_t *replays);
+/* Autotuner of the batch engine (no counterpart in the reference, whose
+ * knobs are compile-time constants).  For four batch sizes (16,384, 65,536,
+ * 262,144 and 1,048,576 edge ops, half inserts of stream window
+ * [ins_first, ...) and half deletes of window [del_first, ...)) it times
+ * forward batches and their inverses (the deleted window inserted back, the
+ * inserted one deleted: the graph and the labels' values return to where
+ * they were) under candidate settings and keeps, per batch-size bucket
+ * (<= 32k, <= 128k, <= 512k, more edge ops), the fastest: the team cap
+ * (passes larger than cap_x4/4 x 1024 vertices per team workgroup go
+ * grid-wide) and the team weights of the four algorithms.  Later batches of
+ * the graph run with the table.  With target_ms > 0 it also returns
+ * batch_cap, the largest serving batch whose measured device time fits a
+ * quarter of the target (the serving loop's own shrink rule), for
+ * rg_serve_params.cap_max.  Every edge the deleted window holds must be
+ * live (it is when the windows are the stream's current window, as in the
+ * serving loop); the degrees are checked afterwards and a graph that did not
+ * return is reported as RG_EINVAL.  The graph's version advances by
+ * `batches`, and the labels' modified sets hold what the batches touched. */
+#define RG_TUNE_BUCKETS 4
+typedef struct rg_autotune_params {
+    uint64_t ins_first, del_first;
+    int directed;
+    uint32_t pairs;      /* timed forward/inverse pairs per setting (0: 3) */
+    double target_ms;    /* serving target for batch_cap (0: not chosen) */
+} rg_autotune_params;
+typedef struct rg_autotune_result {
+    uint32_t ops[RG_TUNE_BUCKETS];          /* edge ops of the timed batches */
+    uint32_t cap_x4[RG_TUNE_BUCKETS];       /* chosen team cap, x 1/4 of the default */
+    uint32_t weight[RG_TUNE_BUCKETS][4];    /* chosen team weights (BFS, SSSP, SSWP, WCC) */
+    double default_ms[RG_TUNE_BUCKETS];     /* device ms of a pair, default settings */
+    double best_ms[RG_TUNE_BUCKETS];        /* device ms of a pair, chosen settings */
+    uint32_t batch_cap;
+    uint64_t batches;
+} rg_autotune_result;
+int rg_autotune(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_autotune_params *p,
+                rg_autotune_result *r);
 /* The shape a graph was created with, and a labels instance's algorithm and
  * root (for handles that rg_snapshot_load made). */
 int rg_graph_shape(rg_graph *g, int *symmetric, int *dual, int *weighted);

@@ -42,6 +42,30 @@ class BatchStats(ctypes.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+TUNE_BUCKETS = 4
+
+
+class AutotuneParams(ctypes.Structure):
+    _fields_ = [("ins_first", ctypes.c_uint64), ("del_first", ctypes.c_uint64), ("directed", ctypes.c_int),
+                ("pairs", ctypes.c_uint32), ("target_ms", ctypes.c_double)]
+
+
+class AutotuneResult(ctypes.Structure):
+    _fields_ = [("ops", ctypes.c_uint32 * TUNE_BUCKETS), ("cap_x4", ctypes.c_uint32 * TUNE_BUCKETS),
+                ("weight", (ctypes.c_uint32 * 4) * TUNE_BUCKETS), ("default_ms", ctypes.c_double * TUNE_BUCKETS),
+                ("best_ms", ctypes.c_double * TUNE_BUCKETS), ("batch_cap", ctypes.c_uint32),
+                ("batches", ctypes.c_uint64)]
+
+    def as_dict(self):
+        buckets = []
+        for b in range(TUNE_BUCKETS):
+            if self.ops[b]:
+                buckets.append({"ops": int(self.ops[b]), "cap_x4": int(self.cap_x4[b]),
+                                "weight": [int(x) for x in self.weight[b]],
+                                "default_ms": round(self.default_ms[b], 4), "best_ms": round(self.best_ms[b], 4)})
+        return {"buckets": buckets, "batch_cap": int(self.batch_cap), "batches": int(self.batches)}
+
+
 class ServeParams(ctypes.Structure):
     _fields_ = [("ins_first", u64), ("del_first", u64), ("n_updates", u64), ("rate", ctypes.c_double),
                 ("seed", u64), ("clients", ctypes.c_uint32), ("batch_cap", ctypes.c_uint32),
@@ -85,6 +109,7 @@ SIGNATURES = {
     "rg_labels_assign": (ci, [vp, vp, vp]),
     "rg_graph_shape": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
     "rg_graph_capture": (ci, [vp, ci]),
+    "rg_autotune": (ci, [vp, vp, ci, vp, vp, vp]),
     "rg_graph_pool_info": (ci, [vp, vp]),
     "rg_graph_capture_info": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ctypes.c_uint32), u64p]),
     "rg_labels_algo": (ci, [vp, ctypes.POINTER(ci), u64p]),
@@ -129,6 +154,7 @@ SIGNATURES = {
     "rg_degrees": (ci, [vp, vp, vp]),
     "rg_labels_phase_time": (ci, [vp, vp, vp]),
     "rg_labels_dense": (ci, [vp, ci, u64p]),
+    "rg_labels_team_weight": (ci, [vp, ctypes.c_uint32]),
     "rg_labels_phase_detail": (ci, [vp, vp, vp, vp]),
 }
 
@@ -359,6 +385,16 @@ class Graph:
         obj.V, obj.symmetric, obj.weighted, obj._labels = int(V.value), bool(sym.value), bool(wt.value), []
         return obj
 
+    def autotune(self, labels, stream, ins_first, del_first, target_ms=0.0, pairs=3, directed=True):
+        """rg_autotune: time forward/inverse batch pairs per batch-size bucket
+        and keep the fastest engine settings; returns the table and, with a
+        target, the serving batch cap."""
+        arr = (vp * max(1, len(labels)))(*[L.h for L in labels])
+        p = AutotuneParams(ins_first, del_first, int(directed), pairs, target_ms)
+        r = AutotuneResult()
+        _check(load_library().rg_autotune(self.h, arr, len(labels), stream.h, ctypes.byref(p), ctypes.byref(r)))
+        return r.as_dict()
+
     def capture(self, on=True):
         """rg_graph_capture: replay stream batches' window/store/split
         launches from captured hipGraphs (on) or launch them one by one."""
@@ -448,6 +484,10 @@ class Labels:
         n = u64()
         _check(load_library().rg_labels_dense(self.h, int(mode), ctypes.byref(n)))
         return int(n.value)
+
+    def team_weight(self, weight=0):
+        """rg_labels_team_weight: share of the batch repair launch (0 = by algorithm)."""
+        _check(load_library().rg_labels_team_weight(self.h, int(weight)))
 
     def build_tree(self):
         r = u64()

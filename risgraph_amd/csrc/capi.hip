@@ -145,6 +145,14 @@ struct rg_graph {
     };
     std::vector<Captured> captured;
     int capture = -1;                 // -1: RG_GRAPH decides (default on)
+    // engine knobs per batch-size bucket, picked by rg_autotune: the team
+    // cap (x 1/4) and the team weights of the four algorithms
+    struct TuneEntry {
+        uint32_t cap_x4 = 4;
+        uint32_t weight[4] = {0, 0, 0, 0};   // 0: the algorithm default
+    };
+    TuneEntry tune_tab[RG_TUNE_BUCKETS];
+    const TuneEntry *run_tune = nullptr;   // the current run's entry
     uint64_t replays = 0;             // batches run through a captured pipeline
     BatchDesc *h_desc = nullptr;      // pinned
     BatchDesc *d_desc = nullptr;
@@ -195,6 +203,7 @@ struct rg_labels {
     uint32_t team_w = 0;           // workgroups per tree in the team launch (0: by batch size)
     uint32_t block_max = RG_AUTO;  // largest pass (vertices, or ops/16) run by the team
     uint32_t run_w = 8;            // team width of the current run
+    uint32_t team_weight = 0;      // share of the team's XCD slots (0: by algorithm)
     uint32_t *init_active = nullptr;
     // device history (rg_history_*): ring of value records per batch version
     struct Hist {
@@ -549,7 +558,63 @@ uint32_t team_width(const rg_labels *L, uint32_t n_ops)
     return w;
 }
 
-uint32_t team_cap(const rg_labels *L) { return L->block_max == RG_AUTO ? 1024u * L->run_w : L->block_max; }
+uint32_t team_cap(const rg_labels *L)
+{
+    if (L->block_max != RG_AUTO) return L->block_max;
+    const uint32_t x4 = L->g->run_tune ? L->g->run_tune->cap_x4 : 4u;
+    return (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, 256ull * L->run_w * x4);
+}
+
+// batch-size bucket of the tuning table: up to 32k, 128k, 512k edge ops, more
+int tune_bucket(uint64_t n_ops)
+{
+    return n_ops <= (32u << 10) ? 0 : n_ops <= (128u << 10) ? 1 : n_ops <= (512u << 10) ? 2 : 3;
+}
+
+// A tree's share of the team launch.  Measured per-tree repair times of
+// all_inc batches at R-MAT scale 24 with equal shares: SSSP 0.76 ms, SSWP
+// 0.61, WCC 0.38, BFS 0.29 (the launch waits for the slowest).
+uint32_t algo_weight(const rg_graph *g, int algo)
+{
+    const auto *t = g->run_tune;
+    if (t && algo >= 0 && algo < 4 && t->weight[algo]) return t->weight[algo];
+    switch (algo) {
+    case RG_SSSP: return 3;
+    case RG_SSWP: return 2;
+    case RG_WCC: return 2;
+    default: return 1;
+    }
+}
+
+uint32_t team_weight(const rg_labels *L) { return L->team_weight ? L->team_weight : algo_weight(L->g, L->algo); }
+
+// The eight XCD slots of a team launch over trees L[0..n): one slot each at
+// least, the rest by weight (largest remainder).
+void split_slots(rg_labels *const *L, int n, uint32_t *k)
+{
+    double q[MAXT], tot = 0;
+    for (int i = 0; i < n; i++) tot += team_weight(L[i]);
+    uint32_t used = 0;
+    for (int i = 0; i < n; i++) {
+        q[i] = XCD_SLOTS * team_weight(L[i]) / tot;
+        k[i] = std::max<uint32_t>(1, (uint32_t)q[i]);
+        used += k[i];
+    }
+    while (used > XCD_SLOTS) {   // only when n trees forced a minimum slot
+        int j = -1;
+        for (int i = 0; i < n; i++)
+            if (k[i] > 1 && (j < 0 || q[i] - k[i] < q[j] - k[j])) j = i;
+        k[j]--;
+        used--;
+    }
+    while (used < XCD_SLOTS) {
+        int j = 0;
+        for (int i = 1; i < n; i++)
+            if (q[i] - k[i] > q[j] - k[j]) j = i;
+        k[j]++;
+        used++;
+    }
+}
 
 // A tree's next pass runs in the persistent team launch when it is not a
 // full rebuild and fits the team's work cap; otherwise it is one grid step.
@@ -628,8 +693,17 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
     hipStream_t st = g->st;
     std::vector<EngineArgs> args(n);
     std::vector<char> active(n, 1), pending_init(n, 0);
+    // the run's team layout: per-tree XCD slots, workgroups per slot sized
+    // by the batch (an even split gives every tree team_width workgroups)
+    uint32_t kslots[MAXT] = {};
+    const bool one_launch = n <= MAXT;
+    g->run_tune = &g->tune_tab[tune_bucket(n_ops)];
+    if (one_launch) split_slots(L, n, kslots);
+    const uint32_t rounds =
+        std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)n * team_width(L[0], n_ops) / XCD_SLOTS,
+                                                 team_resident_max() / XCD_SLOTS));
     for (int i = 0; i < n; i++) {
-        L[i]->run_w = team_width(L[i], n_ops);
+        L[i]->run_w = one_launch ? rounds * kslots[i] : team_width(L[i], n_ops);
         engine_init(L[i], phase0, seed, n_ops, new_idx);
         // a run that starts in the team hands its initial control block to
         // the kernel through the pinned mirror; otherwise it is copied below
@@ -689,28 +763,27 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 if (stats) stats->grid_steps++;
             }
         }
-        // one launch per group of up to MAXT trees with the same team width
-        // and work cap
-        while (!team.empty()) {
-            const uint32_t W = L[team[0]]->run_w, cap = team_cap(L[team[0]]);
-            const EngineArgs *ta[MAXT];
-            uint32_t mask = 0;
-            int m = 0;
-            for (size_t k = 0; k < team.size() && m < MAXT;) {
-                int i = team[k];
-                if (L[i]->run_w != W || team_cap(L[i]) != cap) { k++; continue; }
-                ta[m] = L[i]->dargs;
-                if (pending_init[i]) mask |= 1u << m;
+        // the team trees share one launch on their XCD slots (a run of more
+        // than MAXT trees: one launch per tree, all eight slots each)
+        for (size_t b = 0; b < team.size(); b += one_launch ? team.size() : 1) {
+            TeamLaunch tl{};
+            const size_t e = one_launch ? team.size() : b + 1;
+            for (size_t k = b; k < e; k++) {
+                const int i = team[k], t = one_launch ? i : 0;
+                tl.a[t] = L[i]->dargs;
+                tl.k[t] = one_launch ? kslots[i] : XCD_SLOTS;
+                tl.work_max[t] = team_cap(L[i]);
+                if (pending_init[i]) tl.init_mask |= 1u << t;
                 pending_init[i] = 0;
-                m++;
-                team.erase(team.begin() + (long)k);
             }
+            tl.rounds = one_launch ? rounds : std::max<uint32_t>(1, L[team[b]]->run_w / XCD_SLOTS);
             hipEvent_t ea, eb;
-            prof_begin(g, 1, (uint64_t)m, st, &ea, &eb);
+            prof_begin(g, 1, (uint64_t)(e - b), st, &ea, &eb);
             if (ea) HC(hipEventRecord(ea, st));
-            launch_engine_team(ta, m, W, cap, mask, st);
+            launch_engine_team(tl, st);
             if (eb) HC(hipEventRecord(eb, st));
         }
+        team.clear();
         HC(hipGetLastError());
         if (guard_iter == 0 && hooks && hooks->after_first_enqueue) hooks->after_first_enqueue();
         HC(hipStreamSynchronize(st));
@@ -1717,6 +1790,20 @@ static void repair_batch(rg_graph *g, rg_labels **L, int nL, uint64_t n, bool ha
     }
     g->last_ms = ms;
     if (g->prof) prof_collect(g);
+    // RG_LOG_SLOW=ms: report batches whose device time exceeded ms
+    static const double log_slow = [] {
+        const char *e = getenv("RG_LOG_SLOW");
+        return e ? atof(e) : 0.0;
+    }();
+    if (log_slow > 0 && ms > log_slow) {
+        fprintf(stderr, "[rg] slow batch: %.2f ms, %llu edge ops, %llu unsafe, grid steps %llu;", ms,
+                (unsigned long long)n, (unsigned long long)g->last_unsafe,
+                (unsigned long long)(stats ? stats->grid_steps : 0));
+        for (int i = 0; i < nL; i++)
+            fprintf(stderr, " algo %d levels %llu subtree %u dense %u;", L[i]->algo,
+                    (unsigned long long)L[i]->hc->levels, L[i]->hc->n_tree, L[i]->hc->dense);
+        fprintf(stderr, "\n");
+    }
 }
 
 static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first, uint64_t n_ins,
@@ -2357,6 +2444,100 @@ int rg_stream_apply(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t i
     });
 }
 
+int rg_autotune(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_autotune_params *p,
+                rg_autotune_result *r)
+{
+    if (!g || !s || !p || !r || !L || nL < 1 || nL > MAXT) return RG_EINVAL;
+    return guard([&] {
+        std::lock_guard<std::recursive_mutex> lk(g->mu);
+        for (int i = 0; i < nL; i++)
+            if (!L[i] || L[i]->g != g) throw RgError{RG_EINVAL, "labels belong to another graph"};
+        HC(hipStreamSynchronize(s->st));
+        memset(r, 0, sizeof(*r));
+        const uint64_t V = g->V;
+        std::vector<uint32_t> d0(V), d1(V);
+        copy_degrees(g->lists[0], d0.data(), V, g->st);
+        HC(hipStreamSynchronize(g->st));
+        const uint32_t pairs = p->pairs ? p->pairs : 3u;
+        static const uint32_t sizes[RG_TUNE_BUCKETS] = {16384u, 65536u, 262144u, 1048576u};
+        static const uint32_t caps[] = {2, 8, 16};
+        static const uint32_t wsets[][4] = {{1, 1, 1, 1}, {1, 2, 3, 2}, {1, 3, 3, 1}, {1, 4, 2, 1}};
+        const uint64_t ins0 = p->ins_first % s->n, del0 = p->del_first % s->n;
+        // a forward batch and its inverse; device ms of the pair
+        auto run_pair = [&](uint32_t ops) {
+            const uint64_t half = ops / 2;
+            apply_batch(g, L, nL, s, ins0, half, del0, half, p->directed, nullptr);
+            double ms = g->last_ms;
+            apply_batch(g, L, nL, s, del0, half, ins0, half, p->directed, nullptr);
+            r->batches += 2;
+            return ms + g->last_ms;
+        };
+        auto measure = [&](int b, const rg_graph::TuneEntry &e) {
+            g->tune_tab[b] = e;
+            run_pair(sizes[b]);   // warm-up (and the bucket's captured pipeline)
+            std::vector<double> t;
+            for (uint32_t k = 0; k < pairs; k++) t.push_back(run_pair(sizes[b]));
+            std::sort(t.begin(), t.end());
+            return t[t.size() / 2];
+        };
+        bool used[4] = {false, false, false, false};
+        for (int i = 0; i < nL; i++)
+            if (L[i]->algo >= 0 && L[i]->algo < 4) used[L[i]->algo] = true;
+        double fwd_ms[RG_TUNE_BUCKETS] = {};
+        for (int b = 0; b < RG_TUNE_BUCKETS; b++) {
+            if (sizes[b] / 2 > s->n) break;
+            r->ops[b] = sizes[b];
+            rg_graph::TuneEntry best;
+            double best_ms = measure(b, best);
+            r->default_ms[b] = best_ms;
+            // a setting has to win by 3% to replace the incumbent
+            for (uint32_t c : caps) {
+                rg_graph::TuneEntry e = best;
+                e.cap_x4 = c;
+                const double ms = measure(b, e);
+                if (ms < 0.97 * best_ms) best = e, best_ms = ms;
+            }
+            if (nL > 1) {
+                for (const auto &w : wsets) {
+                    rg_graph::TuneEntry e = best;
+                    for (int a = 0; a < 4; a++) e.weight[a] = w[a];
+                    const double ms = measure(b, e);
+                    if (ms < 0.97 * best_ms) best = e, best_ms = ms;
+                }
+            }
+            g->tune_tab[b] = best;
+            g->run_tune = &g->tune_tab[b];
+            r->cap_x4[b] = best.cap_x4;
+            for (int a = 0; a < 4; a++) r->weight[b][a] = used[a] ? algo_weight(g, a) : 0u;
+            r->best_ms[b] = best_ms;
+            fwd_ms[b] = best_ms / 2;
+        }
+        g->run_tune = nullptr;
+        if (p->target_ms > 0) {
+            // the largest batch whose device time fits a quarter of the
+            // target: measured sizes, then the line through the two largest
+            const double budget = 0.25 * p->target_ms;
+            uint64_t cap = 0;
+            int last = -1;
+            for (int b = 0; b < RG_TUNE_BUCKETS; b++)
+                if (r->ops[b]) {
+                    last = b;
+                    if (fwd_ms[b] <= budget) cap = r->ops[b];
+                }
+            if (last >= 1 && cap == r->ops[last]) {
+                const double x0 = r->ops[last - 1], x1 = r->ops[last];
+                const double slope = (fwd_ms[last] - fwd_ms[last - 1]) / (x1 - x0);
+                if (slope > 0) cap = (uint64_t)std::min(4.0 * x1, x1 + (budget - fwd_ms[last]) / slope);
+            }
+            r->batch_cap = (uint32_t)std::max<uint64_t>(cap, 1024);
+        }
+        copy_degrees(g->lists[0], d1.data(), V, g->st);
+        HC(hipStreamSynchronize(g->st));
+        if (d0 != d1)
+            throw RgError{RG_EINVAL, "autotune windows did not restore the graph (deleted edges must be live)"};
+    });
+}
+
 int rg_stream_plain(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t ins_first, uint64_t del_first,
                     uint64_t n_steps, int directed, double *lat_ns, rg_batch_stats *stats)
 {
@@ -2400,6 +2581,14 @@ int rg_profile_read(rg_graph *g, int kind, uint64_t *launches, double *ms, uint6
         if (ms) *ms = g->p_ms[kind];
         if (units) *units = g->p_units[kind];
     });
+}
+
+int rg_labels_team_weight(rg_labels *L, uint32_t weight)
+{
+    if (!L || weight > 64) return RG_EINVAL;
+    std::lock_guard<std::recursive_mutex> lk(L->g->mu);
+    L->team_weight = weight;
+    return RG_OK;
 }
 
 int rg_labels_dense(rg_labels *L, int mode, uint64_t *dense_passes)

@@ -1136,10 +1136,15 @@ __device__ __forceinline__ void team_sync(Ctl *c, uint32_t *bar, uint32_t W, uin
 
 struct TeamParams {
     const EngineArgs *a[MAXT];   // device copies of the trees' arguments
-    uint32_t ntrees, W;
-    uint32_t work_max;     // a larger pass leaves the team for grid mode
-    uint32_t chunk_max;    // more chunk items than this leave for grid mode
+    uint32_t k[MAXT];            // XCD slots of each tree
+    uint32_t work_max[MAXT];     // a larger pass leaves the team for grid mode
+    uint32_t chunk_max[MAXT];    // more chunk items than this leave for grid mode
+    uint32_t rounds;             // workgroups per XCD slot
     uint32_t init_mask;
+    // per slot s, bits 4s..4s+3: its tree (0xF: none) and its rank among
+    // the tree's slots (packed: a byte array indexed by the slot would be
+    // copied to private memory)
+    uint32_t slot_tree, slot_rank;
 };
 
 // The level-synchronous loop of one tree on its W workgroups.  With W == 1
@@ -1268,8 +1273,16 @@ __device__ __forceinline__ void team_run(const EngineArgs &a, uint32_t W, uint32
     }
 }
 
-// Workgroup b serves tree b % ntrees: with 4 trees, a tree's workgroups sit
-// on two of the eight XCDs (workgroups are dealt to XCDs round-robin).
+template <class T>
+__device__ __forceinline__ T pick(const T (&x)[MAXT], uint32_t t)
+{
+    static_assert(MAXT == 4, "selection below");
+    return t == 0 ? x[0] : t == 1 ? x[1] : t == 2 ? x[2] : x[3];
+}
+
+// Workgroup b runs on XCD b % 8 (workgroups are dealt to the XCDs round-
+// robin) and serves the tree that owns that XCD slot, as workgroup
+// (b / 8) * k + rank of the tree's W = rounds * k.
 __global__ __launch_bounds__(1024) void k_team(TeamParams P)
 {
     __shared__ EngineArgs sa;
@@ -1278,19 +1291,22 @@ __global__ __launch_bounds__(1024) void k_team(TeamParams P)
     __shared__ uint32_t sh[3];
     __shared__ uint32_t stk[STACK_WORDS];
     __shared__ Stage stg;
-    const uint32_t t = blockIdx.x % P.ntrees, wg = blockIdx.x / P.ntrees;
-    static_assert(MAXT == 4, "tree selection below");
+    const uint32_t slot = blockIdx.x % XCD_SLOTS, round = blockIdx.x / XCD_SLOTS;
+    const uint32_t t = (P.slot_tree >> (4 * slot)) & 0xFu;
+    if (t >= (uint32_t)MAXT) return;   // a slot of a tree not in this launch
+    const uint32_t k = pick(P.k, t), W = P.rounds * k, wg = round * k + ((P.slot_rank >> (4 * slot)) & 0xFu);
+    const uint32_t work_max = pick(P.work_max, t), chunk_max = pick(P.chunk_max, t);
     static_assert(sizeof(EngineArgs) % 4 == 0, "word copy below");
     // the tree's arguments are staged in LDS: every pass reads them often
-    const uint32_t *src = (const uint32_t *)(t == 0 ? P.a[0] : t == 1 ? P.a[1] : t == 2 ? P.a[2] : P.a[3]);
+    const uint32_t *src = (const uint32_t *)pick(P.a, t);
     for (uint32_t i = threadIdx.x; i < sizeof(EngineArgs) / 4; i += blockDim.x) ((uint32_t *)&sa)[i] = src[i];
     __syncthreads();
     const bool init = (P.init_mask >> t) & 1u;
     switch (sa.algo) {
-    case BFS: team_run<BFS>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
-    case SSSP: team_run<SSSP>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
-    case SSWP: team_run<SSWP>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
-    default: team_run<WCC>(sa, P.W, P.work_max, P.chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
+    case BFS: team_run<BFS>(sa, W, work_max, chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
+    case SSSP: team_run<SSSP>(sa, W, work_max, chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
+    case SSWP: team_run<SSWP>(sa, W, work_max, chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
+    default: team_run<WCC>(sa, W, work_max, chunk_max, wg, init, &lds_ctl, snap, sh, stk, &stg); break;
     }
 }
 
@@ -1344,7 +1360,7 @@ __global__ void k_advance(EngineArgs a)
 // Team workgroups that can be resident at once on the current device: the
 // team barrier needs every workgroup of a launch resident, so wider requests
 // are narrowed to fit.
-static uint32_t team_resident_max()
+uint32_t team_resident_max()
 {
     static uint32_t cached[64] = {};
     int dev = 0;
@@ -1360,20 +1376,32 @@ static uint32_t team_resident_max()
     return cached[dev];
 }
 
-void launch_engine_team(const EngineArgs *const *a, int n, uint32_t W, uint32_t work_max, uint32_t init_mask,
-                        hipStream_t st)
+void launch_engine_team(const TeamLaunch &tl, hipStream_t st)
 {
-    if (n <= 0 || n > MAXT || W == 0) return;
-    const uint32_t wmax = std::max<uint32_t>(1, team_resident_max() / (uint32_t)n);
-    if (W > wmax) W = wmax;
     TeamParams P{};
-    for (int i = 0; i < MAXT; i++) P.a[i] = a[i < n ? i : 0];
-    P.ntrees = (uint32_t)n;
-    P.W = W;
-    P.work_max = work_max;
-    P.chunk_max = 256u * W;
-    P.init_mask = init_mask;
-    hipLaunchKernelGGL(k_team, dim3(n * W), dim3(1024), 0, st, P);
+    uint32_t s = 0;
+    for (int t = 0; t < MAXT; t++) {
+        for (uint32_t r = 0; r < tl.k[t] && s < XCD_SLOTS; r++, s++) {
+            P.slot_tree |= (uint32_t)t << (4 * s);
+            P.slot_rank |= r << (4 * s);
+        }
+    }
+    if (s == 0 || tl.rounds == 0) return;
+    for (; s < XCD_SLOTS; s++) P.slot_tree |= 0xFu << (4 * s);
+    // every workgroup of the launch must be resident at once
+    const uint32_t rounds = std::max<uint32_t>(1, std::min(tl.rounds, team_resident_max() / XCD_SLOTS));
+    const EngineArgs *any = nullptr;
+    for (int t = 0; t < MAXT; t++)
+        if (tl.k[t]) any = tl.a[t];
+    for (int t = 0; t < MAXT; t++) {
+        P.a[t] = tl.k[t] ? tl.a[t] : any;
+        P.k[t] = tl.k[t];
+        P.work_max[t] = tl.work_max[t];
+        P.chunk_max[t] = 256u * rounds * tl.k[t];
+    }
+    P.rounds = rounds;
+    P.init_mask = tl.init_mask;
+    hipLaunchKernelGGL(k_team, dim3(XCD_SLOTS * rounds), dim3(1024), 0, st, P);
 }
 
 template <int A>

@@ -327,13 +327,25 @@ struct rg_edge_rec {
 // vertex v0 + k (relative to off[0]), off has v1 - v0 + 1 entries
 void launch_list_dump(List l, Slots sl, uint32_t v0, uint32_t v1, const uint64_t *off, rg_edge_rec *out,
                       hipStream_t s);
-// Persistent team launch over trees *a[0..n) (n <= MAXT; device copies of
-// their arguments), W workgroups each.
-// A tree leaves the launch when its next pass exceeds work_max (grid mode
-// takes over) or its phase is done.  Bit t of init_mask: tree t's control
-// block is taken from its pinned mirror (first launch of a run).
-void launch_engine_team(const EngineArgs *const *a, int n, uint32_t W, uint32_t work_max, uint32_t init_mask,
-                        hipStream_t s);
+// Persistent team launch of the trees whose passes fit it.  The launch has
+// XCD_SLOTS * rounds workgroups, dealt to the XCDs round-robin (workgroup b
+// runs on XCD b % 8): tree t owns k[t] of the eight XCD slots, so its
+// W = rounds * k[t] workgroups share those XCDs' L2s, and a tree with more
+// work gets more slots.  k[t] = 0: tree t is not in this launch.  A tree
+// leaves the launch when its next pass exceeds work_max[t] (grid mode takes
+// over) or its phase is done.  Bit t of init_mask: tree t's control block is
+// taken from its pinned mirror (first launch of a run).
+constexpr uint32_t XCD_SLOTS = 8;
+struct TeamLaunch {
+    const EngineArgs *a[MAXT];   // device copies of the trees' arguments
+    uint32_t k[MAXT];
+    uint32_t work_max[MAXT];
+    uint32_t rounds;
+    uint32_t init_mask;
+};
+void launch_engine_team(const TeamLaunch &tl, hipStream_t s);
+// workgroups a team launch may have (all co-resident, as its barrier needs)
+uint32_t team_resident_max();
 // `steps` grid-wide passes enqueued back to back, sized for the pass the host
 // last saw; each is a no-op unless the tree's current pass is grid-sized
 // (checked on the device control block)
