@@ -2704,20 +2704,31 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
         };
         SplitOut so;
         so.flags = p->split != 0;
+        std::vector<uint64_t> unsafe_u;   // the current batch's updates that wait for the repair
+        bool timed = true;                // latencies of the current batch are recorded
         auto run = [&](uint64_t b, uint64_t e) {
             // updates [b, e): adds are the even indices, deletes the odd ones
             uint64_t a0 = (b + 1) / 2, a1 = (e + 1) / 2;
             uint64_t d0 = b / 2, d1 = e / 2;
+            unsafe_u.clear();
+            // once the split's flags are on the host, and while the repair
+            // still runs on the device, the safe updates are answered here:
+            // the host's per-update work overlaps the repair instead of
+            // following it
+            so.on_safe = [&, b, e, a0, a1, d0](const uint8_t *flag) {
+                const double ts = std::chrono::duration<double>(so.t_safe - t0).count();
+                for (uint64_t u = b; u < e; u++) {
+                    // op index of update u: inserts first, then deletes
+                    const uint64_t op = (u & 1) ? (a1 - a0) + (u / 2 - d0) : u / 2 - a0;
+                    const uint8_t w = flag[op];
+                    waited[u] = w;
+                    if (w) unsafe_u.push_back(u);
+                    else if (timed) lat[u] = (ts - arr[u]) * 1e3;
+                }
+            };
             apply_batch(g, L, nL, s, (p->ins_first + a0) % s->n, a1 - a0, (p->del_first + d0) % s->n, d1 - d0,
                         p->directed, &st, p->split ? &so : nullptr);
             gpu_ms += g->last_ms;
-            if (p->split && so.done) {
-                // op index of update u: inserts first, then deletes
-                for (uint64_t u = b; u < e; u++) {
-                    const uint64_t op = (u & 1) ? (a1 - a0) + (u / 2 - d0) : u / 2 - a0;
-                    waited[u] = so.flag[op];
-                }
-            }
         };
         while (next < n) {
             double tn = now();
@@ -2730,8 +2741,11 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
             uint64_t end = std::min<uint64_t>(arrived, next + (uint64_t)std::floor(limit()));
             run(next, end);
             double tc = now();
-            const double ts = so.done ? std::chrono::duration<double>(so.t_safe - t0).count() : tc;
-            for (uint64_t u = next; u < end; u++) lat[u] = ((waited[u] ? tc : ts) - arr[u]) * 1e3;
+            if (so.done) {
+                for (uint64_t u : unsafe_u) lat[u] = (tc - arr[u]) * 1e3;
+            } else {
+                for (uint64_t u = next; u < end; u++) lat[u] = (tc - arr[u]) * 1e3;
+            }
             if (so.done) frac = std::max(0.01, 0.8 * frac + 0.2 * (double)so.unsafe / (double)(end - next));
             if (p->target_ms > 0) {
                 double service_ms = (tc - tn) * 1e3;
@@ -2750,6 +2764,7 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
         double wall = now();
         // an overloaded run still applies the rest (untimed) so the stream
         // window stays consistent for the next run
+        timed = false;
         while (next < n) {
             uint64_t end = std::min<uint64_t>(n, next + (uint64_t)cap_max);
             run(next, end);
