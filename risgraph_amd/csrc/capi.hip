@@ -108,6 +108,7 @@ struct rg_graph {
     Pool *pool = nullptr;
     uint64_t slot_cap = 0, hash_cap = 0;
     uint32_t fcap = 0;                // entries per size-class free list of the slab pools
+    uint32_t lstride = 4;             // list-op slots per edge op of the current batch
     EdgeOps ops{};
     ListOps lops{};
     EdgeOps uops{};                   // a stream batch's unsafe ops (the engine's input after the split)
@@ -327,6 +328,7 @@ StoreArgs store_args(rg_graph *g)
     a.cnt_tmp[0] = g->cnt_tmp[0];
     a.cnt_tmp[1] = g->cnt_tmp[1];
     a.fcap = g->fcap;
+    a.lstride = g->lstride;
     return a;
 }
 
@@ -349,6 +351,7 @@ void upload_edges(rg_graph *g, const rg_edge *e, uint64_t n, uint8_t kind)
     HC(hipMemcpyAsync(g->ops.kind, g->h_kind, n, hipMemcpyHostToDevice, g->st));
     g->ops.n = (uint32_t)n;
     g->lops.n = (uint32_t)(4 * n);
+    g->lstride = 4;   // callers may rewrite the kinds op by op
 }
 
 void store_status_check(rg_graph *g)
@@ -358,6 +361,7 @@ void store_status_check(rg_graph *g)
         g->h_status[0] = 0;
         HC(hipMemsetAsync(g->err, 0, 4, g->st));
         if (e & 1) throw RgError{RG_EINVAL_VERTEX, "VertexId error."};
+        if (e & 2) throw RgError{RG_EDEVICE, "an edge op needed more list-op slots than its batch's stride"};
         throw RgError{RG_EDEVICE, "store op chain corrupt"};
     }
     if (g->h_status[1]) throw RgError{RG_ENOMEM, "adjacency slot or hash pool exhausted"};
@@ -1117,6 +1121,7 @@ int rg_edge_update_batch_static(rg_graph *g, const rg_edge *d_edges, uint64_t n,
         StoreArgs a = store_args(g);
         a.ops = ops;
         a.lops = lops;
+        a.lstride = 4;
         if (!launch_store_batch(a, g->st)) launch_pool_recycle(a, g->st);
         if (d_old) hipLaunchKernelGGL(k_old_out, grid, blk, 0, g->st, ops.res, (uint32_t)n, d_old);
         HC(hipGetLastError());
@@ -1611,6 +1616,7 @@ int rg_stream_load(rg_graph *g, rg_stream *s, uint64_t first, uint64_t n, int di
             launch_ops_from_stream(s->src, s->dst, w, first + b, m, kind, g->ops, 0, g->st);
             g->ops.n = m;
             g->lops.n = 4 * m;
+            g->lstride = 4;
             run_store(g);
         }
         HC(hipStreamSynchronize(g->st));
@@ -1822,6 +1828,9 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
     const bool split = nL > 0 && nL <= MAXT && n > SPLIT_MIN_OPS;
     const bool flags = split && so && so->flags;
     uint8_t ud = directed ? 0 : K_UNDIR;
+    // every op of a stream batch has the same direction: its list ops fit
+    // this many slots (half the records to group for a directed batch)
+    g->lstride = (directed ? 1u : 2u) * (g->symmetric ? 1u : 2u);
     const uint32_t *w = g->weighted ? s->w : nullptr;
     if (g->weighted && !w) throw RgError{RG_EINVAL, "weighted graph needs a weighted stream"};
     // a split batch whose store kernels are not timed one by one runs the
@@ -1832,7 +1841,7 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
         ensure_ops(g, bucket);
         if (flags) ensure_host(g, bucket);
         g->ops.n = (uint32_t)n;
-        g->lops.n = (uint32_t)(4 * n);
+        g->lops.n = (uint32_t)(g->lstride * n);
         hipGraphExec_t exec = captured_pipeline(g, L, nL, s, bucket, (uint8_t)(K_INS | ud), (uint8_t)(K_DEL | ud),
                                                 flags);
         if (exec) {
@@ -1856,7 +1865,7 @@ static void apply_batch(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64
     launch_ops_from_window(s->src, s->dst, w, s->n, ins_first, (uint32_t)n_ins, del_first, (uint32_t)n_del,
                            (uint8_t)(K_INS | ud), (uint8_t)(K_DEL | ud), g->ops, split ? g->ucount : nullptr, g->st);
     g->ops.n = (uint32_t)n;
-    g->lops.n = (uint32_t)(4 * n);
+    g->lops.n = (uint32_t)(g->lstride * n);
     HC(hipEventRecord(g->ev[0], g->st));
     run_store(g);
     HC(hipEventRecord(g->ev[1], g->st));

@@ -296,6 +296,10 @@ struct StoreArgs {
     // bucket's largest batch); null = ops.n
     const uint32_t *dyn_n;
     uint32_t fcap;   // entries per size-class free list
+    // list-op slots per edge op: the most list ops one edge op of the batch
+    // makes (1 directed on a symmetric store, 2 directed with in-lists or
+    // undirected on a symmetric store, 4 undirected with in-lists)
+    uint32_t lstride;
 };
 
 // The window of one captured stream batch: written by the host into pinned

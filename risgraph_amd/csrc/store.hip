@@ -52,7 +52,8 @@ __device__ __forceinline__ uint32_t next_pow2(uint32_t x)
 // ---------------------------------------------------------------------
 // expand edge ops into list ops and link them into per-list chains
 // ---------------------------------------------------------------------
-// Edge op i -> list ops 4i..4i+3 (unused slots get v = NILP).  With `rank`,
+// Edge op i -> list ops S*i..S*i+S-1, S = the batch's lstride (unused slots
+// get v = NILP; an op needing more slots than S flags the batch).  With `rank`,
 // each list op takes its rank on its list with one atomicAdd on the header's
 // nops (the segment path); without, the caller groups the ops itself.
 __device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank, uint32_t n_ops)
@@ -60,8 +61,9 @@ __device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank, uint32_t n_
     uint32_t s = a.ops.src[i], d = a.ops.dst[i], w = a.weighted ? a.ops.w[i] : 0;
     uint8_t kind = a.ops.kind[i];
     a.ops.res[i] = OpResult{0u, 0u};
-    uint32_t base = 4 * i;
-    for (int k = 0; k < 4; k++) a.lops.op[base + k].v = NILP;
+    const uint32_t S = a.lstride;
+    uint32_t base = S * i;
+    for (uint32_t k = 0; k < S; k++) a.lops.op[base + k].v = NILP;
     if (kind & K_NOSTORE) return;
     if (s >= a.V || d >= a.V) {
         atomicOr(a.err, 1u);
@@ -71,6 +73,10 @@ __device__ void expand_op(const StoreArgs &a, uint32_t i, bool rank, uint32_t n_
     bool undir = kind & K_UNDIR;
     uint32_t nl = 0;
     auto emit = [&](uint32_t v, uint32_t nbr, int8_t delta, uint8_t flags) {
+        if (nl >= S) {
+            atomicOr(a.err, 2u);   // the host chose too small a stride
+            return;
+        }
         uint32_t j = base + nl++;
         LOp o{};
         o.v = v;
@@ -251,7 +257,7 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
     __shared__ uint32_t wsum[SEG_BLOCK / WAVE][3];
     __shared__ unsigned long long bbase;
     __shared__ uint32_t hbase;
-    const uint32_t nlops = 4 * op_count(a);
+    const uint32_t nlops = a.lstride * op_count(a);
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t cnt = 0, lt = 0, hv = 0, v = NILP, gcls = 0;
     bool grow = false;
@@ -321,7 +327,7 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
 __global__ void k_scatter(StoreArgs a)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= 4 * op_count(a)) return;
+    if (j >= a.lstride * op_count(a)) return;
     const uint32_t v = a.lops.op[j].v;
     if (v == NILP) return;
     const int L = (a.lops.op[j].flags & L_IN) ? 1 : 0;
@@ -1043,7 +1049,7 @@ bool launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipE
 {
     if (a.ops.n == 0) return false;
     const uint32_t n = a.ops.n;
-    const uint32_t nl = 4 * n;
+    const uint32_t nl = a.lstride * n;
     if (n <= SMALL_OPS) {
         // one launch for the whole batch, pool recycling included
         if (ev_a) (void)hipEventRecord(ev_a, s);
@@ -1067,7 +1073,7 @@ bool launch_store_batch(const StoreArgs &a, hipStream_t s, hipEvent_t ev_a, hipE
 
 void launch_store_batch_dyn(const StoreArgs &a, uint32_t n_max, hipStream_t s)
 {
-    const uint32_t nl = 4 * n_max;
+    const uint32_t nl = a.lstride * n_max;
     hipLaunchKernelGGL(k_expand_link, dim3((n_max + 255) / 256), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_segment, dim3((nl + SEG_BLOCK - 1) / SEG_BLOCK), dim3(SEG_BLOCK), 0, s, a);
     hipLaunchKernelGGL(k_scatter, dim3((nl + 255) / 256), dim3(256), 0, s, a);
@@ -1261,7 +1267,7 @@ __global__ void __launch_bounds__(HB) k_store_small(StoreArgs a)
     __shared__ uint32_t wcnt[HB / WAVE];
     __shared__ uint32_t nvalid, nseg, nheavy;
     const uint32_t tid = threadIdx.x, lane = __lane_id(), wv = tid / WAVE;
-    const uint32_t n = a.ops.n, nl = 4 * n;
+    const uint32_t n = a.ops.n, nl = a.lstride * n;
     if (tid == 0) {
         nvalid = 0;
         nheavy = 0;
