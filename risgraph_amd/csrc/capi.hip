@@ -283,7 +283,7 @@ void ensure_ops(rg_graph *g, uint64_t n)
     g->lops.op = dalloc<LOp>(lc);
     g->lops.segbuf = dalloc<uint32_t>(lc);
     g->lops.heavy = dalloc<uint32_t>(lc);
-    g->lops.owner = dalloc<uint32_t>(lc);
+    g->lops.owner = dalloc<uint64_t>(lc);
     g->d_flag = dalloc<uint8_t>(cap);
     g->uops.src = dalloc<uint32_t>(cap);
     g->uops.dst = dalloc<uint32_t>(cap);
@@ -388,7 +388,7 @@ constexpr size_t SC_RES = SC_KIND + al256(1ull * STATIC_MAX_OPS);
 constexpr size_t SC_LOP = SC_RES + al256(sizeof(OpResult) * STATIC_MAX_OPS);
 constexpr size_t SC_SEG = SC_LOP + al256(sizeof(LOp) * 4ull * STATIC_MAX_OPS);
 constexpr size_t SC_OWNER = SC_SEG + al256(16ull * STATIC_MAX_OPS);
-constexpr size_t SC_HEAVY = SC_OWNER + al256(16ull * STATIC_MAX_OPS);
+constexpr size_t SC_HEAVY = SC_OWNER + al256(32ull * STATIC_MAX_OPS);
 constexpr size_t SCRATCH_BYTES = SC_HEAVY + al256(16ull * STATIC_MAX_OPS);
 
 static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
@@ -1104,7 +1104,7 @@ int rg_edge_update_batch_static(rg_graph *g, const rg_edge *d_edges, uint64_t n,
         ListOps lops{};
         lops.op = (LOp *)(sc + SC_LOP);
         lops.segbuf = (uint32_t *)(sc + SC_SEG);
-        lops.owner = (uint32_t *)(sc + SC_OWNER);
+        lops.owner = (uint64_t *)(sc + SC_OWNER);
         lops.heavy = (uint32_t *)(sc + SC_HEAVY);
         lops.n = (uint32_t)(4 * n);
         // validate and convert first: a batch with a bad id or weight is

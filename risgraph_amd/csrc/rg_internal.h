@@ -158,7 +158,9 @@ static_assert(sizeof(LOp) == 32, "list op record is 32 bytes");
 struct ListOps {
     LOp *op;
     uint32_t *segbuf;   // list-op indices grouped by list (segments)
-    uint32_t *owner;    // rank-0 list ops of the light lists (dense)
+    // the light lists (dense): rank-0 op index | (vertex | in-list << 31) << 32,
+    // so the apply loads a list's header together with its op record
+    uint64_t *owner;
     uint32_t *heavy;    // rank-0 list ops of the heavy lists (dense)
     uint32_t n;
 };

@@ -13,7 +13,7 @@
 // first, then its deletes, so a batch behaves like the reference batch driver
 // (all adds, then all dels: bfs_inc_batch.cpp:142-178).
 // k_apply_all runs both kinds of list side by side in one launch:
-//   light lists    one 4-lane group per list; ops one at a time (lookups
+//   light lists    one 2-lane group per list; ops one at a time (lookups
 //                  scan the list 8 entries per lane per step, or probe its
 //                  open-addressing index once it holds >= INDEX_THRESHOLD
 //                  entries).
@@ -172,7 +172,7 @@ __device__ void pool_free(const StoreArgs &a, int h, uint64_t off, uint64_t n)
 // ---------------------------------------------------------------------
 // The first op of every list reserves the list's segment and an entry in one
 // of two dense owner lists (rank-0 ops): light lists, which k_apply_all
-// walks with a 4-lane group each, and heavy lists, which it gives a
+// walks with a 2-lane group each, and heavy lists, which it gives a
 // workgroup each.  A list is heavy when its segment is a hub segment, or when
 // its inserts (at most its op count) could need an index build or rebuild or
 // a block copy longer than LIGHT_COPY entries.  A block sums its segments and
@@ -320,7 +320,9 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
     }
     __syncthreads();
     if (lt | hv) a.lists[L].h[v].seg = (uint32_t)bbase + wsum[wid][0] + ic - cnt;
-    if (lt) a.lops.owner[(uint32_t)(bbase >> 32) + wsum[wid][1] + il - 1] = j;
+    if (lt)
+        a.lops.owner[(uint32_t)(bbase >> 32) + wsum[wid][1] + il - 1] =
+            (uint64_t)j | ((uint64_t)(v | ((uint32_t)L << 31)) << 32);
     if (hv) a.lops.heavy[hbase + wsum[wid][2] + ih - 1] = j;
 }
 
@@ -551,7 +553,10 @@ __device__ uint32_t list_find(const StoreArgs &a, const ListState &st, uint32_t 
     return NILP;
 }
 
-constexpr int AG = 4;             // lanes per list in the light apply / reserve kernels
+// lanes per list in the light apply: 2 measured best at 300k-op batches
+// (0.50 ms store batch against 0.54 with 4 and 0.59 with 8; 1 was best at
+// million-op batches by 3%)
+constexpr int AG = 2;
 
 template <int G>
 __device__ __forceinline__ int32_t gsum(int32_t x)
@@ -637,14 +642,62 @@ __device__ __forceinline__ bool light_room(const StoreArgs &a, ListState &st, ui
     return true;
 }
 
+// list_find for a list's single op.  An unindexed list's scan loads the
+// count (and weight) quads together with the neighbour quads, so the match's
+// count arrives with it instead of one more dependent load; `c` gets it (0
+// when the key is absent).  Quads start below len inside a block of a power
+// of two >= 4 entries at a multiple-of-4 offset, in all three arrays.
+__device__ __forceinline__ uint32_t find_single(const StoreArgs &a, const ListState &st, uint32_t key_nbr,
+                                                uint32_t key_w, const Grp &g, uint32_t *ins_bucket, uint32_t &c)
+{
+    if (st.hoff != HEMPTY) {
+        const uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, ins_bucket);
+        c = idx == NILP ? 0u : a.slots.cnt[st.off + idx];
+        return idx;
+    }
+    const uint32_t *nb = a.slots.nbr + st.off, *cn = a.slots.cnt + st.off;
+    const uint32_t *wt = a.slots.wt ? a.slots.wt + st.off : nullptr;
+    for (uint32_t b = 0; b < st.len; b += 8 * AG) {
+        uint32_t found = NILP, fc = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t i0 = b + (uint32_t)h * 4 * AG + 4 * (uint32_t)g.lane;
+            uint4 q = make_uint4(NILP, NILP, NILP, NILP), qc = make_uint4(0, 0, 0, 0), qw = make_uint4(0, 0, 0, 0);
+            if (i0 < st.len) {
+                q = *reinterpret_cast<const uint4 *>(nb + i0);
+                qc = *reinterpret_cast<const uint4 *>(cn + i0);
+                if (wt) qw = *reinterpret_cast<const uint4 *>(wt + i0);
+            }
+            const uint32_t e[4] = {q.x, q.y, q.z, q.w}, ec[4] = {qc.x, qc.y, qc.z, qc.w};
+            const uint32_t ew[4] = {qw.x, qw.y, qw.z, qw.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t i = i0 + (uint32_t)k;
+                if (found == NILP && i < st.len && (e[k] & NBR_MASK) == key_nbr && (!wt || ew[k] == key_w)) {
+                    found = i;
+                    fc = ec[k];
+                }
+            }
+        }
+        const uint32_t hit = gballot<AG>(found != NILP, g);
+        if (hit) {
+            const int src = g.base + __ffs(hit) - 1;
+            c = (uint32_t)__shfl((int)fc, src);
+            return (uint32_t)__shfl((int)found, src);
+        }
+    }
+    if (ins_bucket) *ins_bucket = NILP;
+    c = 0;
+    return NILP;
+}
+
 __device__ __forceinline__ void apply_single(const StoreArgs &a, ListState &st, uint32_t k, int8_t delta,
                                              uint8_t flags, uint32_t key_nbr, uint32_t key_w, uint32_t edge,
                                              uint32_t rcap, uint64_t roff, const Grp &g)
 {
     if (!light_room(a, st, st.len + (delta > 0 ? 1u : 0u), rcap, roff, g)) return;
-    uint32_t bucket = NILP;
-    uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, &bucket);
-    uint32_t c = 0;
+    uint32_t bucket = NILP, c = 0;
+    uint32_t idx = find_single(a, st, key_nbr, key_w, g, &bucket, c);
     if (idx == NILP) {
         if (delta > 0) {
             // append a new entry (storage.hpp:371-374, 389-392)
@@ -660,8 +713,6 @@ __device__ __forceinline__ void apply_single(const StoreArgs &a, ListState &st, 
             st.len++;
             if (st.hoff != HEMPTY) st.hcnt++;
         }
-    } else {
-        c = a.slots.cnt[st.off + idx];
     }
     uint32_t nc, ret;
     int32_t dchange;
@@ -719,17 +770,18 @@ __device__ void light_lists_apply(const StoreArgs &a, uint32_t blk, uint32_t nbl
     const uint32_t ng = (nblk * blockDim.x) / AG;
     const uint32_t nown = (uint32_t)(a.pool->seg_pack >> 32);
     for (uint32_t oi = gid; oi < nown; oi += ng) {
-        const uint32_t j = a.lops.owner[oi];
-        // the op's own fields are loaded with its vertex, before the header
-        // arrives: most lists have this one op only
+        const uint64_t ow = a.lops.owner[oi];
+        const uint32_t j = (uint32_t)ow, vl = (uint32_t)(ow >> 32);
+        const uint32_t v = vl & 0x7FFFFFFFu;
+        const int L = (int)(vl >> 31);
+        // the op's record and the list's header load side by side: most
+        // lists have this one op only
         const LOp o = a.lops.op[j];
-        const uint32_t v = o.v;
         const uint8_t fl = o.flags;
         const int8_t dj = o.delta;
         const uint32_t key_nbr = o.nbr;
         const uint32_t key_w = a.slots.wt ? o.w : 0;
         const uint32_t ej = o.edge;
-        const int L = (fl & L_IN) ? 1 : 0;
         ListHdr &hd = a.lists[L].h[v];
         const uint32_t n = hd.nops;
         const uint32_t rcap = hd.ncap;
