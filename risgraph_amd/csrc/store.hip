@@ -642,62 +642,13 @@ __device__ __forceinline__ bool light_room(const StoreArgs &a, ListState &st, ui
     return true;
 }
 
-// list_find for a list's single op.  An unindexed list's scan loads the
-// count (and weight) quads together with the neighbour quads, so the match's
-// count arrives with it instead of one more dependent load; `c` gets it (0
-// when the key is absent).  Quads start below len inside a block of a power
-// of two >= 4 entries at a multiple-of-4 offset, in all three arrays.
-__device__ __forceinline__ uint32_t find_single(const StoreArgs &a, const ListState &st, uint32_t key_nbr,
-                                                uint32_t key_w, const Grp &g, uint32_t *ins_bucket, uint32_t &c)
-{
-    if (st.hoff != HEMPTY) {
-        const uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, ins_bucket);
-        c = idx == NILP ? 0u : a.slots.cnt[st.off + idx];
-        return idx;
-    }
-    const uint32_t *nb = a.slots.nbr + st.off, *cn = a.slots.cnt + st.off;
-    const uint32_t *wt = a.slots.wt ? a.slots.wt + st.off : nullptr;
-    for (uint32_t b = 0; b < st.len; b += 8 * AG) {
-        uint32_t found = NILP, fc = 0;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const uint32_t i0 = b + (uint32_t)h * 4 * AG + 4 * (uint32_t)g.lane;
-            uint4 q = make_uint4(NILP, NILP, NILP, NILP), qc = make_uint4(0, 0, 0, 0), qw = make_uint4(0, 0, 0, 0);
-            if (i0 < st.len) {
-                q = *reinterpret_cast<const uint4 *>(nb + i0);
-                qc = *reinterpret_cast<const uint4 *>(cn + i0);
-                if (wt) qw = *reinterpret_cast<const uint4 *>(wt + i0);
-            }
-            const uint32_t e[4] = {q.x, q.y, q.z, q.w}, ec[4] = {qc.x, qc.y, qc.z, qc.w};
-            const uint32_t ew[4] = {qw.x, qw.y, qw.z, qw.w};
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint32_t i = i0 + (uint32_t)k;
-                if (found == NILP && i < st.len && (e[k] & NBR_MASK) == key_nbr && (!wt || ew[k] == key_w)) {
-                    found = i;
-                    fc = ec[k];
-                }
-            }
-        }
-        const uint32_t hit = gballot<AG>(found != NILP, g);
-        if (hit) {
-            const int src = g.base + __ffs(hit) - 1;
-            c = (uint32_t)__shfl((int)fc, src);
-            return (uint32_t)__shfl((int)found, src);
-        }
-    }
-    if (ins_bucket) *ins_bucket = NILP;
-    c = 0;
-    return NILP;
-}
-
 __device__ __forceinline__ void apply_single(const StoreArgs &a, ListState &st, uint32_t k, int8_t delta,
                                              uint8_t flags, uint32_t key_nbr, uint32_t key_w, uint32_t edge,
                                              uint32_t rcap, uint64_t roff, const Grp &g)
 {
     if (!light_room(a, st, st.len + (delta > 0 ? 1u : 0u), rcap, roff, g)) return;
     uint32_t bucket = NILP, c = 0;
-    uint32_t idx = find_single(a, st, key_nbr, key_w, g, &bucket, c);
+    uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, &bucket);
     if (idx == NILP) {
         if (delta > 0) {
             // append a new entry (storage.hpp:371-374, 389-392)
@@ -713,6 +664,8 @@ __device__ __forceinline__ void apply_single(const StoreArgs &a, ListState &st, 
             st.len++;
             if (st.hoff != HEMPTY) st.hcnt++;
         }
+    } else {
+        c = a.slots.cnt[st.off + idx];
     }
     uint32_t nc, ret;
     int32_t dchange;
