@@ -110,11 +110,14 @@ class Cursor:
 # otherwise shrinks x0.7 when a batch's device time exceeds a quarter of the
 # latency target, up to BATCH_MAX.  (Taking every arrived update per batch, with no cap,
 # met the p99.9 target less often: 6 of 8 half-second rounds at 80M
-# updates/s against 7 of 8.  A cap of 2^21 instead of 2^20 drains the
-# backlog after a large deletion event in fewer batches: at 170M updates/s
-# the half-second window holding such an event had p99.9 25.9 ms instead of
-# 32.0 ms, and the window after it 8.8 ms instead of 14.7 ms.)
-BATCH_MAX = 1 << 21
+# updates/s against 7 of 8.  The stream has periodic regions where twice the
+# usual share of updates is unsafe (17-22% instead of 8%) and batches cost
+# twice as much; a backlog built there drains in batches of the cap.  Over
+# 14 half-second windows at 180M updates/s the two windows of each such
+# region had p99.9 15.6 / 29.4 and 24.7 / 25.5 ms with a cap of 2^20, against
+# 19.2 / 40.5 and 26.3 / 35.3 ms with 2^21 and 25.2 / 49.1 and 29.8 / 43.0 ms
+# with 2^19; the other windows were 5-9 ms under all three.)
+BATCH_MAX = 1 << 20
 
 
 def serve(g, labels, stream, cur, n, rate, cap, cap_max, target_ms, split=True, cap_unsafe=False):

@@ -222,8 +222,9 @@ int rg_graph_capture_info(rg_graph *g, int *on, uint32_t *pipelines, uint64_t *r
  * (passes larger than cap_x4/4 x 1024 vertices per team workgroup go
  * grid-wide) and the team weights of the four algorithms.  Later batches of
  * the graph run with the table.  With target_ms > 0 it also returns
- * batch_cap, the largest serving batch whose measured device time fits a
- * quarter of the target (the serving loop's own shrink rule), for
+ * batch_cap, the largest serving batch whose measured device time, doubled
+ * (for stream regions with twice the usual unsafe share), fits a quarter of
+ * the target (the serving loop's own shrink rule), for
  * rg_serve_params.cap_max.  Every edge the deleted window holds must be
  * live (it is when the windows are the stream's current window, as in the
  * serving loop); the degrees are checked afterwards and a graph that did not
@@ -366,6 +367,9 @@ int rg_stream_plain(rg_graph *g, rg_labels **L, int nL, rg_stream *s, uint64_t i
  * between events recorded before its store kernels and after its last
  * engine kernel. */
 int rg_last_gpu_ms(rg_graph *g, double *ms);
+/* Edge ops of the last stream batch that the split found unsafe (all of
+ * them for a batch that was not split). */
+int rg_last_unsafe(rg_graph *g, uint64_t *n);
 
 /* Kernel timing with HIP events on the streams the kernels run on.  Bit k of
  * `mask` times kernels of kind k: the store apply kernels (kind 0), the

@@ -109,6 +109,7 @@ SIGNATURES = {
     "rg_labels_assign": (ci, [vp, vp, vp]),
     "rg_graph_shape": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci)]),
     "rg_graph_capture": (ci, [vp, ci]),
+    "rg_last_unsafe": (ci, [vp, u64p]),
     "rg_autotune": (ci, [vp, vp, ci, vp, vp, vp]),
     "rg_graph_pool_info": (ci, [vp, vp]),
     "rg_graph_capture_info": (ci, [vp, ctypes.POINTER(ci), ctypes.POINTER(ctypes.c_uint32), u64p]),
@@ -446,6 +447,12 @@ class Graph:
         ms = ctypes.c_double()
         _check(load_library().rg_last_gpu_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    def last_unsafe(self):
+        """rg_last_unsafe: unsafe edge ops of the last stream batch."""
+        n = u64()
+        _check(load_library().rg_last_unsafe(self.h, ctypes.byref(n)))
+        return int(n.value)
 
 
 class Labels:

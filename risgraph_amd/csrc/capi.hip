@@ -1802,8 +1802,10 @@ static void repair_batch(rg_graph *g, rg_labels **L, int nL, uint64_t n, bool ha
         return e ? atof(e) : 0.0;
     }();
     if (log_slow > 0 && ms > log_slow) {
-        fprintf(stderr, "[rg] slow batch: %.2f ms, %llu edge ops, %llu unsafe, grid steps %llu;", ms,
-                (unsigned long long)n, (unsigned long long)g->last_unsafe,
+        float store_ms = 0;
+        (void)hipEventElapsedTime(&store_ms, g->ev[0], g->ev[1]);
+        fprintf(stderr, "[rg] slow batch: %.2f ms (store %.2f), %llu edge ops, %llu unsafe, grid steps %llu;", ms,
+                store_ms, (unsigned long long)n, (unsigned long long)g->last_unsafe,
                 (unsigned long long)(stats ? stats->grid_steps : 0));
         for (int i = 0; i < nL; i++)
             fprintf(stderr, " algo %d levels %llu subtree %u dense %u;", L[i]->algo,
@@ -2523,22 +2525,32 @@ int rg_autotune(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_autot
         }
         g->run_tune = nullptr;
         if (p->target_ms > 0) {
-            // the largest batch whose device time fits a quarter of the
-            // target: measured sizes, then the line through the two largest
-            const double budget = 0.25 * p->target_ms;
-            uint64_t cap = 0;
-            int last = -1;
-            for (int b = 0; b < RG_TUNE_BUCKETS; b++)
-                if (r->ops[b]) {
-                    last = b;
-                    if (fwd_ms[b] <= budget) cap = r->ops[b];
+            // the largest batch whose device time, doubled, fits a quarter of
+            // the target (the serving loop's shrink rule): measured sizes,
+            // then the line through the two largest.  Doubled because stream
+            // regions where twice the usual share of updates is unsafe make
+            // batches cost about twice the tuning pairs' time, and a backlog
+            // built there drains in batches of the cap.
+            const double budget = 0.125 * p->target_ms;
+            // piecewise-linear device time over the measured sizes (the
+            // last segment extended, up to 4x the largest size)
+            double cap = 0;
+            int m = 0;
+            while (m < RG_TUNE_BUCKETS && r->ops[m]) m++;
+            for (int b = 0; b < m; b++) {
+                if (fwd_ms[b] <= budget) cap = r->ops[b];
+                const bool last = b + 1 == m;
+                if (b == 0 || fwd_ms[b] <= budget) {
+                    if (!last && fwd_ms[b + 1] > budget && fwd_ms[b + 1] > fwd_ms[b])
+                        cap = r->ops[b] + (budget - fwd_ms[b]) / (fwd_ms[b + 1] - fwd_ms[b]) *
+                                              ((double)r->ops[b + 1] - r->ops[b]);
                 }
-            if (last >= 1 && cap == r->ops[last]) {
-                const double x0 = r->ops[last - 1], x1 = r->ops[last];
-                const double slope = (fwd_ms[last] - fwd_ms[last - 1]) / (x1 - x0);
-                if (slope > 0) cap = (uint64_t)std::min(4.0 * x1, x1 + (budget - fwd_ms[last]) / slope);
+                if (last && b >= 1 && fwd_ms[b] <= budget && fwd_ms[b] > fwd_ms[b - 1]) {
+                    const double slope = (fwd_ms[b] - fwd_ms[b - 1]) / ((double)r->ops[b] - r->ops[b - 1]);
+                    cap = std::min(4.0 * r->ops[b], r->ops[b] + (budget - fwd_ms[b]) / slope);
+                }
             }
-            r->batch_cap = (uint32_t)std::max<uint64_t>(cap, 1024);
+            r->batch_cap = (uint32_t)std::max(cap, 1024.0);
         }
         copy_degrees(g->lists[0], d1.data(), V, g->st);
         HC(hipStreamSynchronize(g->st));
@@ -2567,6 +2579,13 @@ int rg_last_gpu_ms(rg_graph *g, double *ms)
 {
     if (!g || !ms) return RG_EINVAL;
     *ms = g->last_ms;
+    return RG_OK;
+}
+
+int rg_last_unsafe(rg_graph *g, uint64_t *n)
+{
+    if (!g || !n) return RG_EINVAL;
+    *n = g->last_unsafe;
     return RG_OK;
 }
 
