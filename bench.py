@@ -73,6 +73,11 @@ def parse():
     ap.add_argument("--serve-only", type=float, default=0.0, metavar="RATE",
                     help="for profiling: skip the plain leg and the ramp, serve the warmup and timed steps at RATE "
                          "updates/s (the line's plain and ramp fields are then empty)")
+    ap.add_argument("--precheck-steps", type=int, default=6,
+                    help="untimed serving steps at a candidate rate before its warmup and timed steps "
+                         "(0: none); a candidate whose p99.9 misses the target backs off x0.9 at once")
+    ap.add_argument("--attempt-seconds", type=float, default=240.0,
+                    help="wall-clock budget of the timed attempts; past it each back-off is x0.75")
     ap.add_argument("--no-split", action="store_true",
                     help="every update of a batch waits for the batch's repair (no safe/unsafe split)")
     ap.add_argument("--cap", choices=["unsafe", "total"], default="unsafe",
@@ -125,6 +130,31 @@ def serve(g, labels, stream, cur, n, rate, cap, cap_max, target_ms, split=True, 
     res, lat = g.serve(labels, stream, ins_first, del_first, n, rate, batch_cap=cap, cap_max=cap_max,
                        target_ms=target_ms, latencies=True, split=split, cap_unsafe=cap_unsafe)
     return res, lat
+
+
+LAT_BINS = 1 << 21   # 1-us latency bins up to ~2.1 s; the last bin holds everything above
+
+
+def lat_hist(lat):
+    """1-us histogram of one round's latencies (ms); unserved updates of an
+    overloaded round (+inf) land in the last bin."""
+    b = np.multiply(lat, 1000.0)
+    np.clip(b, 0.0, LAT_BINS - 1, out=b)
+    return np.bincount(b.astype(np.int64), minlength=LAT_BINS)
+
+
+def hist_percentiles(h, qs):
+    """Percentiles (ms) of the latencies counted in h: the upper edge of the
+    1-us bin that holds rank ceil(q n) - 1, so at most 1 us above the exact
+    order statistic and never below it (2097.152 ms stands for anything
+    beyond the histogram)."""
+    c = np.cumsum(h)
+    n = int(c[-1])
+    out = {}
+    for q in qs:
+        k = min(n - 1, int(np.ceil(q / 100.0 * n)) - 1)
+        out[q] = (int(np.searchsorted(c, k + 1)) + 1) / 1000.0
+    return out
 
 
 def serve_batched(g, labels, stream, cur, n, rate, args):
@@ -253,12 +283,14 @@ def main():
         n_step = int(max(20000, rate * args.step_seconds))
         for _ in range(args.warmup):
             serve_batched(g, labels, stream, cur, n_step, rate, args)
-        lats, acc = [], dict(wall=0.0, total=0, batches=0, grid_steps=0, gpu_ms=0.0, overloaded=0, unsafe=0,
-                             safe_p50=[], unsafe_p50=[], unsafe_p999=[])
+        hist = np.zeros(LAT_BINS, np.int64)
+        acc = dict(wall=0.0, total=0, batches=0, grid_steps=0, gpu_ms=0.0, overloaded=0, unsafe=0,
+                   safe_p50=[], unsafe_p50=[], unsafe_p999=[])
         tp0 = g.profile_read(1)   # the timed steps' engine team launches (HIP events)
         for k in range(args.steps):
             res, lat = serve_batched(g, labels, stream, cur, n_step, rate, args)
-            lats.append(lat)
+            hist += lat_hist(lat)
+            del lat
             acc["wall"] += res["wall_s"]
             acc["total"] += res["served"]
             acc["batches"] += res["batches"]
@@ -275,18 +307,42 @@ def main():
         tp1 = g.profile_read(1)
         acc["team_launches"] = tp1["launches"] - tp0["launches"]
         acc["team_ms"] = tp1["ms"] - tp0["ms"]
-        lat_all = np.concatenate(lats)      # unserved updates of an overloaded step count as +inf
-        p = {q: float(np.percentile(lat_all, q)) for q in (50, 99, 99.9)}
+        p = hist_percentiles(hist, (50, 99, 99.9))   # unserved updates of an overloaded step count as +inf
         value = acc["total"] / acc["wall"]
         log(f"timed {args.steps} steps at {rate:.0f}/s: {value:.0f} upd/s p50={p[50]:.2f} p99={p[99]:.2f} "
             f"p99.9={p[99.9]:.2f}ms")
         return value, p, acc, n_step
 
+    # a short untimed check at the rate before the warmup and timed steps
+    # are spent on it: the stream's slow regions recur every few seconds
+    # (one circle of the stream), which a few steps already cover
+    def precheck(rate):
+        n_step = int(max(20000, rate * args.step_seconds))
+        hist, over = np.zeros(LAT_BINS, np.int64), 0
+        for _ in range(args.precheck_steps):
+            res, lat = serve_batched(g, labels, stream, cur, n_step, rate, args)
+            hist += lat_hist(lat)
+            del lat
+            over += res["overloaded"]
+        p999 = hist_percentiles(hist, (99.9,))[99.9]
+        ok = p999 <= args.target_ms and not over
+        log(f"precheck {args.precheck_steps} steps at {rate:.0f}/s: p99.9={p999:.2f}ms {'ok' if ok else 'miss'}")
+        return ok, p999
+
     attempts = []
-    for i in range(8):
+    deadline = time.time() + args.attempt_seconds
+    met, result = False, None
+    for i in range(12):
         if i:
-            rate *= 0.9
-        value, p, acc, n_step = timed(rate)
+            # past the time budget the rate drops faster, so the run ends in time
+            rate *= 0.9 if time.time() < deadline else 0.75
+        if args.precheck_steps and i < 11:
+            ok, p999 = precheck(rate)
+            if not ok:
+                attempts.append({"offered": round(rate, 1), "precheck_p99.9_ms": round(p999, 3)})
+                continue
+        result = timed(rate)
+        value, p, acc, n_step = result
         met = p[99.9] <= args.target_ms and not acc["overloaded"]
         attempts.append({"offered": round(rate, 1), "value": round(value, 1), "p99.9_ms": round(p[99.9], 3)})
         if met:
@@ -376,7 +432,7 @@ def serve_only(g, labels, stream, cur, args, snap, names, root, V, E, n0, info, 
     kernel trace of the command holds the serving kernels alone."""
     rate = args.serve_only
     n_step = int(max(20000, rate * args.step_seconds))
-    lats, wall, total, batches = [], 0.0, 0, 0
+    hist, wall, total, batches = np.zeros(LAT_BINS, np.int64), 0.0, 0, 0
     tp0 = None
     for k in range(args.warmup + args.steps):
         if k == args.warmup:
@@ -385,17 +441,16 @@ def serve_only(g, labels, stream, cur, args, snap, names, root, V, E, n0, info, 
         log(f"serve-only step {k}: {res['updates_per_s']:.0f} upd/s p99.9={res['p999_ms']:.2f}ms "
             f"mean_batch={res['mean_batch']:.1f}")
         if k >= args.warmup:
-            lats.append(lat)
+            hist += lat_hist(lat)
             wall += res["wall_s"]
             total += res["served"]
             batches += res["batches"]
-    lat_all = np.concatenate(lats)
     tp1 = g.profile_read(1)
     roof, roof_store, batch_roof = roofline_sample(g, labels, stream, cur, rate, n_step, args,
                                                    int(total / max(batches, 1)),
                                                    (tp1["launches"] - tp0["launches"], tp1["ms"] - tp0["ms"]))
     out = {"mode": "serve-only profiling run", "offered_rate": rate, "value": round(total / max(wall, 1e-9), 1),
-           "p99.9_ms": round(float(np.percentile(lat_all, 99.9)), 3), "mean_batch": round(total / max(batches, 1), 1),
+           "p99.9_ms": round(hist_percentiles(hist, (99.9,))[99.9], 3), "mean_batch": round(total / max(batches, 1), 1),
            "roofline": roof, "roofline_store": roof_store, "batch_roofline": batch_roof,
            "autotune": args.autotune, "batch_max": args.batch_max}
     print(json.dumps(out), flush=True)

@@ -80,7 +80,7 @@ int rg_edge_update_batch(rg_graph *g, const rg_edge *edges, uint64_t n, int delt
  * d_old (n u64 return values, may be NULL) are DEVICE pointers on the
  * graph's device; the caller's writes to d_edges must be complete before the
  * call.  The batch's working arrays live in a static device array of
- * RG_STATIC_MAX_OPS edge ops' worth (about 3 MB per device, held until the
+ * RG_STATIC_MAX_OPS edge ops' worth (about 5 MB per device, held until the
  * process ends); nothing is allocated and no hipMalloc / hipFree is called.
  * n > RG_STATIC_MAX_OPS returns RG_EINVAL.  A batch holding an id >= V or a
  * weight over 32 bits is rejected whole (RG_EINVAL_VERTEX / RG_EINVAL).

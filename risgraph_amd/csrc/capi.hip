@@ -23,6 +23,7 @@
 #include <limits>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/risgraph_amd.h"
@@ -283,7 +284,7 @@ void ensure_ops(rg_graph *g, uint64_t n)
     g->lops.op = dalloc<LOp>(lc);
     g->lops.segbuf = dalloc<uint32_t>(lc);
     g->lops.heavy = dalloc<uint32_t>(lc);
-    g->lops.owner = dalloc<uint64_t>(lc);
+    g->lops.owner = dalloc<LOp>(lc);
     g->d_flag = dalloc<uint8_t>(cap);
     g->uops.src = dalloc<uint32_t>(cap);
     g->uops.dst = dalloc<uint32_t>(cap);
@@ -388,7 +389,7 @@ constexpr size_t SC_RES = SC_KIND + al256(1ull * STATIC_MAX_OPS);
 constexpr size_t SC_LOP = SC_RES + al256(sizeof(OpResult) * STATIC_MAX_OPS);
 constexpr size_t SC_SEG = SC_LOP + al256(sizeof(LOp) * 4ull * STATIC_MAX_OPS);
 constexpr size_t SC_OWNER = SC_SEG + al256(16ull * STATIC_MAX_OPS);
-constexpr size_t SC_HEAVY = SC_OWNER + al256(32ull * STATIC_MAX_OPS);
+constexpr size_t SC_HEAVY = SC_OWNER + al256(sizeof(LOp) * 4ull * STATIC_MAX_OPS);
 constexpr size_t SCRATCH_BYTES = SC_HEAVY + al256(16ull * STATIC_MAX_OPS);
 
 static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
@@ -1104,7 +1105,7 @@ int rg_edge_update_batch_static(rg_graph *g, const rg_edge *d_edges, uint64_t n,
         ListOps lops{};
         lops.op = (LOp *)(sc + SC_LOP);
         lops.segbuf = (uint32_t *)(sc + SC_SEG);
-        lops.owner = (uint64_t *)(sc + SC_OWNER);
+        lops.owner = (LOp *)(sc + SC_OWNER);
         lops.heavy = (uint32_t *)(sc + SC_HEAVY);
         lops.n = (uint32_t)(4 * n);
         // validate and convert first: a batch with a bad id or weight is
@@ -2683,13 +2684,88 @@ int rg_degrees(rg_graph *g, uint32_t *out_deg, uint32_t *in_deg)
     });
 }
 
-static double percentile(std::vector<double> &v, double q)
+extern "C++" {
+
+// Latency percentiles of a serving round: for each q, the k-th smallest
+// latency (k = ceil(q m) - 1) among the m updates u < n whose want(u) holds,
+// the value nth_element would put at rank k.  Rounds hold tens of millions of
+// updates, so nothing is copied whole: a histogram of 1-us bins finds the bin
+// that holds rank k, and only that bin's values are ordered.
+template <class Want>
+static void percentiles_of(const std::vector<double> &lat, uint64_t n, Want want, const double *q, double *out,
+                           int nq, uint64_t *count)
 {
-    if (v.empty()) return 0;
-    size_t k = (size_t)std::min<double>((double)v.size() - 1, std::ceil(q * (double)v.size()) - 1);
-    std::nth_element(v.begin(), v.begin() + k, v.end());
-    return v[k];
+    constexpr uint32_t NB = 1u << 21;   // up to ~2.1 s; the last bin takes everything above
+    auto bin = [](double x) -> uint32_t {
+        const double b = x * 1000.0;
+        return b >= (double)(NB - 1) ? NB - 1 : (b > 0 ? (uint32_t)b : 0u);
+    };
+    std::vector<uint32_t> h(NB, 0u);
+    uint64_t m = 0;
+    for (uint64_t u = 0; u < n; u++)
+        if (want(u)) {
+            h[bin(lat[u])]++;
+            m++;
+        }
+    if (count) *count = m;
+    std::vector<double> in;
+    for (int i = 0; i < nq; i++) {
+        if (!m) {
+            out[i] = 0;
+            continue;
+        }
+        const uint64_t k = (uint64_t)std::min<double>((double)m - 1, std::ceil(q[i] * (double)m) - 1);
+        uint64_t below = 0;
+        uint32_t b = 0;
+        while (below + h[b] <= k) below += h[b++];
+        in.clear();
+        for (uint64_t u = 0; u < n; u++)
+            if (want(u) && bin(lat[u]) == b) in.push_back(lat[u]);
+        std::nth_element(in.begin(), in.begin() + (k - below), in.end());
+        out[i] = in[k - below];
+    }
 }
+
+// Poisson arrival times of a round: exponential gaps from a counter-based
+// uniform draw per update, summed per chunk on several host threads, then
+// offset by the chunks before (the round's clock starts at 0)
+static void poisson_arrivals(std::vector<double> &arr, uint64_t seed, double rate)
+{
+    const uint64_t n = arr.size();
+    const uint64_t key = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    const unsigned nt = n < (1u << 20) ? 1u : std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+    const uint64_t chunk = (n + nt - 1) / nt;
+    std::vector<double> sum(nt, 0.0);
+    auto fill = [&](unsigned c) {
+        double t = 0;
+        for (uint64_t u = c * chunk; u < std::min(n, (c + 1) * chunk); u++) {
+            uint64_t z = key + (u + 1) * 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            const double uni = ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+            t += -std::log(uni) / rate;
+            arr[u] = t;
+        }
+        sum[c] = t;
+    };
+    auto shift = [&](unsigned c, double off) {
+        for (uint64_t u = c * chunk; u < std::min(n, (c + 1) * chunk); u++) arr[u] += off;
+    };
+    std::vector<std::thread> th;
+    for (unsigned c = 1; c < nt; c++) th.emplace_back(fill, c);
+    fill(0);
+    for (auto &x : th) x.join();
+    th.clear();
+    double off = 0;
+    for (unsigned c = 1; c < nt; c++) {
+        off += sum[c - 1];
+        th.emplace_back(shift, c, off);
+    }
+    for (auto &x : th) x.join();
+}
+
+}  // extern "C++"
 
 int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_params *p, rg_serve_result *r,
              double *lat_ms)
@@ -2701,17 +2777,7 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
         const uint64_t n = p->n_updates;
         // Poisson arrivals: exponential gaps from a counter-based uniform draw
         std::vector<double> arr(n), lat(n);
-        double t = 0;
-        uint64_t key = p->seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
-        for (uint64_t u = 0; u < n; u++) {
-            uint64_t z = key + (u + 1) * 0x9E3779B97F4A7C15ull;
-            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-            z ^= z >> 31;
-            double uni = ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
-            t += -std::log(uni) / p->rate;
-            arr[u] = t;
-        }
+        poisson_arrivals(arr, p->seed, p->rate);
         rg_batch_stats st{};
         // cap: updates per batch, or unsafe updates per batch with cap_unsafe
         // (a batch then admits cap / frac updates, frac = the running unsafe
@@ -2799,13 +2865,16 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
             next = end;
         }
         {
-            std::vector<double> ls, lu;
-            for (uint64_t u = 0; u < served; u++) (waited[u] ? lu : ls).push_back(lat[u]);
-            r->unsafe = lu.size();
-            r->safe_frac = served ? (double)ls.size() / (double)served : 0;
-            r->p50_safe_ms = percentile(ls, 0.50);
-            r->p50_unsafe_ms = percentile(lu, 0.50);
-            r->p999_unsafe_ms = percentile(lu, 0.999);
+            const double q50 = 0.50, qu[2] = {0.50, 0.999};
+            double vu[2];
+            uint64_t n_safe = 0, n_unsafe = 0;
+            percentiles_of(lat, served, [&](uint64_t u) { return waited[u] == 0; }, &q50, &r->p50_safe_ms, 1,
+                           &n_safe);
+            percentiles_of(lat, served, [&](uint64_t u) { return waited[u] != 0; }, qu, vu, 2, &n_unsafe);
+            r->unsafe = n_unsafe;
+            r->safe_frac = served ? (double)n_safe / (double)served : 0;
+            r->p50_unsafe_ms = vu[0];
+            r->p999_unsafe_ms = vu[1];
         }
         r->overloaded = overloaded ? 1 : 0;
         r->served = served;
@@ -2817,16 +2886,23 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
         r->mean_batch = batches ? (double)served / (double)batches : 0;
         r->grid_steps = st.grid_steps;
         r->rebuilds = st.rebuilds;
-        double sum = 0;
-        for (uint64_t u = 0; u < served; u++) sum += lat[u];
+        double sum = 0, mx = 0;
+        for (uint64_t u = 0; u < served; u++) {
+            sum += lat[u];
+            mx = std::max(mx, lat[u]);
+        }
         r->mean_ms = served ? sum / (double)served : 0;
+        r->max_ms = mx;
+        {
+            const double q[3] = {0.50, 0.99, 0.999};
+            double v[3];
+            percentiles_of(lat, served, [](uint64_t) { return true; }, q, v, 3, nullptr);
+            r->p50_ms = v[0];
+            r->p99_ms = v[1];
+            r->p999_ms = v[2];
+        }
         for (uint64_t u = served; u < n; u++) lat[u] = std::numeric_limits<double>::infinity();
         if (lat_ms) std::copy(lat.begin(), lat.end(), lat_ms);
-        std::vector<double> tmp(lat.begin(), lat.begin() + served);
-        r->p50_ms = percentile(tmp, 0.50);
-        r->p99_ms = percentile(tmp, 0.99);
-        r->p999_ms = percentile(tmp, 0.999);
-        r->max_ms = tmp.empty() ? 0 : *std::max_element(tmp.begin(), tmp.end());
     });
 }
 

@@ -10,10 +10,11 @@
 //   full build       build_tree                        graph.hpp:873-985
 //   label update     update_label (16-B CAS form)      graph.hpp:1861-1887
 //
-// The label word is 64 bits: value << 32 | parent vertex.  A relax is a CAS
-// loop that only writes when update_func strictly improves, exactly as the
-// reference's CAS form does, so equal-value races never re-parent a vertex
-// and parent pointers stay acyclic.  Parent identity is the vertex only; a
+// The label word is 64 bits: value << 32 | parent vertex.  A relax writes
+// only when update_func strictly improves, as the reference's CAS form does:
+// a CAS loop for SSWP and WCC, one 64-bit atomicMin for BFS and SSSP (whose
+// parents always hold strictly smaller values), so parent pointers stay
+// acyclic either way (relax() below).  Parent identity is the vertex only; a
 // delete of any (p -> v) edge whose multiplicity reached 0 while v's parent is
 // p marks v, and the rescue step re-parents v onto a surviving parallel edge.
 //
@@ -169,11 +170,28 @@ __device__ __forceinline__ void take_snap_launch(Ctl *c, Snap &s)
     s.seq = l.seq;
 }
 
-// strict-improvement CAS on the packed label word (update_label, CAS form)
+// Strict-improvement update of the packed label word (update_label, CAS
+// form, graph.hpp:1861-1887); true when this call lowered (SSWP: raised) the
+// value.  The callers have already seen the candidate beat a plain load.
+//
+// BFS, and SSSP over an edge of weight >= 1, take one 64-bit atomicMin on
+// value << 32 | parent instead of a CAS loop (no retries when many edges
+// relax one hub at once).  A candidate equal to the stored value can then
+// replace the parent with a smaller id, but such a parent's value is
+// strictly below the child's, so parent chains still strictly decrease and
+// stay acyclic.  SSWP (min(s, w) can equal s), WCC (a child's value equals
+// its parent's) and zero-weight SSSP edges keep the CAS loop, which never
+// re-parents on a tie.
 template <int A>
-__device__ __forceinline__ bool relax(uint64_t *word, uint64_t c, uint32_t parent)
+__device__ __forceinline__ bool relax(uint64_t *word, uint64_t c, uint32_t parent, bool tie_acyclic)
 {
     unsigned long long *p = (unsigned long long *)word;
+    if constexpr (A == BFS || A == SSSP) {
+        if (tie_acyclic) {
+            const unsigned long long prev = atomicMin(p, (unsigned long long)pack((uint32_t)c, parent));
+            return c < (uint64_t)vval(prev);
+        }
+    }
     unsigned long long cur = *p;
     while (better<A>(c, vval(cur))) {
         unsigned long long nw = pack((uint32_t)c, parent);
@@ -431,7 +449,7 @@ __device__ void push_range(const EngineArgs &a, const Snap &s, uint32_t u, uint3
         entry_w<A>(a, off + i, w);
         uint64_t c = cand<A>(uv, w);
         if (!better<A>(c, vval(a.vr[n].val))) continue;
-        if (relax<A>(&a.vr[n].val, c, u)) {
+        if (relax<A>(&a.vr[n].val, c, u, A == BFS || w != 0)) {
             push_next(a, s, n);
             mark_mod(a, s, n);
         }
@@ -546,7 +564,7 @@ template <int A>
 __device__ __forceinline__ void pull_apply(const EngineArgs &a, const Snap &s, uint32_t x, uint64_t best)
 {
     if (A == SSWP ? best == 0ull : best == ~0ull) return;
-    if (relax<A>(&a.vr[x].val, vval(best), vpar(best))) {
+    if (relax<A>(&a.vr[x].val, vval(best), vpar(best), A == BFS)) {
         push_next(a, s, x);
         mark_mod(a, s, x);
     }
@@ -569,7 +587,7 @@ __device__ void seed_edge(const EngineArgs &a, const Snap &s, uint32_t src, uint
 {
     uint64_t c = cand<A>(vval(a.vr[src].val), w);
     if (!better<A>(c, vval(a.vr[dst].val))) return;
-    if (relax<A>(&a.vr[dst].val, c, src)) {
+    if (relax<A>(&a.vr[dst].val, c, src, A == BFS || w != 0)) {
         push_next(a, s, dst);
         mark_mod(a, s, dst);
     }

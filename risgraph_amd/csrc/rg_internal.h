@@ -158,9 +158,11 @@ static_assert(sizeof(LOp) == 32, "list op record is 32 bytes");
 struct ListOps {
     LOp *op;
     uint32_t *segbuf;   // list-op indices grouped by list (segments)
-    // the light lists (dense): rank-0 op index | (vertex | in-list << 31) << 32,
-    // so the apply loads a list's header together with its op record
-    uint64_t *owner;
+    // the light lists (dense): a copy of each list's rank-0 op record with
+    // v = vertex | in-list << 31 and rank = that op's index, written by the
+    // segment kernel in index order and read in order by the apply, so a
+    // single-op list's fields cost no random line
+    LOp *owner;
     uint32_t *heavy;    // rank-0 list ops of the heavy lists (dense)
     uint32_t n;
 };

@@ -320,9 +320,12 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
     }
     __syncthreads();
     if (lt | hv) a.lists[L].h[v].seg = (uint32_t)bbase + wsum[wid][0] + ic - cnt;
-    if (lt)
-        a.lops.owner[(uint32_t)(bbase >> 32) + wsum[wid][1] + il - 1] =
-            (uint64_t)j | ((uint64_t)(v | ((uint32_t)L << 31)) << 32);
+    if (lt) {
+        LOp o = a.lops.op[j];
+        o.v = v | ((uint32_t)L << 31);
+        o.rank = j;
+        a.lops.owner[(uint32_t)(bbase >> 32) + wsum[wid][1] + il - 1] = o;
+    }
     if (hv) a.lops.heavy[hbase + wsum[wid][2] + ih - 1] = j;
 }
 
@@ -692,7 +695,7 @@ __device__ __forceinline__ void apply_single(const StoreArgs &a, ListState &st, 
             if (flags & L_RET) a.ops.res[edge].old = ret;
             if (flags & L_SLOT) a.ops.res[edge].live = live;
         }
-        a.lops.op[k].slot = idx;
+        // (no slot write-back: only multi-op lists read it, in write_liveness)
     }
 }
 
@@ -723,13 +726,12 @@ __device__ void light_lists_apply(const StoreArgs &a, uint32_t blk, uint32_t nbl
     const uint32_t ng = (nblk * blockDim.x) / AG;
     const uint32_t nown = (uint32_t)(a.pool->seg_pack >> 32);
     for (uint32_t oi = gid; oi < nown; oi += ng) {
-        const uint64_t ow = a.lops.owner[oi];
-        const uint32_t j = (uint32_t)ow, vl = (uint32_t)(ow >> 32);
-        const uint32_t v = vl & 0x7FFFFFFFu;
-        const int L = (int)(vl >> 31);
-        // the op's record and the list's header load side by side: most
-        // lists have this one op only
-        const LOp o = a.lops.op[j];
+        // the owner entry carries the rank-0 op's record (most lists have
+        // this one op only), so only the header line is a random access
+        const LOp o = a.lops.owner[oi];
+        const uint32_t j = o.rank;
+        const uint32_t v = o.v & 0x7FFFFFFFu;
+        const int L = (int)(o.v >> 31);
         const uint8_t fl = o.flags;
         const int8_t dj = o.delta;
         const uint32_t key_nbr = o.nbr;
