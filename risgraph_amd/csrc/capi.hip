@@ -973,8 +973,7 @@ int rg_graph_create(uint64_t vertices, uint64_t hint_edges, int symmetric, int d
         g->slots.nbr = dalloc<uint32_t>(g->slot_cap);
         g->slots.cnt = dalloc<uint32_t>(g->slot_cap);
         g->slots.wt = weighted ? dalloc<uint32_t>(g->slot_cap) : nullptr;
-        g->slots.hkey = dalloc<uint64_t>(g->hash_cap);
-        g->slots.hval = dalloc<uint32_t>(g->hash_cap);
+        g->slots.hb = dalloc<HBucket>(g->hash_cap);
         g->pool = dalloc<Pool>(1);
         Pool p{};
         p.slot_cap = g->slot_cap;
@@ -1002,7 +1001,7 @@ int rg_graph_destroy(rg_graph *g)
     if (g->wal_fd >= 0) (void)close(g->wal_fd);
     free_list(g->lists[0]);
     if (!g->symmetric) free_list(g->lists[1]);
-    dfree(g->slots.nbr); dfree(g->slots.cnt); dfree(g->slots.wt); dfree(g->slots.hkey); dfree(g->slots.hval);
+    dfree(g->slots.nbr); dfree(g->slots.cnt); dfree(g->slots.wt); dfree(g->slots.hb);
     dfree(g->pool); dfree(g->err);
     dfree(g->free_off); dfree(g->pend_off); dfree(g->cnt_tmp[0]); dfree(g->cnt_tmp[1]);
     free_ops(g);

@@ -6,10 +6,10 @@
 //               one contiguous block [off, off+cap) of the pool; `len` entries
 //               are used, tombstones (cnt == 0) keep their place, as
 //               IndexedEdgeStorage does (storage.hpp:369-394).
-//   hash pool   hkey[H] (u64 key = nbr | w << 32, ~0 = empty), hval[H] (u32
-//               local slot) -- per-list open-addressing index built once a
-//               list reaches INDEX_THRESHOLD entries (the reference's is
-//               512, storage.hpp:304, 358-367).
+//   hash pool   hb[H]: 16-byte buckets {key (u64 nbr | w << 32, ~0 = empty),
+//               local slot, copy of the entry's count} -- per-list open-
+//               addressing index built once a list reaches INDEX_THRESHOLD
+//               entries (the reference's is 512, storage.hpp:304, 358-367).
 //   headers     per direction (out, in): one 64-byte ListHdr per vertex with
 //               off, len, cap, hoff, hcap, hcnt, deg, and nops / seg (the
 //               current batch's ops on the list and their segment in the
@@ -48,12 +48,23 @@ constexpr uint32_t CHUNK = 128;             // entries per chunk item (two wave 
 constexpr int GROUP = 16;                   // lanes per vertex in vertex passes
 constexpr int WAVE = 64;
 
+// One bucket of a list's hash index: the key, the entry's local index and a
+// copy of the entry's count.  cnt[] stays the count of record (traversal,
+// copies, dumps and host reads use it); every count write on an indexed list
+// updates the copy too, so a probe that hits reads the count from the line it
+// already loaded instead of two more dependent lines (index, then count).
+struct alignas(16) HBucket {
+    unsigned long long key;
+    uint32_t idx;
+    uint32_t cnt;
+};
+static_assert(sizeof(HBucket) == 16, "hash bucket is 16 bytes");
+
 struct Slots {
     uint32_t *nbr;
     uint32_t *wt;
     uint32_t *cnt;
-    uint64_t *hkey;
-    uint32_t *hval;
+    HBucket *hb;
 };
 
 // One 64-byte record per (direction, vertex): a list op of the store touches

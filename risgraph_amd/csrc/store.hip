@@ -409,20 +409,22 @@ __device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, co
     }
     h0 = gbcast<G>(h0, g);
     if (h0 == HEMPTY) return false;
-    for (uint32_t b = g.lane; b < hcap; b += G) a.slots.hkey[h0 + b] = HEMPTY;
+    for (uint32_t b = g.lane; b < hcap; b += G) a.slots.hb[h0 + b].key = HEMPTY;
     __threadfence_block();
     const uint32_t mask = hcap - 1;
     for (uint32_t i0 = g.lane; i0 < st.len; i0 += G * UNR) {
         uint64_t key[UNR];
-        uint32_t bk[UNR];
+        uint32_t bk[UNR], cn[UNR];
 #pragma unroll
         for (int u = 0; u < UNR; u++) {
             const uint32_t i = i0 + u * G;
             key[u] = HEMPTY;
+            cn[u] = 0;
             if (i < st.len) {
                 const uint32_t nb = a.slots.nbr[st.off + i];
                 const uint32_t w = a.slots.wt ? a.slots.wt[st.off + i] : 0;
                 key[u] = hkey_of(nb, w);
+                cn[u] = a.slots.cnt[st.off + i];
             }
             bk[u] = (uint32_t)mix64(key[u]) & mask;
         }
@@ -430,8 +432,8 @@ __device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, co
 #pragma unroll
         for (int u = 0; u < UNR; u++)
             prev[u] = key[u] == HEMPTY ? 0ull
-                                       : atomicCAS((unsigned long long *)&a.slots.hkey[h0 + bk[u]],
-                                                   (unsigned long long)HEMPTY, (unsigned long long)key[u]);
+                                       : atomicCAS(&a.slots.hb[h0 + bk[u]].key, (unsigned long long)HEMPTY,
+                                                   (unsigned long long)key[u]);
 #pragma unroll
         for (int u = 0; u < UNR; u++) {
             if (key[u] == HEMPTY) continue;
@@ -439,11 +441,15 @@ __device__ bool build_index(const StoreArgs &a, ListState &st, uint32_t hcap, co
             while (prev[u] != HEMPTY) {   // taken: linear probing (cannot fill below 50% load)
                 b = (b + 1) & mask;
                 if (++probes >= hcap) break;
-                prev[u] = atomicCAS((unsigned long long *)&a.slots.hkey[h0 + b], (unsigned long long)HEMPTY,
+                prev[u] = atomicCAS(&a.slots.hb[h0 + b].key, (unsigned long long)HEMPTY,
                                     (unsigned long long)key[u]);
             }
-            if (probes >= hcap) atomicOr(&a.pool->oom, 2u);
-            else a.slots.hval[h0 + b] = i0 + u * G;
+            if (probes >= hcap) {
+                atomicOr(&a.pool->oom, 2u);
+            } else {
+                a.slots.hb[h0 + b].idx = i0 + u * G;
+                a.slots.hb[h0 + b].cnt = cn[u];
+            }
         }
     }
     __threadfence_block();
@@ -502,11 +508,13 @@ __device__ void move_block(const StoreArgs &a, ListState &st, uint64_t o, uint32
     st.cap = ncap;
 }
 
-// find key in the list: local index or NILP.  With `ins_bucket`, returns the
-// empty hash bucket where the key would go (hashed lists only).
+// find key in the list: local index or NILP.  Hashed lists also return in
+// *bucket the bucket that holds the key, or the empty bucket where it would
+// go (NILP: index full), and on a hit the count copy of that bucket in
+// *hcnt (the probe's one 16-byte load per lane brings key, index and count).
 template <int G>
 __device__ uint32_t list_find(const StoreArgs &a, const ListState &st, uint32_t key_nbr, uint32_t key_w,
-                              const Grp &g, uint32_t *ins_bucket)
+                              const Grp &g, uint32_t *bucket, uint32_t *hcnt)
 {
     if (st.hoff != HEMPTY) {
         uint64_t key = hkey_of(key_nbr, key_w);
@@ -514,19 +522,24 @@ __device__ uint32_t list_find(const StoreArgs &a, const ListState &st, uint32_t 
         uint32_t h = (uint32_t)mix64(key) & mask;
         for (uint32_t probe = 0; probe < st.hcap; probe += G) {
             uint32_t b = (h + probe + g.lane) & mask;
-            uint64_t k = a.slots.hkey[st.hoff + b];
-            uint32_t hit = gballot<G>(k == key, g);
-            uint32_t emp = gballot<G>(k == HEMPTY, g);
+            const HBucket e = a.slots.hb[st.hoff + b];
+            uint32_t hit = gballot<G>(e.key == key, g);
+            uint32_t emp = gballot<G>(e.key == HEMPTY, g);
             uint32_t any = hit | emp;
             if (any) {
                 int first = __ffs(any) - 1;
-                uint32_t fb = (h + probe + first) & mask;
-                if ((hit >> first) & 1u) return a.slots.hval[st.hoff + fb];
-                if (ins_bucket) *ins_bucket = fb;
+                *bucket = (h + probe + first) & mask;
+                const int src = g.base + first;
+                const uint32_t idx = (uint32_t)__shfl((int)e.idx, src);
+                const uint32_t c = (uint32_t)__shfl((int)e.cnt, src);
+                if ((hit >> first) & 1u) {
+                    *hcnt = c;
+                    return idx;
+                }
                 return NILP;
             }
         }
-        if (ins_bucket) *ins_bucket = NILP;
+        *bucket = NILP;
         return NILP;
     }
     // Linear scan, 8 entries per lane per step (two 16-byte loads in flight):
@@ -580,8 +593,10 @@ __device__ void apply_serial(const StoreArgs &a, ListState &st, const uint32_t *
             if ((pass == 0) != (delta > 0)) continue;
             uint32_t key_nbr = a.lops.op[k].nbr;
             uint32_t key_w = a.slots.wt ? a.lops.op[k].w : 0;
-            uint32_t bucket = NILP;
-            uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, &bucket);
+            const bool hashed = st.hoff != HEMPTY;
+            uint32_t bucket = NILP, hc = 0;
+            uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, &bucket, &hc);
+            uint32_t c = 0;
             if (idx == NILP && delta > 0) {
                 // append a new entry with count 0 (storage.hpp:371-374, 389-392)
                 idx = st.len;
@@ -589,16 +604,15 @@ __device__ void apply_serial(const StoreArgs &a, ListState &st, const uint32_t *
                     a.slots.nbr[st.off + idx] = key_nbr | DEAD;
                     a.slots.cnt[st.off + idx] = 0;
                     if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
-                    if (st.hoff != HEMPTY && bucket != NILP) {
-                        a.slots.hkey[st.hoff + bucket] = hkey_of(key_nbr, key_w);
-                        a.slots.hval[st.hoff + bucket] = idx;
-                    }
+                    if (hashed && bucket != NILP)
+                        a.slots.hb[st.hoff + bucket] = HBucket{hkey_of(key_nbr, key_w), idx, 0u};
                 }
                 st.len++;
-                if (st.hoff != HEMPTY) st.hcnt++;
+                if (hashed) st.hcnt++;
                 __threadfence_block();
+            } else if (idx != NILP) {
+                c = hashed ? hc : a.slots.cnt[st.off + idx];
             }
-            uint32_t c = idx == NILP ? 0 : a.slots.cnt[st.off + idx];
             uint32_t nc, ret;
             int32_t dchange;
             switch (delta) {
@@ -613,6 +627,7 @@ __device__ void apply_serial(const StoreArgs &a, ListState &st, const uint32_t *
             }
             if (idx != NILP && g.lane == 0) {
                 a.slots.cnt[st.off + idx] = nc;
+                if (hashed && bucket != NILP) a.slots.hb[st.hoff + bucket].cnt = nc;
                 if ((c == 0) != (nc == 0))
                     a.slots.nbr[st.off + idx] = key_nbr | (nc ? 0u : DEAD);
             }
@@ -650,25 +665,24 @@ __device__ __forceinline__ void apply_single(const StoreArgs &a, ListState &st, 
                                              uint32_t rcap, uint64_t roff, const Grp &g)
 {
     if (!light_room(a, st, st.len + (delta > 0 ? 1u : 0u), rcap, roff, g)) return;
-    uint32_t bucket = NILP, c = 0;
-    uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, &bucket);
+    const bool hashed = st.hoff != HEMPTY;
+    uint32_t bucket = NILP, c = 0, hc = 0;
+    bool appended = false;
+    uint32_t idx = list_find<AG>(a, st, key_nbr, key_w, g, &bucket, &hc);
     if (idx == NILP) {
         if (delta > 0) {
             // append a new entry (storage.hpp:371-374, 389-392)
             idx = st.len;
+            appended = true;
             if (g.lane == 0) {
                 a.slots.nbr[st.off + idx] = key_nbr | DEAD;
                 if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
-                if (st.hoff != HEMPTY && bucket != NILP) {
-                    a.slots.hkey[st.hoff + bucket] = hkey_of(key_nbr, key_w);
-                    a.slots.hval[st.hoff + bucket] = idx;
-                }
             }
             st.len++;
-            if (st.hoff != HEMPTY) st.hcnt++;
+            if (hashed) st.hcnt++;
         }
     } else {
-        c = a.slots.cnt[st.off + idx];
+        c = hashed ? hc : a.slots.cnt[st.off + idx];
     }
     uint32_t nc, ret;
     int32_t dchange;
@@ -686,6 +700,10 @@ __device__ __forceinline__ void apply_single(const StoreArgs &a, ListState &st, 
     if (g.lane == 0) {
         if (idx != NILP) {
             a.slots.cnt[st.off + idx] = nc;
+            if (hashed && bucket != NILP) {
+                if (appended) a.slots.hb[st.hoff + bucket] = HBucket{hkey_of(key_nbr, key_w), idx, nc};
+                else a.slots.hb[st.hoff + bucket].cnt = nc;
+            }
             if ((c == 0) != (nc == 0)) a.slots.nbr[st.off + idx] = key_nbr | (nc ? 0u : DEAD);
         }
         const uint32_t live = (idx != NILP && nc > 0) ? 1u : 0u;
@@ -834,25 +852,29 @@ __device__ bool wg_build_index(const StoreArgs &a, ListState &st, uint32_t hcap,
     const uint64_t h0 = sh.bcast;
     __syncthreads();
     if (h0 == HEMPTY) return false;
-    for (uint32_t b = threadIdx.x; b < hcap; b += HB) a.slots.hkey[h0 + b] = HEMPTY;
+    for (uint32_t b = threadIdx.x; b < hcap; b += HB) a.slots.hb[h0 + b].key = HEMPTY;
     __syncthreads();
     const uint32_t mask = hcap - 1;
     for (uint32_t i0 = threadIdx.x; i0 < st.len; i0 += HB * HUNR) {
         uint64_t key[HUNR];
-        uint32_t bk[HUNR];
+        uint32_t bk[HUNR], cn[HUNR];
         unsigned long long prev[HUNR];
 #pragma unroll
         for (int u = 0; u < HUNR; u++) {
             const uint32_t i = i0 + u * HB;
             key[u] = HEMPTY;
-            if (i < st.len) key[u] = hkey_of(a.slots.nbr[st.off + i], a.slots.wt ? a.slots.wt[st.off + i] : 0);
+            cn[u] = 0;
+            if (i < st.len) {
+                key[u] = hkey_of(a.slots.nbr[st.off + i], a.slots.wt ? a.slots.wt[st.off + i] : 0);
+                cn[u] = a.slots.cnt[st.off + i];
+            }
             bk[u] = (uint32_t)mix64(key[u]) & mask;
         }
 #pragma unroll
         for (int u = 0; u < HUNR; u++)
             prev[u] = key[u] == HEMPTY ? 0ull
-                                       : atomicCAS((unsigned long long *)&a.slots.hkey[h0 + bk[u]],
-                                                   (unsigned long long)HEMPTY, (unsigned long long)key[u]);
+                                       : atomicCAS(&a.slots.hb[h0 + bk[u]].key, (unsigned long long)HEMPTY,
+                                                   (unsigned long long)key[u]);
 #pragma unroll
         for (int u = 0; u < HUNR; u++) {
             if (key[u] == HEMPTY) continue;
@@ -860,11 +882,15 @@ __device__ bool wg_build_index(const StoreArgs &a, ListState &st, uint32_t hcap,
             while (prev[u] != HEMPTY) {   // linear probing (cannot fill below 50% load)
                 b = (b + 1) & mask;
                 if (++probes >= hcap) break;
-                prev[u] = atomicCAS((unsigned long long *)&a.slots.hkey[h0 + b], (unsigned long long)HEMPTY,
+                prev[u] = atomicCAS(&a.slots.hb[h0 + b].key, (unsigned long long)HEMPTY,
                                     (unsigned long long)key[u]);
             }
-            if (probes >= hcap) atomicOr(&a.pool->oom, 2u);
-            else a.slots.hval[h0 + b] = i0 + u * HB;
+            if (probes >= hcap) {
+                atomicOr(&a.pool->oom, 2u);
+            } else {
+                a.slots.hb[h0 + b].idx = i0 + u * HB;
+                a.slots.hb[h0 + b].cnt = cn[u];
+            }
         }
     }
     __syncthreads();
@@ -919,20 +945,21 @@ __device__ void wg_apply_rounds(const StoreArgs &a, ListState &st, const uint32_
             }
             __syncthreads();
             if (act && pre == 0) {
-                uint32_t idx = NILP, bucket = NILP;
+                uint32_t idx = NILP, bucket = NILP, hb_at = NILP, c = 0;
                 uint32_t b = (uint32_t)mix64(key) & mask;
                 for (uint32_t p = 0; p < st.hcap; p++, b = (b + 1) & mask) {
-                    const uint64_t kk = a.slots.hkey[st.hoff + b];
-                    if (kk == key) {
-                        idx = a.slots.hval[st.hoff + b];
+                    const HBucket e = a.slots.hb[st.hoff + b];
+                    if (e.key == key) {
+                        idx = e.idx;
+                        c = e.cnt;
+                        hb_at = b;
                         break;
                     }
-                    if (kk == HEMPTY) {
+                    if (e.key == HEMPTY) {
                         bucket = b;
                         break;
                     }
                 }
-                uint32_t c = 0;
                 if (idx == NILP && pass == 0 && bucket != NILP) {
                     // a key new to the list: append an entry with count 0
                     idx = st.len + atomicAdd(&sh.n_app, 1u);
@@ -941,21 +968,21 @@ __device__ void wg_apply_rounds(const StoreArgs &a, ListState &st, const uint32_
                     if (a.slots.wt) a.slots.wt[st.off + idx] = key_w;
                     uint32_t bb = bucket;
                     for (uint32_t p = 0; p < st.hcap; p++, bb = (bb + 1) & mask) {
-                        if (atomicCAS((unsigned long long *)&a.slots.hkey[st.hoff + bb], (unsigned long long)HEMPTY,
+                        if (atomicCAS(&a.slots.hb[st.hoff + bb].key, (unsigned long long)HEMPTY,
                                       (unsigned long long)key) == HEMPTY) {
-                            a.slots.hval[st.hoff + bb] = idx;
+                            a.slots.hb[st.hoff + bb].idx = idx;
+                            hb_at = bb;
                             break;
                         }
                     }
-                } else if (idx != NILP) {
-                    c = a.slots.cnt[st.off + idx];
-                } else if (pass == 0) {
+                } else if (idx == NILP && pass == 0) {
                     atomicOr(&a.pool->oom, 2u);   // index full (cannot happen at load <= 1/2)
                 }
                 if (idx != NILP) {
                     const uint32_t tot = sh.cnt[ls];
                     const uint32_t cf = pass == 0 ? c + tot : (c > tot ? c - tot : 0);
                     a.slots.cnt[st.off + idx] = cf;
+                    if (hb_at != NILP) a.slots.hb[st.hoff + hb_at].cnt = cf;
                     if ((c == 0) != (cf == 0)) a.slots.nbr[st.off + idx] = key_nbr | (cf ? 0u : DEAD);
                     atomicAdd(&sh.sum, (int32_t)cf - (int32_t)c);
                 }
