@@ -204,6 +204,7 @@ struct rg_labels {
     Tuning tune{};
     uint32_t team_w = 0;           // workgroups per tree in the team launch (0: by batch size)
     uint32_t block_max = RG_AUTO;  // largest pass (vertices, or ops/16) run by the team
+    uint32_t chunk_max = RG_AUTO;  // most chunk items of a pass run by the team
     uint32_t run_w = 8;            // team width of the current run
     uint32_t team_weight = 0;      // share of the team's XCD slots (0: by algorithm)
     uint32_t *init_active = nullptr;
@@ -570,6 +571,15 @@ uint32_t team_cap(const rg_labels *L)
     return (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, 256ull * L->run_w * x4);
 }
 
+// chunk items a team pass keeps (256 per workgroup, scaled like team_cap);
+// a pass with more hands its chunk stage to grid mode
+uint32_t team_chunk_cap(const rg_labels *L)
+{
+    if (L->chunk_max != RG_AUTO) return L->chunk_max;
+    const uint32_t x4 = L->g->run_tune ? L->g->run_tune->cap_x4 : 4u;
+    return (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, 64ull * L->run_w * x4);
+}
+
 // batch-size bucket of the tuning table: up to 32k, 128k, 512k edge ops, more
 int tune_bucket(uint64_t n_ops)
 {
@@ -627,6 +637,18 @@ bool team_ok(const rg_labels *L, const Ctl &h)
 {
     return h.phase != PH_BUILD_INIT && !(h.phase == PH_PUSH && (h.flags & F_DENSE)) && h.stage == 0 &&
            host_work(h, L->g->V) <= team_cap(L);
+}
+
+// The first pass of a split run works on the batch's unsafe ops, whose count
+// the split kernel publishes on the device after the host has routed the
+// pass: the host mirror still holds the whole batch's op count then, so an
+// op pass (seed or deletion roots) of such a run starts in the team.  The
+// team checks the published count itself and hands a pass that is still too
+// large to grid mode before running it.
+bool team_ok_first(const rg_labels *L, const Ctl &h, bool split)
+{
+    if (split && h.stage == 0 && (h.phase == PH_SEED || h.phase == PH_DEL_ROOTS)) return true;
+    return team_ok(L, h);
 }
 
 // seed: a mixed batch starts with its deletions and seeds its inserts in the
@@ -712,7 +734,7 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
         engine_init(L[i], phase0, seed, n_ops, new_idx);
         // a run that starts in the team hands its initial control block to
         // the kernel through the pinned mirror; otherwise it is copied below
-        if (team_ok(L[i], *L[i]->hc)) pending_init[i] = 1;
+        if (team_ok_first(L[i], *L[i]->hc, split)) pending_init[i] = 1;
     }
     if (hooks && hooks->before_launch) hooks->before_launch();
     for (int i = 0; i < n; i++) {
@@ -747,7 +769,7 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
             on_own[i] = 0;
             if (!active[i]) continue;
             const Ctl &h = *L[i]->hc;
-            if (team_ok(L[i], h)) {
+            if (guard_iter == 0 ? team_ok_first(L[i], h, split) : team_ok(L[i], h)) {
                 team.push_back(i);
             } else {
                 // a grid-wide step on the tree's own stream; it turns into a
@@ -778,6 +800,7 @@ void engine_run_multi(rg_labels **L, int n, uint32_t phase0, bool seed, uint32_t
                 tl.a[t] = L[i]->dargs;
                 tl.k[t] = one_launch ? kslots[i] : XCD_SLOTS;
                 tl.work_max[t] = team_cap(L[i]);
+                tl.chunk_max[t] = team_chunk_cap(L[i]);
                 if (pending_init[i]) tl.init_mask |= 1u << t;
                 pending_init[i] = 0;
             }
@@ -1264,6 +1287,7 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         if (const char *tw = getenv("RG_TEAM_W"); tw && *tw)
             L->team_w = (uint32_t)std::max(1, std::min(64, atoi(tw)));
         if (const char *bm = getenv("RG_BLOCK_MAX"); bm && *bm) L->block_max = (uint32_t)strtoul(bm, nullptr, 0);
+        if (const char *cm = getenv("RG_CHUNK_MAX"); cm && *cm) L->chunk_max = (uint32_t)strtoul(cm, nullptr, 0);
         HC(hipMemcpyAsync(L->ctl, L->hc, sizeof(Ctl), hipMemcpyHostToDevice, g->st));
         HC(hipStreamSynchronize(g->st));
     });

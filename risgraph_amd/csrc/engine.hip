@@ -325,16 +325,22 @@ __device__ __forceinline__ void mark_mod(const EngineArgs &a, const Snap &s, uin
 
 enum ChunkOp : uint32_t { CO_PUSH = 0, CO_RESCUE = 1, CO_EXPAND = 2, CO_PULL = 3 };
 
-// the first `len` entries of v's list at slot `off`, as chunk items
+__device__ __forceinline__ uint32_t scan_rot(uint32_t d, int32_t idx);
+
+// the first `len` entries of v's list at slot `off`, as chunk items; a
+// rescue scan's items start at a rotated chunk (scan_rot), since the first
+// items of the queue are the first to run
 __device__ __forceinline__ void defer_chunks(const EngineArgs &a, const Snap &s, uint32_t v, uint64_t off,
                                              uint32_t len, ChunkOp op)
 {
     const uint32_t nch = (len + CHUNK - 1) / CHUNK;
+    const uint32_t c0 = op == CO_RESCUE ? scan_rot(v, s.idx) % nch : 0u;
     auto item = [&](uint32_t k) {
+        const uint32_t ck = k + c0 < nch ? k + c0 : k + c0 - nch;
         ChunkItem it;
-        it.off = off + (uint64_t)k * CHUNK;
+        it.off = off + (uint64_t)ck * CHUNK;
         it.v = v;
-        it.n_op = min(CHUNK, len - k * CHUNK) | ((uint32_t)op << 16);
+        it.n_op = min(CHUNK, len - ck * CHUNK) | ((uint32_t)op << 16);
         return it;
     };
     // the items that fit go to the stage, the rest straight to the queue
@@ -467,11 +473,34 @@ __device__ void push_range(const EngineArgs &a, const Snap &s, uint32_t u, uint3
 // d's value: d's new value is at least that (the neighbour's tree path
 // survives the run), which lets the expand leave alone the children whose
 // value does not exceed it.
+//
+// A scan starts at a per-run hash of d and wraps around (the reference scans
+// from the front, graph.hpp:1330-1359).  List positions follow the order in
+// which keys first entered the list, so a front-first scan makes rescues
+// adopt the oldest entries; a sliding window over a circular stream deletes
+// those together when it comes round to them, and the adopted tree edges
+// then fall in one burst (on R-MAT scale 24, every circle of the stream
+// brought a stretch with two to three times the usual share of unsafe
+// updates).  Every candidate the test accepts gives d the same value.
+__device__ __forceinline__ uint32_t scan_rot(uint32_t d, int32_t idx)
+{
+    uint32_t h = d * 0x9E3779B1u ^ (uint32_t)idx * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return h;
+}
+
 template <int A>
 __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uint32_t dv, uint64_t off,
                              uint32_t b, uint32_t e, int lane, int stride, uint32_t &tb)
 {
-    for (uint32_t i = b + lane; i < e; i += stride) {
+    const uint32_t len = e - b;
+    const uint32_t rot = len ? scan_rot(d, s.idx) % len : 0u;
+    for (uint32_t j = (uint32_t)lane; j < len; j += stride) {
+        const uint32_t i = b + (j + rot < len ? j + rot : j + rot - len);
         if (a.vr[d].inv != s.idx) break;
         uint32_t n = a.slots.nbr[off + i];
         if (n & DEAD) continue;
@@ -1415,7 +1444,7 @@ void launch_engine_team(const TeamLaunch &tl, hipStream_t st)
         P.a[t] = tl.k[t] ? tl.a[t] : any;
         P.k[t] = tl.k[t];
         P.work_max[t] = tl.work_max[t];
-        P.chunk_max[t] = 256u * rounds * tl.k[t];
+        P.chunk_max[t] = tl.chunk_max[t];
     }
     P.rounds = rounds;
     P.init_mask = tl.init_mask;

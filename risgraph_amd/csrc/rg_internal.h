@@ -359,6 +359,7 @@ struct TeamLaunch {
     const EngineArgs *a[MAXT];   // device copies of the trees' arguments
     uint32_t k[MAXT];
     uint32_t work_max[MAXT];
+    uint32_t chunk_max[MAXT];    // a pass with more chunk items hands its chunk stage to grid mode
     uint32_t rounds;
     uint32_t init_mask;
 };
