@@ -15,13 +15,14 @@ Legs:
   serve   continuous batching with the safe/unsafe split (safe updates
           complete once their store batch is applied and classified; unsafe
           ones after the repair) and the adaptive admission cap, which counts
-          unsafe updates (up to 2^21 updates per batch): offered rate
+          unsafe updates (up to 2^20 updates per batch): offered rate
           ramped x1.5 per short calibration round while the server keeps up
           (>= 95% served, p99 within the target; two misses in a row end the
-          ramp), then two bisection rounds; the W warmup and K timed
-          steps run at the best passing rate (a step = one serving round of
-          ~step_seconds of arrivals) and must themselves meet the target,
-          else the rate backs off x0.9 (up to 8 attempts) and they run again;
+          ramp), then two bisection rounds; a candidate rate is prechecked
+          for a few untimed steps, then the W warmup and K timed steps run
+          at it (a step = one serving round of ~step_seconds of arrivals)
+          and must themselves meet the target, else the rate backs off x0.95
+          (up to 12 attempts, within a wall-clock budget) and both run again;
   check   every incremental tree is compared with a fresh full rebuild on the
           final graph (values bit-exact);
   cpu     the reference's rt serving loop restated on all host threads
@@ -75,7 +76,7 @@ def parse():
                          "updates/s (the line's plain and ramp fields are then empty)")
     ap.add_argument("--precheck-steps", type=int, default=6,
                     help="untimed serving steps at a candidate rate before its warmup and timed steps "
-                         "(0: none); a candidate whose p99.9 misses the target backs off x0.9 at once")
+                         "(0: none); a candidate whose p99.9 misses the target backs off x0.95 at once")
     ap.add_argument("--attempt-seconds", type=float, default=240.0,
                     help="wall-clock budget of the timed attempts; past it each back-off is x0.75")
     ap.add_argument("--no-split", action="store_true",
@@ -111,17 +112,13 @@ class Cursor:
 
 
 # Batched serving uses the library's adaptive batch cap (rg_serve): it
-# starts at 64 updates, grows x1.5 while the backlog exceeds it and
+# starts at 16 unsafe updates, grows x1.5 while the backlog exceeds it and
 # otherwise shrinks x0.7 when a batch's device time exceeds a quarter of the
-# latency target, up to BATCH_MAX.  (Taking every arrived update per batch, with no cap,
-# met the p99.9 target less often: 6 of 8 half-second rounds at 80M
-# updates/s against 7 of 8.  The stream has periodic regions where twice the
-# usual share of updates is unsafe (17-22% instead of 8%) and batches cost
-# twice as much; a backlog built there drains in batches of the cap.  Over
-# 14 half-second windows at 180M updates/s the two windows of each such
-# region had p99.9 15.6 / 29.4 and 24.7 / 25.5 ms with a cap of 2^20, against
-# 19.2 / 40.5 and 26.3 / 35.3 ms with 2^21 and 25.2 / 49.1 and 29.8 / 43.0 ms
-# with 2^19; the other windows were 5-9 ms under all three.)
+# latency target, up to BATCH_MAX updates.  (Taking every arrived update per
+# batch, with no cap, met the p99.9 target less often: 6 of 8 half-second
+# rounds at 80M updates/s against 7 of 8.  Over 14 half-second windows at
+# 250M updates/s offered, a cap of 2^20 gave p99.9 19.2 ms and 2^21 21.7 ms;
+# at 235M both gave 16.5 ms.)
 BATCH_MAX = 1 << 20
 
 
@@ -335,7 +332,7 @@ def main():
     for i in range(12):
         if i:
             # past the time budget the rate drops faster, so the run ends in time
-            rate *= 0.9 if time.time() < deadline else 0.75
+            rate *= 0.95 if time.time() < deadline else 0.75
         if args.precheck_steps and i < 11:
             ok, p999 = precheck(rate)
             if not ok:
