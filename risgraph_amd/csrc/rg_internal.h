@@ -36,12 +36,13 @@ constexpr uint32_t DEAD = 0x80000000u;
 constexpr uint32_t NBR_MASK = 0x7FFFFFFFu;
 constexpr uint32_t NILP = 0xFFFFFFFFu;
 // A list gets its open-addressing index at this many entries.  The
-// reference indexes at 512 (storage.hpp:304); on the device a 4-lane scan of
-// a few hundred entries is a chain of dependent loads, so lists are indexed
-// from 128 entries (indexes start at 256 buckets).  Returns and counts do
-// not depend on it.
-constexpr uint32_t INDEX_THRESHOLD = 128;
-constexpr uint32_t INDEX_MIN_CAP = 256;
+// reference indexes at 512 (storage.hpp:304); on the device a 2-lane scan of
+// even a hundred entries is a chain of dependent loads, while a 16-byte
+// bucket brings key, slot and count in one line, so lists are indexed from
+// 32 entries (indexes start at 64 buckets).  Returns and counts do not
+// depend on it.
+constexpr uint32_t INDEX_THRESHOLD = 32;
+constexpr uint32_t INDEX_MIN_CAP = 64;
 constexpr uint64_t HEMPTY = ~0ull;
 constexpr uint32_t LONG_LIST = 32;          // longer lists go to the chunk queue
 constexpr uint32_t CHUNK = 128;             // entries per chunk item (two wave steps)
