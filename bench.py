@@ -18,7 +18,7 @@ Legs:
           unsafe updates (up to 2^20 updates per batch): offered rate
           ramped x1.5 per short calibration round while the server keeps up
           (>= 95% served, p99 within the target; two misses in a row end the
-          ramp), then two bisection rounds; a candidate rate is prechecked
+          ramp), then three bisection rounds; a candidate rate is prechecked
           for a few untimed steps, then the W warmup and K timed steps run
           at it (a step = one serving round of ~step_seconds of arrivals)
           and must themselves meet the target, else the rate backs off x0.95
@@ -235,7 +235,7 @@ def main():
     if plain is None:
         plain = dict(res, offered=rate)
 
-    # ---- batched serve: calibration ramp (x1.5), then two bisection steps ----
+    # ---- batched serve: calibration ramp (x1.5), then three bisection steps ----
     # a calibration round passes when the server keeps up: not overloaded,
     # >= 95% of the offered rate served and p99 within the target.  The p99.9
     # of a short round is set by one or two large deletion subtrees, so it
@@ -264,7 +264,7 @@ def main():
                 break
         rate *= 1.5
     if best is not None and fail is not None:
-        for _ in range(2):
+        for _ in range(3):
             mid = (best * fail) ** 0.5
             if calibrate(mid):
                 best = mid
