@@ -219,12 +219,13 @@ int rg_graph_capture_info(rg_graph *g, int *on, uint32_t *pipelines, uint64_t *r
  * inserted one deleted: the graph and the labels' values return to where
  * they were) under candidate settings and keeps, per batch-size bucket
  * (<= 32k, <= 128k, <= 512k, more edge ops), the fastest: the team cap
- * (passes larger than cap_x4/4 x 1024 vertices per team workgroup go
- * grid-wide) and the team weights of the four algorithms.  Later batches of
- * the graph run with the table.  With target_ms > 0 it also returns
- * batch_cap, the largest serving batch whose measured device time, doubled
- * (for stream regions with twice the usual unsafe share), fits a quarter of
- * the target (the serving loop's own shrink rule), for
+ * (passes larger than cap_x4/4 x 1024 vertices, or cap_x4/4 x 1024 chunk
+ * items, per team workgroup go grid-wide) and the team weights of the four
+ * algorithms.  Later batches of the graph run with the table.  With
+ * target_ms > 0 it also returns batch_cap, the largest serving batch whose
+ * measured device time, doubled (an update of a capped batch also waits for
+ * the batch before it), fits a quarter of the target (the serving loop's
+ * own shrink rule), for
  * rg_serve_params.cap_max.  Every edge the deleted window holds must be
  * live (it is when the windows are the stream's current window, as in the
  * serving loop); the degrees are checked afterwards and a graph that did not
