@@ -319,7 +319,14 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segment(StoreArgs a)
         hbase = th ? atomicAdd(&a.pool->n_heavy, th) : 0u;
     }
     __syncthreads();
-    if (lt | hv) a.lists[L].h[v].seg = (uint32_t)bbase + wsum[wid][0] + ic - cnt;
+    if (lt | hv) {
+        const uint32_t seg = (uint32_t)bbase + wsum[wid][0] + ic - cnt;
+        a.lists[L].h[v].seg = seg;
+        // the rank-0 op takes the first place of its segment here (segments
+        // of neighbouring ops lie side by side, so these writes coalesce);
+        // a light list with this one op never reads its segment
+        if (hv || cnt > 1) a.lops.segbuf[seg] = j;
+    }
     if (lt) {
         LOp o = a.lops.op[j];
         o.v = v | ((uint32_t)L << 31);
@@ -334,7 +341,9 @@ __global__ void k_scatter(StoreArgs a)
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= a.lstride * op_count(a)) return;
     const uint32_t v = a.lops.op[j].v;
-    if (v == NILP) return;
+    // rank-0 ops were placed by k_segment: only lists with several ops of
+    // the batch reach the header read below
+    if (v == NILP || a.lops.op[j].rank == 0) return;
     const int L = (a.lops.op[j].flags & L_IN) ? 1 : 0;
     a.lops.segbuf[a.lists[L].h[v].seg + a.lops.op[j].rank] = j;
 }
