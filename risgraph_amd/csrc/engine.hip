@@ -1526,16 +1526,19 @@ void launch_classify(const EngineArgs &a, uint32_t n, uint8_t *unsafe, hipStream
     }
 }
 
+// need_update_tree_add / need_update_tree_del (graph.hpp:987-1000,
+// 1220-1233) from the label words of the op's source (ws) and target (wd)
 template <int A>
-__device__ __forceinline__ bool op_unsafe(const VRec *vr, bool ins, bool both, uint32_t s, uint32_t d, uint32_t w)
+__device__ __forceinline__ bool op_unsafe(uint64_t ws, uint64_t wd, bool ins, bool both, uint32_t s, uint32_t d,
+                                          uint32_t w)
 {
     if (ins) {
-        bool u = better<A>(cand<A>(vval(vr[s].val), w), vval(vr[d].val));
-        if (both) u = u || better<A>(cand<A>(vval(vr[d].val), w), vval(vr[s].val));
+        bool u = better<A>(cand<A>(vval(ws), w), vval(wd));
+        if (both) u = u || better<A>(cand<A>(vval(wd), w), vval(ws));
         return u;
     }
-    bool u = vpar(vr[d].val) == s;
-    if (both) u = u || vpar(vr[s].val) == d;
+    bool u = vpar(wd) == s;
+    if (both) u = u || vpar(ws) == d;
     return u;
 }
 
@@ -1562,13 +1565,27 @@ __global__ void k_split(SplitParams P, uint32_t n)
         const bool ins = (k & K_INS) != 0;
         const bool cand = (k & K_FORCE) || (ins ? (r.old == 0 && r.live != 0) : ((k & K_DEL) && r.old == 1));
         if (cand) {
-            for (int t = 0; t < P.ntrees && !u; t++) {
+            // every tree's label words are loaded before any is tested
+            // (most ops are safe in every tree and need them all)
+            uint64_t ws[MAXT], wd[MAXT];
+#pragma unroll
+            for (int t = 0; t < MAXT; t++) {
+                ws[t] = wd[t] = 0;
+                if (t < P.ntrees) {
+                    const bool both = P.undirected[t] || (k & K_UNDIR);
+                    wd[t] = P.vr[t][d].val;
+                    if (ins || both) ws[t] = P.vr[t][s].val;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < MAXT; t++) {
+                if (t >= P.ntrees) continue;
                 const bool both = P.undirected[t] || (k & K_UNDIR);
                 switch (P.algo[t]) {
-                case BFS: u = op_unsafe<BFS>(P.vr[t], ins, both, s, d, w); break;
-                case SSSP: u = op_unsafe<SSSP>(P.vr[t], ins, both, s, d, w); break;
-                case SSWP: u = op_unsafe<SSWP>(P.vr[t], ins, both, s, d, w); break;
-                default: u = op_unsafe<WCC>(P.vr[t], ins, both, s, d, w); break;
+                case BFS: u |= op_unsafe<BFS>(ws[t], wd[t], ins, both, s, d, w); break;
+                case SSSP: u |= op_unsafe<SSSP>(ws[t], wd[t], ins, both, s, d, w); break;
+                case SSWP: u |= op_unsafe<SSWP>(ws[t], wd[t], ins, both, s, d, w); break;
+                default: u |= op_unsafe<WCC>(ws[t], wd[t], ins, both, s, d, w); break;
                 }
             }
         }
