@@ -801,7 +801,10 @@ __device__ void light_lists_apply(const StoreArgs &a, uint32_t blk, uint32_t nbl
 // ---------------------------------------------------------------------
 constexpr int HB = 256;           // threads per heavy list
 constexpr uint32_t HT = 2 * HB;   // LDS key table of one round (load <= 1/2)
-constexpr int HUNR = 8;           // entries per thread in flight in copies
+// entries per thread in flight in copies and index builds: 4, not 8, keeps
+// k_apply_all at 74 VGPRs (6 waves per SIMD instead of 4), which its light
+// lists' dependent loads need more (a 512k-update batch's apply 386 -> 314 us)
+constexpr int HUNR = 4;
 
 struct HeavyShared {
     unsigned long long key[HT];
