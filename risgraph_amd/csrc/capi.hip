@@ -2828,6 +2828,15 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
         SplitOut so;
         so.flags = p->split != 0;
         std::vector<uint64_t> unsafe_u;   // the current batch's updates that wait for the repair
+        // the previous batch's, completed at t_prev: their latencies are
+        // written while the next batch runs on the device (in its safe-answer
+        // step), not between the two batches, where the device would idle
+        std::vector<uint64_t> unsafe_prev;
+        double t_prev = 0;
+        auto settle_prev = [&] {
+            for (uint64_t u : unsafe_prev) lat[u] = (t_prev - arr[u]) * 1e3;
+            unsafe_prev.clear();
+        };
         bool timed = true;                // latencies of the current batch are recorded
         auto run = [&](uint64_t b, uint64_t e) {
             // updates [b, e): adds are the even indices, deletes the odd ones
@@ -2840,6 +2849,7 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
             // following it
             so.on_safe = [&, b, e, a0, a1, d0](const uint8_t *flag) {
                 const double ts = std::chrono::duration<double>(so.t_safe - t0).count();
+                settle_prev();
                 for (uint64_t u = b; u < e; u++) {
                     // op index of update u: inserts first, then deletes
                     const uint64_t op = (u & 1) ? (a1 - a0) + (u / 2 - d0) : u / 2 - a0;
@@ -2864,8 +2874,10 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
             uint64_t end = std::min<uint64_t>(arrived, next + (uint64_t)std::floor(limit()));
             run(next, end);
             double tc = now();
+            settle_prev();   // a batch without the split had no safe-answer step
             if (so.done) {
-                for (uint64_t u : unsafe_u) lat[u] = (tc - arr[u]) * 1e3;
+                std::swap(unsafe_u, unsafe_prev);
+                t_prev = tc;
             } else {
                 for (uint64_t u = next; u < end; u++) lat[u] = (tc - arr[u]) * 1e3;
             }
@@ -2885,6 +2897,7 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
         }
         const uint64_t served = next;
         double wall = now();
+        settle_prev();
         // an overloaded run still applies the rest (untimed) so the stream
         // window stays consistent for the next run
         timed = false;
