@@ -15,8 +15,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cerrno>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <fcntl.h>
 #include <unistd.h>
 #include <functional>
@@ -2755,6 +2757,76 @@ static void percentiles_of(const std::vector<double> &lat, uint64_t n, Want want
     }
 }
 
+// Persistent helper threads of a serving loop.  post() hands out jobs
+// 0..n-1 and returns at once; wait() returns when every job of the last
+// post has run (and rethrows the first exception a job threw).
+class HostPool {
+public:
+    explicit HostPool(unsigned n)
+    {
+        for (unsigned i = 0; i < n; i++) th_.emplace_back([this] { loop(); });
+    }
+    ~HostPool()
+    {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void post(unsigned n, std::function<void(unsigned)> f)
+    {
+        wait();
+        {
+            std::lock_guard<std::mutex> l(m_);
+            f_ = std::move(f);
+            n_ = n;
+            next_ = 0;
+            left_ = n;
+        }
+        cv_.notify_all();
+    }
+    void wait()
+    {
+        std::unique_lock<std::mutex> l(m_);
+        done_.wait(l, [this] { return left_ == 0; });
+        if (err_) {
+            std::exception_ptr e = err_;
+            err_ = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+
+private:
+    void loop()
+    {
+        std::unique_lock<std::mutex> l(m_);
+        for (;;) {
+            cv_.wait(l, [this] { return stop_ || next_ < n_; });
+            if (stop_) return;
+            const unsigned j = next_++;
+            l.unlock();
+            std::exception_ptr e;
+            try {
+                f_(j);
+            } catch (...) {
+                e = std::current_exception();
+            }
+            l.lock();
+            if (e && !err_) err_ = e;
+            if (--left_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    std::function<void(unsigned)> f_;
+    unsigned n_ = 0, next_ = 0, left_ = 0;
+    bool stop_ = false;
+    std::exception_ptr err_;
+};
+
 // Poisson arrival times of a round: exponential gaps from a counter-based
 // uniform draw per update, summed per chunk on several host threads, then
 // offset by the chunks before (the round's clock starts at 0)
@@ -2827,40 +2899,63 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
         };
         SplitOut so;
         so.flags = p->split != 0;
-        std::vector<uint64_t> unsafe_u;   // the current batch's updates that wait for the repair
-        // the previous batch's, completed at t_prev: their latencies are
-        // written while the next batch runs on the device (in its safe-answer
-        // step), not between the two batches, where the device would idle
-        std::vector<uint64_t> unsafe_prev;
+        // The safe-answer step of a batch runs on helper threads while this
+        // thread goes on driving the repair (a tree that leaves the team
+        // launch needs the host for its grid-wide steps), split into SAFE_PARTS
+        // ranges; at a million updates one thread took as long as the repair.
+        // Each range keeps the updates that wait for the repair; those of the
+        // previous batch, completed at t_prev, get their latencies in the
+        // same step (one more job), so the device does not idle between
+        // batches for it.
+        constexpr unsigned SAFE_PARTS = 4;
+        HostPool pool(SAFE_PARTS);
+        std::vector<std::vector<uint64_t>> unsafe_u(SAFE_PARTS), unsafe_prev(SAFE_PARTS);
         double t_prev = 0;
         auto settle_prev = [&] {
-            for (uint64_t u : unsafe_prev) lat[u] = (t_prev - arr[u]) * 1e3;
-            unsafe_prev.clear();
+            for (auto &part : unsafe_prev) {
+                for (uint64_t u : part) lat[u] = (t_prev - arr[u]) * 1e3;
+                part.clear();
+            }
         };
         bool timed = true;                // latencies of the current batch are recorded
+        double t_repair = 0;              // the current batch's repair completed
         auto run = [&](uint64_t b, uint64_t e) {
             // updates [b, e): adds are the even indices, deletes the odd ones
             uint64_t a0 = (b + 1) / 2, a1 = (e + 1) / 2;
             uint64_t d0 = b / 2, d1 = e / 2;
-            unsafe_u.clear();
-            // once the split's flags are on the host, and while the repair
-            // still runs on the device, the safe updates are answered here:
-            // the host's per-update work overlaps the repair instead of
-            // following it
             so.on_safe = [&, b, e, a0, a1, d0](const uint8_t *flag) {
                 const double ts = std::chrono::duration<double>(so.t_safe - t0).count();
-                settle_prev();
-                for (uint64_t u = b; u < e; u++) {
-                    // op index of update u: inserts first, then deletes
-                    const uint64_t op = (u & 1) ? (a1 - a0) + (u / 2 - d0) : u / 2 - a0;
-                    const uint8_t w = flag[op];
-                    waited[u] = w;
-                    if (w) unsafe_u.push_back(u);
-                    else if (timed) lat[u] = (ts - arr[u]) * 1e3;
-                }
+                pool.post(SAFE_PARTS + 1, [&, b, e, a0, a1, d0, flag, ts](unsigned j) {
+                    if (j == SAFE_PARTS) {
+                        settle_prev();
+                        return;
+                    }
+                    auto &mine = unsafe_u[j];
+                    mine.clear();
+                    const uint64_t pb = b + (e - b) * j / SAFE_PARTS, pe = b + (e - b) * (j + 1) / SAFE_PARTS;
+                    for (uint64_t u = pb; u < pe; u++) {
+                        // op index of update u: inserts first, then deletes
+                        const uint64_t op = (u & 1) ? (a1 - a0) + (u / 2 - d0) : u / 2 - a0;
+                        const uint8_t w = flag[op];
+                        waited[u] = w;
+                        if (w) mine.push_back(u);
+                        else if (timed) lat[u] = (ts - arr[u]) * 1e3;
+                    }
+                });
             };
-            apply_batch(g, L, nL, s, (p->ins_first + a0) % s->n, a1 - a0, (p->del_first + d0) % s->n, d1 - d0,
-                        p->directed, &st, p->split ? &so : nullptr);
+            try {
+                apply_batch(g, L, nL, s, (p->ins_first + a0) % s->n, a1 - a0, (p->del_first + d0) % s->n,
+                            d1 - d0, p->directed, &st, p->split ? &so : nullptr);
+            } catch (...) {
+                // the helpers may still read the batch's flags
+                try {
+                    pool.wait();
+                } catch (...) {
+                }
+                throw;
+            }
+            t_repair = now();
+            pool.wait();
             gpu_ms += g->last_ms;
         };
         while (next < n) {
@@ -2873,7 +2968,7 @@ int rg_serve(rg_graph *g, rg_labels **L, int nL, rg_stream *s, const rg_serve_pa
             uint64_t arrived = (uint64_t)(std::upper_bound(arr.begin() + next, arr.end(), tn) - arr.begin());
             uint64_t end = std::min<uint64_t>(arrived, next + (uint64_t)std::floor(limit()));
             run(next, end);
-            double tc = now();
+            const double tc = t_repair;
             settle_prev();   // a batch without the split had no safe-answer step
             if (so.done) {
                 std::swap(unsafe_u, unsafe_prev);
