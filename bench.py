@@ -21,8 +21,10 @@ Legs:
           ramp), then three bisection rounds; a candidate rate is prechecked
           for a few untimed steps, then the W warmup and K timed steps run
           at it (a step = one serving round of ~step_seconds of arrivals)
-          and must themselves meet the target, else the rate backs off x0.95
-          (up to 12 attempts, within a wall-clock budget) and both run again;
+          and must themselves meet the target, else the rate backs off (x0.95
+          after a precheck miss, x0.97 after timed steps whose precheck passed:
+          the rate is then near the limit) and both run again (up to 12
+          attempts, within a wall-clock budget);
   check   every incremental tree is compared with a fresh full rebuild on the
           final graph (values bit-exact);
   cpu     the reference's rt serving loop restated on all host threads
@@ -329,10 +331,14 @@ def main():
     attempts = []
     deadline = time.time() + args.attempt_seconds
     met, result = False, None
+    timed_miss = False
     for i in range(12):
         if i:
-            # past the time budget the rate drops faster, so the run ends in time
-            rate *= 0.95 if time.time() < deadline else 0.75
+            # a rate whose precheck passed but whose timed steps missed is near
+            # the limit: a finer step; past the time budget the rate drops
+            # faster, so the run ends in time
+            rate *= (0.97 if timed_miss else 0.95) if time.time() < deadline else 0.75
+        timed_miss = False
         if args.precheck_steps and i < 11:
             ok, p999 = precheck(rate)
             if not ok:
@@ -344,6 +350,7 @@ def main():
         attempts.append({"offered": round(rate, 1), "value": round(value, 1), "p99.9_ms": round(p[99.9], 3)})
         if met:
             break
+        timed_miss = True
     wall, total, batches = acc["wall"], acc["total"], acc["batches"]
     grid_steps, gpu_ms, overloaded = acc["grid_steps"], acc["gpu_ms"], acc["overloaded"]
 
