@@ -506,6 +506,14 @@ __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uin
         if (n & DEAD) continue;
         uint32_t w;
         entry_w<A>(a, off + i, w);
+        // the weight alone rules most entries out, before their label load:
+        // SSSP needs val[n] + w == dv, so w <= dv; SSWP min(val[n], w) is at
+        // most w, which must reach dv, or pass tb for the bound
+        if constexpr (A == SSSP) {
+            if (w > dv) continue;
+        } else if constexpr (A == SSWP) {
+            if (w < dv && w <= tb) continue;
+        }
         const uint64_t c = cand<A>(vval(a.vr[n].val), w);
         if (c != (uint64_t)dv) {
             if constexpr (A == SSWP) {
@@ -948,7 +956,8 @@ __device__ void pass_chunks(const EngineArgs &a, const Snap &s, uint32_t tid, ui
             push_range<A>(a, s, v, vval(a.vr[v].val), off, b, e, lane, WAVE);
         } else if (op == CO_RESCUE) {
             if (a.vr[v].inv != s.idx) continue;
-            uint32_t tb = 0;
+            // SSWP: offers at or under v's bound so far add nothing
+            uint32_t tb = A == SSWP ? a.bnd[v] : 0u;
             rescue_range<A>(a, s, v, vval(a.vr[v].val), off, b, e, lane, WAVE, tb);
             if constexpr (A == SSWP) {
                 for (int m = WAVE / 2; m > 0; m >>= 1) tb = max(tb, (uint32_t)__shfl_xor((int)tb, m, WAVE));
