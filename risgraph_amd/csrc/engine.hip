@@ -539,6 +539,16 @@ __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uin
 // still equals val[c]: c keeps its value and its parent, and so does its whole
 // subtree (values only shrink along a path).  Such children are not marked;
 // the others inherit bx.
+// SSWP: a child reached over an entry of weight w <= bx has val[c] =
+// min(val[x], w) <= bx, so that entry is passed over before the child's
+// label load (another entry to the same child, the tree edge, still counts)
+template <int A>
+__device__ __forceinline__ bool expand_skip(const EngineArgs &a, uint64_t slot, uint32_t bx)
+{
+    if constexpr (A == SSWP) return a.slots.wt[slot] <= bx;
+    else return false;
+}
+
 template <int A>
 __device__ __forceinline__ bool expand_child(const EngineArgs &a, const Snap &s, uint32_t x, uint32_t c,
                                              uint32_t bx)
@@ -559,7 +569,7 @@ __device__ void expand_range(const EngineArgs &a, const Snap &s, uint32_t x, uin
 {
     for (uint32_t i = b + lane; i < e; i += stride) {
         uint32_t c = a.slots.nbr[off + i];
-        if (c & DEAD) continue;
+        if ((c & DEAD) || expand_skip<A>(a, off + i, bx)) continue;
         if (expand_child<A>(a, s, x, c, bx)) stage_next(a, s, c);
     }
 }
@@ -734,7 +744,7 @@ __device__ void expand_cont(const EngineArgs &a, const Snap &s, uint32_t x, uint
         uint32_t c = 0;
         if (i < len) {
             c = a.slots.nbr[off + i];
-            if (!(c & DEAD)) mk = expand_child<A>(a, s, x, c, bx);
+            if (!(c & DEAD) && !expand_skip<A>(a, off + i, bx)) mk = expand_child<A>(a, s, x, c, bx);
         }
         const bool keep = mk && depth < CONT_DEPTH;
         const uint32_t kb = group_ballot<G>(keep);
@@ -990,7 +1000,8 @@ __device__ __forceinline__ void advance_state(const EngineArgs &a, Ctl &c)
         c.levels += 1;
     };
     auto rescue_on = [&](uint32_t level, uint32_t n) {
-        return (a.tune.rescue_level != RESCUE_NEVER && level <= a.tune.rescue_level) || n < a.tune.rescue_frontier;
+        return (a.tune.rescue_level != RESCUE_NEVER && level <= a.tune.rescue_level) || n < a.tune.rescue_frontier ||
+               (a.tune.rescue_tree && (uint64_t)c.n_tree + n >= a.tune.rescue_tree);
     };
     auto recompute = [&](uint32_t level, uint32_t n) {
         uint64_t tot = (uint64_t)c.n_tree + n;
