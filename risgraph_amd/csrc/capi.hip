@@ -1290,9 +1290,6 @@ int rg_labels_create(rg_graph *g, int algo, uint64_t root, const uint64_t *init_
         // frontier are cheap atomics here)
         L->tune.dense = 0;
         if (const char *de = getenv("RG_DENSE"); de && *de == '1') L->tune.dense = 1;
-        L->tune.rescue_tree = 0;
-        if (const char *rt = getenv("RG_RESCUE_TREE"); rt && *rt && (algo == RG_SSSP || algo == RG_BFS))
-            L->tune.rescue_tree = (uint32_t)atol(rt);
         // environment overrides; empty values are ignored
         if (const char *tw = getenv("RG_TEAM_W"); tw && *tw)
             L->team_w = (uint32_t)std::max(1, std::min(64, atoi(tw)));

@@ -1000,8 +1000,7 @@ __device__ __forceinline__ void advance_state(const EngineArgs &a, Ctl &c)
         c.levels += 1;
     };
     auto rescue_on = [&](uint32_t level, uint32_t n) {
-        return (a.tune.rescue_level != RESCUE_NEVER && level <= a.tune.rescue_level) || n < a.tune.rescue_frontier ||
-               (a.tune.rescue_tree && (uint64_t)c.n_tree + n >= a.tune.rescue_tree);
+        return (a.tune.rescue_level != RESCUE_NEVER && level <= a.tune.rescue_level) || n < a.tune.rescue_frontier;
     };
     auto recompute = [&](uint32_t level, uint32_t n) {
         uint64_t tot = (uint64_t)c.n_tree + n;

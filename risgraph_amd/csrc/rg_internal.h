@@ -255,7 +255,6 @@ struct Tuning {
     float recompute_frac;         // ... and > frac * V
     uint32_t walk_limit;          // has_parents step cap (graph.hpp:1900)
     uint32_t dense;               // builds switch to pull passes by the reference's rule (0: push only)
-    uint32_t rescue_tree;         // ... or subtree + frontier >= this (0: never)
 };
 
 // One chunk of a long list for a pass's chunk stage.  It carries where its
