@@ -420,7 +420,10 @@ int rg_degrees(rg_graph *g, uint32_t *out_deg, uint32_t *in_deg);
  * target is the plain one-update-at-a-time path.  When the oldest waiting
  * request is older than max(10 * target, 200 ms) the run is overloaded: the
  * rest is applied untimed in cap_max batches and r->overloaded is set.
- * Latency = completion time - arrival time (no coordinated omission). */
+ * Latency = completion time - arrival time (no coordinated omission).
+ * The per-update bookkeeping of a split batch (flags, latencies) runs on
+ * four host threads of the call's own while the calling thread drives the
+ * batch's repair. */
 typedef struct rg_serve_params {
     uint64_t ins_first, del_first, n_updates;
     double rate;
