@@ -579,25 +579,6 @@ __device__ void expand_range(const EngineArgs &a, const Snap &s, uint32_t x, uin
 // being reset in the same pass (a build's dense passes have no subtree: its
 // init pass cleared every mark)
 template <int A>
-__device__ uint64_t pull_range(const EngineArgs &a, const Snap &s, uint64_t off, uint32_t b, uint32_t e,
-                               int lane, int stride)
-{
-    uint64_t best = (A == SSWP) ? 0ull : ~0ull;
-    for (uint32_t i = b + lane; i < e; i += stride) {
-        uint32_t n = a.slots.nbr[off + i];
-        if (n & DEAD) continue;
-        if (!(s.flags & F_BUILD) && a.vr[n].inv == s.idx) continue;
-        uint32_t w;
-        entry_w<A>(a, off + i, w);
-        uint64_t c = cand<A>(vval(a.vr[n].val), w);
-        if (A != SSWP && c > 0xFFFFFFFFull) continue;
-        uint64_t word = pack((uint32_t)c, n);
-        if (A == SSWP ? word > best : word < best) best = word;
-    }
-    return best;
-}
-
-template <int A>
 __device__ __forceinline__ uint64_t reduce_best(uint64_t x, int width)
 {
     for (int m = width / 2; m > 0; m >>= 1) {
@@ -605,6 +586,39 @@ __device__ __forceinline__ uint64_t reduce_best(uint64_t x, int width)
         if (A == SSWP ? y > x : y < x) x = y;
     }
     return x;
+}
+
+// Steps are uniform across the G-lane group (the bounds depend on the list
+// only).  Between steps the group shares its best candidate, and an entry
+// whose weight alone shows it cannot beat that is skipped before its label
+// load: an SSSP candidate val + w is at least w, an SSWP min(val, w) at most
+// w (strictly, so ties still compare parents).
+template <int A>
+__device__ uint64_t pull_range(const EngineArgs &a, const Snap &s, uint64_t off, uint32_t b, uint32_t e,
+                               int lane, int stride)
+{
+    uint64_t best = (A == SSWP) ? 0ull : ~0ull;
+    for (uint32_t base = b; base < e; base += stride) {
+        const uint32_t i = base + (uint32_t)lane;
+        if (i < e) {
+            const uint32_t n = a.slots.nbr[off + i];
+            uint32_t w;
+            entry_w<A>(a, off + i, w);
+            bool skip = (n & DEAD) != 0;
+            if constexpr (A == SSSP) skip |= w > vval(best);
+            else if constexpr (A == SSWP) skip |= w < vval(best);
+            if (!skip && !((s.flags & F_BUILD) == 0 && a.vr[n].inv == s.idx)) {
+                const uint64_t c = cand<A>(vval(a.vr[n].val), w);
+                if (A == SSWP || c <= 0xFFFFFFFFull) {
+                    const uint64_t word = pack((uint32_t)c, n);
+                    if (A == SSWP ? word > best : word < best) best = word;
+                }
+            }
+        }
+        if constexpr (A == SSSP || A == SSWP)
+            if (base + stride < e) best = reduce_best<A>(best, stride);
+    }
+    return best;
 }
 
 template <int A>
