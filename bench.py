@@ -21,9 +21,9 @@ Legs:
           ramp), then three bisection rounds; a candidate rate is prechecked
           for a few untimed steps, then the W warmup and K timed steps run
           at it (a step = one serving round of ~step_seconds of arrivals)
-          and must themselves meet the target, else the rate backs off (x0.95
-          after a precheck miss, x0.97 after timed steps whose precheck passed:
-          the rate is then near the limit) and both run again (up to 12
+          and must themselves meet the target, else the rate backs off (x0.97
+          when it is near the limit: its precheck passed, or missed by at most
+          half the target; else x0.95) and both run again (up to 12
           attempts, within a wall-clock budget);
   check   every incremental tree is compared with a fresh full rebuild on the
           final graph (values bit-exact);
@@ -78,7 +78,8 @@ def parse():
                          "updates/s (the line's plain and ramp fields are then empty)")
     ap.add_argument("--precheck-steps", type=int, default=6,
                     help="untimed serving steps at a candidate rate before its warmup and timed steps "
-                         "(0: none); a candidate whose p99.9 misses the target backs off x0.95 at once")
+                         "(0: none); a candidate whose p99.9 misses the target backs off at once "
+                         "(x0.97 when it missed by at most half the target, else x0.95)")
     ap.add_argument("--attempt-seconds", type=float, default=240.0,
                     help="wall-clock budget of the timed attempts; past it each back-off is x0.75")
     ap.add_argument("--no-split", action="store_true",
@@ -331,18 +332,19 @@ def main():
     attempts = []
     deadline = time.time() + args.attempt_seconds
     met, result = False, None
-    timed_miss = False
+    near = False
     for i in range(12):
         if i:
-            # a rate whose precheck passed but whose timed steps missed is near
-            # the limit: a finer step; past the time budget the rate drops
-            # faster, so the run ends in time
-            rate *= (0.97 if timed_miss else 0.95) if time.time() < deadline else 0.75
-        timed_miss = False
+            # a rate that missed narrowly (its precheck passed, or missed by at
+            # most half the target) is near the limit: a finer step; past the
+            # time budget the rate drops faster, so the run ends in time
+            rate *= (0.97 if near else 0.95) if time.time() < deadline else 0.75
+        near = False
         if args.precheck_steps and i < 11:
             ok, p999 = precheck(rate)
             if not ok:
                 attempts.append({"offered": round(rate, 1), "precheck_p99.9_ms": round(p999, 3)})
+                near = p999 <= 1.5 * args.target_ms
                 continue
         result = timed(rate)
         value, p, acc, n_step = result
@@ -350,7 +352,7 @@ def main():
         attempts.append({"offered": round(rate, 1), "value": round(value, 1), "p99.9_ms": round(p[99.9], 3)})
         if met:
             break
-        timed_miss = True
+        near = True
     wall, total, batches = acc["wall"], acc["total"], acc["batches"]
     grid_steps, gpu_ms, overloaded = acc["grid_steps"], acc["gpu_ms"], acc["overloaded"]
 
