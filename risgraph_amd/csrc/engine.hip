@@ -508,11 +508,15 @@ __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uin
         entry_w<A>(a, off + i, w);
         // the weight alone rules most entries out, before their label load:
         // SSSP needs val[n] + w == dv, so w <= dv; SSWP min(val[n], w) is at
-        // most w, which must reach dv, or pass tb for the bound
+        // most w, which must reach dv, or pass tb for the bound; BFS needs
+        // val[n] == dv - 1
         if constexpr (A == SSSP) {
             if (w > dv) continue;
         } else if constexpr (A == SSWP) {
             if (w < dv && w <= tb) continue;
+        } else if constexpr (A == BFS) {
+            // value 0 is the root's alone: a vertex at level 1 has no other parent
+            if (dv == 1 && n != a.root) continue;
         }
         const uint64_t c = cand<A>(vval(a.vr[n].val), w);
         if (c != (uint64_t)dv) {
