@@ -576,7 +576,7 @@ def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch, team
     tot_bytes, eng_bytes, tot_ms, changed = 0, 0, 0.0, 0
     team_l, team_ms, grid_ms = 0, 0.0, 0.0
     batch = max(2, mean_batch - mean_batch % 2)
-    nb = 3
+    nb = 8   # change sets vary widely from batch to batch (3 batches gave 54-110 MB per launch)
     for _ in range(nb):
         before = [L.read()[0] for L in labels]
         t0, r0 = g.profile_read(1), g.profile_read(2)
@@ -604,7 +604,7 @@ def roofline_sample(g, labels, stream, cur, rate, n_step, args, mean_batch, team
         "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
         "algorithmic_bytes_per_launch": round(per_launch, 1),
-        "bytes_basis": "changed-vertex set of 3 sample batches at the operating batch size "
+        "bytes_basis": f"changed-vertex set of {nb} sample batches at the operating batch size "
                        f"({batch} updates): {B_VERTEX} B per changed vertex + live degree x "
                        f"{B_EDGE} B per tree, over their {team_l} team launches",
         "avg_launch_ms": round(team_avg_ms, 5), "launches": team_l_timed,
