@@ -518,7 +518,10 @@ __device__ bool rescue_range(const EngineArgs &a, const Snap &s, uint32_t d, uin
             // value 0 is the root's alone: a vertex at level 1 has no other parent
             if (dv == 1 && n != a.root) continue;
         }
-        const uint64_t c = cand<A>(vval(a.vr[n].val), w);
+        const uint64_t nv = a.vr[n].val;
+        // a child of d hangs below d's mark: its walk would end at d at once
+        if (vpar(nv) == d) continue;
+        const uint64_t c = cand<A>(vval(nv), w);
         if (c != (uint64_t)dv) {
             if constexpr (A == SSWP) {
                 if (c > tb && c < dv && !has_parents<A>(a, s, n, s.idx)) tb = (uint32_t)c;
